@@ -1,0 +1,133 @@
+/*
+ * midfusion.h — C ABI of the MI355X (gfx950) implementation of MID-Fusion's
+ * per-frame dense path: depth preprocessing, ICP + photometric Gauss-Newton
+ * tracking of the camera and of moving objects, octree TSDF block
+ * allocation, projective integration and raycasting.
+ *
+ * The entry points mirror the reference pipeline facade `Kfusion`
+ * (apps/kfusion/include/kernels.h:22-361) and the per-object calls of
+ * `Object` / `Tracking` (object/object.h:33-154, tracking/tracking.h:34-409).
+ * Plain pointers and sizes only.  Host pointers unless a name ends in
+ * `_device`.  Every call is stream-ordered on the context's HIP stream and
+ * returns 0 (or a non-negative count/id) on success, a negative MF_E_* code
+ * on failure; nothing calls exit().
+ */
+#ifndef MIDFUSION_H
+#define MIDFUSION_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF_OK 0
+#define MF_E_ARG (-1)       /* bad argument / image ratio (preprocessing.cpp:76-88 exits) */
+#define MF_E_OOM (-2)       /* device allocation failed */
+#define MF_E_HIP (-3)       /* HIP runtime error */
+#define MF_E_CAPACITY (-4)  /* block pool of a volume is full */
+
+#define MF_ROBUST_NONE 0
+#define MF_ROBUST_HUBER 1
+#define MF_ROBUST_TUKEY 2
+#define MF_ROBUST_CAUCHY 3   /* reference default, k = 2.3849 (tracking.h:342-345) */
+
+typedef struct mf_ctx mf_ctx;
+
+/* Configuration: the `Configuration` fields the dense path reads
+ * (include/config.h:15-63, default_parameters.h:22-57). */
+typedef struct mf_config {
+  int width, height;    /* computation size */
+  int n_levels;         /* pyramid levels, <= 4 (reference: 3) */
+  int iters[4];         /* GN iterations per level, [0] = finest (reference {10,5,4}) */
+  int use_icp, use_rgb; /* Tracking(..., using_ICP, using_RGB) */
+  int robust;           /* MF_ROBUST_* */
+  double robust_k;      /* robust-function constant */
+  float mu;             /* TSDF truncation (default_parameters.h:25) */
+  int track_rate, integ_rate;
+  int init_frame;       /* segment_startFrame_: the frame that only seeds the reference */
+  int batch_objects;    /* 1: all moving objects share one launch per GN iteration */
+} mf_config;
+
+/* Kfusion::Kfusion (kernels.h:115-170): allocates every frame buffer and the
+ * background object (id 0, identity pose, volume res^3 of side size_m). */
+mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size_m, int bg_res, long bg_capacity);
+void mf_destroy(mf_ctx* ctx);
+/* Last error message of the context (static storage of the context). */
+const char* mf_last_error(mf_ctx* ctx);
+
+/* Object::Object (object/object.cpp:19-46).  Returns the instance id
+ * (= instance_label_), capacity = max voxel blocks (0: default). */
+int mf_object_create(mf_ctx* ctx, float size_m, int res, const float T_w_o[16], long capacity);
+/* Object::set_static_state (object.cpp:317-320): moving objects are tracked
+ * by trackEachObject and masked out of the camera re-track. */
+int mf_object_set_moving(mf_ctx* ctx, int id, int moving);
+/* Generalised integration mask of an instance (instance_seg::generalize_label
+ * output, segmentation.cpp:10-47): fg value >= 0 inside, -1 outside.  NULL
+ * clears it (the instance is then not integrated). */
+int mf_object_set_mask(mf_ctx* ctx, int id, const float* mask);
+int mf_object_set_mask_device(mf_ctx* ctx, int id, const float* d_mask);
+
+/* Kfusion::preprocessing(depth, rgb, inSize, filter) (kfusion.cpp:210-235):
+ * mm->metres, grey + rgb/255, optional bilateral filter of the depth. */
+int mf_preprocess(mf_ctx* ctx, const uint16_t* depth_mm, const uint8_t* rgb8, int in_w, int in_h, int filter);
+int mf_preprocess_device(mf_ctx* ctx, const uint16_t* d_depth_mm, const uint8_t* d_rgb8, int in_w, int in_h,
+                         int filter);
+/* Kfusion::tracking(k, rate, frame) (kfusion.cpp:237-488).  Returns 1 if
+ * tracking ran, 0 if skipped (rate / seeding frame). */
+int mf_track(mf_ctx* ctx, const float k[4], unsigned frame);
+/* Kfusion::integration(k, rate, mu, frame) (kfusion.cpp:573-677). */
+int mf_integrate(mf_ctx* ctx, const float k[4], unsigned frame);
+/* Kfusion::raycasting(k, mu, frame) (kfusion.cpp:539-569). */
+int mf_raycast(mf_ctx* ctx, const float k[4], unsigned frame);
+/* Kfusion::synchroniseDevices (kfusion.cpp:1697-1699).  Also reports a
+ * pending capacity overflow as MF_E_CAPACITY. */
+int mf_sync(mf_ctx* ctx);
+
+/* Kfusion::getPose (kernels.h:282-284) for id < 0; Object::volume_pose_
+ * otherwise.  Row-major 4x4. */
+int mf_get_pose(mf_ctx* ctx, int id, float T[16]);
+int mf_set_pose(mf_ctx* ctx, int id, const float T[16]);
+
+/* Raycast output (Kfusion::vertex/normal + raycast label image): AoS float3
+ * per pixel; any pointer may be NULL. */
+int mf_get_maps(mf_ctx* ctx, float* vertex, float* normal, int32_t* label);
+int mf_set_maps(mf_ctx* ctx, const float* vertex, const float* normal, const int32_t* label);
+/* Object::m_vertex / m_normal. */
+int mf_get_object_maps(mf_ctx* ctx, int id, float* vertex, float* normal);
+int mf_set_object_maps(mf_ctx* ctx, int id, const float* vertex, const float* normal);
+
+/* Allocated voxel-block keys of an instance in the reference encoding
+ * (keyops::encode, octant_ops.hpp:51-55), ascending.  Returns the count. */
+long mf_object_keys(mf_ctx* ctx, int id, uint64_t* out, long cap);
+/* Voxels of the block holding voxel (x,y,z): 512 x {tsdf, w, r, g, b, c, fg,
+ * fgw} as double, index x + 8y + 64z.  Returns 1, or 0 if unallocated. */
+int mf_object_block(mf_ctx* ctx, int id, int x, int y, int z, double* out);
+/* Number of internal octree nodes implied by the allocated blocks. */
+long mf_object_node_count(mf_ctx* ctx, int id);
+
+/* Level-0 tracking result codes of an instance's last pass (TrackData.result
+ * of Object::trackresult_, ICP rows then RGB rows: 2*W*H int32). */
+int mf_get_trackresult(mf_ctx* ctx, int id, int32_t* codes);
+/* Per-iteration GN trace since the last clear: rows of 22 doubles {instance,
+ * level, iteration, error, inliers, flag, pose[16]}; returns the row count. */
+long mf_get_trace(mf_ctx* ctx, double* out, long cap);
+void mf_clear_trace(mf_ctx* ctx);
+/* Pyramid buffers of the live frame: which = 0 depth, 1 grey, 2 vertex (3/px),
+ * 3 normal (3/px), 4 grad x, 5 grad y, 6 icp covariance (3/px). */
+long mf_get_level(mf_ctx* ctx, int level, int which, float* out);
+/* Preprocessed frame: depth (m), grey, rgb (3/px). */
+int mf_get_frame(mf_ctx* ctx, float* depth, float* grey, float* rgb);
+
+/* Kernel timing: when enabled every launch is bracketed by HIP events on the
+ * context stream.  out[i] = {total_ms, launches} for each kernel class in
+ * mf_kernel_names order; returns the number of classes. */
+int mf_enable_timing(mf_ctx* ctx, int on);
+int mf_get_timing(mf_ctx* ctx, double* out, int cap);
+const char* mf_kernel_name(int i);
+/* Device bytes held by the context. */
+long mf_device_bytes(mf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

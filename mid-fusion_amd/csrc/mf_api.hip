@@ -1,0 +1,666 @@
+// C-ABI implementation: the Kfusion per-frame facade on one MI355X.
+//
+// Reference sequencing: Kfusion::preprocessing / tracking / integration /
+// raycasting (apps/kfusion/src/cpp/kfusion.cpp:165-677), Tracking
+// (tracking/tracking.cpp:142-357, 580-708, 1401-1414), Object
+// (object/object.cpp:19-46, 128-195).  Poses, GN state and every frame
+// buffer stay in HBM; a frame is a stream-ordered sequence of launches with
+// no host synchronisation until mf_sync / a getter.
+#include "mf_kernels.h"
+#include "../../include/midfusion.h"
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+#include <algorithm>
+
+using namespace mf;
+
+namespace {
+
+enum KClass {
+  KC_CONVERT, KC_BILATERAL, KC_HALF, KC_LEVEL, KC_V2D, KC_TRACK, KC_FINALIZE, KC_GNSMALL, KC_PARAMS, KC_ALLOC,
+  KC_COMMIT, KC_INTEGRATE, KC_RAYCAST, KC_COUNT
+};
+const char* kNames[KC_COUNT] = {"convert", "bilateral", "half_sample", "level_maps", "vertex2depth", "track_reduce",
+                                "gn_finalize", "gn_control", "obj_params", "alloc_band", "alloc_commit",
+                                "integrate", "raycast"};
+
+struct Object {
+  int id = 0;
+  float size = 0;
+  int res = 0;
+  VolDev v;
+  bool moving = false;
+  bool has_mask = false;
+  float* mask = nullptr;
+  float* mv[3] = {nullptr, nullptr, nullptr};
+  float* mn[3] = {nullptr, nullptr, nullptr};
+  int8_t* codes = nullptr;       // level-0 result codes of this instance's last pass
+  float* rD[MAX_LEVELS] = {};    // rendered reference depth pyramid
+  ObjParams* prm = nullptr;
+};
+
+}  // namespace
+
+struct mf_ctx {
+  mf_config cfg;
+  hipStream_t stream = nullptr;
+  int W = 0, H = 0;
+  size_t N0 = 0;
+  std::vector<void*> allocs;
+  long bytes = 0;
+  std::string err;
+  // frame
+  uint16_t* in_depth = nullptr;
+  uint8_t* in_rgb = nullptr;
+  size_t in_cap = 0;
+  float *depth = nullptr, *grey = nullptr, *rgb[3] = {};
+  float* gauss = nullptr;
+  Level live[MAX_LEVELS], ref[MAX_LEVELS];
+  // model maps (raycast output)
+  float *vtx[3] = {}, *nrm[3] = {};
+  int32_t* label = nullptr;
+  // poses on the device: [0] camera, [1] T_w_r, [2] inverse(T_w_r), [3 + id] objects
+  Mat4* mats = nullptr;
+  GNState* gn = nullptr;
+  TrackInst* inst = nullptr;     // device descriptors [MAX_INST]
+  float* partial = nullptr;
+  int nblk0 = 0;
+  uint8_t* moving = nullptr;     // [MAX_INST] label-indexed flags
+  TraceRec* trace = nullptr;
+  int* trace_n = nullptr;
+  int trace_cap = 4096;
+  RayObj* rayobjs = nullptr;
+  TrackInst inst_host[MAX_INST];   // mirrors of the device descriptor tables
+  RayObj ray_host[MAX_INST];
+  int ray_n = -1;
+  std::vector<Object*> objs;
+  float ray_eps = -1.f;
+  bool have_ref = false;
+  // timing
+  bool timing = false;
+  struct Ev { hipEvent_t a, b; int cls; };
+  std::vector<Ev> evs;
+  std::vector<hipEvent_t> pool;
+  double acc_ms[KC_COUNT] = {};
+  long acc_n[KC_COUNT] = {};
+};
+
+namespace {
+
+template <typename T>
+T* dalloc(mf_ctx* c, size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+  hipMemsetAsync(p, 0, n * sizeof(T), c->stream);
+  c->allocs.push_back(p);
+  c->bytes += (long)(n * sizeof(T));
+  return (T*)p;
+}
+
+hipEvent_t ev_get(mf_ctx* c) {
+  if (!c->pool.empty()) { hipEvent_t e = c->pool.back(); c->pool.pop_back(); return e; }
+  hipEvent_t e; hipEventCreate(&e); return e;
+}
+struct Timed {
+  mf_ctx* c; int cls; hipEvent_t a = nullptr, b = nullptr;
+  Timed(mf_ctx* cc, int k) : c(cc), cls(k) { if (c->timing) { a = ev_get(c); b = ev_get(c); hipEventRecord(a, c->stream); } }
+  ~Timed() { if (c->timing) { hipEventRecord(b, c->stream); c->evs.push_back({a, b, cls}); } }
+};
+void drain_timing(mf_ctx* c) {
+  if (c->evs.empty()) return;
+  hipStreamSynchronize(c->stream);
+  for (auto& e : c->evs) {
+    float ms = 0; hipEventElapsedTime(&ms, e.a, e.b);
+    c->acc_ms[e.cls] += ms; c->acc_n[e.cls]++;
+    c->pool.push_back(e.a); c->pool.push_back(e.b);
+  }
+  c->evs.clear();
+}
+
+Mat4 m4(const float* T) { Mat4 r; for (int i = 0; i < 16; i++) r.m[i / 4][i % 4] = T[i]; return r; }
+int fail(mf_ctx* c, int code, const char* msg) { c->err = msg; return code; }
+
+bool alloc_level(mf_ctx* c, Level& L, int w, int h) {
+  const size_t n = (size_t)w * h;
+  L.w = w; L.h = h;
+  L.D = dalloc<float>(c, n); L.I = dalloc<float>(c, n); L.gx = dalloc<float>(c, n); L.gy = dalloc<float>(c, n);
+  for (int k = 0; k < 3; k++) { L.V[k] = dalloc<float>(c, n); L.N[k] = dalloc<float>(c, n); L.cov[k] = dalloc<float>(c, n); }
+  return L.D && L.I && L.gx && L.gy && L.V[2] && L.N[2] && L.cov[2];
+}
+
+int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
+  if (res < 16 || (res & (res - 1)) || size <= 0) return fail(c, MF_E_ARG, "object resolution must be a power of two >= 16");
+  if ((int)c->objs.size() >= MAX_INST - 3) return fail(c, MF_E_ARG, "too many objects");
+  Object* o = new Object();
+  o->id = (int)c->objs.size();
+  o->size = size; o->res = res;
+  VolDev& v = o->v;
+  v.res = res; v.size = size; v.voxel = size / (float)res;
+  v.max_level = 0; while ((1 << v.max_level) < res) v.max_level++;
+  v.leaf_level = v.max_level - 3;
+  v.nb = res / 8;
+  const long nb3 = (long)v.nb * v.nb * v.nb;
+  if (cap <= 0) cap = std::min(nb3, (long)262144);
+  cap = std::min(cap, nb3);
+  v.cap = cap;
+  v.slot = dalloc<int32_t>(c, nb3);
+  if (v.slot) hipMemsetAsync(v.slot, 0xff, nb3 * sizeof(int32_t), c->stream);
+  v.req = dalloc<uint8_t>(c, nb3);
+  long off = 0;
+  for (int l = 0; l < 24; l++) v.node_off[l] = 0;
+  for (int l = 1; l < v.leaf_level; l++) { v.node_off[l] = off; off += 1L << (3 * l); }
+  v.nodes = dalloc<uint8_t>(c, off);
+  v.blk_morton = dalloc<int32_t>(c, cap);
+  v.blk_active = dalloc<uint8_t>(c, cap);
+  const size_t nv = (size_t)cap * BLOCK_VOX;
+  v.tsdf = dalloc<float>(c, nv); v.wt = dalloc<float>(c, nv);
+  v.r = dalloc<float>(c, nv); v.g = dalloc<float>(c, nv); v.b = dalloc<float>(c, nv);
+  v.c = dalloc<int32_t>(c, nv); v.fgw = dalloc<int32_t>(c, nv); v.fg = dalloc<double>(c, nv);
+  v.counters = dalloc<int32_t>(c, 8);
+  v.active_list = dalloc<int32_t>(c, cap);
+  o->mask = dalloc<float>(c, c->N0);
+  for (int k = 0; k < 3; k++) { o->mv[k] = dalloc<float>(c, c->N0); o->mn[k] = dalloc<float>(c, c->N0); }
+  o->codes = dalloc<int8_t>(c, 2 * c->N0);
+  for (int l = 0; l < c->cfg.n_levels; l++) o->rD[l] = dalloc<float>(c, (c->N0 >> (2 * l)) + 1);
+  o->prm = dalloc<ObjParams>(c, 1);
+  if (!v.slot || !v.req || !v.nodes || !v.tsdf || !v.fg || !v.c || !v.fgw || !v.active_list || !o->mask ||
+      !o->codes || !o->prm || !o->mv[2] || !o->mn[2]) {
+    delete o;
+    return fail(c, MF_E_OOM, "device allocation failed");
+  }
+  // INVALID normals in the model maps until the first raycast
+  std::vector<float> inv(c->N0, (float)INVALID);
+  hipMemcpyAsync(o->mn[0], inv.data(), c->N0 * sizeof(float), hipMemcpyHostToDevice, c->stream);
+  const Mat4 M = m4(T);
+  hipMemcpyAsync(c->mats + 3 + o->id, &M, sizeof(Mat4), hipMemcpyHostToDevice, c->stream);
+  hipStreamSynchronize(c->stream);
+  c->objs.push_back(o);
+  return o->id;
+}
+
+TrackLaunch base_launch(mf_ctx* c, const float k[4], int level, int object) {
+  TrackLaunch p;
+  memset(&p, 0, sizeof p);
+  p.level = level; p.w = c->live[level].w; p.h = c->live[level].h; p.W = c->W; p.H = c->H;
+  p.use_icp = c->cfg.use_icp; p.use_rgb = c->cfg.use_rgb; p.object = object;
+  p.robust = c->cfg.robust; p.robust_k = c->cfg.robust_k;
+  p.fx = k[0]; p.fy = k[1]; p.cx = k[2]; p.cy = k[3];
+  // sigma_bright_ = sqrtf(100.0)/255.0 (tracking.cpp:764,769); inv_cov = 1.0/sigma
+  const float sigma_bright = (float)(sqrtf(100.0f) / 255.0);
+  p.inv_cov_rgb = (float)(1.0 / (double)sigma_bright);
+  p.live = c->live[level]; p.ref = c->ref[level];
+  p.label = nullptr; p.moving = c->moving; p.n_labels = MAX_INST;
+  return p;
+}
+
+// Rendered reference depth pyramid of one instance (tracking.cpp:218-238):
+// vertex2depth at T_w_r (inverse in mats[2]) then robust half-sampling.
+void render_reference(mf_ctx* c, float* const rD[MAX_LEVELS], float* const mv[3]) {
+  { Timed t(c, KC_V2D); launch_vertex2depth(c->stream, mv, (int)c->N0, c->mats + 2, rD[0]); }
+  for (int l = 1; l < c->cfg.n_levels; l++) {
+    Timed t(c, KC_HALF);
+    launch_half(c->stream, rD[l - 1], rD[l], nullptr, nullptr, c->live[l - 1].w, c->live[l - 1].h, 0.1f * 3);
+  }
+}
+
+// One coarse-to-fine GN run over `n` instances sharing a launch
+// (trackLiveFrame / trackEachObject loops, tracking.cpp:242-335, 624-690).
+void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, const int32_t* label_mask) {
+  for (int l = c->cfg.n_levels - 1; l >= 0; --l) {
+    { Timed t(c, KC_GNSMALL); launch_gn_level_begin(c->stream, c->gn, MAX_INST); }
+    TrackLaunch p = base_launch(c, k, l, object);
+    p.label = label_mask;
+    const int nblk = track_blocks(p.w, p.h);
+    for (int it = 0; it < c->cfg.iters[l]; ++it) {
+      p.iter = it;
+      { Timed t(c, KC_TRACK); launch_track(c->stream, p, c->inst + first_inst, n, c->gn, c->partial, nblk); }
+      { Timed t(c, KC_FINALIZE);
+        launch_finalize(c->stream, p, n, c->inst + first_inst, c->gn, c->partial, nblk, c->trace, c->trace_n,
+                        c->trace_cap); }
+    }
+  }
+}
+
+void set_inst(mf_ctx* c, int slot, float* const mv[3], float* const mn[3], float* const rD[MAX_LEVELS],
+              int8_t* codes, int state) {
+  TrackInst ti;
+  memset(&ti, 0, sizeof ti);
+  for (int k = 0; k < 3; k++) { ti.mv[k] = mv[k]; ti.mn[k] = mn[k]; }
+  for (int l = 0; l < MAX_LEVELS; l++) ti.rD[l] = rD[l];
+  ti.codes = codes; ti.state = state;
+  // descriptors only change with the object set: upload on change, ordered
+  // after every pending launch that may still read the old table
+  if (memcmp(&ti, &c->inst_host[slot], sizeof ti) != 0) {
+    c->inst_host[slot] = ti;
+    hipMemcpyAsync(c->inst + slot, &c->inst_host[slot], sizeof ti, hipMemcpyHostToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
+  }
+}
+
+void download_planes(mf_ctx* c, float* const src[3], float* dst_aos, size_t n) {
+  std::vector<float> tmp(3 * n);
+  for (int k = 0; k < 3; k++) hipMemcpyAsync(tmp.data() + k * n, src[k], n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  for (size_t i = 0; i < n; i++) for (int k = 0; k < 3; k++) dst_aos[3 * i + k] = tmp[k * n + i];
+}
+void upload_planes(mf_ctx* c, float* const dst[3], const float* src_aos, size_t n) {
+  std::vector<float> tmp(3 * n);
+  for (size_t i = 0; i < n; i++) for (int k = 0; k < 3; k++) tmp[k * n + i] = src_aos[3 * i + k];
+  for (int k = 0; k < 3; k++) hipMemcpyAsync(dst[k], tmp.data() + k * n, n * sizeof(float), hipMemcpyHostToDevice, c->stream);
+  hipStreamSynchronize(c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size_m, int bg_res, long bg_capacity) {
+  if (!cfg || cfg->width <= 0 || cfg->height <= 0 || cfg->n_levels < 1 || cfg->n_levels > MAX_LEVELS) return nullptr;
+  mf_ctx* c = new mf_ctx();
+  memset(c->inst_host, 0, sizeof c->inst_host);
+  memset(c->ray_host, 0, sizeof c->ray_host);
+  c->cfg = *cfg;
+  if (c->cfg.track_rate <= 0) c->cfg.track_rate = 1;
+  if (c->cfg.integ_rate <= 0) c->cfg.integ_rate = 1;
+  c->W = cfg->width; c->H = cfg->height; c->N0 = (size_t)c->W * c->H;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return nullptr; }
+  bool ok = true;
+  c->depth = dalloc<float>(c, c->N0); c->grey = dalloc<float>(c, c->N0);
+  for (int k = 0; k < 3; k++) c->rgb[k] = dalloc<float>(c, c->N0);
+  for (int l = 0; l < cfg->n_levels; l++) {
+    ok = ok && alloc_level(c, c->live[l], c->W >> l, c->H >> l);
+    ok = ok && alloc_level(c, c->ref[l], c->W >> l, c->H >> l);
+  }
+  for (int k = 0; k < 3; k++) { c->vtx[k] = dalloc<float>(c, c->N0); c->nrm[k] = dalloc<float>(c, c->N0); }
+  c->label = dalloc<int32_t>(c, c->N0);
+  c->mats = dalloc<Mat4>(c, 3 + MAX_INST);
+  c->gn = dalloc<GNState>(c, MAX_INST);
+  c->inst = dalloc<TrackInst>(c, MAX_INST);
+  c->nblk0 = track_blocks(c->W, c->H);
+  c->partial = dalloc<float>(c, (size_t)MAX_INST * c->nblk0 * 32);
+  c->moving = dalloc<uint8_t>(c, MAX_INST);
+  c->trace = dalloc<TraceRec>(c, c->trace_cap);
+  c->trace_n = dalloc<int>(c, 1);
+  c->rayobjs = dalloc<RayObj>(c, MAX_INST);
+  c->gauss = dalloc<float>(c, 5);
+  ok = ok && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
+       c->inst && c->partial && c->moving && c->trace && c->trace_n && c->rayobjs && c->gauss;
+  if (!ok) { mf_destroy(c); return nullptr; }
+  // Gaussian of the bilateral filter (kfusion.cpp:80-88): expf(-(x*x)/(2*delta^2)), delta = 4
+  float g[5];
+  for (int i = 0; i < 5; i++) { int x = i - 2; g[i] = expf((float)(-(x * x)) / (2.0f * 4.0f * 4.0f)); }
+  hipMemcpyAsync(c->gauss, g, sizeof g, hipMemcpyHostToDevice, c->stream);
+  const Mat4 P = m4(init_pose);
+  hipMemcpyAsync(c->mats, &P, sizeof P, hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(c->mats + 1, &P, sizeof P, hipMemcpyHostToDevice, c->stream);
+  // model normals INVALID until the first raycast
+  std::vector<float> inv(c->N0, (float)INVALID);
+  hipMemcpyAsync(c->nrm[0], inv.data(), c->N0 * sizeof(float), hipMemcpyHostToDevice, c->stream);
+  std::vector<int32_t> il(c->N0, INVALID);
+  hipMemcpyAsync(c->label, il.data(), c->N0 * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+  hipStreamSynchronize(c->stream);
+  // background object, identity pose (kernels.h:146-154)
+  float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  if (make_object(c, bg_size_m, bg_res, I, bg_capacity) != 0) { mf_destroy(c); return nullptr; }
+  return c;
+}
+
+void mf_destroy(mf_ctx* c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto& e : c->evs) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+  for (auto e : c->pool) hipEventDestroy(e);
+  for (void* p : c->allocs) hipFree(p);
+  for (Object* o : c->objs) delete o;
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* mf_last_error(mf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mf_object_create(mf_ctx* c, float size_m, int res, const float T_w_o[16], long capacity) {
+  return make_object(c, size_m, res, T_w_o, capacity);
+}
+
+int mf_object_set_moving(mf_ctx* c, int id, int moving) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  c->objs[id]->moving = id > 0 && moving != 0;
+  uint8_t f = c->objs[id]->moving ? 1 : 0;
+  hipMemcpyAsync(c->moving + id, &f, 1, hipMemcpyHostToDevice, c->stream);
+  hipStreamSynchronize(c->stream);
+  return 0;
+}
+
+int mf_object_set_mask(mf_ctx* c, int id, const float* mask) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  Object* o = c->objs[id];
+  if (!mask) { o->has_mask = false; return 0; }
+  hipMemcpyAsync(o->mask, mask, c->N0 * sizeof(float), hipMemcpyHostToDevice, c->stream);
+  hipStreamSynchronize(c->stream);
+  o->has_mask = true;
+  return 0;
+}
+int mf_object_set_mask_device(mf_ctx* c, int id, const float* d_mask) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  Object* o = c->objs[id];
+  if (!d_mask) { o->has_mask = false; return 0; }
+  hipMemcpyAsync(o->mask, d_mask, c->N0 * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+  o->has_mask = true;
+  return 0;
+}
+
+int mf_preprocess_device(mf_ctx* c, const uint16_t* d_depth, const uint8_t* d_rgb, int iw, int ih, int filter) {
+  const int W = c->W, H = c->H;
+  if (iw < W || ih < H || iw % W || ih % H || iw / W != ih / H) return fail(c, MF_E_ARG, "Invalid ratio.");
+  { Timed t(c, KC_CONVERT); launch_convert(c->stream, d_depth, d_rgb, iw, iw / W, W, H, c->depth, c->grey, c->rgb[0], c->rgb[1], c->rgb[2]); }
+  if (filter) {
+    // bilateral of the integration depth (kfusion.cpp:218-221), out of place
+    float* tmp = c->ref[0].gx;   // scratch plane of the same size, rewritten every frame
+    hipMemcpyAsync(tmp, c->depth, c->N0 * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+    Timed t(c, KC_BILATERAL);
+    launch_bilateral(c->stream, tmp, c->depth, nullptr, nullptr, W, H, 0.1f, c->gauss);
+  }
+  return 0;
+}
+
+int mf_preprocess(mf_ctx* c, const uint16_t* depth, const uint8_t* rgb, int iw, int ih, int filter) {
+  const size_t n = (size_t)iw * ih;
+  if (n > c->in_cap) {
+    c->in_depth = dalloc<uint16_t>(c, n);
+    c->in_rgb = dalloc<uint8_t>(c, 3 * n);
+    if (!c->in_depth || !c->in_rgb) return fail(c, MF_E_OOM, "input staging");
+    c->in_cap = n;
+  }
+  hipMemcpyAsync(c->in_depth, depth, n * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream);
+  if (rgb) hipMemcpyAsync(c->in_rgb, rgb, 3 * n, hipMemcpyHostToDevice, c->stream);
+  return mf_preprocess_device(c, c->in_depth, rgb ? c->in_rgb : nullptr, iw, ih, filter);
+}
+
+int mf_track(mf_ctx* c, const float k[4], unsigned frame) {
+  if (frame % c->cfg.track_rate != 0) return 0;
+  hipStream_t s = c->stream;
+  const int L = c->cfg.n_levels;
+  { Timed t(c, KC_GNSMALL); launch_copy_mat(s, c->mats + 0, c->mats + 1); }   // T_w_r = camera_pose
+  // set_params_frame + buildPyramid (tracking.cpp:142-171, 776-818)
+  { Timed t(c, KC_BILATERAL); launch_bilateral(s, c->grey, c->live[0].I, c->depth, c->live[0].D, c->W, c->H, 0.1f, c->gauss); }
+  for (int l = 0; l < L; l++) {
+    if (l > 0) { Timed t(c, KC_HALF); launch_half(s, c->live[l - 1].D, c->live[l].D, c->live[l - 1].I, c->live[l].I, c->live[l - 1].w, c->live[l - 1].h, 0.1f * 3); }
+    const float sc = (float)(1 << l);
+    const Mat4 invK = camKinv(k[0] / sc, k[1] / sc, k[2] / sc, k[3] / sc);
+    Timed t(c, KC_LEVEL);
+    launch_level(s, c->live[l], invK, k[1] < 0 ? 1 : 0, k[0] / sc, 1.0f, 1.0f, 0.075f, c->cfg.use_rgb != 0, c->cfg.use_icp != 0);
+  }
+  if ((int)frame == c->cfg.init_frame || !c->have_ref) {
+    for (int l = 0; l < L; l++) std::swap(c->live[l], c->ref[l]);   // setRefImgFromCurr
+    c->have_ref = true;
+    return 0;
+  }
+  Object* bg = c->objs[0];
+  { Timed t(c, KC_GNSMALL); launch_invert_mat(s, c->mats + 1, c->mats + 2); }
+  // camera: trackLiveFrame against the global model maps
+  float* camD[MAX_LEVELS];
+  for (int l = 0; l < MAX_LEVELS; l++) camD[l] = bg->rD[l];
+  set_inst(c, 0, c->vtx, c->nrm, camD, bg->codes, 0);
+  if (c->cfg.use_rgb) render_reference(c, camD, c->vtx);
+  { Timed t(c, KC_GNSMALL); launch_gn_begin_camera(s, c->gn + 0, c->mats + 0, c->mats + 1, k[0], k[1], k[2], k[3]); }
+  gn_run(c, k, 0, 0, 1, nullptr);
+  { Timed t(c, KC_GNSMALL); launch_gn_end_camera(s, c->gn + 0, c->mats + 0); }
+  std::vector<Object*> mov;
+  for (size_t i = 1; i < c->objs.size(); i++) if (c->objs[i]->moving) mov.push_back(c->objs[i]);
+  if (!mov.empty()) {
+    // re-track the camera with the moving objects' raycast pixels masked
+    // (kfusion.cpp:359-392, trackLiveFrame mask overload)
+    { Timed t(c, KC_GNSMALL); launch_gn_begin_camera(s, c->gn + 0, c->mats + 0, c->mats + 1, k[0], k[1], k[2], k[3]); }
+    gn_run(c, k, 0, 0, 1, c->label);
+    { Timed t(c, KC_GNSMALL); launch_gn_end_camera(s, c->gn + 0, c->mats + 0); }
+    // trackEachObject for every moving object (kfusion.cpp:395-414)
+    for (size_t j = 0; j < mov.size(); j++) {
+      Object* o = mov[j];
+      set_inst(c, 1 + (int)j, o->mv, o->mn, o->rD, o->codes, 1 + (int)j);
+      if (c->cfg.use_rgb) render_reference(c, o->rD, o->mv);
+      Timed t(c, KC_GNSMALL);
+      launch_gn_begin_object(s, c->gn + 1 + j, c->mats + 0, c->mats + 1, c->mats + 3 + o->id);
+    }
+    if (c->cfg.batch_objects) gn_run(c, k, 1, 1, (int)mov.size(), nullptr);
+    else for (size_t j = 0; j < mov.size(); j++) gn_run(c, k, 1, 1 + (int)j, 1, nullptr);
+    for (size_t j = 0; j < mov.size(); j++) {
+      Timed t(c, KC_GNSMALL);
+      launch_gn_end_object(s, c->gn + 1 + j, c->mats + 0, c->mats + 3 + mov[j]->id);
+    }
+  }
+  for (int l = 0; l < L; l++) std::swap(c->live[l], c->ref[l]);   // setRefImgFromCurr
+  return 1;
+}
+
+int mf_integrate(mf_ctx* c, const float k[4], unsigned frame) {
+  if (!((frame % c->cfg.integ_rate) == 0 || frame <= 3)) return 0;
+  hipStream_t s = c->stream;
+  const float band = 2.0f * c->cfg.mu;
+  for (Object* o : c->objs) {
+    if (!o->has_mask) continue;
+    // refine_mask_use_motion with the instance's last tracking rows; static
+    // objects carry the camera's rows (kfusion.cpp:398-401)
+    const int8_t* codes = (o->id == 0 || o->moving) ? o->codes : c->objs[0]->codes;
+    const int refine = frame > 1 ? 1 : 0;
+    { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats + 3 + o->id, k[0], k[1], k[2], k[3], o->prm); }
+    { Timed t(c, KC_ALLOC); launch_alloc(s, o->v, o->prm, c->depth, o->mask, codes, refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, band); }
+    { Timed t(c, KC_COMMIT); launch_alloc_commit(s, o->v); }
+    { Timed t(c, KC_INTEGRATE);
+      launch_integrate(s, o->v, o->prm, k[0], k[1], k[2], k[3], c->depth, c->rgb[0], c->rgb[1], c->rgb[2], o->mask, codes,
+                       refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, c->cfg.mu, 100); }
+  }
+  return 1;
+}
+
+int mf_raycast(mf_ctx* c, const float k[4], unsigned frame) {
+  (void)frame;
+  hipStream_t s = c->stream;
+  if (c->ray_eps < 0) c->ray_eps = exp2f(-(float)c->objs[0]->v.max_level);   // ray_iterator's static epsilon
+  const int n = (int)c->objs.size();
+  for (int i = 0; i < n; i++) {
+    Object* o = c->objs[i];
+    { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats + 3 + o->id, k[0], k[1], k[2], k[3], o->prm); }
+  }
+  if (c->ray_n != n) {   // descriptor table changes only with the object set
+    for (int i = 0; i < n; i++) {
+      Object* o = c->objs[i];
+      RayObj& r = c->ray_host[i];
+      r.v = o->v; r.prm = o->prm; r.label = o->id;
+      r.mvx = o->mv[0]; r.mvy = o->mv[1]; r.mvz = o->mv[2];
+      r.mnx = o->mn[0]; r.mny = o->mn[1]; r.mnz = o->mn[2];
+    }
+    hipMemcpyAsync(c->rayobjs, c->ray_host, n * sizeof(RayObj), hipMemcpyHostToDevice, s);
+    hipStreamSynchronize(s);
+    c->ray_n = n;
+  }
+  Timed t(c, KC_RAYCAST);
+  launch_raycast(s, c->rayobjs, n, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, c->vtx[0], c->vtx[1],
+                 c->vtx[2], c->nrm[0], c->nrm[1], c->nrm[2], c->label);
+  return 1;
+}
+
+int mf_sync(mf_ctx* c) {
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(e));
+  for (Object* o : c->objs) {
+    int32_t cnt[4];
+    hipMemcpy(cnt, o->v.counters, sizeof cnt, hipMemcpyDeviceToHost);
+    if (cnt[1]) return fail(c, MF_E_CAPACITY, "voxel block pool full");
+  }
+  return 0;
+}
+
+int mf_get_pose(mf_ctx* c, int id, float T[16]) {
+  if (id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  Mat4 M;
+  hipMemcpyAsync(&M, c->mats + (id < 0 ? 0 : 3 + id), sizeof M, hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  for (int i = 0; i < 16; i++) T[i] = M.m[i / 4][i % 4];
+  return 0;
+}
+int mf_set_pose(mf_ctx* c, int id, const float T[16]) {
+  if (id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  const Mat4 M = m4(T);
+  hipMemcpyAsync(c->mats + (id < 0 ? 0 : 3 + id), &M, sizeof M, hipMemcpyHostToDevice, c->stream);
+  hipStreamSynchronize(c->stream);
+  return 0;
+}
+
+int mf_get_maps(mf_ctx* c, float* vertex, float* normal, int32_t* label) {
+  if (vertex) download_planes(c, c->vtx, vertex, c->N0);
+  if (normal) download_planes(c, c->nrm, normal, c->N0);
+  if (label) { hipMemcpyAsync(label, c->label, c->N0 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream); hipStreamSynchronize(c->stream); }
+  return 0;
+}
+int mf_set_maps(mf_ctx* c, const float* vertex, const float* normal, const int32_t* label) {
+  if (vertex) upload_planes(c, c->vtx, vertex, c->N0);
+  if (normal) upload_planes(c, c->nrm, normal, c->N0);
+  if (label) { hipMemcpyAsync(c->label, label, c->N0 * sizeof(int32_t), hipMemcpyHostToDevice, c->stream); hipStreamSynchronize(c->stream); }
+  return 0;
+}
+int mf_get_object_maps(mf_ctx* c, int id, float* vertex, float* normal) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  if (vertex) download_planes(c, c->objs[id]->mv, vertex, c->N0);
+  if (normal) download_planes(c, c->objs[id]->mn, normal, c->N0);
+  return 0;
+}
+int mf_set_object_maps(mf_ctx* c, int id, const float* vertex, const float* normal) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  if (vertex) upload_planes(c, c->objs[id]->mv, vertex, c->N0);
+  if (normal) upload_planes(c, c->objs[id]->mn, normal, c->N0);
+  return 0;
+}
+
+long mf_object_keys(mf_ctx* c, int id, uint64_t* out, long cap) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  const VolDev& v = c->objs[id]->v;
+  int32_t cnt[4];
+  hipMemcpyAsync(cnt, v.counters, sizeof cnt, hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  const int n = std::min(cnt[0], (int)v.cap);
+  std::vector<int32_t> m(std::max(n, 1));
+  hipMemcpy(m.data(), v.blk_morton, n * sizeof(int32_t), hipMemcpyDeviceToHost);
+  std::vector<uint64_t> keys(n);
+  for (int i = 0; i < n; i++) keys[i] = ((uint64_t)(uint32_t)m[i] << 9) | (uint64_t)v.leaf_level;
+  std::sort(keys.begin(), keys.end());
+  for (long i = 0; i < n && i < cap; i++) out[i] = keys[i];
+  return n;
+}
+
+int mf_object_block(mf_ctx* c, int id, int x, int y, int z, double* out) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  const VolDev& v = c->objs[id]->v;
+  if (x < 0 || y < 0 || z < 0 || x >= v.res || y >= v.res || z >= v.res) return 0;
+  const uint64_t m = morton3((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3);
+  int32_t s = -1;
+  hipMemcpyAsync(&s, v.slot + m, sizeof s, hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  if (s < 0) return 0;
+  std::vector<float> f(5 * BLOCK_VOX);
+  std::vector<int32_t> iv(2 * BLOCK_VOX);
+  std::vector<double> fg(BLOCK_VOX);
+  const size_t o = (size_t)s * BLOCK_VOX, bs = BLOCK_VOX * sizeof(float);
+  hipMemcpy(f.data(), v.tsdf + o, bs, hipMemcpyDeviceToHost);
+  hipMemcpy(f.data() + BLOCK_VOX, v.wt + o, bs, hipMemcpyDeviceToHost);
+  hipMemcpy(f.data() + 2 * BLOCK_VOX, v.r + o, bs, hipMemcpyDeviceToHost);
+  hipMemcpy(f.data() + 3 * BLOCK_VOX, v.g + o, bs, hipMemcpyDeviceToHost);
+  hipMemcpy(f.data() + 4 * BLOCK_VOX, v.b + o, bs, hipMemcpyDeviceToHost);
+  hipMemcpy(iv.data(), v.c + o, BLOCK_VOX * sizeof(int32_t), hipMemcpyDeviceToHost);
+  hipMemcpy(iv.data() + BLOCK_VOX, v.fgw + o, BLOCK_VOX * sizeof(int32_t), hipMemcpyDeviceToHost);
+  hipMemcpy(fg.data(), v.fg + o, BLOCK_VOX * sizeof(double), hipMemcpyDeviceToHost);
+  for (int i = 0; i < BLOCK_VOX; i++) {
+    double* d = out + 8 * i;
+    d[0] = f[i]; d[1] = f[BLOCK_VOX + i]; d[2] = f[2 * BLOCK_VOX + i]; d[3] = f[3 * BLOCK_VOX + i];
+    d[4] = f[4 * BLOCK_VOX + i]; d[5] = iv[i]; d[6] = fg[i]; d[7] = iv[BLOCK_VOX + i];
+  }
+  return 1;
+}
+
+long mf_object_node_count(mf_ctx* c, int id) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  const VolDev& v = c->objs[id]->v;
+  long tot = 0;
+  for (int l = 1; l < v.leaf_level; l++) tot += 1L << (3 * l);
+  std::vector<uint8_t> b(std::max(tot, 1L));
+  hipStreamSynchronize(c->stream);
+  hipMemcpy(b.data(), v.nodes, tot, hipMemcpyDeviceToHost);
+  long n = 0;
+  for (long i = 0; i < tot; i++) n += b[i] != 0;
+  return n;
+}
+
+int mf_get_trackresult(mf_ctx* c, int id, int32_t* codes) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  Object* o = c->objs[id];
+  const int8_t* src = (id == 0 || o->moving) ? o->codes : c->objs[0]->codes;
+  std::vector<int8_t> t(2 * c->N0);
+  hipMemcpyAsync(t.data(), src, t.size(), hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  for (size_t i = 0; i < t.size(); i++) codes[i] = t[i];
+  return 0;
+}
+
+long mf_get_trace(mf_ctx* c, double* out, long cap) {
+  int n = 0;
+  hipMemcpyAsync(&n, c->trace_n, sizeof n, hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  n = std::min(n, c->trace_cap);
+  std::vector<TraceRec> r(std::max(n, 1));
+  hipMemcpy(r.data(), c->trace, n * sizeof(TraceRec), hipMemcpyDeviceToHost);
+  for (long i = 0; i < n && i < cap; i++) {
+    double* d = out + 22 * i;
+    d[0] = r[i].inst; d[1] = r[i].level; d[2] = r[i].iter; d[3] = r[i].err; d[4] = r[i].s28; d[5] = r[i].flag;
+    for (int j = 0; j < 16; j++) d[6 + j] = r[i].pose[j];
+  }
+  return n;
+}
+void mf_clear_trace(mf_ctx* c) {
+  hipMemsetAsync(c->trace_n, 0, sizeof(int), c->stream);
+  hipStreamSynchronize(c->stream);
+}
+
+long mf_get_level(mf_ctx* c, int level, int which, float* out) {
+  if (level < 0 || level >= c->cfg.n_levels) return fail(c, MF_E_ARG, "bad level");
+  // after tracking the live pyramid has been handed over to the reference slot
+  const Level& L = c->ref[level];
+  const size_t n = (size_t)L.w * L.h;
+  hipStreamSynchronize(c->stream);
+  switch (which) {
+    case 0: hipMemcpy(out, L.D, n * 4, hipMemcpyDeviceToHost); break;
+    case 1: hipMemcpy(out, L.I, n * 4, hipMemcpyDeviceToHost); break;
+    case 2: download_planes(c, L.V, out, n); break;
+    case 3: download_planes(c, L.N, out, n); break;
+    case 4: hipMemcpy(out, L.gx, n * 4, hipMemcpyDeviceToHost); break;
+    case 5: hipMemcpy(out, L.gy, n * 4, hipMemcpyDeviceToHost); break;
+    case 6: download_planes(c, L.cov, out, n); break;
+    default: return fail(c, MF_E_ARG, "bad buffer");
+  }
+  return (long)n;
+}
+
+int mf_get_frame(mf_ctx* c, float* depth, float* grey, float* rgb) {
+  hipStreamSynchronize(c->stream);
+  if (depth) hipMemcpy(depth, c->depth, c->N0 * 4, hipMemcpyDeviceToHost);
+  if (grey) hipMemcpy(grey, c->grey, c->N0 * 4, hipMemcpyDeviceToHost);
+  if (rgb) download_planes(c, c->rgb, rgb, c->N0);
+  return 0;
+}
+
+int mf_enable_timing(mf_ctx* c, int on) {
+  drain_timing(c);
+  c->timing = on != 0;
+  for (int i = 0; i < KC_COUNT; i++) { c->acc_ms[i] = 0; c->acc_n[i] = 0; }
+  return 0;
+}
+int mf_get_timing(mf_ctx* c, double* out, int cap) {
+  drain_timing(c);
+  for (int i = 0; i < KC_COUNT && i < cap; i++) { out[2 * i] = c->acc_ms[i]; out[2 * i + 1] = (double)c->acc_n[i]; }
+  return KC_COUNT;
+}
+const char* mf_kernel_name(int i) { return (i >= 0 && i < KC_COUNT) ? kNames[i] : ""; }
+long mf_device_bytes(mf_ctx* c) { return c->bytes; }
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// Internal device-side data layout of the MID-Fusion dense path.
+//
+// HBM layout (all SoA, 4-byte planes unless stated):
+//   frame:    depth, grey, rgb r/g/b                    [W*H]
+//   pyramid:  per level l: D, I, V x/y/z, N x/y/z, gx, gy, cov x/y/z  [W*H/4^l]
+//   model:    raycast vertex x/y/z, normal x/y/z, label [W*H]
+//   volume:   Morton-indexed block table slot[(res/8)^3] (int32, -1 missing),
+//             internal-node occupancy bytes per octree level, block pool with
+//             per-field voxel planes [capacity*512] (tsdf, w, r, g, b, c,
+//             fgw as 4 B; fg as 8 B).
+#pragma once
+#include "mf_math.h"
+#include <vector>
+#include <string>
+
+namespace mf {
+
+static constexpr int MAX_LEVELS = 4;
+static constexpr int MAX_INST = 64;     // camera + objects per context
+static constexpr int BLOCK_VOX = 512;   // 8^3 voxels per block (octree_defines.h:38)
+
+// Device view of one object volume (passed by value to kernels).
+struct VolDev {
+  int res, nb, max_level, leaf_level;
+  float size, voxel;
+  long cap;
+  int32_t* slot;        // [nb^3] block slot by Morton index, -1 missing
+  uint8_t* req;         // [nb^3] allocation request flags
+  uint8_t* nodes;       // internal-node occupancy, levels 1..leaf-1
+  long node_off[24];    // byte offset of level l in `nodes`
+  int32_t* blk_morton;  // [cap] Morton index of each slot
+  uint8_t* blk_active;  // [cap] VoxelBlock::active_
+  float *tsdf, *wt, *r, *g, *b;
+  int32_t *c, *fgw;
+  double* fg;
+  int32_t* counters;    // [0] allocated blocks, [1] overflow flag, [2] active-list size, [3] new this frame
+  int32_t* active_list; // [cap]
+};
+
+// Matrices derived from the poses, computed on the device once per stage.
+struct ObjParams {
+  Mat4 pose, inv_pose;      // T_w_o and its inverse
+  Mat4 Tco, KT;             // inverse(T_w_c)*T_w_o, K*Tco (integration)
+  Mat4 kPose;               // T_w_c * inverse(K) (allocation)
+  f3 o_cam;                 // camera centre in the object frame
+  Mat4 view;                // T_w_c * getInverseCameraMatrix(k) (raycast)
+};
+
+// Gauss-Newton state of one tracked instance (device resident).
+struct GNState {
+  Mat4 T;          // camera: T_w_l; object: T_c2_o2
+  Mat4 prev;       // previous_pose (carried across levels)
+  Mat4 A;          // camera: T_w_r;            object: T_c1_o1
+  Mat4 B;          // camera: K * inverse(T_w_r); object: inverse(T_w_o1)
+  Mat4 C;          // object: inverse(T_c1_o1)
+  Mat4 T_wo1;      // object: pose at the previous frame
+  float prev_err;
+  int done;        // current level finished (break taken)
+  int iters_run;
+  int pad;
+  double sums[32]; // last reduction (FP64)
+};
+
+// Per-instance inputs of one tracking pass.
+struct TrackInst {
+  const float* mv[3];   // model vertex planes (world frame), full res
+  const float* mn[3];   // model normal planes
+  const float* rD[MAX_LEVELS];   // rendered reference depth pyramid
+  int8_t* codes;        // level-0 result codes out [2*W*H] (ICP, RGB) or null
+  int state;            // index into GNState array
+  int pad;
+};
+
+// Pyramid buffers of one frame (live or reference).
+struct Level {
+  int w, h;
+  float *D, *I, *V[3], *N[3], *gx, *gy, *cov[3];
+};
+
+struct TraceRec { int inst, level, iter, flag; float err, s28; float pose[16]; };
+
+}  // namespace mf

@@ -1,0 +1,65 @@
+// Host-side launchers of the gfx950 kernels (one translation unit per stage).
+#pragma once
+#include "mf_internal.h"
+
+namespace mf {
+
+// preprocessing (mf_pre.hip)
+void launch_convert(hipStream_t s, const uint16_t* din, const uint8_t* rgb, int iw, int ratio, int ow, int oh,
+                    float* depth, float* grey, float* r, float* g, float* b);
+void launch_bilateral(hipStream_t s, const float* in0, float* out0, const float* in1, float* out1, int w, int h,
+                      float e_d, const float* gauss);
+void launch_half(hipStream_t s, const float* in0, float* out0, const float* in1, float* out1, int iw, int ih,
+                 float e_d);
+void launch_level(hipStream_t s, const Level& L, const Mat4& invK, int negY, float focal, float sxy, float sdisp,
+                  float baseline, bool grads, bool cov);
+void launch_vertex2depth(hipStream_t s, const float* const v[3], int n, const Mat4* invT, float* out);
+
+// tracking (mf_track.hip)
+struct TrackLaunch {
+  int level, iter, w, h, W, H;         // level size and full size
+  int use_icp, use_rgb, object;        // instance kind
+  int robust; double robust_k;
+  float fx, fy, cx, cy;                // full-resolution intrinsics
+  float inv_cov_rgb;                   // 1/sigma_bright
+  Level live, ref;                     // live pyramid level, reference (previous frame) level
+  const int32_t* label;                // camera re-track: outlier mask from raycast labels
+  const uint8_t* moving;               // per-object moving flags (label-indexed)
+  int n_labels;
+};
+int track_blocks(int w, int h);
+void launch_track(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                  float* partial, int nblk);
+void launch_finalize(hipStream_t s, const TrackLaunch& p, int ninst, const TrackInst* inst, GNState* st,
+                     const float* partial, int nblk, TraceRec* trace, int* trace_n, int trace_cap);
+void launch_gn_begin_camera(hipStream_t s, GNState* st, const Mat4* T_w_l, const Mat4* T_w_r, float fx, float fy,
+                            float cx, float cy);
+void launch_gn_begin_object(hipStream_t s, GNState* st, const Mat4* T_w_c2, const Mat4* T_w_c1, const Mat4* T_wo1);
+void launch_gn_level_begin(hipStream_t s, GNState* st, int n);
+void launch_gn_end_camera(hipStream_t s, const GNState* st, Mat4* T_w_l);
+void launch_gn_end_object(hipStream_t s, const GNState* st, const Mat4* T_w_c2, Mat4* volume_pose);
+void launch_copy_mat(hipStream_t s, const Mat4* src, Mat4* dst);
+void launch_invert_mat(hipStream_t s, const Mat4* src, Mat4* dst);
+
+// mapping (mf_map.hip)
+void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* T_w_o, float fx, float fy, float cx, float cy,
+                       ObjParams* out);
+void launch_alloc(hipStream_t s, const VolDev& v, const ObjParams* prm, const float* depth, const float* mask,
+                  const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float band);
+void launch_alloc_commit(hipStream_t s, const VolDev& v);
+void launch_integrate(hipStream_t s, const VolDev& v, const ObjParams* prm, float fx, float fy, float cx, float cy,
+                      const float* depth, const float* r, const float* g, const float* b, const float* mask,
+                      const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float mu,
+                      int maxweight);
+
+// raycasting (mf_raycast.hip)
+struct RayObj {
+  VolDev v;
+  const ObjParams* prm;
+  int label;
+  float* mvx; float* mvy; float* mvz; float* mnx; float* mny; float* mnz;   // per-object output maps
+};
+void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,
+                    float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label);
+
+}  // namespace mf

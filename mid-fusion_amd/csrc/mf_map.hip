@@ -1,0 +1,221 @@
+// Voxel-block allocation and projective TSDF integration on gfx950.
+//
+// Reference: buildVolumeAllocationList (kfusion/alloc_impl.hpp:121-191),
+// Octree::alloc_update (lib/octree.hpp:761-863), projective_functor
+// (lib/functors/projective_functor.hpp:54-156), in_frustum
+// (lib/algorithms/filter.hpp:42-51), sdf_update (kfusion/mapping_impl.hpp:
+// 15-72), Object::integrate_volume_kernel / refine_mask_use_motion
+// (object/object.cpp:128-195, 255-315).
+//
+// The octree is replaced by a Morton-indexed block table: slot[morton(bx,by,
+// bz)] holds the pool slot of an allocated 8^3 block, so a band sample's
+// lookup is one 4-byte load and allocation is a request flag plus a commit
+// pass in Morton order.  Internal-node occupancy bytes per level keep what
+// the ESVO raycast traversal needs.  Allocated key sets are bit-identical to
+// the reference's keyops::encode keys.
+#include "mf_kernels.h"
+
+namespace mf {
+
+__device__ __forceinline__ float refined_mask(const float* mask, const int8_t* codes, size_t i, size_t n, int refine,
+                                              int use_icp, int use_rgb) {
+  float m = mask[i];
+  if (refine && m >= 0.f && codes) {
+    if (use_icp && (codes[i] == -4 || codes[i] == -6)) return -1.f;
+    if (use_rgb && codes[n + i] < -3) return -1.f;
+  }
+  return m;
+}
+
+__global__ void k_obj_params(const Mat4* T_w_c, const Mat4* T_w_o, float fx, float fy, float cx, float cy,
+                             ObjParams* out) {
+  const Mat4 Twc = *T_w_c, Two = *T_w_o;
+  const Mat4 K = camK(fx, fy, cx, cy);
+  ObjParams p;
+  p.pose = Two;
+  p.inv_pose = inverse(Two);
+  p.Tco = mul(inverse(Twc), Two);
+  p.KT = mul(K, p.Tco);
+  p.kPose = mul(Twc, inverse(K));
+  p.o_cam = affine(p.inv_pose, translation(Twc));
+  p.view = mul(Twc, camKinv(fx, fy, cx, cy));
+  *out = p;
+}
+
+__global__ void k_alloc_reset(VolDev v) {
+  v.counters[2] = 0;
+  v.counters[3] = v.counters[0];
+}
+
+// Band ray-march of buildVolumeAllocationList: every sample in a missing
+// block raises that block's request flag; a sample in an allocated block sets
+// the block active.
+__global__ void k_alloc(VolDev v, const ObjParams* __restrict__ prm, const float* __restrict__ depth,
+                        const float* __restrict__ mask, const int8_t* __restrict__ codes, int refine, int use_icp,
+                        int use_rgb, int w, int h, float band) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w || y >= h) return;
+  const size_t i = (size_t)x + (size_t)y * w;
+  const float d = depth[i];
+  if (d == 0.0f || refined_mask(mask, codes, i, (size_t)w * h, refine, use_icp, use_rgb) < 0.f) return;
+  const float inv_vox = 1.0f / v.voxel;
+  const int num_steps = (int)ceilf(band * inv_vox);
+  const f3 wv = affine(prm->kPose, mk3(((float)x + 0.5f) * d, ((float)y + 0.5f) * d, d));
+  const f3 ov = affine(prm->inv_pose, wv);
+  const f3 dir = normalize(prm->o_cam - ov);
+  const f3 origin = ov - (band * 0.5f) * dir;
+  const f3 step = (dir * band) / (float)num_steps;
+  const float size = (float)v.res;
+  f3 p = origin;
+  for (int s = 0; s < num_steps; s++) {
+    const float sx = floorf(p.x * inv_vox), sy = floorf(p.y * inv_vox), sz = floorf(p.z * inv_vox);
+    if (sx < size && sy < size && sz < size && sx >= 0 && sy >= 0 && sz >= 0) {
+      const uint32_t vx = (uint32_t)(int)sx, vy = (uint32_t)(int)sy, vz = (uint32_t)(int)sz;
+      const uint64_t m = morton3(vx >> 3, vy >> 3, vz >> 3);
+      const int32_t sl = v.slot[m];
+      if (sl < 0) v.req[m] = 1;
+      else v.blk_active[sl] = 1;
+    }
+    p = p + step;
+  }
+}
+
+// Commit: requested blocks receive pool slots (Octree::updateLevel: new leaf
+// blocks start active, octree.hpp:846-851); their ancestors are marked in the
+// per-level node-occupancy bytes.
+__global__ void k_alloc_commit(VolDev v, long nb3) {
+  for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < nb3; m += (long)gridDim.x * blockDim.x) {
+    if (!v.req[m]) continue;
+    v.req[m] = 0;
+    const int s = atomicAdd(&v.counters[0], 1);
+    if (s >= v.cap) { v.counters[1] = 1; atomicSub(&v.counters[0], 1); continue; }
+    v.slot[m] = s;
+    v.blk_morton[s] = (int32_t)m;
+    v.blk_active[s] = 1;
+    for (int l = v.leaf_level - 1; l >= 1; l--) v.nodes[v.node_off[l] + (m >> (3 * (v.leaf_level - l)))] = 1;
+  }
+}
+
+// Voxel initialisation of this frame's new blocks (voxel_traits<SDF>::initValue,
+// continuous/volume_traits.hpp:38).
+__global__ void k_init_new(VolDev v) {
+  const int first = v.counters[3], last = min(v.counters[0], (int)v.cap);
+  for (long i = (long)first * BLOCK_VOX + blockIdx.x * (long)blockDim.x + threadIdx.x; i < (long)last * BLOCK_VOX;
+       i += (long)gridDim.x * blockDim.x) {
+    v.tsdf[i] = 1.f; v.wt[i] = 0.f; v.r[i] = 0.f; v.g[i] = 0.f; v.b[i] = 0.f; v.c[i] = 0; v.fg[i] = 0.0; v.fgw[i] = 0;
+  }
+}
+
+__device__ __forceinline__ void block_coords(int32_t m, int& x, int& y, int& z) {
+  x = (int)compact21((uint64_t)m) * 8; y = (int)compact21((uint64_t)m >> 1) * 8; z = (int)compact21((uint64_t)m >> 2) * 8;
+}
+
+// projective_functor::build_active_list: active || in_frustum(block corner).
+__global__ void k_active_list(VolDev v, const ObjParams* __restrict__ prm, int w, int h) {
+  const int n = min(v.counters[0], (int)v.cap);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  bool keep = v.blk_active[s] != 0;
+  if (!keep) {
+    int bx, by, bz;
+    block_coords(v.blk_morton[s], bx, by, bz);
+    const f3 bc = mk3((float)bx * v.voxel, (float)by * v.voxel, (float)bz * v.voxel);
+    const f3 vc = affine(prm->KT, bc);
+    const int px = (int)(vc.x / vc.z), py = (int)(vc.y / vc.z);
+    keep = px >= 0 && px < w && py >= 0 && py < h;
+  }
+  if (keep) v.active_list[atomicAdd(&v.counters[2], 1)] = s;
+}
+
+// update_block + sdf_update: one 512-thread workgroup per active block, one
+// voxel per thread (x fastest, the block's storage order).  The row start
+// and x-increment are formed exactly as the reference forms them.
+__global__ __launch_bounds__(512) void k_integrate(VolDev v, const ObjParams* __restrict__ prm, float fx, float fy,
+                                                   float cx, float cy, const float* __restrict__ depth,
+                                                   const float* __restrict__ rr, const float* __restrict__ gg,
+                                                   const float* __restrict__ bb, const float* __restrict__ mask,
+                                                   const int8_t* __restrict__ codes, int refine, int use_icp,
+                                                   int use_rgb, int w, int h, float mu, int maxweight) {
+  const int nact = v.counters[2];
+  const int tid = threadIdx.x;
+  const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
+  const Mat4 Tco = prm->Tco;
+  const Mat4 K = camK(fx, fy, cx, cy);
+  const float vs = v.voxel;
+  const f3 delta = rotate(Tco, mk3(vs, 0.f, 0.f));
+  const f3 cdelta = rotate(K, delta);
+  const size_t npx = (size_t)w * h;
+  for (int a = blockIdx.x; a < nact; a += gridDim.x) {
+    const int s = v.active_list[a];
+    int bx, by, bz;
+    block_coords(v.blk_morton[s], bx, by, bz);
+    const int y = by + ly, z = bz + lz;
+    const f3 start = affine(Tco, mk3((float)bx * vs, (float)y * vs, (float)z * vs));
+    const f3 cstart = affine(K, start);
+    const f3 cv = cstart + (float)lx * cdelta;
+    const f3 pos = start + (float)lx * delta;
+    bool vis = false;
+    if (!(pos.z < 0.0001f)) {
+      const float invd = 1.f / cv.z;
+      const float pxf = cv.x * invd + 0.5f, pyf = cv.y * invd + 0.5f;
+      if (!(pxf < 0.5f || pxf > (float)w - 1.5f || pyf < 0.5f || pyf > (float)h - 1.5f)) {
+        vis = true;
+        const size_t pi = (size_t)(unsigned)pxf + (size_t)w * (unsigned)pyf;
+        const float ds = depth[pi];
+        const float m = refined_mask(mask, codes, pi, npx, refine, use_icp, use_rgb);
+        if (!(ds <= 0.f || m < 0.f)) {
+          const float qx = pos.x / pos.z, qy = pos.y / pos.z;
+          const float diff = (ds - pos.z) * sqrtf(1.f + qx * qx + qy * qy);
+          if (diff > -mu) {
+            const float sdf = fmin_r(1.f, diff / mu);
+            const size_t vi = (size_t)s * BLOCK_VOX + tid;
+            const float t0 = v.tsdf[vi], w0 = v.wt[vi];
+            v.tsdf[vi] = fmax_r(-1.f, fmin_r((w0 * t0 + sdf) / (w0 + 1), 1.f));
+            v.wt[vi] = fmin_r(w0 + 1, (float)maxweight);
+            const int c0 = v.c[vi];
+            const float fc = (float)c0, fc1 = (float)(c0 + 1);
+            v.r[vi] = (rr[pi] + v.r[vi] * fc) / fc1;
+            v.g[vi] = (gg[pi] + v.g[vi] * fc) / fc1;
+            v.b[vi] = (bb[pi] + v.b[vi] * fc) / fc1;
+            v.c[vi] = c0 + 1;
+            const int fw = v.fgw[vi];
+            v.fg[vi] = ((double)m + v.fg[vi] * (double)fw) / (double)(fw + 1);
+            v.fgw[vi] = fw + 1;
+          }
+        }
+      }
+    }
+    const int any = __syncthreads_or(vis ? 1 : 0);
+    if (tid == 0) v.blk_active[s] = any ? 1 : 0;
+  }
+}
+
+// ------------------------------------------------------------ launchers
+void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* T_w_o, float fx, float fy, float cx, float cy,
+                       ObjParams* out) {
+  k_obj_params<<<1, 1, 0, s>>>(T_w_c, T_w_o, fx, fy, cx, cy, out);
+}
+void launch_alloc(hipStream_t s, const VolDev& v, const ObjParams* prm, const float* depth, const float* mask,
+                  const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float band) {
+  k_alloc_reset<<<1, 1, 0, s>>>(v);
+  dim3 blk(256), grd((w + 255) / 256, h);
+  k_alloc<<<grd, blk, 0, s>>>(v, prm, depth, mask, codes, refine, use_icp, use_rgb, w, h, band);
+}
+void launch_alloc_commit(hipStream_t s, const VolDev& v) {
+  const long nb3 = (long)v.nb * v.nb * v.nb;
+  long g = (nb3 + 255) / 256;
+  if (g > 4096) g = 4096;
+  k_alloc_commit<<<(int)g, 256, 0, s>>>(v, nb3);
+  k_init_new<<<2048, 256, 0, s>>>(v);
+}
+void launch_integrate(hipStream_t s, const VolDev& v, const ObjParams* prm, float fx, float fy, float cx, float cy,
+                      const float* depth, const float* r, const float* g, const float* b, const float* mask,
+                      const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float mu,
+                      int maxweight) {
+  const long cap = v.cap;
+  k_active_list<<<(int)((cap + 255) / 256), 256, 0, s>>>(v, prm, w, h);
+  k_integrate<<<4096, 512, 0, s>>>(v, prm, fx, fy, cx, cy, depth, r, g, b, mask, codes, refine, use_icp, use_rgb, w,
+                                   h, mu, maxweight);
+}
+
+}  // namespace mf

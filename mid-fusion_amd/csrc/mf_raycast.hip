@@ -1,0 +1,276 @@
+// Multi-object raycasting on gfx950.
+//
+// Reference: raycastObjectList (rendering/rendering.cpp:9-181), ray_iterator
+// (lib/octree.hpp:919-1130), raycast() (kfusion/rendering_impl.hpp:80-140),
+// Octree::interp / grad / get_fine (lib/octree.hpp:359-379, 479-501, 593-680)
+// and gather_points (lib/interpolation/interp_gather.hpp:107-237).
+//
+// One thread per pixel walks every object: ESVO traversal over the per-level
+// occupancy bytes and the Morton block table gives the entry distance of the
+// first allocated block (the same float operations as ray_iterator, so the
+// march starts at the same t), then the SDF march reads voxels through the
+// block table (missing voxels read as tsdf 1, fg 0).
+#include "mf_kernels.h"
+
+namespace mf {
+
+__device__ __forceinline__ int32_t vslot(const VolDev& v, int x, int y, int z) {
+  if (x < 0 || y < 0 || z < 0 || x >= v.res || y >= v.res || z >= v.res) return -1;
+  return v.slot[morton3((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3)];
+}
+__device__ __forceinline__ float tsdf_at(const VolDev& v, int x, int y, int z) {
+  const int32_t s = vslot(v, x, y, z);
+  if (s < 0) return 1.f;
+  return v.tsdf[(size_t)s * BLOCK_VOX + (x & 7) + 8 * (y & 7) + 64 * (z & 7)];
+}
+__device__ __forceinline__ float fg_at(const VolDev& v, int x, int y, int z) {
+  const int32_t s = vslot(v, x, y, z);
+  if (s < 0) return 0.f;
+  return (float)v.fg[(size_t)s * BLOCK_VOX + (x & 7) + 8 * (y & 7) + 64 * (z & 7)];
+}
+
+// Octree::interp (octree.hpp:479-501): voxel values at integer corners, base
+// clamped at 0, fraction of the unclamped position.
+template <bool FG>
+__device__ __forceinline__ float interp(const VolDev& v, f3 pos) {
+  const float flx = floorf(pos.x), fly = floorf(pos.y), flz = floorf(pos.z);
+  const int bx = (int)flx, by = (int)fly, bz = (int)flz;
+  const float fx = pos.x - flx, fy = pos.y - fly, fz = pos.z - flz;
+  const int lx = max(bx, 0), ly = max(by, 0), lz = max(bz, 0);
+  float p[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    p[i] = FG ? fg_at(v, lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1))
+              : tsdf_at(v, lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1));
+  return (((p[0] * (1 - fx) + p[1] * fx) * (1 - fy) + (p[2] * (1 - fx) + p[3] * fx) * fy) * (1 - fz) +
+          ((p[4] * (1 - fx) + p[5] * fx) * (1 - fy) + (p[6] * (1 - fx) + p[7] * fx) * fy) * fz);
+}
+
+// Octree::grad(pos, select) (octree.hpp:593-680), tsdf field.
+__device__ f3 grad(const VolDev& v, f3 pos) {
+  const float fl[3] = {floorf(pos.x), floorf(pos.y), floorf(pos.z)};
+  const int b[3] = {(int)fl[0], (int)fl[1], (int)fl[2]};
+  const float f[3] = {pos.x - fl[0], pos.y - fl[1], pos.z - fl[2]};
+  int ll[3], lu[3], ul[3], uu[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    ll[a] = max(b[a] - 1, 0); lu[a] = max(b[a], 0);
+    ul[a] = min(b[a] + 1, v.res - 1); uu[a] = min(b[a] + 2, v.res - 1);
+  }
+  float g[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    float D[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int id[3] = {q & 1, (q >> 1) & 1, (q >> 2) & 1};
+      int hi[3], lo[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        if (c == a) { hi[c] = id[c] ? uu[c] : ul[c]; lo[c] = id[c] ? lu[c] : ll[c]; }
+        else { hi[c] = lo[c] = id[c] ? ul[c] : lu[c]; }
+      }
+      D[q] = tsdf_at(v, hi[0], hi[1], hi[2]) - tsdf_at(v, lo[0], lo[1], lo[2]);
+    }
+    g[a] = (((D[0] * (1 - f[0]) + D[1] * f[0]) * (1 - f[1]) + (D[2] * (1 - f[0]) + D[3] * f[0]) * f[1]) * (1 - f[2]) +
+            ((D[4] * (1 - f[0]) + D[5] * f[0]) * (1 - f[1]) + (D[6] * (1 - f[0]) + D[7] * f[0]) * f[1]) * f[2]);
+  }
+  const float s = v.size / (float)v.res;
+  return mk3(g[0] * s * 0.5f, g[1] * s * 0.5f, g[2] * s * 0.5f);
+}
+
+// Child occupancy of an octree node: level `lv` (1 = children of the root),
+// Morton prefix `pre` of the child.
+__device__ __forceinline__ bool child_exists(const VolDev& v, int lv, uint64_t pre) {
+  if (lv == v.leaf_level) return v.slot[pre] >= 0;
+  return v.nodes[v.node_off[lv] + pre] != 0;
+}
+
+// ray_iterator::next() first call (octree.hpp:923-1093): entry distance, in
+// voxel units, of the first allocated leaf block along the ray; -1 if none.
+// Same float operations as the reference traversal over [1,2]^3.
+__device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, float farp, float eps) {
+  constexpr int DEPTH = 23;
+  const int lg = v.leaf_level;                   // log2(res/8)
+  const int min_scale = DEPTH - lg;
+  float d[3] = {d_in.x, d_in.y, d_in.z};
+  for (int a = 0; a < 3; a++) if (fabsf(d[a]) < eps) d[a] = copysignf(eps, d[a]);
+  const float oo[3] = {o.x, o.y, o.z};
+  float t_coef[3], t_bias[3], pos[3] = {1.f, 1.f, 1.f};
+  for (int a = 0; a < 3; a++) {
+    const float so = oo[a] / v.size + 1.f;
+    t_coef[a] = -1.f / fabsf(d[a]);
+    t_bias[a] = t_coef[a] * so;
+  }
+  int octant_mask = 7;
+  for (int a = 0; a < 3; a++)
+    if (d[a] > 0.0f) { octant_mask ^= (1 << a); t_bias[a] = 3.0f * t_coef[a] - t_bias[a]; }
+  float t_min = fmax_r(fmax_r(2.0f * t_coef[0] - t_bias[0], 2.0f * t_coef[1] - t_bias[1]), 2.0f * t_coef[2] - t_bias[2]);
+  float t_max = fmin_r(fmin_r(t_coef[0] - t_bias[0], t_coef[1] - t_bias[1]), t_coef[2] - t_bias[2]);
+  float h = t_max;
+  t_min = fmax_r(t_min, nearp / v.size);
+  t_max = fmin_r(t_max, farp / v.size);
+  int idx = 0;
+  for (int a = 0; a < 3; a++)
+    if (1.5f * t_coef[a] - t_bias[a] > t_min) { idx ^= (1 << a); pos[a] = 1.5f; }
+  int scale = DEPTH - 1;
+  float scale_exp2 = 0.5f;
+  uint64_t parent = 0;          // Morton prefix of the parent node (root: 0)
+  bool parent_ok = true;
+  uint64_t st_parent[DEPTH + 1];
+  float st_tmax[DEPTH + 1];
+  bool st_ok[DEPTH + 1];
+  for (int i = 0; i <= DEPTH; i++) st_ok[i] = false;
+  float t_corner[3] = {0, 0, 0}, tc_max = 0.f;
+  const float vs = v.size / (float)v.res;
+  for (int guard = 0; scale < DEPTH; guard++) {
+    if (!parent_ok || guard >= 8192) return -1.f;   // bounded walk
+    for (int a = 0; a < 3; a++) t_corner[a] = pos[a] * t_coef[a] - t_bias[a];
+    tc_max = fmin_r(fmin_r(t_corner[0], t_corner[1]), t_corner[2]);
+    const int cidx = idx ^ octant_mask ^ 7;
+    const int clevel = DEPTH - scale;            // level of the child examined
+    const uint64_t child = (parent << 3) | (uint64_t)cidx;
+    const bool has_child = child_exists(v, clevel, child);
+    if (scale == min_scale && has_child) return t_min * v.size / vs;
+    if (has_child && t_min <= t_max) {
+      // descend
+      const float tv_max = fmin_r(t_max, tc_max);
+      const float half = scale_exp2 * 0.5f;
+      float t_center[3];
+      for (int a = 0; a < 3; a++) t_center[a] = half * t_coef[a] + t_corner[a];
+      if (tc_max < h) { st_parent[scale] = parent; st_tmax[scale] = t_max; st_ok[scale] = parent_ok; }
+      h = tc_max;
+      parent = child;
+      scale--;
+      scale_exp2 = half;
+      idx = (t_center[0] > t_min) | ((t_center[1] > t_min) << 1) | ((t_center[2] > t_min) << 2);
+      for (int a = 0; a < 3; a++) pos[a] += scale_exp2 * (float)(bool)(idx & (1 << a));
+      t_max = tv_max;
+      continue;
+    }
+    // advance
+    const int step_mask = (t_corner[0] <= tc_max) | ((t_corner[1] <= tc_max) << 1) | ((t_corner[2] <= tc_max) << 2);
+    for (int a = 0; a < 3; a++) pos[a] -= scale_exp2 * (float)(bool)(step_mask & (1 << a));
+    t_min = tc_max;
+    idx ^= step_mask;
+    if ((idx & step_mask) != 0) {
+      unsigned diff = 0;
+      for (int a = 0; a < 3; a++)
+        if (step_mask & (1 << a)) diff |= (unsigned)(__float_as_int(pos[a]) ^ __float_as_int(pos[a] + scale_exp2));
+      scale = (__float_as_int((float)diff) >> 23) - 127;
+      scale_exp2 = __int_as_float((scale - DEPTH + 127) << 23);
+      if (scale < 0 || scale > DEPTH) return -1.f;
+      parent = st_parent[scale];
+      t_max = st_tmax[scale];
+      parent_ok = st_ok[scale];
+      int sh[3];
+      for (int a = 0; a < 3; a++) { sh[a] = __float_as_int(pos[a]) >> scale; pos[a] = __int_as_float(sh[a] << scale); }
+      idx = (sh[0] & 1) | ((sh[1] & 1) << 1) | ((sh[2] & 1) << 2);
+      h = 0.0f;
+    }
+  }
+  return -1.f;
+}
+
+// raycast(volume, origin, dir, tnear, tfar, mu, step, largestep)
+// (kfusion/rendering_impl.hpp:80-140); returns t (0: no hit).
+__device__ float march(const VolDev& v, f3 origin, f3 dir, float nearp, float farp, float mu, float step,
+                       float largestep, f3& hit) {
+  const float size = v.size;
+  const float inv_vs = (float)v.res / v.size;
+  const f3 invR = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+  const f3 tbot = cmul((-1.f) * invR, origin);
+  const f3 ttop = cmul(invR, mk3(size - origin.x, size - origin.y, size - origin.z));
+  const f3 tmn = mk3(fmin_r(ttop.x, tbot.x), fmin_r(ttop.y, tbot.y), fmin_r(ttop.z, tbot.z));
+  const f3 tmx = mk3(fmax_r(ttop.x, tbot.x), fmax_r(ttop.y, tbot.y), fmax_r(ttop.z, tbot.z));
+  const float largest_tmin = fmax_r(fmax_r(tmn.x, tmn.y), fmax_r(tmn.x, tmn.z));
+  const float smallest_tmax = fmin_r(fmin_r(tmx.x, tmx.y), fmin_r(tmx.x, tmx.z));
+  const float tnear = fmax_r(largest_tmin, nearp);
+  const float tfar = fmin_r(smallest_tmax, farp);
+  hit = mk3(0.f, 0.f, 0.f);
+  if (!(tnear < tfar)) return 0.f;
+  float t = tnear, stepsize = largestep;
+  f3 p = origin + dir * t;
+  float f_t = interp<false>(v, p * inv_vs);
+  float f_tt = 0;
+  if (f_t > 0) {
+    for (; t < tfar; t += stepsize) {
+      const f3 q = p * inv_vs;
+      f_tt = tsdf_at(v, (int)q.x, (int)q.y, (int)q.z);
+      if (f_tt < 0.1f && f_tt >= -0.5f) f_tt = interp<false>(v, q);   // (double)f <= 0.1
+      if (f_tt < 0) break;
+      stepsize = fmax_r(f_tt * mu, step);
+      p = p + stepsize * dir;
+      f_t = f_tt;
+    }
+    if (f_tt < 0) {
+      t = t + stepsize * f_tt / (f_t - f_tt);
+      hit = origin + dir * t;
+      return t;
+    }
+  }
+  return 0.f;
+}
+
+__global__ __launch_bounds__(256) void k_raycast(const RayObj* __restrict__ objs, int nobj, int w, int h, float mu,
+                                                 float nearp, float farp, float eps, float* __restrict__ vx,
+                                                 float* __restrict__ vy, float* __restrict__ vz,
+                                                 float* __restrict__ nx, float* __restrict__ ny,
+                                                 float* __restrict__ nz, int32_t* __restrict__ label) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w || y >= h) return;
+  const size_t pix = (size_t)x + (size_t)y * w;
+  float nearest = 1.0e3f;
+  double high_fg = 0;
+  int miss = 0;
+  f3 out_v = mk3(0.f, 0.f, 0.f), out_n = mk3((float)INVALID, 0.f, 0.f);
+  int out_l = INVALID;
+  for (int oi = 0; oi < nobj; oi++) {
+    const RayObj& O = objs[oi];
+    const VolDev& v = O.v;
+    const ObjParams& P = *O.prm;
+    const float step = v.size / (float)v.res;
+    const float largestep = step * 8;
+    const f3 wo = translation(P.view);
+    const f3 oo = affine(P.inv_pose, wo);
+    const f3 wd = rotate(P.view, mk3((float)x, (float)y, 1.f));
+    const f3 od = normalize(rotate(P.inv_pose, wd));
+    const float nfar = fmin_r(nearest, farp);
+    const float tmin = ray_first_block(v, oo, od, nearp, nfar, eps);
+    f3 hit = mk3(0.f, 0.f, 0.f);
+    float hw = 0.f;
+    if (tmin > 0.f) hw = march(v, oo, od, tmin * v.size / (float)v.res, nfar, mu, step, largestep, hit);
+    const float hd = (O.label == 0) ? hw + 0.05f : hw;
+    if (hd > nearest) { miss++; continue; }
+    const float inv_vs = (float)v.res / v.size;
+    const double fg = (double)interp<true>(v, hit * inv_vs);
+    if (fg <= 0.5f) { miss++; continue; }
+    bool take = false;
+    if (hd == nearest) { if (fg > high_fg) take = true; }
+    else if (hd < nearest) { nearest = hd; take = true; }
+    if (!take) { miss++; continue; }
+    high_fg = fg;
+    out_l = O.label;
+    out_v = affine(P.pose, hit);
+    const f3 g = grad(v, hit * inv_vs);
+    out_n = (length(g) == 0) ? mk3((float)INVALID, 0.f, 0.f) : normalize(rotate(P.pose, g));
+  }
+  if (miss == nobj) { out_v = mk3(0.f, 0.f, 0.f); out_n = mk3((float)INVALID, 0.f, 0.f); out_l = INVALID; }
+  vx[pix] = out_v.x; vy[pix] = out_v.y; vz[pix] = out_v.z;
+  nx[pix] = out_n.x; ny[pix] = out_n.y; nz[pix] = out_n.z;
+  label[pix] = out_l;
+  for (int oi = 0; oi < nobj; oi++) {
+    const RayObj& O = objs[oi];
+    const bool mine = (out_l == O.label);
+    O.mvx[pix] = mine ? out_v.x : 0.f; O.mvy[pix] = mine ? out_v.y : 0.f; O.mvz[pix] = mine ? out_v.z : 0.f;
+    O.mnx[pix] = mine ? out_n.x : (float)INVALID; O.mny[pix] = mine ? out_n.y : 0.f; O.mnz[pix] = mine ? out_n.z : 0.f;
+  }
+}
+
+void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,
+                    float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label) {
+  dim3 blk(256), grd((w + 255) / 256, h);
+  k_raycast<<<grd, blk, 0, s>>>(objs, nobj, w, h, mu, nearp, farp, eps, vx, vy, vz, nx, ny, nz, label);
+}
+
+}  // namespace mf

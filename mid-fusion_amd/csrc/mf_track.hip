@@ -1,0 +1,485 @@
+// ICP + photometric Gauss-Newton tracking on gfx950.
+//
+// One fused kernel per GN iteration computes every pixel's ICP and RGB rows
+// (Tracking::track_ICP_Kernel / trackRGB / track_Obj_ICP_Kernel /
+// track_obj_RGB_kernel, tracking.cpp:392-578, 923-1111) in registers and
+// reduces the 32 robust-weighted sums (Tracking::reduce, tracking.cpp:
+// 1208-1352) per workgroup through an LDS transpose; rows never reach HBM.
+// A one-workgroup-per-instance finalize kernel accumulates the workgroup
+// partials in FP64, solves the 6x6 normal equations in FP64 and applies the
+// reference GN control flow (step-back, break-without-update, zero-inlier
+// break; tracking.cpp:300-334, 664-688) on the device, so a whole pyramid of
+// iterations is enqueued without host synchronisation.  A launch covers any
+// number of instances (camera or objects) through its grid's y dimension.
+#include "mf_kernels.h"
+
+namespace mf {
+
+static constexpr int TB = 256;   // threads per tracking workgroup
+
+__device__ __forceinline__ f3 ld3(const float* const* p, size_t i) { return mk3(p[0][i], p[1][i], p[2][i]); }
+
+// bilinear_interp<T> (utils/math_utils.h:149-166)
+__device__ __forceinline__ float bilin(const float* __restrict__ f, int w, float px, float py) {
+  const float fpx = floorf(px), fpy = floorf(py);
+  const int bx = (int)fpx, by = (int)fpy;
+  const float fx = px - fpx, fy = py - fpy;
+  const size_t i0 = (size_t)bx + (size_t)by * w;
+  const float a = f[i0], b = f[i0 + 1], c = f[i0 + w], d = f[i0 + w + 1];
+  return a * (1.f - fx) * (1.f - fy) + b * (1.f - fy) * fx + c * (1.f - fx) * fy + d * fx * fy;
+}
+__device__ __forceinline__ f3 bilin3(const float* const* f, int w, float px, float py) {
+  const float fpx = floorf(px), fpy = floorf(py);
+  const int bx = (int)fpx, by = (int)fpy;
+  const float fx = px - fpx, fy = py - fpy;
+  const size_t i0 = (size_t)bx + (size_t)by * w;
+  const float w0 = 1.f - fx, w1 = 1.f - fy;
+  f3 r;
+  float* rr = &r.x;
+  for (int k = 0; k < 3; k++) {
+    const float* p = f[k];
+    rr[k] = p[i0] * w0 * w1 + p[i0 + 1] * w1 * fx + p[i0 + w] * w0 * fy + p[i0 + w + 1] * fx * fy;
+  }
+  return r;
+}
+
+// calc_robust_weight / calc_robust_residual (tracking.cpp:1474-1532).  The
+// reference's sq() takes float: (e/k) in double, squared in float.
+__device__ __forceinline__ void robust(float e, int fn, double k, float& w, float& loss) {
+  switch (fn) {
+    case 0: w = 1.f; loss = (float)(0.5 * e * e); break;
+    case 1:
+      if (fabsf(e) <= k) { w = 1.f; loss = (float)(0.5 * e * e); }
+      else { w = (float)(k / fabs((double)e)); loss = (float)(k * (fabs((double)e) - 0.5 * k)); }
+      break;
+    case 2:
+      if (fabsf(e) <= k) {
+        const float q = (e * e) / ((float)k * (float)k);
+        const float a = (float)(1.0 - (double)q);
+        w = a * a;
+        const float r = (float)((double)e / k);
+        loss = (float)(k * k / 6.0 * (1.0 - pow(1.0 - (double)(r * r), 3)));
+      } else { w = 0.f; loss = (float)(k * k / 6.0); }
+      break;
+    default: {
+      const float r = (float)((double)e / k);
+      const float q = r * r;
+      w = 1.0f / (1.0f + q);
+      loss = (float)(0.5 * k * k * log(1.0 + (double)q));
+    }
+  }
+}
+
+struct Mats { Mat4 T, invT, A, B, C; };
+
+// ICP row (camera), track_ICP_Kernel mask overload (tracking.cpp:923-1020)
+__device__ __forceinline__ int icp_row(const TrackLaunch& p, const TrackInst& in, const Mats& M, int x, int y,
+                                       size_t i, bool masked, float& e, float J[6]) {
+  if (masked) return -6;
+  const f3 nl = mk3(p.live.N[0][i], p.live.N[1][i], p.live.N[2][i]);
+  if (nl.x == INVALID) return -1;
+  const f3 vl = mk3(p.live.V[0][i], p.live.V[1][i], p.live.V[2][i]);
+  const f3 pv = affine(M.T, vl);
+  const f3 pp = affine(M.B, pv);
+  const float ux = pp.x / pp.z, uy = pp.y / pp.z;
+  if (ux < 1 || ux > (float)(p.W - 2) || uy < 1 || uy > (float)(p.H - 2) || isnan(ux) || isnan(uy)) return -2;
+  const unsigned rx = (unsigned)ux, ry = (unsigned)uy;
+  const size_t ri = rx + (size_t)ry * p.W;
+  const f3 rn = ld3(in.mn, ri);
+  if (rn.x == INVALID) return -3;
+  const f3 diff = ld3(in.mv, ri) - pv;
+  const f3 pn = rotate(M.T, nl);
+  if (length(diff) > 0.1f) return -4;
+  if (dot(pn, rn) < 0.8f) return -5;
+  const f3 P = mk3(p.live.cov[0][i], p.live.cov[1][i], p.live.cov[2][i]);
+  const float sig = rn.x * rn.x * P.x + rn.y * rn.y * P.y + rn.z * rn.z * P.z;
+  const float w = sqrtf((float)(1.0 / (double)sig));
+  e = w * dot(rn, diff);
+  const f3 j0 = w * rn, j1 = w * cross(pv, rn);
+  J[0] = j0.x; J[1] = j0.y; J[2] = j0.z; J[3] = j1.x; J[4] = j1.y; J[5] = j1.z;
+  return 1;
+}
+
+// Object ICP row, track_Obj_ICP_Kernel (tracking.cpp:392-486)
+__device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst& in, const Mats& M, int x, int y,
+                                           size_t i, float& e, float J[6]) {
+  const f3 nl = mk3(p.live.N[0][i], p.live.N[1][i], p.live.N[2][i]);
+  if (nl.x == INVALID) return -1;
+  const f3 c2v = mk3(p.live.V[0][i], p.live.V[1][i], p.live.V[2][i]);
+  if (c2v.x == 0 && c2v.y == 0 && c2v.z == 0) return -1;
+  const f3 o2v = affine(M.invT, c2v);
+  const f3 c1v = affine(M.A, o2v);
+  const f3 pj = rotate(camK(p.fx, p.fy, p.cx, p.cy), c1v);
+  const float ux = pj.x / pj.z, uy = pj.y / pj.z;
+  if (ux < 1 || ux > (float)(p.W - 2) || uy < 1 || uy > (float)(p.H - 2) || isnan(ux) || isnan(uy)) return -2;
+  const f3 o2n = rotate(M.invT, nl);
+  const f3 wrv = bilin3(in.mv, p.W, ux, uy);
+  const f3 wrn = bilin3(in.mn, p.W, ux, uy);
+  if (wrn.x == INVALID) return -3;
+  const f3 o1rv = affine(M.B, wrv);
+  const f3 o1rn = rotate(M.B, wrn);
+  const f3 diff = o2v - o1rv;
+  const float residual = dot(o1rn, diff);
+  if (length(diff) > 0.1f) return -4;
+  if (dot(o2n, o1rn) < 0.8f) return -5;
+  const f3 P = mk3(p.live.cov[0][i], p.live.cov[1][i], p.live.cov[2][i]);
+  const float sig = o1rn.x * o1rn.x * P.x + o1rn.y * o1rn.y * P.y + o1rn.z * o1rn.z * P.z;
+  const float w = sqrtf((float)(1.0 / (double)sig));
+  e = w * residual;
+  const f3 jt = rotate(M.T, o1rn);
+  const f3 j0 = ((-1.0f) * w) * ((-1.0f) * jt);
+  const f3 j1 = w * cross(c2v, jt);
+  J[0] = j0.x; J[1] = j0.y; J[2] = j0.z; J[3] = j1.x; J[4] = j1.y; J[5] = j1.z;
+  return 1;
+}
+
+// Photometric rows: trackRGB (tracking.cpp:1023-1111) for the camera,
+// track_obj_RGB_kernel (tracking.cpp:491-578) for objects.
+template <bool OBJ>
+__device__ __forceinline__ int rgb_row(const TrackLaunch& p, const TrackInst& in, const Mats& M, int x, int y,
+                                       size_t i, float& e, float J[6]) {
+  const float s = (float)(1 << p.level);
+  const float fx = p.fx / s, fy = p.fy / s, cx = p.cx / s, cy = p.cy / s;
+  const float dr = in.rD[p.level][i];
+  const f3 vr = dr > 0.0f ? dr * rotate(camKinv(fx, fy, cx, cy), mk3((float)x, (float)y, 1.0f)) : mk3(0.f, 0.f, 0.f);
+  const f3 vl = mk3(p.ref.V[0][i], p.ref.V[1][i], p.ref.V[2][i]);
+  if (vr.z <= 0.f || vr.z == INVALID) return -1;
+  if (length(vr - vl) > 0.1f) { if (vl.z > 0.f) return -3; }
+  f3 lv, wv;
+  if (OBJ) {
+    const f3 o1v = affine(M.C, vr);
+    lv = affine(M.T, o1v);
+  } else {
+    wv = affine(M.A, vr);
+    lv = affine(M.invT, wv);
+  }
+  const f3 pj = rotate(camK(fx, fy, cx, cy), lv);
+  const float ux = pj.x / pj.z, uy = pj.y / pj.z;
+  if (OBJ) {
+    if (ux < 1 || ux > (float)(p.w - 2) || uy < 1 || uy > (float)(p.h - 2)) return -2;
+  } else {
+    if (ux < 1 || ux > (float)(p.w - 2) || uy < 1 || uy > (float)(p.h - 2) || isnan(ux) || isnan(uy)) return -2;
+  }
+  const float li = bilin(p.live.I, p.w, ux, uy);
+  const float res = p.ref.I[i] - li;
+  const float g_x = bilin(p.live.gx, p.w, ux, uy), g_y = bilin(p.live.gy, p.w, ux, uy);
+  const float sc = 1.f / lv.z;
+  f3 jt = sc * mk3(g_x * fx, g_y * fy, -(g_x * lv.x * fx + g_y * lv.y * fy) / lv.z);
+  f3 jr;
+  if (OBJ) {
+    jr = cross(lv, jt);
+  } else {
+    jt = (-1.f) * rotate(M.T, jt);
+    jr = cross(wv, jt);
+  }
+  const float ic = p.inv_cov_rgb;
+  const f3 j0 = ic * jt, j1 = ic * jr;
+  J[0] = j0.x; J[1] = j0.y; J[2] = j0.z; J[3] = j1.x; J[4] = j1.y; J[5] = j1.z;
+  e = ic * res;
+  return 1;
+}
+
+__device__ __forceinline__ void accum(float* s, int res, float e, const float* J, int fn, double k) {
+  if (res < 1) {
+    s[29] += res == -4 ? 1.f : 0.f;
+    s[30] += res == -5 ? 1.f : 0.f;
+    s[31] += res > -4 ? 1.f : 0.f;
+    return;
+  }
+  float w, loss;
+  robust(e, fn, k, w, loss);
+  s[0] += loss;
+  const float we = w * e;
+#pragma unroll
+  for (int a = 0; a < 6; a++) s[1 + a] += we * J[a];
+  int c = 7;
+#pragma unroll
+  for (int a = 0; a < 6; a++) {
+    const float wa = w * J[a];
+#pragma unroll
+    for (int b = a; b < 6; b++) s[c++] += wa * J[b];
+  }
+  s[28] += 1.f;
+}
+
+// Camera re-track outlier mask from the raycast labels of moving objects,
+// resized as cv::resize does for a 0/255 mask (tracking.cpp:1782-1788).
+__device__ __forceinline__ bool dyn_masked(const TrackLaunch& p, int x, int y) {
+  if (!p.label) return false;
+  const int s = 1 << p.level;
+  const int o = (p.level <= 1) ? 0 : (s / 2 - 1);
+  const int n = (p.level == 0) ? 1 : 2;
+  bool m = false;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      const int l = p.label[(size_t)(x * s + o + i) + (size_t)(y * s + o + j) * p.W];
+      if (l > 0 && l < p.n_labels && p.moving[l]) m = true;
+    }
+  return m;
+}
+
+template <bool ICP, bool RGB, bool OBJ>
+__global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __restrict__ insts,
+                                              GNState* __restrict__ st, float* __restrict__ partial, int nblk) {
+  __shared__ float red[32][TB + 1];
+  __shared__ Mats Msh;
+  const TrackInst in = insts[blockIdx.y];
+  GNState& S = st[in.state];
+  if (S.done) return;   // uniform per workgroup
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    Msh.T = S.T; Msh.invT = inverse(S.T); Msh.A = S.A; Msh.B = S.B; Msh.C = S.C;
+  }
+  __syncthreads();
+  const Mats& M = Msh;
+  float s[32];
+#pragma unroll
+  for (int k = 0; k < 32; k++) s[k] = 0.f;
+  const int n = p.w * p.h;
+  const size_t N0 = (size_t)p.W * p.H;
+  for (int i = blockIdx.x * TB + tid; i < n; i += nblk * TB) {
+    const int x = i % p.w, y = i / p.w;
+    float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
+    if (ICP) {
+      int r;
+      if (OBJ) r = obj_icp_row(p, in, M, x, y, (size_t)i, e, J);
+      else r = icp_row(p, in, M, x, y, (size_t)i, dyn_masked(p, x, y), e, J);
+      accum(s, r, e, J, p.robust, p.robust_k);
+      if (in.codes && p.level == 0) in.codes[(size_t)x + (size_t)y * p.W] = (int8_t)r;
+    }
+    if (RGB) {
+      const int r = rgb_row<OBJ>(p, in, M, x, y, (size_t)i, e, J);
+      accum(s, r, e, J, p.robust, p.robust_k);
+      if (in.codes && p.level == 0) in.codes[N0 + (size_t)x + (size_t)y * p.W] = (int8_t)r;
+    }
+  }
+  // LDS transpose reduction: 32 sums x 256 threads -> 32 sums.
+#pragma unroll
+  for (int k = 0; k < 32; k++) red[k][tid] = s[k];
+  __syncthreads();
+  {
+    const int k = tid >> 3, seg = tid & 7;   // 8 threads per sum
+    float a = 0.f;
+    for (int j = seg * (TB / 8); j < (seg + 1) * (TB / 8); j++) a += red[k][j];
+    a += __shfl_xor(a, 1);
+    a += __shfl_xor(a, 2);
+    a += __shfl_xor(a, 4);
+    if (seg == 0) partial[((size_t)blockIdx.y * nblk + blockIdx.x) * 32 + k] = a;
+  }
+}
+
+// ---- FP64 6x6 solve (commons.h:672-696 semantics: TooN GR_SVD backsub with
+// condition 1e6).  LDLT when tr(C)*tr(C^-1) < 1e6 certifies that no singular
+// value is truncated; otherwise a Jacobi eigendecomposition pseudo-inverse.
+__device__ void jacobi_pinv(const double C[6][6], const double b[6], double x[6]) {
+  double A[6][6], V[6][6];
+  for (int i = 0; i < 6; i++) for (int j = 0; j < 6; j++) { A[i][j] = C[i][j]; V[i][j] = (i == j); }
+  for (int sweep = 0; sweep < 60; sweep++) {
+    double off = 0;
+    for (int pp = 0; pp < 6; pp++) for (int q = pp + 1; q < 6; q++) off += A[pp][q] * A[pp][q];
+    if (off == 0) break;
+    for (int pp = 0; pp < 6; pp++)
+      for (int q = pp + 1; q < 6; q++) {
+        if (A[pp][q] == 0) continue;
+        const double theta = (A[q][q] - A[pp][pp]) / (2 * A[pp][q]);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+        const double c = 1 / sqrt(t * t + 1), sn = t * c;
+        for (int k = 0; k < 6; k++) { const double akp = A[k][pp], akq = A[k][q]; A[k][pp] = c * akp - sn * akq; A[k][q] = sn * akp + c * akq; }
+        for (int k = 0; k < 6; k++) { const double apk = A[pp][k], aqk = A[q][k]; A[pp][k] = c * apk - sn * aqk; A[q][k] = sn * apk + c * aqk; }
+        for (int k = 0; k < 6; k++) { const double vkp = V[k][pp], vkq = V[k][q]; V[k][pp] = c * vkp - sn * vkq; V[k][q] = sn * vkp + c * vkq; }
+      }
+  }
+  double mx = 0;
+  for (int i = 0; i < 6; i++) mx = fmax(mx, fabs(A[i][i]));
+  for (int i = 0; i < 6; i++) x[i] = 0;
+  for (int i = 0; i < 6; i++) {
+    if (fabs(A[i][i]) * 1e6 <= mx) continue;
+    double pr = 0;
+    for (int k = 0; k < 6; k++) pr += V[k][i] * b[k];
+    pr /= A[i][i];
+    for (int k = 0; k < 6; k++) x[k] += pr * V[k][i];
+  }
+}
+__device__ void solve6(const double* s27, double x[6]) {
+  double b[6], C[6][6];
+  for (int i = 0; i < 6; i++) b[i] = s27[i];
+  int c = 6;
+  for (int i = 0; i < 6; i++) for (int j = i; j < 6; j++) { C[i][j] = s27[c]; C[j][i] = s27[c]; c++; }
+  // LDL^T
+  double L[6][6], D[6];
+  bool ok = true;
+  for (int j = 0; j < 6 && ok; j++) {
+    double d = C[j][j];
+    for (int k = 0; k < j; k++) d -= L[j][k] * L[j][k] * D[k];
+    if (!(d > 0)) { ok = false; break; }
+    D[j] = d; L[j][j] = 1;
+    for (int i = j + 1; i < 6; i++) {
+      double v = C[i][j];
+      for (int k = 0; k < j; k++) v -= L[i][k] * L[j][k] * D[k];
+      L[i][j] = v / d;
+    }
+  }
+  if (ok) {
+    // tr(C^-1) = sum_j (1/D_j) sum_i (L^-1)_{ji}^2 ; Li = L^-1 (unit lower)
+    double Li[6][6];
+    for (int i = 0; i < 6; i++)
+      for (int j = 0; j < 6; j++) {
+        if (j > i) { Li[i][j] = 0; continue; }
+        if (j == i) { Li[i][j] = 1; continue; }
+        double v = 0;
+        for (int k = j; k < i; k++) v -= L[i][k] * Li[k][j];
+        Li[i][j] = v;
+      }
+    double trC = 0, trI = 0;
+    for (int i = 0; i < 6; i++) trC += C[i][i];
+    for (int j = 0; j < 6; j++) {
+      double ss = 0;
+      for (int i = j; i < 6; i++) ss += Li[i][j] * Li[i][j];
+      trI += ss / D[j];
+    }
+    if (trC * trI < 1e6) {
+      double z[6];
+      for (int i = 0; i < 6; i++) { double v = b[i]; for (int k = 0; k < i; k++) v -= L[i][k] * z[k]; z[i] = v; }
+      for (int i = 0; i < 6; i++) z[i] /= D[i];
+      for (int i = 5; i >= 0; i--) { double v = z[i]; for (int k = i + 1; k < 6; k++) v -= L[k][i] * x[k]; x[i] = v; }
+      return;
+    }
+  }
+  jacobi_pinv(C, b, x);
+}
+// SE3 exponential, x[0..2] translation, x[3..5] rotation (TooN SE3::exp as
+// used at tracking.cpp:1361): Rodrigues + left Jacobian in FP64.
+__device__ Mat4 se3_exp(const double x[6]) {
+  const double w0 = x[3], w1 = x[4], w2 = x[5];
+  const double th2 = w0 * w0 + w1 * w1 + w2 * w2;
+  double A, B, C;
+  if (th2 < 1e-8) { A = 1.0 - th2 / 6.0; B = 0.5 - th2 / 24.0; C = 1.0 / 6.0 - th2 / 120.0; }
+  else if (th2 < 1e-6) { C = (1.0 / 6.0) * (1.0 - th2 / 20.0); A = 1.0 - th2 * C; B = 0.5 - th2 / 24.0; }
+  else { const double th = sqrt(th2); A = sin(th) / th; B = (1.0 - cos(th)) / th2; C = (1.0 - A) / th2; }
+  const double W[3][3] = {{0, -w2, w1}, {w2, 0, -w0}, {-w1, w0, 0}};
+  double W2[3][3];
+  for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) { double s = 0; for (int k = 0; k < 3; k++) s += W[i][k] * W[k][j]; W2[i][j] = s; }
+  Mat4 M = ident();
+  for (int i = 0; i < 3; i++) {
+    for (int j = 0; j < 3; j++) M.m[i][j] = (float)((i == j) + A * W[i][j] + B * W2[i][j]);
+    double s = x[i];
+    for (int k = 0; k < 3; k++) s += (B * W[i][k] + C * W2[i][k]) * x[k];
+    M.m[i][3] = (float)s;
+  }
+  return M;
+}
+
+__global__ __launch_bounds__(256) void k_finalize(int object, int level, int iter, int ninst,
+                                                  const TrackInst* __restrict__ insts, GNState* __restrict__ st,
+                                                  const float* __restrict__ partial, int nblk,
+                                                  TraceRec* __restrict__ trace, int* trace_n, int trace_cap) {
+  __shared__ double acc[32][9];
+  const int inst = blockIdx.x;
+  GNState& S = st[insts[inst].state];
+  if (S.done) return;
+  const int tid = threadIdx.x;
+  const int k = tid >> 3, seg = tid & 7;
+  double a = 0;
+  for (int b = seg; b < nblk; b += 8) a += (double)partial[((size_t)inst * nblk + b) * 32 + k];
+  acc[k][seg] = a;
+  __syncthreads();
+  if (tid < 32) {
+    double t = 0;
+    for (int j = 0; j < 8; j++) t += acc[tid][j];
+    S.sums[tid] = t;
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  const float s0 = (float)S.sums[0], s28 = (float)S.sums[28];
+  const float err = s0 / s28;
+  int flag = 0;
+  S.iters_run++;
+  if (object && s28 == 0) { S.done = 1; flag = 3; }
+  else if (err > S.prev_err) { S.T = S.prev; S.done = 1; flag = 2; }
+  else {
+    S.prev_err = err;
+    double x[6];
+    solve6(S.sums + 1, x);
+    const Mat4 upd = se3_exp(x);
+    const double nx = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+    if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) { S.done = 1; flag = 1; }
+    else { S.prev = S.T; S.T = mul(upd, S.prev); }
+  }
+  if (trace) {
+    const int ti = atomicAdd(trace_n, 1);
+    if (ti < trace_cap) {
+      TraceRec& r = trace[ti];
+      r.inst = insts[inst].state; r.level = level; r.iter = iter; r.flag = flag; r.err = err; r.s28 = s28;
+      for (int q = 0; q < 16; q++) r.pose[q] = S.T.m[q / 4][q % 4];
+    }
+  }
+}
+
+__global__ void k_gn_begin_camera(GNState* st, const Mat4* T_w_l, const Mat4* T_w_r, float fx, float fy, float cx,
+                                  float cy) {
+  GNState& S = *st;
+  S.T = *T_w_l; S.prev = S.T; S.A = *T_w_r;
+  S.B = mul(camK(fx, fy, cx, cy), inverse(*T_w_r));
+  S.C = ident(); S.T_wo1 = ident();
+  S.prev_err = INFINITY; S.done = 0; S.iters_run = 0;
+}
+__global__ void k_gn_begin_object(GNState* st, const Mat4* T_w_c2, const Mat4* T_w_c1, const Mat4* T_wo1) {
+  GNState& S = *st;
+  const Mat4 To = *T_wo1;
+  S.T_wo1 = To;
+  S.A = mul(inverse(*T_w_c1), To);
+  S.T = mul(inverse(*T_w_c2), To);
+  S.prev = S.T;
+  S.B = inverse(To);
+  S.C = inverse(S.A);
+  S.prev_err = INFINITY; S.done = 0; S.iters_run = 0;
+}
+__global__ void k_gn_level_begin(GNState* st, int n) {
+  const int i = threadIdx.x;
+  if (i < n) { st[i].prev_err = INFINITY; st[i].done = 0; }
+}
+__global__ void k_gn_end_camera(const GNState* st, Mat4* T_w_l) { *T_w_l = st->T; }
+__global__ void k_gn_end_object(const GNState* st, const Mat4* T_w_c2, Mat4* pose) { *pose = mul(*T_w_c2, st->T); }
+__global__ void k_copy_mat(const Mat4* src, Mat4* dst) { *dst = *src; }
+__global__ void k_invert_mat(const Mat4* src, Mat4* dst) { *dst = inverse(*src); }
+
+// ------------------------------------------------------------ launchers
+int track_blocks(int w, int h) {
+  const int n = w * h;
+  int nb = (n + TB * 4 - 1) / (TB * 4);   // ~4 pixels per thread
+  return nb < 1 ? 1 : nb;
+}
+void launch_track(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                  float* partial, int nblk) {
+  dim3 grd(nblk, ninst), blk(TB);
+  if (p.object) {
+    if (p.use_icp && p.use_rgb) k_track<true, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+    else if (p.use_icp) k_track<true, false, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+    else k_track<false, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+  } else {
+    if (p.use_icp && p.use_rgb) k_track<true, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+    else if (p.use_icp) k_track<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+    else k_track<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+  }
+}
+void launch_finalize(hipStream_t s, const TrackLaunch& p, int ninst, const TrackInst* inst, GNState* st,
+                     const float* partial, int nblk, TraceRec* trace, int* trace_n, int trace_cap) {
+  k_finalize<<<ninst, 256, 0, s>>>(p.object, p.level, p.iter, ninst, inst, st, partial, nblk, trace, trace_n,
+                                   trace_cap);
+}
+void launch_gn_begin_camera(hipStream_t s, GNState* st, const Mat4* T_w_l, const Mat4* T_w_r, float fx, float fy,
+                            float cx, float cy) {
+  k_gn_begin_camera<<<1, 1, 0, s>>>(st, T_w_l, T_w_r, fx, fy, cx, cy);
+}
+void launch_gn_begin_object(hipStream_t s, GNState* st, const Mat4* T_w_c2, const Mat4* T_w_c1, const Mat4* T_wo1) {
+  k_gn_begin_object<<<1, 1, 0, s>>>(st, T_w_c2, T_w_c1, T_wo1);
+}
+void launch_gn_level_begin(hipStream_t s, GNState* st, int n) { k_gn_level_begin<<<1, 64, 0, s>>>(st, n); }
+void launch_gn_end_camera(hipStream_t s, const GNState* st, Mat4* T_w_l) { k_gn_end_camera<<<1, 1, 0, s>>>(st, T_w_l); }
+void launch_gn_end_object(hipStream_t s, const GNState* st, const Mat4* T_w_c2, Mat4* pose) {
+  k_gn_end_object<<<1, 1, 0, s>>>(st, T_w_c2, pose);
+}
+void launch_copy_mat(hipStream_t s, const Mat4* src, Mat4* dst) { k_copy_mat<<<1, 1, 0, s>>>(src, dst); }
+void launch_invert_mat(hipStream_t s, const Mat4* src, Mat4* dst) { k_invert_mat<<<1, 1, 0, s>>>(src, dst); }
+
+}  // namespace mf

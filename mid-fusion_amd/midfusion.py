@@ -1,0 +1,241 @@
+"""Python mirror of the reference `Kfusion` per-frame facade over libmidfusion.so.
+
+Method names follow apps/kfusion/include/kernels.h:242-284 (preprocessing,
+tracking, integration, raycasting, getPose).  Everything runs in the HIP
+library; this module only marshals arrays.  It raises if the library is
+missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libmidfusion.so')
+
+ROBUST = {'none': 0, 'huber': 1, 'tukey': 2, 'cauchy': 3}
+ROBUST_K = {'none': 1.0, 'huber': 1.345, 'tukey': 4.6851, 'cauchy': 2.3849}
+
+
+class MFConfig(C.Structure):
+    _fields_ = [('width', C.c_int), ('height', C.c_int), ('n_levels', C.c_int), ('iters', C.c_int * 4),
+                ('use_icp', C.c_int), ('use_rgb', C.c_int), ('robust', C.c_int), ('robust_k', C.c_double),
+                ('mu', C.c_float), ('track_rate', C.c_int), ('integ_rate', C.c_int), ('init_frame', C.c_int),
+                ('batch_objects', C.c_int)]
+
+
+class MFError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+# name: (restype, argtypes) for every entry point of include/midfusion.h
+P = C.c_void_p
+f32p = np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS')
+f64p = np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS')
+u64p = np.ctypeslib.ndpointer(np.uint64, flags='C_CONTIGUOUS')
+i32p = np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS')
+u16p = np.ctypeslib.ndpointer(np.uint16, flags='C_CONTIGUOUS')
+u8p = np.ctypeslib.ndpointer(np.uint8, flags='C_CONTIGUOUS')
+SIGNATURES = {
+    'mf_create': (P, [C.POINTER(MFConfig), f32p, C.c_float, C.c_int, C.c_long]),
+    'mf_destroy': (None, [P]),
+    'mf_last_error': (C.c_char_p, [P]),
+    'mf_object_create': (C.c_int, [P, C.c_float, C.c_int, f32p, C.c_long]),
+    'mf_object_set_moving': (C.c_int, [P, C.c_int, C.c_int]),
+    'mf_object_set_mask': (C.c_int, [P, C.c_int, f32p]),
+    'mf_object_set_mask_device': (C.c_int, [P, C.c_int, P]),
+    'mf_preprocess': (C.c_int, [P, u16p, u8p, C.c_int, C.c_int, C.c_int]),
+    'mf_preprocess_device': (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_int]),
+    'mf_track': (C.c_int, [P, f32p, C.c_uint]),
+    'mf_integrate': (C.c_int, [P, f32p, C.c_uint]),
+    'mf_raycast': (C.c_int, [P, f32p, C.c_uint]),
+    'mf_sync': (C.c_int, [P]),
+    'mf_get_pose': (C.c_int, [P, C.c_int, f32p]),
+    'mf_set_pose': (C.c_int, [P, C.c_int, f32p]),
+    'mf_get_maps': (C.c_int, [P, f32p, f32p, i32p]),
+    'mf_set_maps': (C.c_int, [P, f32p, f32p, i32p]),
+    'mf_get_object_maps': (C.c_int, [P, C.c_int, f32p, f32p]),
+    'mf_set_object_maps': (C.c_int, [P, C.c_int, f32p, f32p]),
+    'mf_object_keys': (C.c_long, [P, C.c_int, u64p, C.c_long]),
+    'mf_object_block': (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, f64p]),
+    'mf_object_node_count': (C.c_long, [P, C.c_int]),
+    'mf_get_trackresult': (C.c_int, [P, C.c_int, i32p]),
+    'mf_get_trace': (C.c_long, [P, f64p, C.c_long]),
+    'mf_clear_trace': (None, [P]),
+    'mf_get_level': (C.c_long, [P, C.c_int, C.c_int, f32p]),
+    'mf_get_frame': (C.c_int, [P, f32p, f32p, f32p]),
+    'mf_enable_timing': (C.c_int, [P, C.c_int]),
+    'mf_get_timing': (C.c_int, [P, f64p, C.c_int]),
+    'mf_kernel_name': (C.c_char_p, [C.c_int]),
+    'mf_device_bytes': (C.c_long, [P]),
+}
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library; raises if it was not built (no fallback)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise MFError('libmidfusion.so is not built: run mid-fusion_amd/build.py (no CPU fallback exists)')
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+class Kfusion:
+    """GPU per-frame dense path of one camera + object instances."""
+
+    def __init__(self, width, height, init_pose, bg_size=5.12, bg_res=512, iters=(10, 5, 4), use_icp=True,
+                 use_rgb=True, robust='cauchy', robust_k=None, mu=0.1, track_rate=1, integ_rate=1, init_frame=0,
+                 batch_objects=True, bg_capacity=0):
+        self.L = load()
+        self.w, self.h = width, height
+        cfg = MFConfig()
+        cfg.width, cfg.height, cfg.n_levels = width, height, len(iters)
+        for i, v in enumerate(iters):
+            cfg.iters[i] = v
+        cfg.use_icp, cfg.use_rgb = int(use_icp), int(use_rgb)
+        cfg.robust = ROBUST[robust]
+        cfg.robust_k = ROBUST_K[robust] if robust_k is None else robust_k
+        cfg.mu, cfg.track_rate, cfg.integ_rate, cfg.init_frame = mu, track_rate, integ_rate, init_frame
+        cfg.batch_objects = int(batch_objects)
+        self.ctx = self.L.mf_create(C.byref(cfg), _f32(np.asarray(init_pose).reshape(16)), bg_size, bg_res,
+                                    bg_capacity)
+        if not self.ctx:
+            raise MFError('mf_create failed (no HIP device or out of memory)')
+
+    def close(self):
+        if getattr(self, 'ctx', None):
+            self.L.mf_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, r):
+        if r < 0:
+            raise MFError(self.L.mf_last_error(self.ctx).decode() + ' (code %d)' % r)
+        return r
+
+    # ---- objects (Object, object.h:48-51)
+    def add_object(self, size, res, pose, capacity=0):
+        return self._chk(self.L.mf_object_create(self.ctx, size, res, _f32(np.asarray(pose).reshape(16)), capacity))
+
+    def set_moving(self, oid, moving):
+        self._chk(self.L.mf_object_set_moving(self.ctx, oid, int(moving)))
+
+    def set_mask(self, oid, mask):
+        self._chk(self.L.mf_object_set_mask(self.ctx, oid, _f32(mask).reshape(-1)))
+
+    # ---- per-frame stages (kernels.h:242-250)
+    def preprocessing(self, depth, rgb, filt=False):
+        d = np.ascontiguousarray(depth, np.uint16)
+        r = np.ascontiguousarray(rgb, np.uint8)
+        return self._chk(self.L.mf_preprocess(self.ctx, d.reshape(-1), r.reshape(-1), d.shape[1], d.shape[0],
+                                              int(filt)))
+
+    def preprocessing_device(self, d_depth, d_rgb, in_w, in_h, filt=False):
+        return self._chk(self.L.mf_preprocess_device(self.ctx, d_depth, d_rgb, in_w, in_h, int(filt)))
+
+    def tracking(self, k, frame):
+        return self._chk(self.L.mf_track(self.ctx, _f32(k), frame))
+
+    def integration(self, k, frame):
+        return self._chk(self.L.mf_integrate(self.ctx, _f32(k), frame))
+
+    def raycasting(self, k, frame):
+        return self._chk(self.L.mf_raycast(self.ctx, _f32(k), frame))
+
+    def sync(self):
+        return self._chk(self.L.mf_sync(self.ctx))
+
+    # ---- state
+    def getPose(self, oid=-1):
+        T = np.zeros(16, np.float32)
+        self._chk(self.L.mf_get_pose(self.ctx, oid, T))
+        return T.reshape(4, 4)
+
+    def setPose(self, oid, T):
+        self._chk(self.L.mf_set_pose(self.ctx, oid, _f32(np.asarray(T).reshape(16))))
+
+    def maps(self):
+        n = self.w * self.h
+        v, nn, lab = np.zeros(3 * n, np.float32), np.zeros(3 * n, np.float32), np.zeros(n, np.int32)
+        self._chk(self.L.mf_get_maps(self.ctx, v, nn, lab))
+        return v.reshape(self.h, self.w, 3), nn.reshape(self.h, self.w, 3), lab.reshape(self.h, self.w)
+
+    def set_maps(self, vtx, nrm, lab):
+        self._chk(self.L.mf_set_maps(self.ctx, _f32(vtx).reshape(-1), _f32(nrm).reshape(-1),
+                                     np.ascontiguousarray(lab, np.int32).reshape(-1)))
+
+    def object_maps(self, oid):
+        n = self.w * self.h
+        v, nn = np.zeros(3 * n, np.float32), np.zeros(3 * n, np.float32)
+        self._chk(self.L.mf_get_object_maps(self.ctx, oid, v, nn))
+        return v.reshape(self.h, self.w, 3), nn.reshape(self.h, self.w, 3)
+
+    def set_object_maps(self, oid, vtx, nrm):
+        self._chk(self.L.mf_set_object_maps(self.ctx, oid, _f32(vtx).reshape(-1), _f32(nrm).reshape(-1)))
+
+    def keys(self, oid):
+        n = self._chk(self.L.mf_object_keys(self.ctx, oid, np.zeros(1, np.uint64), 0))
+        out = np.zeros(max(n, 1), np.uint64)
+        self.L.mf_object_keys(self.ctx, oid, out, n)
+        return out[:n]
+
+    def block(self, oid, x, y, z):
+        out = np.zeros(512 * 8)
+        ok = self._chk(self.L.mf_object_block(self.ctx, oid, x, y, z, out))
+        return out.reshape(512, 8) if ok else None
+
+    def node_count(self, oid):
+        return self._chk(self.L.mf_object_node_count(self.ctx, oid))
+
+    def trackresult(self, oid):
+        out = np.zeros(2 * self.w * self.h, np.int32)
+        self._chk(self.L.mf_get_trackresult(self.ctx, oid, out))
+        return out
+
+    def trace(self):
+        n = self.L.mf_get_trace(self.ctx, np.zeros(22), 0)
+        out = np.zeros(22 * max(n, 1))
+        self.L.mf_get_trace(self.ctx, out, n)
+        return out[:22 * n].reshape(n, 22)
+
+    def trace_clear(self):
+        self.L.mf_clear_trace(self.ctx)
+
+    def level(self, level, which):
+        comp = 3 if which in (2, 3, 6) else 1
+        n = (self.w >> level) * (self.h >> level)
+        out = np.zeros(n * comp, np.float32)
+        self._chk(self.L.mf_get_level(self.ctx, level, which, out))
+        shape = ((self.h >> level), (self.w >> level))
+        return out.reshape(shape + (3,)) if comp == 3 else out.reshape(shape)
+
+    def frame_buffers(self):
+        n = self.w * self.h
+        d, g, r = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(3 * n, np.float32)
+        self._chk(self.L.mf_get_frame(self.ctx, d, g, r))
+        return d.reshape(self.h, self.w), g.reshape(self.h, self.w), r.reshape(self.h, self.w, 3)
+
+    def enable_timing(self, on=True):
+        self.L.mf_enable_timing(self.ctx, int(on))
+
+    def timing(self):
+        out = np.zeros(64)
+        n = self.L.mf_get_timing(self.ctx, out, 32)
+        return {self.L.mf_kernel_name(i).decode(): (out[2 * i], int(out[2 * i + 1])) for i in range(n)}
+
+    def device_bytes(self):
+        return self.L.mf_device_bytes(self.ctx)

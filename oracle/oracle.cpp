@@ -1,0 +1,730 @@
+/*
+ * CPU ORACLE — TEST INFRASTRUCTURE ONLY (see oracle_core.h).
+ *
+ * Pipeline-level restatement of the reference's per-frame dense path:
+ * Tracking (tracking/tracking.cpp), Object (object/object.cpp:128-195,
+ * 255-315), raycastObjectList (rendering/rendering.cpp:9-181) and the
+ * Kfusion stage sequencing (kfusion.cpp:165-677), behind a C API that the
+ * tests load with ctypes.  Built twice: float (reference semantics) and
+ * double (-DORC_REAL=double, the FP64 oracle).
+ */
+#include "oracle_pre.h"
+#include "oracle_track.h"
+#include "oracle_map.h"
+#include <cstdio>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using namespace orc;
+
+namespace {
+
+struct Cfg {
+  int w, h;
+  int nlev;
+  int iters[4];
+  int use_icp, use_rgb;
+  int robust;       // Robust enum
+  double robust_k;
+  float mu;
+  int track_rate, integ_rate;
+  int init_frame;
+  int alloc_quirks;  // 1: reproduce the reference allocation-list quirks
+};
+
+static M4 m4_from(const float* T) { M4 r; for (int i = 0; i < 16; i++) r.m[i / 4][i % 4] = (R)T[i]; return r; }
+static void m4_to(const M4& M, double* T) { for (int i = 0; i < 16; i++) T[i] = (double)M.m[i / 4][i % 4]; }
+
+struct TraceRec { int inst, level, iter; double err, s28; int flag; double pose[16]; };
+
+// ---------------------------------------------------------------- Tracking
+struct Tracker {
+  Cfg c;
+  int W, H;
+  std::vector<int> lw, lh;
+  std::vector<std::vector<R>> l_D, l_I, gx, gy, r_I, r_D_render;
+  std::vector<std::vector<V3>> l_V, l_N, cov, r_V_live, r_V_render;
+  std::vector<std::vector<TrackRow>> J;   // per level, 2*W*H rows (stride W)
+  R fx, fy, cx, cy;
+  R sigma_bright, sigma_disp, sigma_xy, baseline;
+  int stack;
+  std::vector<TraceRec>* trace = nullptr;
+  int trace_inst = 0;
+  bool have_ref = false;
+
+  void init(const Cfg& cfg) {
+    c = cfg; W = c.w; H = c.h;
+    lw.resize(c.nlev); lh.resize(c.nlev);
+    auto rs = [&](auto& v, size_t n) { v.resize(c.nlev); for (auto& e : v) e.assign(n, {}); };
+    for (int l = 0; l < c.nlev; l++) { lw[l] = W >> l; lh[l] = H >> l; }
+    l_D.resize(c.nlev); l_I.resize(c.nlev); gx.resize(c.nlev); gy.resize(c.nlev); r_I.resize(c.nlev);
+    r_D_render.resize(c.nlev); l_V.resize(c.nlev); l_N.resize(c.nlev); cov.resize(c.nlev);
+    r_V_live.resize(c.nlev); r_V_render.resize(c.nlev); J.resize(c.nlev);
+    (void)rs;
+    for (int l = 0; l < c.nlev; l++) {
+      size_t n = (size_t)lw[l] * lh[l];
+      l_D[l].assign(n, 0); l_I[l].assign(n, 0); gx[l].assign(n, 0); gy[l].assign(n, 0); r_I[l].assign(n, 0);
+      r_D_render[l].assign(n, 0);
+      l_V[l].assign(n, v3(0, 0, 0)); l_N[l].assign(n, v3(0, 0, 0)); cov[l].assign(n, v3(0, 0, 0));
+      r_V_live[l].assign(n, v3(0, 0, 0)); r_V_render[l].assign(n, v3(0, 0, 0));
+      TrackRow z; memset(&z, 0, sizeof z);
+      J[l].assign((size_t)2 * W * H, z);
+    }
+    stack = (c.use_icp && c.use_rgb) ? 2 : 1;
+  }
+  // Tracking::obtainErrorParameters (tracking.cpp:728-773): TUM-style
+  // parameters for k.y > 0, ICL-NUIM for k.y < 0 (same values on this path).
+  void error_params() {
+    sigma_disp = 1.0; sigma_xy = 1.0;
+    sigma_bright = (R)(std::sqrt((float)100.0) / 255.0);
+    baseline = (R)0.075;
+  }
+  // Tracking::set_params_frame + buildPyramid (tracking.cpp:142-171,776-818)
+  void set_params_frame(const float k[4], const R* grey, const R* depth) {
+    fx = k[0]; fy = k[1]; cx = k[2]; cy = k[3];
+    error_params();
+    for (int l = 0; l < c.nlev; l++) {
+      if (l == 0) {
+        bilateral(l_I[0].data(), grey, W, H, R(0.1), 2);
+        bilateral(l_D[0].data(), depth, W, H, R(0.1), 2);
+      } else {
+        half_sample(l_D[l].data(), l_D[l - 1].data(), lw[l - 1], lh[l - 1], R(0.1f * 3), 1);
+        half_sample(l_I[l].data(), l_I[l - 1].data(), lw[l - 1], lh[l - 1], R(0.1f * 3), 1);
+      }
+      const R s = (R)(1 << l);
+      M4 invK = camKinv(fx / s, fy / s, cx / s, cy / s);
+      depth2vertex(l_V[l].data(), l_D[l].data(), lw[l], lh[l], invK);
+      vertex2normal(l_N[l].data(), l_V[l].data(), lw[l], lh[l], k[1] < 0);
+      if (c.use_rgb) gradients(gx[l].data(), gy[l].data(), l_I[l].data(), lw[l], lh[l]);
+    }
+    if (c.use_icp)
+      for (int l = c.nlev - 1; l >= 0; --l) {
+        const R s = (R)(1 << l);
+        icp_cov(cov[l].data(), l_D[l].data(), lw[l], lh[l], fx / s, sigma_xy, sigma_disp, baseline);
+      }
+  }
+  // Tracking::setRefImgFromCurr (tracking.cpp:1401-1414)
+  void set_ref_from_curr() {
+    if (c.use_rgb)
+      for (int l = 0; l < c.nlev; l++) { r_I[l] = l_I[l]; r_V_live[l] = l_V[l]; }
+    have_ref = true;
+  }
+  // Rendered reference depth/vertex pyramid (tracking.cpp:218-238, 599-618)
+  void render_reference(const V3* model_vertex, const M4& T_w_r) {
+    for (int l = 0; l < c.nlev; l++) {
+      const R s = (R)(1 << l);
+      M4 invK = camKinv(fx / s, fy / s, cx / s, cy / s);
+      if (l == 0) vertex2depth(r_D_render[0].data(), model_vertex, W, H, T_w_r);
+      else half_sample(r_D_render[l].data(), r_D_render[l - 1].data(), lw[l - 1], lh[l - 1], R(0.1f * 3), 1);
+      depth2vertex(r_V_render[l].data(), r_D_render[l].data(), lw[l], lh[l], invK);
+    }
+  }
+  void reduce_level(int l, double sums[32]) {
+    R out[8 * 32];
+    const TrackRow* base = J[l].data() + ((!c.use_icp && c.use_rgb) ? (size_t)W * H : 0);
+    orc::reduce(out, base, W, H, lw[l], lh[l], stack, c.robust, c.robust_k);
+    for (int i = 0; i < 32; i++) sums[i] = (double)out[i];
+  }
+  void record(int level, int iter, double err, double s28, int flag, const M4& T) {
+    if (!trace) return;
+    TraceRec r; r.inst = trace_inst; r.level = level; r.iter = iter; r.err = err; r.s28 = s28; r.flag = flag;
+    m4_to(T, r.pose);
+    trace->push_back(r);
+  }
+  // Downscaled outlier masks: cv::resize of a 0/255 mask (tracking.cpp:1782-1788);
+  // level 1 (x1/2): nonzero iff any pixel of the 2x2 block is; level 2 (x1/4,
+  // bilinear at the block centre): iff any of the central 2x2 pixels is.
+  std::vector<std::vector<uint8_t>> level_masks(const uint8_t* m) {
+    std::vector<std::vector<uint8_t>> out(c.nlev);
+    for (int l = 0; l < c.nlev; l++) {
+      out[l].assign((size_t)lw[l] * lh[l], 0);
+      if (!m) continue;
+      const int s = 1 << l;
+      const int o = (l <= 1) ? 0 : (s / 2 - 1);
+      const int n = (l == 0) ? 1 : 2;
+      for (int y = 0; y < lh[l]; y++)
+        for (int x = 0; x < lw[l]; x++) {
+          uint8_t v = 0;
+          for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++) v |= m[(x * s + o + i) + (y * s + o + j) * W] ? 1 : 0;
+          out[l][x + y * lw[l]] = v ? 255 : 0;
+        }
+    }
+    return out;
+  }
+  // Tracking::trackLiveFrame (tracking.cpp:208-357; mask overload :1750-1878)
+  void track_live(M4& T_w_l, const M4& T_w_r, const V3* mvert, const V3* mnorm, const uint8_t* mask) {
+    if (c.use_rgb) render_reference(mvert, T_w_r);
+    auto masks = level_masks(mask);
+    M4 prev = T_w_l;
+    const M4 view = mul(camK(fx, fy, cx, cy), inverse(T_w_r));
+    const R inv_cov = (R)(1.0 / (double)sigma_bright);
+    for (int l = c.nlev - 1; l >= 0; --l) {
+      R prev_err = (R)INFINITY;
+      for (int it = 0; it < c.iters[l]; ++it) {
+        const M4 inv_T = inverse(T_w_l);
+        const R s = (R)(1 << l);
+#pragma omp parallel for
+        for (int y = 0; y < lh[l]; y++)
+          for (int x = 0; x < lw[l]; x++) {
+            if (c.use_icp)
+              icp_row(J[l][x + y * W], x, y, l_V[l].data(), l_N[l].data(), lw[l], mvert, mnorm, W, H, T_w_l, view,
+                      R(0.1), R(0.8), cov[l].data(), mask ? masks[l].data() : nullptr);
+            if (c.use_rgb)
+              rgb_row(J[l][(size_t)W * H + x + y * W], x, y, r_V_render[l].data(), r_V_live[l].data(), r_I[l].data(),
+                      lw[l], l_I[l].data(), lw[l], lh[l], gx[l].data(), gy[l].data(), T_w_r, T_w_l, inv_T, fx / s,
+                      fy / s, cx / s, cy / s, inv_cov, R(0.1), nullptr);
+          }
+        double sums[32];
+        reduce_level(l, sums);
+        const R err = (R)sums[0] / (R)sums[28];
+        if (err > prev_err) { T_w_l = prev; record(l, it, err, sums[28], 2, T_w_l); break; }
+        prev_err = err;
+        M4 upd;
+        if (solve_pose(upd, sums, nullptr)) { record(l, it, err, sums[28], 1, T_w_l); break; }
+        prev = T_w_l;
+        T_w_l = mul(upd, prev);
+        record(l, it, err, sums[28], 0, T_w_l);
+      }
+    }
+  }
+  // Tracking::trackEachObject (tracking.cpp:580-708).  Returns T_c2_o2.
+  void track_object(M4& volume_pose, M4& virtual_cam, const V3* mvert, const V3* mnorm, const M4& T_w_c2,
+                    const M4& T_w_c1) {
+    const M4 T_wo1 = volume_pose;
+    const M4 T_c1_o1 = mul(inverse(T_w_c1), T_wo1);
+    M4 T_c2_o2 = mul(inverse(T_w_c2), T_wo1);
+    if (c.use_rgb) render_reference(mvert, T_w_c1);
+    M4 prev = T_c2_o2;
+    const M4 inv_T_w_o1 = inverse(T_wo1);
+    const M4 inv_T_c1_o1 = inverse(T_c1_o1);
+    const M4 K0 = camK(fx, fy, cx, cy);
+    const R inv_cov = (R)(1.0 / (double)sigma_bright);
+    for (int l = c.nlev - 1; l >= 0; --l) {
+      R prev_err = (R)INFINITY;
+      for (int it = 0; it < c.iters[l]; ++it) {
+        const M4 inv_T = inverse(T_c2_o2);
+        const R s = (R)(1 << l);
+#pragma omp parallel for
+        for (int y = 0; y < lh[l]; y++)
+          for (int x = 0; x < lw[l]; x++) {
+            if (c.use_icp)
+              obj_icp_row(J[l][x + y * W], x, y, l_V[l].data(), l_N[l].data(), lw[l], mvert, mnorm, W, H, T_c2_o2,
+                          inv_T, T_c1_o1, inv_T_w_o1, K0, R(0.1), R(0.8), cov[l].data(), nullptr);
+            if (c.use_rgb)
+              obj_rgb_row(J[l][(size_t)W * H + x + y * W], x, y, r_V_render[l].data(), r_V_live[l].data(),
+                          r_I[l].data(), lw[l], l_I[l].data(), lw[l], lh[l], gx[l].data(), gy[l].data(), T_c2_o2,
+                          inv_T_c1_o1, fx / s, fy / s, cx / s, cy / s, inv_cov, R(0.1), nullptr);
+          }
+        double sums[32];
+        reduce_level(l, sums);
+        const R err = (R)sums[0] / (R)sums[28];
+        if ((R)sums[28] == 0) { record(l, it, err, sums[28], 3, T_c2_o2); break; }
+        if (err > prev_err) { T_c2_o2 = prev; record(l, it, err, sums[28], 2, T_c2_o2); break; }
+        prev_err = err;
+        M4 upd;
+        if (solve_pose(upd, sums, nullptr)) { record(l, it, err, sums[28], 1, T_c2_o2); break; }
+        prev = T_c2_o2;
+        T_c2_o2 = mul(upd, prev);
+        record(l, it, err, sums[28], 0, T_c2_o2);
+      }
+    }
+    volume_pose = mul(T_w_c2, T_c2_o2);
+    virtual_cam = mul(T_wo1, inverse(T_c2_o2));
+  }
+};
+
+// ----------------------------------------------------------------- Objects
+struct Object {
+  int label;
+  R size; int res;
+  Octree vol;
+  M4 pose, virtual_cam;
+  bool is_static = true;
+  std::vector<V3> m_vertex, m_normal, m_vertex_bef, m_normal_bef;
+  std::vector<TrackRow> trackresult;
+  std::vector<R> mask;
+  bool has_mask = false;
+  std::vector<uint64_t> quirk_list;   // persistent allocation list (quirks mode)
+  std::vector<uint64_t> last_keys;    // keys handed to alloc_update last frame
+  size_t last_emitted = 0;
+};
+
+struct Pipe {
+  Cfg c;
+  std::vector<R> depth, grey;
+  std::vector<V3> rgb;
+  std::vector<V3> vertex, normal;
+  std::vector<int> label;
+  M4 camera_pose, T_w_r;
+  Tracker trk;
+  std::vector<Object*> objs;
+  std::vector<TraceRec> trace;
+  float ray_eps = -1.f;   // ray_iterator's static epsilon: set by the first raycast volume
+
+  ~Pipe() { for (Object* o : objs) { o->vol.release(); delete o; } }
+};
+
+// Object::refine_mask_use_motion (object.cpp:255-315), absorb_outlier_bg=false
+static void refine_mask(std::vector<R>& m, const std::vector<TrackRow>& tr, int W, int H, bool icp, bool rgb) {
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      const size_t ip = x + (size_t)y * W, rp = ip + (size_t)W * H;
+      if (m[ip] >= 0) {
+        if (icp && (tr[ip].result == -4 || tr[ip].result == -6)) { m[ip] = -1; continue; }
+        if (rgb && tr[rp].result < -3) { m[ip] = -1; continue; }
+      }
+    }
+}
+
+// Object::integrate_volume_kernel (object.cpp:128-195)
+static void integrate_object(Pipe& p, Object& o, const std::vector<R>& mask, const float k[4]) {
+  const R voxel = o.size / (R)o.res;
+  const M4 K = camK(k[0], k[1], k[2], k[3]);
+  std::vector<uint64_t> keys;
+  o.last_emitted = build_alloc_list(keys, o.vol, p.camera_pose, o.pose, K, p.depth.data(), mask.data(), p.c.w, p.c.h,
+                                    voxel, R(2) * (R)p.c.mu);
+  if (p.c.alloc_quirks) {
+    // allocationList[++count] = key; alloc_update(list, count)  (alloc_impl.hpp:175-189)
+    size_t need = keys.size() + 1;
+    if (o.quirk_list.size() < need) o.quirk_list.resize(need, 0);
+    for (size_t i = 0; i < keys.size(); i++) o.quirk_list[i + 1] = keys[i];
+    o.last_keys.assign(o.quirk_list.begin(), o.quirk_list.begin() + keys.size());
+    o.vol.alloc_update(o.quirk_list.data(), (int)keys.size());
+  } else {
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    o.last_keys = keys;
+    if (!keys.empty()) o.vol.alloc_update(keys.data(), (int)keys.size());
+  }
+  const M4 Tco = mul(inverse(p.camera_pose), o.pose);
+  integrate(o.vol, Tco, K, p.depth.data(), p.rgb.data(), mask.data(), p.c.w, p.c.h, (R)p.c.mu, 100);
+}
+
+// raycastObjectList (rendering.cpp:9-181), all objects, has_integrated selects
+// the per-object output maps.
+static void raycast_all(Pipe& p, const float k[4], bool has_integrated) {
+  const int W = p.c.w, H = p.c.h;
+  const M4 view = mul(p.camera_pose, camKinv(k[0], k[1], k[2], k[3]));
+  const R nearp = R(0.1), farp = R(4.0);
+  if (p.ray_eps < 0 && !p.objs.empty()) {
+    int lg = 0; while ((1 << lg) < p.objs[0]->res) lg++;
+    p.ray_eps = std::exp2f(-(float)lg);
+  }
+  std::vector<M4> inv_pose(p.objs.size());
+  for (size_t i = 0; i < p.objs.size(); i++) inv_pose[i] = inverse(p.objs[i]->pose);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      const size_t pix = x + (size_t)y * W;
+      R nearest = R(1.0e3);
+      double high_fg = 0;
+      size_t miss = 0;
+      for (size_t oi = 0; oi < p.objs.size(); oi++) {
+        Object& o = *p.objs[oi];
+        const R step = std::min(o.size, o.size) / (R)o.res;
+        const R largestep = step * 8;
+        const V3 wo = translation(view);
+        const V3 oo = affine(inv_pose[oi], wo);
+        const V3 wd = rotate(view, v3(R(x), R(y), R(1)));
+        const V3 od = normalize(rotate(inv_pose[oi], wd));
+        const R nfar = fmin_r(nearest, farp);
+        const float of[3] = {(float)oo.x, (float)oo.y, (float)oo.z}, df[3] = {(float)od.x, (float)od.y, (float)od.z};
+        RayIter it(o.vol, of, df, (float)nearp, (float)nfar, p.ray_eps);
+        const R tmin = (R)it.first();
+        V3 hit = v3(0, 0, 0); R hw = 0;
+        if (tmin > 0) raycast_ray(o.vol, oo, od, tmin * o.size / (R)o.res, nfar, (R)p.c.mu, step, largestep, hit, hw);
+        const R hd = (o.label == 0) ? hw + R(0.05) : hw;
+        if (hd > nearest) { miss++; continue; }
+        const double fg = (double)interp(o.vol, hit * ((R)o.res / o.size), F_FG);
+        if (fg <= 0.5f) { miss++; continue; }
+        bool take = false;
+        if (hd == nearest) { if (fg > high_fg) take = true; }
+        else if (hd < nearest) { nearest = hd; take = true; }
+        if (!take) { miss++; continue; }
+        high_fg = fg;
+        p.label[pix] = o.label;
+        p.vertex[pix] = affine(o.pose, hit);
+        const V3 g = grad(o.vol, hit * ((R)o.res / o.size));
+        if (length(g) == 0) p.normal[pix] = v3(R(INVALID), 0, 0);
+        else p.normal[pix] = normalize(rotate(o.pose, g));
+      }
+      if (miss == p.objs.size()) { p.vertex[pix] = v3(0, 0, 0); p.normal[pix] = v3(R(INVALID), 0, 0); p.label[pix] = INVALID; }
+      for (Object* o : p.objs) {
+        V3 ov = v3(0, 0, 0), on = v3(R(INVALID), 0, 0);
+        if (p.label[pix] == o->label) { ov = p.vertex[pix]; on = p.normal[pix]; }
+        if (has_integrated) { o->m_vertex[pix] = ov; o->m_normal[pix] = on; }
+        else { o->m_vertex_bef[pix] = ov; o->m_normal_bef[pix] = on; }
+      }
+    }
+}
+
+}  // namespace
+
+// ============================================================== C API ====
+extern "C" {
+
+typedef struct {
+  int w, h, nlev, iters[4], use_icp, use_rgb, robust;
+  double robust_k;
+  float mu;
+  int track_rate, integ_rate, init_frame, alloc_quirks;
+} orc_config;
+
+int orc_real_bytes(void) { return (int)sizeof(R); }
+
+void* orc_create(const orc_config* cfg, const float init_pose[16]) {
+  Pipe* p = new Pipe();
+  Cfg& c = p->c;
+  c.w = cfg->w; c.h = cfg->h; c.nlev = cfg->nlev;
+  for (int i = 0; i < 4; i++) c.iters[i] = cfg->iters[i];
+  c.use_icp = cfg->use_icp; c.use_rgb = cfg->use_rgb; c.robust = cfg->robust; c.robust_k = cfg->robust_k;
+  c.mu = cfg->mu; c.track_rate = cfg->track_rate; c.integ_rate = cfg->integ_rate; c.init_frame = cfg->init_frame;
+  c.alloc_quirks = cfg->alloc_quirks;
+  const size_t n = (size_t)c.w * c.h;
+  p->depth.assign(n, 0); p->grey.assign(n, 0); p->rgb.assign(n, v3(0, 0, 0));
+  p->vertex.assign(n, v3(0, 0, 0)); p->normal.assign(n, v3(0, 0, 0)); p->label.assign(n, INVALID);
+  p->camera_pose = m4_from(init_pose);
+  p->T_w_r = p->camera_pose;
+  p->trk.init(c);
+  p->trk.trace = &p->trace;
+  return p;
+}
+void orc_destroy(void* h) { delete (Pipe*)h; }
+
+// Object(...) (object.cpp:19-46); label == index in the object list.
+int orc_object_create(void* h, float size_m, int res, const float T_w_o[16]) {
+  Pipe* p = (Pipe*)h;
+  Object* o = new Object();
+  o->label = (int)p->objs.size();
+  o->size = size_m; o->res = res;
+  o->vol.init(res, size_m);
+  o->pose = m4_from(T_w_o); o->virtual_cam = o->pose;
+  const size_t n = (size_t)p->c.w * p->c.h;
+  o->m_vertex.assign(n, v3(0, 0, 0)); o->m_normal.assign(n, v3(0, 0, 0));
+  o->m_vertex_bef.assign(n, v3(0, 0, 0)); o->m_normal_bef.assign(n, v3(0, 0, 0));
+  TrackRow z; memset(&z, 0, sizeof z);
+  o->trackresult.assign(2 * n, z);
+  p->objs.push_back(o);
+  return o->label;
+}
+int orc_object_set_moving(void* h, int id, int moving) { ((Pipe*)h)->objs[id]->is_static = !moving; return 0; }
+int orc_object_set_mask(void* h, int id, const float* mask) {
+  Pipe* p = (Pipe*)h; Object* o = p->objs[id];
+  const size_t n = (size_t)p->c.w * p->c.h;
+  if (!mask) { o->has_mask = false; return 0; }
+  o->mask.resize(n);
+  for (size_t i = 0; i < n; i++) o->mask[i] = mask[i];
+  o->has_mask = true;
+  return 0;
+}
+
+// Kfusion::preprocessing (kfusion.cpp:210-235).  filter: bilateral applied
+// out of place (the reference's in-place call races on its own stencil).
+int orc_preprocess(void* h, const uint16_t* depth, const uint8_t* rgb, int iw, int ih, int filter) {
+  Pipe* p = (Pipe*)h;
+  const int W = p->c.w, H = p->c.h;
+  if (rgb && !rgb2intensity(p->grey.data(), p->rgb.data(), W, H, rgb, iw, ih)) return -1;
+  if (!mm2meters(p->depth.data(), W, H, depth, iw, ih)) return -1;
+  if (filter) {
+    std::vector<R> tmp(p->depth);
+    bilateral(p->depth.data(), tmp.data(), W, H, R(0.1), 2);
+  }
+  return 0;
+}
+
+// Kfusion::tracking (kfusion.cpp:237-488) with the moving-object set given
+// by the caller (orc_object_set_moving) instead of the motion-residual test.
+int orc_track(void* h, const float k[4], unsigned frame) {
+  Pipe* p = (Pipe*)h;
+  Tracker& t = p->trk;
+  const int W = p->c.w, H = p->c.h;
+  const size_t n = (size_t)W * H;
+  if (frame % p->c.track_rate != 0) return 0;
+  p->T_w_r = p->camera_pose;
+  t.set_params_frame(k, p->grey.data(), p->depth.data());
+  if ((int)frame == p->c.init_frame) { t.set_ref_from_curr(); return 0; }
+  t.trace_inst = 0;
+  t.track_live(p->camera_pose, p->T_w_r, p->vertex.data(), p->normal.data(), nullptr);
+  p->objs[0]->virtual_cam = p->camera_pose;
+  memcpy(p->objs[0]->trackresult.data(), t.J[0].data(), sizeof(TrackRow) * 2 * n);
+  std::vector<uint8_t> dyn(n, 0);
+  bool any_moving = false;
+  for (size_t i = 1; i < p->objs.size(); i++) if (!p->objs[i]->is_static) any_moving = true;
+  if (any_moving) {
+    for (size_t i = 0; i < n; i++) {
+      int l = p->label[i];
+      if (l > 0 && l < (int)p->objs.size() && !p->objs[l]->is_static) dyn[i] = 255;
+    }
+    t.track_live(p->camera_pose, p->T_w_r, p->vertex.data(), p->normal.data(), dyn.data());
+    p->objs[0]->virtual_cam = p->camera_pose;
+    memcpy(p->objs[0]->trackresult.data(), t.J[0].data(), sizeof(TrackRow) * 2 * n);
+  }
+  for (size_t i = 1; i < p->objs.size(); i++) {
+    Object* o = p->objs[i];
+    if (o->is_static) { o->trackresult = p->objs[0]->trackresult; continue; }
+    t.trace_inst = (int)i;
+    t.track_object(o->pose, o->virtual_cam, o->m_vertex.data(), o->m_normal.data(), p->camera_pose, p->T_w_r);
+    memcpy(o->trackresult.data(), t.J[0].data(), sizeof(TrackRow) * 2 * n);
+  }
+  t.set_ref_from_curr();
+  return 1;
+}
+
+// Kfusion::integration (kfusion.cpp:573-677), objects with a mask set.
+int orc_integrate(void* h, const float k[4], unsigned frame) {
+  Pipe* p = (Pipe*)h;
+  if (!((frame % p->c.integ_rate) == 0 || frame <= 3)) return 0;
+  for (Object* o : p->objs) {
+    if (!o->has_mask) continue;
+    std::vector<R> m(o->mask);
+    if (frame > 1) refine_mask(m, o->trackresult, p->c.w, p->c.h, p->c.use_icp, p->c.use_rgb);
+    integrate_object(*p, *o, m, k);
+  }
+  return 1;
+}
+
+// Kfusion::raycasting (kfusion.cpp:539-569), every object.
+int orc_raycast(void* h, const float k[4], unsigned frame) {
+  (void)frame;
+  raycast_all(*(Pipe*)h, k, true);
+  return 1;
+}
+
+void orc_get_pose(void* h, int id, double T[16]) {
+  Pipe* p = (Pipe*)h;
+  m4_to(id < 0 ? p->camera_pose : p->objs[id]->pose, T);
+}
+void orc_set_pose(void* h, int id, const float T[16]) {
+  Pipe* p = (Pipe*)h;
+  if (id < 0) p->camera_pose = m4_from(T); else p->objs[id]->pose = m4_from(T);
+}
+// Raycast maps (AoS float3 as float[3*n]) and labels.
+void orc_get_maps(void* h, float* vtx, float* nrm, int* lab) {
+  Pipe* p = (Pipe*)h;
+  const size_t n = (size_t)p->c.w * p->c.h;
+  for (size_t i = 0; i < n; i++) {
+    if (vtx) { vtx[3 * i] = (float)p->vertex[i].x; vtx[3 * i + 1] = (float)p->vertex[i].y; vtx[3 * i + 2] = (float)p->vertex[i].z; }
+    if (nrm) { nrm[3 * i] = (float)p->normal[i].x; nrm[3 * i + 1] = (float)p->normal[i].y; nrm[3 * i + 2] = (float)p->normal[i].z; }
+    if (lab) lab[i] = p->label[i];
+  }
+}
+// Overwrite the global model maps (AoS float3) and labels, e.g. to track
+// against an exact reference surface.
+void orc_set_maps(void* h, const float* vtx, const float* nrm, const int* lab) {
+  Pipe* p = (Pipe*)h;
+  const size_t n = (size_t)p->c.w * p->c.h;
+  for (size_t i = 0; i < n; i++) {
+    if (vtx) p->vertex[i] = v3(vtx[3 * i], vtx[3 * i + 1], vtx[3 * i + 2]);
+    if (nrm) p->normal[i] = v3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+    if (lab) p->label[i] = lab[i];
+  }
+}
+void orc_set_object_maps(void* h, int id, const float* vtx, const float* nrm) {
+  Pipe* p = (Pipe*)h; Object* o = p->objs[id];
+  const size_t n = (size_t)p->c.w * p->c.h;
+  for (size_t i = 0; i < n; i++) {
+    o->m_vertex[i] = v3(vtx[3 * i], vtx[3 * i + 1], vtx[3 * i + 2]);
+    o->m_normal[i] = v3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+  }
+}
+void orc_get_object_maps(void* h, int id, float* vtx, float* nrm) {
+  Pipe* p = (Pipe*)h; Object* o = p->objs[id];
+  const size_t n = (size_t)p->c.w * p->c.h;
+  for (size_t i = 0; i < n; i++) {
+    vtx[3 * i] = (float)o->m_vertex[i].x; vtx[3 * i + 1] = (float)o->m_vertex[i].y; vtx[3 * i + 2] = (float)o->m_vertex[i].z;
+    nrm[3 * i] = (float)o->m_normal[i].x; nrm[3 * i + 1] = (float)o->m_normal[i].y; nrm[3 * i + 2] = (float)o->m_normal[i].z;
+  }
+}
+// Allocated leaf block keys (node codes, level in the low bits), sorted.
+long orc_object_keys(void* h, int id, uint64_t* out, long cap) {
+  Object* o = ((Pipe*)h)->objs[id];
+  std::vector<uint64_t> k;
+  for (Node* b : o->vol.blocks) k.push_back(b->code);
+  std::sort(k.begin(), k.end());
+  for (long i = 0; i < (long)k.size() && i < cap; i++) out[i] = k[i];
+  return (long)k.size();
+}
+long orc_object_node_count(void* h, int id) { return (long)((Pipe*)h)->objs[id]->vol.nodes.size(); }
+// Voxel payload of the block holding voxel (x,y,z): 512 x {tsdf, w, r, g, b,
+// c, fg, fgw} as double; returns 0 if the block is missing.
+int orc_object_block(void* h, int id, int x, int y, int z, double* out) {
+  Object* o = ((Pipe*)h)->objs[id];
+  Node* b = o->vol.fetch(x, y, z);
+  if (!b) return 0;
+  for (int i = 0; i < 512; i++) {
+    const Voxel& v = b->vox[i];
+    double* d = out + 8 * i;
+    d[0] = v.x; d[1] = v.y; d[2] = v.r; d[3] = v.g; d[4] = v.b; d[5] = v.c; d[6] = v.fg; d[7] = v.w;
+  }
+  return 1;
+}
+int orc_object_block_active(void* h, int id, int x, int y, int z) {
+  Object* o = ((Pipe*)h)->objs[id];
+  Node* b = o->vol.fetch(x, y, z);
+  return b ? (b->active ? 1 : 0) : -1;
+}
+long orc_last_emitted(void* h, int id) { return (long)((Pipe*)h)->objs[id]->last_emitted; }
+long orc_last_keys(void* h, int id, uint64_t* out, long cap) {
+  Object* o = ((Pipe*)h)->objs[id];
+  for (long i = 0; i < (long)o->last_keys.size() && i < cap; i++) out[i] = o->last_keys[i];
+  return (long)o->last_keys.size();
+}
+// Level-0 tracking rows of an object's last pass: result codes (2*n).
+void orc_object_trackresult(void* h, int id, int* res, float* err) {
+  Pipe* p = (Pipe*)h; Object* o = p->objs[id];
+  const size_t n = (size_t)2 * p->c.w * p->c.h;
+  for (size_t i = 0; i < n; i++) { res[i] = o->trackresult[i].result; if (err) err[i] = (float)o->trackresult[i].error; }
+}
+long orc_trace(void* h, double* out, long cap) {
+  Pipe* p = (Pipe*)h;
+  long n = (long)p->trace.size();
+  for (long i = 0; i < n && i < cap; i++) {
+    const TraceRec& r = p->trace[i];
+    double* d = out + 22 * i;
+    d[0] = r.inst; d[1] = r.level; d[2] = r.iter; d[3] = r.err; d[4] = r.s28; d[5] = r.flag;
+    for (int j = 0; j < 16; j++) d[6 + j] = r.pose[j];
+  }
+  return n;
+}
+void orc_trace_clear(void* h) { ((Pipe*)h)->trace.clear(); }
+// Tracker pyramid level buffers for stage checks: which = 0 depth, 1 grey,
+// 2 vertex (3/px), 3 normal (3/px), 4 gradx, 5 grady, 6 icp cov (3/px),
+// 7 rendered reference depth, 8 rendered reference vertex (3/px)
+long orc_tracker_level(void* h, int level, int which, float* out) {
+  Tracker& t = ((Pipe*)h)->trk;
+  const size_t n = (size_t)t.lw[level] * t.lh[level];
+  auto c1 = [&](const std::vector<R>& v) { for (size_t i = 0; i < n; i++) out[i] = (float)v[i]; };
+  auto c3 = [&](const std::vector<V3>& v) { for (size_t i = 0; i < n; i++) { out[3 * i] = (float)v[i].x; out[3 * i + 1] = (float)v[i].y; out[3 * i + 2] = (float)v[i].z; } };
+  switch (which) {
+    case 0: c1(t.l_D[level]); break; case 1: c1(t.l_I[level]); break; case 2: c3(t.l_V[level]); break;
+    case 3: c3(t.l_N[level]); break; case 4: c1(t.gx[level]); break; case 5: c1(t.gy[level]); break;
+    case 6: c3(t.cov[level]); break; case 7: c1(t.r_D_render[level]); break; case 8: c3(t.r_V_render[level]); break;
+    default: return -1;
+  }
+  return (long)n;
+}
+void orc_get_frame(void* h, float* depth, float* grey, float* rgb) {
+  Pipe* p = (Pipe*)h;
+  const size_t n = (size_t)p->c.w * p->c.h;
+  for (size_t i = 0; i < n; i++) {
+    if (depth) depth[i] = (float)p->depth[i];
+    if (grey) grey[i] = (float)p->grey[i];
+    if (rgb) { rgb[3 * i] = (float)p->rgb[i].x; rgb[3 * i + 1] = (float)p->rgb[i].y; rgb[3 * i + 2] = (float)p->rgb[i].z; }
+  }
+}
+
+// ----------------------------------------------------- stage-level checks
+int orc_bilateral(float* out, const float* in, int w, int h) {
+  std::vector<R> a(in, in + (size_t)w * h), b((size_t)w * h);
+  bilateral(b.data(), a.data(), w, h, R(0.1), 2);
+  for (size_t i = 0; i < b.size(); i++) out[i] = (float)b[i];
+  return 0;
+}
+int orc_half_sample(float* out, const float* in, int w, int h) {
+  std::vector<R> a(in, in + (size_t)w * h), b((size_t)(w / 2) * (h / 2));
+  half_sample(b.data(), a.data(), w, h, R(0.1f * 3), 1);
+  for (size_t i = 0; i < b.size(); i++) out[i] = (float)b[i];
+  return 0;
+}
+float orc_bilinear(const float* img, int w, float px, float py) {
+  std::vector<R> a(img, img + (size_t)w * w);
+  return (float)bilin(a.data(), w, px, py);
+}
+uint64_t orc_encode(int x, int y, int z, int level, int max_depth) { return encode(x, y, z, level, max_depth); }
+uint64_t orc_morton(uint32_t x, uint32_t y, uint32_t z) { return morton(x, y, z); }
+void orc_unpack(uint64_t c, uint32_t* xyz) { int x, y, z; unpack(c, x, y, z); xyz[0] = x; xyz[1] = y; xyz[2] = z; }
+int orc_unique_ms(uint64_t* keys, int n, int level) {
+  return level < 0 ? unique_ms(keys, n) : unique_ms_level(keys, n, (unsigned)level);
+}
+// 6x6 pseudo-inverse solve + SE3 exponential of the reduction sums.
+int orc_solve(const double sums[32], double x[6], float upd[16]) {
+  M4 u;
+  bool conv = solve_pose(u, sums, x);
+  for (int i = 0; i < 16; i++) upd[i] = (float)u.m[i / 4][i % 4];
+  return conv ? 1 : 0;
+}
+void orc_se3_exp(const double x[6], double T[16]) {
+  double Rm[3][3], t[3];
+  se3_exp(x, Rm, t);
+  for (int i = 0; i < 16; i++) T[i] = 0;
+  for (int i = 0; i < 3; i++) { for (int j = 0; j < 3; j++) T[4 * i + j] = Rm[i][j]; T[4 * i + 3] = t[i]; }
+  T[15] = 1;
+}
+void orc_inverse(const float A[16], float B[16]) {
+  M4 r = inverse(m4_from(A));
+  for (int i = 0; i < 16; i++) B[i] = (float)r.m[i / 4][i % 4];
+}
+// Reduction of caller-provided rows (result, error, J[6] as float) with the
+// reference's 8-way interleave.
+void orc_reduce_rows(const int* result, const float* err, const float* J, int jw, int jh, int w, int h, int stack,
+                     int robust, double k, double out[32]) {
+  std::vector<TrackRow> rows((size_t)stack * jw * jh);
+  for (size_t i = 0; i < rows.size(); i++) {
+    rows[i].result = result[i]; rows[i].error = err[i];
+    for (int j = 0; j < 6; j++) rows[i].J[j] = J[6 * i + j];
+  }
+  R o[8 * 32];
+  reduce(o, rows.data(), jw, jh, w, h, stack, robust, k);
+  for (int i = 0; i < 32; i++) out[i] = o[i];
+}
+float orc_robust_weight(float r, int fn, double k) { return (float)robust_weight(r, fn, k); }
+float orc_robust_loss(float r, int fn, double k) { return (float)robust_loss(r, fn, k); }
+
+// Standalone octree for interp/grad/ray checks: allocate blocks from keys,
+// write voxel tsdf/fg, query.
+void* orc_octree_create(int res, float dim) { Octree* o = new Octree(); o->init(res, dim); return o; }
+void orc_octree_destroy(void* h) { Octree* o = (Octree*)h; o->release(); delete o; }
+void orc_octree_alloc(void* h, uint64_t* keys, int n) { ((Octree*)h)->alloc_update(keys, n); }
+long orc_octree_keys(void* h, uint64_t* out, long cap) {
+  Octree* o = (Octree*)h;
+  std::vector<uint64_t> k;
+  for (Node* b : o->blocks) k.push_back(b->code);
+  std::sort(k.begin(), k.end());
+  for (long i = 0; i < (long)k.size() && i < cap; i++) out[i] = k[i];
+  return (long)k.size();
+}
+long orc_octree_node_keys(void* h, uint64_t* out, long cap) {
+  Octree* o = (Octree*)h;
+  std::vector<uint64_t> k;
+  for (Node* b : o->nodes) k.push_back(b->code);
+  std::sort(k.begin(), k.end());
+  for (long i = 0; i < (long)k.size() && i < cap; i++) out[i] = k[i];
+  return (long)k.size();
+}
+int orc_octree_set(void* h, int x, int y, int z, float tsdf, float fg) {
+  Octree* o = (Octree*)h;
+  Node* b = o->fetch(x, y, z);
+  if (!b) return 0;
+  Voxel& v = b->vox[(x - b->cx) + 8 * (y - b->cy) + 64 * (z - b->cz)];
+  v.x = tsdf; v.fg = fg; v.y = 1;
+  return 1;
+}
+float orc_octree_interp(void* h, float x, float y, float z, int field) { return (float)interp(*(Octree*)h, v3(x, y, z), field); }
+void orc_octree_grad(void* h, float x, float y, float z, float g[3]) {
+  V3 r = grad(*(Octree*)h, v3(x, y, z));
+  g[0] = (float)r.x; g[1] = (float)r.y; g[2] = (float)r.z;
+}
+float orc_octree_ray_tmin(void* h, const float o[3], const float d[3], float nearp, float farp, float eps) {
+  RayIter it(*(Octree*)h, o, d, nearp, farp, eps);
+  return it.first();
+}
+// raycast() of one ray in metric units: returns t (0: no hit), hit[3].
+float orc_octree_raycast(void* h, const float o[3], const float d[3], float nearp, float farp, float mu, float step,
+                         float largestep, float hit[3]) {
+  V3 hv; R t;
+  raycast_ray(*(Octree*)h, v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), nearp, farp, mu, step, largestep, hv, t);
+  hit[0] = (float)hv.x; hit[1] = (float)hv.y; hit[2] = (float)hv.z;
+  return (float)t;
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, always as the checker.  ``Oracle(fp64=True)`` loads the double build.
+"""
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+ROBUST = {'none': 0, 'huber': 1, 'tukey': 2, 'cauchy': 3}
+ROBUST_K = {'none': 1.0, 'huber': 1.345, 'tukey': 4.6851, 'cauchy': 2.3849}
+
+
+class OrcConfig(C.Structure):
+    _fields_ = [('w', C.c_int), ('h', C.c_int), ('nlev', C.c_int), ('iters', C.c_int * 4),
+                ('use_icp', C.c_int), ('use_rgb', C.c_int), ('robust', C.c_int), ('robust_k', C.c_double),
+                ('mu', C.c_float), ('track_rate', C.c_int), ('integ_rate', C.c_int), ('init_frame', C.c_int),
+                ('alloc_quirks', C.c_int)]
+
+
+def build():
+    import subprocess
+    subprocess.check_call(['make', '-s', '-C', _HERE])
+
+
+def _lib(fp64):
+    path = os.path.join(_HERE, 'liborc64.so' if fp64 else 'liborc32.so')
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    f32p = np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS')
+    f64p = np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS')
+    u64p = np.ctypeslib.ndpointer(np.uint64, flags='C_CONTIGUOUS')
+    i32p = np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS')
+    u16p = np.ctypeslib.ndpointer(np.uint16, flags='C_CONTIGUOUS')
+    u8p = np.ctypeslib.ndpointer(np.uint8, flags='C_CONTIGUOUS')
+    sig = {
+        'orc_create': (P, [C.POINTER(OrcConfig), f32p]),
+        'orc_destroy': (None, [P]),
+        'orc_object_create': (C.c_int, [P, C.c_float, C.c_int, f32p]),
+        'orc_object_set_moving': (C.c_int, [P, C.c_int, C.c_int]),
+        'orc_object_set_mask': (C.c_int, [P, C.c_int, f32p]),
+        'orc_preprocess': (C.c_int, [P, u16p, u8p, C.c_int, C.c_int, C.c_int]),
+        'orc_track': (C.c_int, [P, f32p, C.c_uint]),
+        'orc_integrate': (C.c_int, [P, f32p, C.c_uint]),
+        'orc_raycast': (C.c_int, [P, f32p, C.c_uint]),
+        'orc_get_pose': (None, [P, C.c_int, f64p]),
+        'orc_set_pose': (None, [P, C.c_int, f32p]),
+        'orc_get_maps': (None, [P, f32p, f32p, i32p]),
+        'orc_get_object_maps': (None, [P, C.c_int, f32p, f32p]),
+        'orc_set_maps': (None, [P, f32p, f32p, i32p]),
+        'orc_set_object_maps': (None, [P, C.c_int, f32p, f32p]),
+        'orc_object_keys': (C.c_long, [P, C.c_int, u64p, C.c_long]),
+        'orc_object_node_count': (C.c_long, [P, C.c_int]),
+        'orc_object_block': (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, f64p]),
+        'orc_object_block_active': (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
+        'orc_last_emitted': (C.c_long, [P, C.c_int]),
+        'orc_last_keys': (C.c_long, [P, C.c_int, u64p, C.c_long]),
+        'orc_object_trackresult': (None, [P, C.c_int, i32p, f32p]),
+        'orc_trace': (C.c_long, [P, f64p, C.c_long]),
+        'orc_trace_clear': (None, [P]),
+        'orc_tracker_level': (C.c_long, [P, C.c_int, C.c_int, f32p]),
+        'orc_get_frame': (None, [P, f32p, f32p, f32p]),
+        'orc_bilateral': (C.c_int, [f32p, f32p, C.c_int, C.c_int]),
+        'orc_half_sample': (C.c_int, [f32p, f32p, C.c_int, C.c_int]),
+        'orc_bilinear': (C.c_float, [f32p, C.c_int, C.c_float, C.c_float]),
+        'orc_encode': (C.c_uint64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        'orc_morton': (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32]),
+        'orc_unpack': (None, [C.c_uint64, np.ctypeslib.ndpointer(np.uint32, flags='C_CONTIGUOUS')]),
+        'orc_unique_ms': (C.c_int, [u64p, C.c_int, C.c_int]),
+        'orc_solve': (C.c_int, [f64p, f64p, f32p]),
+        'orc_se3_exp': (None, [f64p, f64p]),
+        'orc_inverse': (None, [f32p, f32p]),
+        'orc_reduce_rows': (None, [i32p, f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_double, f64p]),
+        'orc_robust_weight': (C.c_float, [C.c_float, C.c_int, C.c_double]),
+        'orc_robust_loss': (C.c_float, [C.c_float, C.c_int, C.c_double]),
+        'orc_octree_create': (P, [C.c_int, C.c_float]),
+        'orc_octree_destroy': (None, [P]),
+        'orc_octree_alloc': (None, [P, u64p, C.c_int]),
+        'orc_octree_keys': (C.c_long, [P, u64p, C.c_long]),
+        'orc_octree_node_keys': (C.c_long, [P, u64p, C.c_long]),
+        'orc_octree_set': (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]),
+        'orc_octree_interp': (C.c_float, [P, C.c_float, C.c_float, C.c_float, C.c_int]),
+        'orc_octree_grad': (None, [P, C.c_float, C.c_float, C.c_float, f32p]),
+        'orc_octree_ray_tmin': (C.c_float, [P, f32p, f32p, C.c_float, C.c_float, C.c_float]),
+        'orc_octree_raycast': (C.c_float, [P, f32p, f32p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                           f32p]),
+        'orc_num_threads': (C.c_int, []),
+        'orc_real_bytes': (C.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIBS = {}
+
+
+def lib(fp64=False):
+    if fp64 not in _LIBS:
+        _LIBS[fp64] = _lib(fp64)
+    return _LIBS[fp64]
+
+
+def f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+class Oracle:
+    """Kfusion-style per-frame pipeline on the CPU oracle."""
+
+    def __init__(self, w, h, init_pose, iters=(10, 5, 4), use_icp=True, use_rgb=True, robust='cauchy',
+                 robust_k=None, mu=0.1, track_rate=1, integ_rate=1, init_frame=0, alloc_quirks=False, fp64=False):
+        self.L = lib(fp64)
+        self.w, self.h = w, h
+        cfg = OrcConfig()
+        cfg.w, cfg.h, cfg.nlev = w, h, len(iters)
+        for i, v in enumerate(iters):
+            cfg.iters[i] = v
+        cfg.use_icp, cfg.use_rgb = int(use_icp), int(use_rgb)
+        cfg.robust = ROBUST[robust]
+        cfg.robust_k = ROBUST_K[robust] if robust_k is None else robust_k
+        cfg.mu, cfg.track_rate, cfg.integ_rate, cfg.init_frame = mu, track_rate, integ_rate, init_frame
+        cfg.alloc_quirks = int(alloc_quirks)
+        self.h_ = self.L.orc_create(C.byref(cfg), f32(np.asarray(init_pose).reshape(16)))
+        self.n_obj = 0
+
+    def __del__(self):
+        if getattr(self, 'h_', None):
+            self.L.orc_destroy(self.h_)
+            self.h_ = None
+
+    def add_object(self, size, res, pose):
+        self.n_obj += 1
+        return self.L.orc_object_create(self.h_, size, res, f32(np.asarray(pose).reshape(16)))
+
+    def set_moving(self, oid, moving):
+        self.L.orc_object_set_moving(self.h_, oid, int(moving))
+
+    def set_mask(self, oid, mask):
+        self.L.orc_object_set_mask(self.h_, oid, f32(mask).reshape(-1))
+
+    def preprocess(self, depth, rgb, filt=False):
+        d = np.ascontiguousarray(depth, np.uint16)
+        r = np.ascontiguousarray(rgb, np.uint8)
+        return self.L.orc_preprocess(self.h_, d.reshape(-1), r.reshape(-1), d.shape[1], d.shape[0], int(filt))
+
+    def track(self, k, frame):
+        return self.L.orc_track(self.h_, f32(k), frame)
+
+    def integrate(self, k, frame):
+        return self.L.orc_integrate(self.h_, f32(k), frame)
+
+    def raycast(self, k, frame):
+        return self.L.orc_raycast(self.h_, f32(k), frame)
+
+    def pose(self, oid=-1):
+        T = np.zeros(16)
+        self.L.orc_get_pose(self.h_, oid, T)
+        return T.reshape(4, 4)
+
+    def maps(self):
+        n = self.w * self.h
+        v, nn, lab = np.zeros(3 * n, np.float32), np.zeros(3 * n, np.float32), np.zeros(n, np.int32)
+        self.L.orc_get_maps(self.h_, v, nn, lab)
+        return v.reshape(self.h, self.w, 3), nn.reshape(self.h, self.w, 3), lab.reshape(self.h, self.w)
+
+    def set_maps(self, vtx, nrm, lab):
+        self.L.orc_set_maps(self.h_, f32(vtx).reshape(-1), f32(nrm).reshape(-1),
+                            np.ascontiguousarray(lab, np.int32).reshape(-1))
+
+    def set_object_maps(self, oid, vtx, nrm):
+        self.L.orc_set_object_maps(self.h_, oid, f32(vtx).reshape(-1), f32(nrm).reshape(-1))
+
+    def object_maps(self, oid):
+        n = self.w * self.h
+        v, nn = np.zeros(3 * n, np.float32), np.zeros(3 * n, np.float32)
+        self.L.orc_get_object_maps(self.h_, oid, v, nn)
+        return v.reshape(self.h, self.w, 3), nn.reshape(self.h, self.w, 3)
+
+    def keys(self, oid):
+        n = self.L.orc_object_keys(self.h_, oid, np.zeros(1, np.uint64), 0)
+        out = np.zeros(max(n, 1), np.uint64)
+        self.L.orc_object_keys(self.h_, oid, out, n)
+        return out[:n]
+
+    def last_keys(self, oid):
+        n = self.L.orc_last_keys(self.h_, oid, np.zeros(1, np.uint64), 0)
+        out = np.zeros(max(n, 1), np.uint64)
+        self.L.orc_last_keys(self.h_, oid, out, n)
+        return out[:n]
+
+    def block(self, oid, x, y, z):
+        out = np.zeros(512 * 8)
+        ok = self.L.orc_object_block(self.h_, oid, x, y, z, out)
+        return out.reshape(512, 8) if ok else None
+
+    def trace(self):
+        n = self.L.orc_trace(self.h_, np.zeros(22), 0)
+        out = np.zeros(22 * max(n, 1))
+        self.L.orc_trace(self.h_, out, n)
+        return out[:22 * n].reshape(n, 22)
+
+    def trace_clear(self):
+        self.L.orc_trace_clear(self.h_)
+
+    def trackresult(self, oid):
+        n = 2 * self.w * self.h
+        r, e = np.zeros(n, np.int32), np.zeros(n, np.float32)
+        self.L.orc_object_trackresult(self.h_, oid, r, e)
+        return r, e
+
+    def level(self, level, which):
+        comp = 3 if which in (2, 3, 6, 8) else 1
+        n = (self.w >> level) * (self.h >> level)
+        out = np.zeros(n * comp, np.float32)
+        self.L.orc_tracker_level(self.h_, level, which, out)
+        return out.reshape((self.h >> level), (self.w >> level), comp) if comp == 3 else \
+            out.reshape((self.h >> level), (self.w >> level))
+
+    def frame_buffers(self):
+        n = self.w * self.h
+        d, g, r = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(3 * n, np.float32)
+        self.L.orc_get_frame(self.h_, d, g, r)
+        return d.reshape(self.h, self.w), g.reshape(self.h, self.w), r.reshape(self.h, self.w, 3)
