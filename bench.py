@@ -1,0 +1,204 @@
+"""Frames/s of MID-Fusion's per-frame dense path on one MI355X.
+
+A step is one frame: preprocessing (inputs already in HBM), camera + moving
+object tracking (ICP + photometric, 3 levels {10,5,4}), allocation +
+integration of every instance, and raycasting.  Workload: 640x480
+(fx=fy=525, cx=319.5, cy=239.5), background volume 512^3 at 1 cm, one
+moving textured box 0.2x0.15x0.1 m with its own 128^3 volume at 2 mm,
+seeded synthetic frames.  Prints one JSON line.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, 'mid-fusion_amd')
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+HBM_PEAK_GBS = 8000.0
+W, H = 640, 480
+K = np.array([525.0, 525.0, 319.5, 239.5], np.float32)
+
+
+def make_scene(scene, n_objects):
+    movers = []
+    rng = np.random.default_rng(2)
+    for i in range(n_objects):
+        T0 = np.eye(4)
+        T0[:3, 3] = [2.30 + 0.25 * (i % 4) - 0.128, 2.45 + 0.2 * (i // 4) - 0.128, 2.45]
+        tw = np.r_[rng.uniform(-0.004, 0.004, 3), rng.uniform(-0.015, 0.015, 3)]
+        movers.append(scene.MovingBox([0.1, 0.075, 0.05], T0, tw, rng.uniform(0.3, 0.9, 3), 0.256, 128))
+    return scene.Scene(seed=2, width=W, height=H, K=tuple(K), moving=movers), movers
+
+
+def render_frames(sc, n_objects, n):
+    scene = sys.modules['mf_scene']
+    frames = []
+    for f in range(n):
+        d, rgb, lab = sc.render(f)
+        frames.append((d, rgb, scene.masks_for_labels(lab, n_objects)))
+    return frames
+
+
+def track_bytes(trace, use_rgb=True):
+    """Algorithmic bytes of the executed tracking iterations: per pixel of the
+    level, ICP reads live v, n, cov (36 B) + model v, n (24 B) + mask (1 B);
+    RGB reads rendered-reference depth, live/reference vertices, two
+    intensities and two gradients (41 B); plus the 32-float reduction."""
+    per_px = 61 + (41 if use_rgb else 0)
+    tot = 0
+    for r in trace:
+        lvl = int(r[1])
+        tot += (W >> lvl) * (H >> lvl) * per_px + 256
+    return tot
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=30)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--objects', type=int, default=1)
+    ap.add_argument('--plain', action='store_true', help='one object per launch instead of batched launches')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline leg')
+    ap.add_argument('--cpu-frames', type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build runs the dense path on a single MI355X '
+              '(multi-GPU sharding by instance is a design note only); use --gpus 1', file=sys.stderr)
+        sys.exit(2)
+
+    mf = _load('midfusion', os.path.join(PKG, 'midfusion.py'))
+    scene = _load('mf_scene', os.path.join(PKG, 'scene.py'))
+    hiprt = _load('mf_hiprt', os.path.join(PKG, 'hiprt.py'))
+
+    n_obj = args.objects
+    sc, movers = make_scene(scene, n_obj)
+    n_frames = args.warmup + args.steps
+    t0 = time.time()
+    frames = render_frames(sc, n_obj, n_frames + 3)
+    print('rendered %d frames in %.1fs' % (len(frames), time.time() - t0), file=sys.stderr, flush=True)
+
+    P0 = sc.camera_pose(0).astype(np.float32)
+    g = mf.Kfusion(W, H, P0, bg_size=5.12, bg_res=512, batch_objects=not args.plain)
+    for i, m in enumerate(movers):
+        g.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
+        g.set_moving(i + 1, True)
+    # inputs resident in HBM before timing
+    dev = []
+    for d, rgb, masks in frames:
+        dev.append((hiprt.DeviceBuffer(d), hiprt.DeviceBuffer(rgb), [hiprt.DeviceBuffer(m) for m in masks]))
+
+    def step(f):
+        dd, dr, dm = dev[f]
+        for i, m in enumerate(dm):
+            g.L.mf_object_set_mask_device(g.ctx, i, m.ptr)
+        g.preprocessing_device(dd.ptr, dr.ptr, W, H)
+        g.tracking(K, f)
+        g.integration(K, f)
+        g.raycasting(K, f)
+
+    for f in range(args.warmup):
+        step(f)
+    g.sync()
+    t = time.perf_counter()
+    for f in range(args.warmup, n_frames):
+        step(f)
+    g.sync()
+    el = time.perf_counter() - t
+    fps = args.steps / el
+    ms = 1000.0 * el / args.steps
+    pose_err = float(np.linalg.norm(g.getPose()[:3, 3] - sc.camera_pose(n_frames - 1)[:3, 3]))
+
+    # per-kernel timing + work counters on 3 more frames (events on the context stream)
+    g.enable_timing(True)
+    g.reset_stats()
+    g.trace_clear()
+    for f in range(n_frames, n_frames + 3):
+        step(f)
+    g.sync()
+    tm = g.timing()
+    st = g.stats()
+    tr = g.trace()
+    tot_ms = sum(v[0] for v in tm.values())
+    stage = {k: round(v[0] / 3, 4) for k, v in tm.items() if v[1]}
+    # algorithmic bytes per kernel class over the 3 frames
+    N0 = W * H
+    alg = {
+        'track_reduce': track_bytes(tr),
+        'integrate': 20 * N0 * (1 + n_obj) + 72 * st['voxel_updates'] + 16 * st['blocks_integrated'],
+        'raycast': 12 * st['march_steps'] + 32 * st['trilinear_evals'] + 128 * st['hits'] + 28 * N0 * 3,
+        'alloc_band': 8 * st['alloc_pixels'] + 8 * st['band_samples'],
+    }
+    dom = max(tm, key=lambda k: tm[k][0])
+    dom_ms, dom_n = tm[dom]
+    if dom in alg:
+        achieved = alg[dom] / (dom_ms / 1000.0) / 1e9
+        roof = {'kernel': dom, 'bound': 'hbm', 'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                'frac': round(achieved / HBM_PEAK_GBS, 5), 'traffic': None,
+                'avg_launch_us': round(1000.0 * dom_ms / dom_n, 3), 'launches': dom_n,
+                'alg_bytes_per_launch': int(alg[dom] / dom_n)}
+    else:
+        roof = {'kernel': dom, 'bound': 'hbm', 'achieved': None, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': None,
+                'traffic': None}
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import oracle
+        o = oracle.Oracle(W, H, P0)
+        o.add_object(5.12, 512, np.eye(4))
+        for i, m in enumerate(movers):
+            o.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
+            o.set_moving(i + 1, True)
+        nthreads = o.L.orc_num_threads()
+        nc = max(2, args.cpu_frames)
+        times = []
+        for f in range(nc):
+            d, rgb, masks = frames[f]
+            t1 = time.perf_counter()
+            for i, m in enumerate(masks):
+                o.set_mask(i, m)
+            o.preprocess(d, rgb)
+            o.track(K, f)
+            o.integrate(K, f)
+            o.raycast(K, f)
+            times.append(time.perf_counter() - t1)
+        cpu_fps = 1.0 / float(np.mean(times[1:]))   # frame 0 only seeds the tracker
+        cpu = {'value': round(cpu_fps, 4), 'unit': 'frames/s', 'cores': nthreads, 'kind': 'port',
+               'sample': 'oracle (OpenMP C++ restatement, -O2) on frames 1..%d of the same workload, frame 0 '
+                         'as warm-up' % (nc - 1)}
+
+    out = {
+        'metric': 'frames/s (track+integrate+raycast, 640x480, cam+K objs)',
+        'value': round(fps, 3), 'unit': 'frames/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(ms, 3), 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': 'cfg2: 640x480 RGB-D, bg 512^3 @1cm + %d moving box(es) 128^3 @2mm, ICP+RGB {10,5,4}, '
+                               '%s' % (n_obj, 'one object per launch' if args.plain else 'batched objects'),
+                   'frames': args.steps},
+        'roofline': roof,
+        'cpu_baseline': cpu,
+        'stage_ms_per_frame': stage,
+        'kernel_ms_per_frame_total': round(tot_ms / 3, 3),
+        'work_per_frame': {k: v // 3 for k, v in st.items()},
+        'camera_translation_error_vs_gt_m': round(pose_err, 5),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

@@ -124,6 +124,12 @@ int mf_get_frame(mf_ctx* ctx, float* depth, float* grey, float* rgb);
 int mf_enable_timing(mf_ctx* ctx, int on);
 int mf_get_timing(mf_ctx* ctx, double* out, int cap);
 const char* mf_kernel_name(int i);
+/* Device work counters since the last reset, for algorithmic-byte
+ * accounting: {voxel updates, blocks integrated, march steps, trilinear
+ * evaluations, raycast hits, allocation band samples, pixels allocated from,
+ * ray-object pairs}.  Returns the number of counters. */
+int mf_get_stats(mf_ctx* ctx, uint64_t* out, int cap);
+int mf_reset_stats(mf_ctx* ctx);
 /* Device bytes held by the context. */
 long mf_device_bytes(mf_ctx* ctx);
 

@@ -71,6 +71,7 @@ struct mf_ctx {
   int* trace_n = nullptr;
   int trace_cap = 4096;
   RayObj* rayobjs = nullptr;
+  unsigned long long* stats = nullptr;   // device work counters [N_STATS]
   TrackInst inst_host[MAX_INST];   // mirrors of the device descriptor tables
   RayObj ray_host[MAX_INST];
   int ray_n = -1;
@@ -285,7 +286,8 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->trace_n = dalloc<int>(c, 1);
   c->rayobjs = dalloc<RayObj>(c, MAX_INST);
   c->gauss = dalloc<float>(c, 5);
-  ok = ok && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
+  c->stats = dalloc<unsigned long long>(c, N_STATS);
+  ok = ok && c->stats && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
        c->inst && c->partial && c->moving && c->trace && c->trace_n && c->rayobjs && c->gauss;
   if (!ok) { mf_destroy(c); return nullptr; }
   // Gaussian of the bilateral filter (kfusion.cpp:80-88): expf(-(x*x)/(2*delta^2)), delta = 4
@@ -445,11 +447,11 @@ int mf_integrate(mf_ctx* c, const float k[4], unsigned frame) {
     const int8_t* codes = (o->id == 0 || o->moving) ? o->codes : c->objs[0]->codes;
     const int refine = frame > 1 ? 1 : 0;
     { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats + 3 + o->id, k[0], k[1], k[2], k[3], o->prm); }
-    { Timed t(c, KC_ALLOC); launch_alloc(s, o->v, o->prm, c->depth, o->mask, codes, refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, band); }
+    { Timed t(c, KC_ALLOC); launch_alloc(s, o->v, o->prm, c->depth, o->mask, codes, refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, band, c->stats); }
     { Timed t(c, KC_COMMIT); launch_alloc_commit(s, o->v); }
     { Timed t(c, KC_INTEGRATE);
       launch_integrate(s, o->v, o->prm, k[0], k[1], k[2], k[3], c->depth, c->rgb[0], c->rgb[1], c->rgb[2], o->mask, codes,
-                       refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, c->cfg.mu, 100); }
+                       refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, c->cfg.mu, 100, c->stats); }
   }
   return 1;
 }
@@ -477,7 +479,7 @@ int mf_raycast(mf_ctx* c, const float k[4], unsigned frame) {
   }
   Timed t(c, KC_RAYCAST);
   launch_raycast(s, c->rayobjs, n, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, c->vtx[0], c->vtx[1],
-                 c->vtx[2], c->nrm[0], c->nrm[1], c->nrm[2], c->label);
+                 c->vtx[2], c->nrm[0], c->nrm[1], c->nrm[2], c->label, c->stats);
   return 1;
 }
 
@@ -659,6 +661,17 @@ int mf_get_timing(mf_ctx* c, double* out, int cap) {
   drain_timing(c);
   for (int i = 0; i < KC_COUNT && i < cap; i++) { out[2 * i] = c->acc_ms[i]; out[2 * i + 1] = (double)c->acc_n[i]; }
   return KC_COUNT;
+}
+int mf_get_stats(mf_ctx* c, uint64_t* out, int cap) {
+  unsigned long long v[N_STATS];
+  hipMemcpyAsync(v, c->stats, sizeof v, hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  for (int i = 0; i < N_STATS && i < cap; i++) out[i] = v[i];
+  return N_STATS;
+}
+int mf_reset_stats(mf_ctx* c) {
+  hipMemsetAsync(c->stats, 0, N_STATS * sizeof(unsigned long long), c->stream);
+  return 0;
 }
 const char* mf_kernel_name(int i) { return (i >= 0 && i < KC_COUNT) ? kNames[i] : ""; }
 long mf_device_bytes(mf_ctx* c) { return c->bytes; }

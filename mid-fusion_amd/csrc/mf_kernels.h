@@ -44,13 +44,19 @@ void launch_invert_mat(hipStream_t s, const Mat4* src, Mat4* dst);
 // mapping (mf_map.hip)
 void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* T_w_o, float fx, float fy, float cx, float cy,
                        ObjParams* out);
+// Work counters accumulated on the device (algorithmic-byte accounting):
+// [0] voxel updates, [1] blocks integrated, [2] march steps, [3] trilinear
+// evaluations, [4] raycast hits, [5] band samples, [6] pixels allocated from,
+// [7] ray-object pairs traversed.
+static constexpr int N_STATS = 8;
 void launch_alloc(hipStream_t s, const VolDev& v, const ObjParams* prm, const float* depth, const float* mask,
-                  const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float band);
+                  const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float band,
+                  unsigned long long* stats);
 void launch_alloc_commit(hipStream_t s, const VolDev& v);
 void launch_integrate(hipStream_t s, const VolDev& v, const ObjParams* prm, float fx, float fy, float cx, float cy,
                       const float* depth, const float* r, const float* g, const float* b, const float* mask,
                       const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float mu,
-                      int maxweight);
+                      int maxweight, unsigned long long* stats);
 
 // raycasting (mf_raycast.hip)
 struct RayObj {
@@ -60,6 +66,7 @@ struct RayObj {
   float* mvx; float* mvy; float* mvz; float* mnx; float* mny; float* mnz;   // per-object output maps
 };
 void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,
-                    float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label);
+                    float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label,
+                    unsigned long long* stats);
 
 }  // namespace mf

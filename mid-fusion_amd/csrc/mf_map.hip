@@ -52,14 +52,20 @@ __global__ void k_alloc_reset(VolDev v) {
 // the block active.
 __global__ void k_alloc(VolDev v, const ObjParams* __restrict__ prm, const float* __restrict__ depth,
                         const float* __restrict__ mask, const int8_t* __restrict__ codes, int refine, int use_icp,
-                        int use_rgb, int w, int h, float band) {
+                        int use_rgb, int w, int h, float band, unsigned long long* __restrict__ stats) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= w || y >= h) return;
   const size_t i = (size_t)x + (size_t)y * w;
-  const float d = depth[i];
-  if (d == 0.0f || refined_mask(mask, codes, i, (size_t)w * h, refine, use_icp, use_rgb) < 0.f) return;
+  const bool valid = x < w && y < h && depth[i] != 0.0f &&
+                     refined_mask(mask, codes, i, (size_t)w * h, refine, use_icp, use_rgb) >= 0.f;
   const float inv_vox = 1.0f / v.voxel;
   const int num_steps = (int)ceilf(band * inv_vox);
+  const int nvalid = __syncthreads_count(valid);
+  if (threadIdx.x == 0 && nvalid) {
+    atomicAdd(&stats[6], (unsigned long long)nvalid);
+    atomicAdd(&stats[5], (unsigned long long)nvalid * num_steps);
+  }
+  if (!valid) return;
+  const float d = depth[i];
   const f3 wv = affine(prm->kPose, mk3(((float)x + 0.5f) * d, ((float)y + 0.5f) * d, d));
   const f3 ov = affine(prm->inv_pose, wv);
   const f3 dir = normalize(prm->o_cam - ov);
@@ -135,7 +141,8 @@ __global__ __launch_bounds__(512) void k_integrate(VolDev v, const ObjParams* __
                                                    const float* __restrict__ rr, const float* __restrict__ gg,
                                                    const float* __restrict__ bb, const float* __restrict__ mask,
                                                    const int8_t* __restrict__ codes, int refine, int use_icp,
-                                                   int use_rgb, int w, int h, float mu, int maxweight) {
+                                                   int use_rgb, int w, int h, float mu, int maxweight,
+                                                   unsigned long long* __restrict__ stats) {
   const int nact = v.counters[2];
   const int tid = threadIdx.x;
   const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
@@ -154,7 +161,7 @@ __global__ __launch_bounds__(512) void k_integrate(VolDev v, const ObjParams* __
     const f3 cstart = affine(K, start);
     const f3 cv = cstart + (float)lx * cdelta;
     const f3 pos = start + (float)lx * delta;
-    bool vis = false;
+    bool vis = false, upd = false;
     if (!(pos.z < 0.0001f)) {
       const float invd = 1.f / cv.z;
       const float pxf = cv.x * invd + 0.5f, pyf = cv.y * invd + 0.5f;
@@ -167,6 +174,7 @@ __global__ __launch_bounds__(512) void k_integrate(VolDev v, const ObjParams* __
           const float qx = pos.x / pos.z, qy = pos.y / pos.z;
           const float diff = (ds - pos.z) * sqrtf(1.f + qx * qx + qy * qy);
           if (diff > -mu) {
+            upd = true;
             const float sdf = fmin_r(1.f, diff / mu);
             const size_t vi = (size_t)s * BLOCK_VOX + tid;
             const float t0 = v.tsdf[vi], w0 = v.wt[vi];
@@ -186,7 +194,12 @@ __global__ __launch_bounds__(512) void k_integrate(VolDev v, const ObjParams* __
       }
     }
     const int any = __syncthreads_or(vis ? 1 : 0);
-    if (tid == 0) v.blk_active[s] = any ? 1 : 0;
+    const int nupd = __syncthreads_count(upd ? 1 : 0);
+    if (tid == 0) {
+      v.blk_active[s] = any ? 1 : 0;
+      atomicAdd(&stats[0], (unsigned long long)nupd);
+      atomicAdd(&stats[1], 1ULL);
+    }
   }
 }
 
@@ -196,10 +209,11 @@ void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* T_w_o, floa
   k_obj_params<<<1, 1, 0, s>>>(T_w_c, T_w_o, fx, fy, cx, cy, out);
 }
 void launch_alloc(hipStream_t s, const VolDev& v, const ObjParams* prm, const float* depth, const float* mask,
-                  const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float band) {
+                  const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float band,
+                  unsigned long long* stats) {
   k_alloc_reset<<<1, 1, 0, s>>>(v);
   dim3 blk(256), grd((w + 255) / 256, h);
-  k_alloc<<<grd, blk, 0, s>>>(v, prm, depth, mask, codes, refine, use_icp, use_rgb, w, h, band);
+  k_alloc<<<grd, blk, 0, s>>>(v, prm, depth, mask, codes, refine, use_icp, use_rgb, w, h, band, stats);
 }
 void launch_alloc_commit(hipStream_t s, const VolDev& v) {
   const long nb3 = (long)v.nb * v.nb * v.nb;
@@ -211,11 +225,11 @@ void launch_alloc_commit(hipStream_t s, const VolDev& v) {
 void launch_integrate(hipStream_t s, const VolDev& v, const ObjParams* prm, float fx, float fy, float cx, float cy,
                       const float* depth, const float* r, const float* g, const float* b, const float* mask,
                       const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float mu,
-                      int maxweight) {
+                      int maxweight, unsigned long long* stats) {
   const long cap = v.cap;
   k_active_list<<<(int)((cap + 255) / 256), 256, 0, s>>>(v, prm, w, h);
   k_integrate<<<4096, 512, 0, s>>>(v, prm, fx, fy, cx, cy, depth, r, g, b, mask, codes, refine, use_icp, use_rgb, w,
-                                   h, mu, maxweight);
+                                   h, mu, maxweight, stats);
 }
 
 }  // namespace mf

@@ -175,7 +175,7 @@ __device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, fl
 // raycast(volume, origin, dir, tnear, tfar, mu, step, largestep)
 // (kfusion/rendering_impl.hpp:80-140); returns t (0: no hit).
 __device__ float march(const VolDev& v, f3 origin, f3 dir, float nearp, float farp, float mu, float step,
-                       float largestep, f3& hit) {
+                       float largestep, f3& hit, unsigned& nsteps, unsigned& ninterp) {
   const float size = v.size;
   const float inv_vs = (float)v.res / v.size;
   const f3 invR = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
@@ -192,12 +192,14 @@ __device__ float march(const VolDev& v, f3 origin, f3 dir, float nearp, float fa
   float t = tnear, stepsize = largestep;
   f3 p = origin + dir * t;
   float f_t = interp<false>(v, p * inv_vs);
+  ninterp++;
   float f_tt = 0;
   if (f_t > 0) {
     for (; t < tfar; t += stepsize) {
       const f3 q = p * inv_vs;
+      nsteps++;
       f_tt = tsdf_at(v, (int)q.x, (int)q.y, (int)q.z);
-      if (f_tt < 0.1f && f_tt >= -0.5f) f_tt = interp<false>(v, q);   // (double)f <= 0.1
+      if (f_tt < 0.1f && f_tt >= -0.5f) { f_tt = interp<false>(v, q); ninterp++; }   // (double)f <= 0.1
       if (f_tt < 0) break;
       stepsize = fmax_r(f_tt * mu, step);
       p = p + stepsize * dir;
@@ -216,9 +218,14 @@ __global__ __launch_bounds__(256) void k_raycast(const RayObj* __restrict__ objs
                                                  float nearp, float farp, float eps, float* __restrict__ vx,
                                                  float* __restrict__ vy, float* __restrict__ vz,
                                                  float* __restrict__ nx, float* __restrict__ ny,
-                                                 float* __restrict__ nz, int32_t* __restrict__ label) {
+                                                 float* __restrict__ nz, int32_t* __restrict__ label,
+                                                 unsigned long long* __restrict__ stats) {
+  __shared__ unsigned cnt[4];
+  if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+  __syncthreads();
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= w || y >= h) return;
+  unsigned nsteps = 0, ninterp = 0, nhits = 0, npairs = 0;
+  if (x < w && y < h) {
   const size_t pix = (size_t)x + (size_t)y * w;
   float nearest = 1.0e3f;
   double high_fg = 0;
@@ -239,11 +246,13 @@ __global__ __launch_bounds__(256) void k_raycast(const RayObj* __restrict__ objs
     const float tmin = ray_first_block(v, oo, od, nearp, nfar, eps);
     f3 hit = mk3(0.f, 0.f, 0.f);
     float hw = 0.f;
-    if (tmin > 0.f) hw = march(v, oo, od, tmin * v.size / (float)v.res, nfar, mu, step, largestep, hit);
+    npairs++;
+    if (tmin > 0.f) hw = march(v, oo, od, tmin * v.size / (float)v.res, nfar, mu, step, largestep, hit, nsteps, ninterp);
     const float hd = (O.label == 0) ? hw + 0.05f : hw;
     if (hd > nearest) { miss++; continue; }
     const float inv_vs = (float)v.res / v.size;
     const double fg = (double)interp<true>(v, hit * inv_vs);
+    ninterp++;
     if (fg <= 0.5f) { miss++; continue; }
     bool take = false;
     if (hd == nearest) { if (fg > high_fg) take = true; }
@@ -254,6 +263,7 @@ __global__ __launch_bounds__(256) void k_raycast(const RayObj* __restrict__ objs
     out_v = affine(P.pose, hit);
     const f3 g = grad(v, hit * inv_vs);
     out_n = (length(g) == 0) ? mk3((float)INVALID, 0.f, 0.f) : normalize(rotate(P.pose, g));
+    nhits++;
   }
   if (miss == nobj) { out_v = mk3(0.f, 0.f, 0.f); out_n = mk3((float)INVALID, 0.f, 0.f); out_l = INVALID; }
   vx[pix] = out_v.x; vy[pix] = out_v.y; vz[pix] = out_v.z;
@@ -265,12 +275,20 @@ __global__ __launch_bounds__(256) void k_raycast(const RayObj* __restrict__ objs
     O.mvx[pix] = mine ? out_v.x : 0.f; O.mvy[pix] = mine ? out_v.y : 0.f; O.mvz[pix] = mine ? out_v.z : 0.f;
     O.mnx[pix] = mine ? out_n.x : (float)INVALID; O.mny[pix] = mine ? out_n.y : 0.f; O.mnz[pix] = mine ? out_n.z : 0.f;
   }
+  }
+  atomicAdd(&cnt[0], nsteps); atomicAdd(&cnt[1], ninterp); atomicAdd(&cnt[2], nhits); atomicAdd(&cnt[3], npairs);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&stats[2], (unsigned long long)cnt[0]); atomicAdd(&stats[3], (unsigned long long)cnt[1]);
+    atomicAdd(&stats[4], (unsigned long long)cnt[2]); atomicAdd(&stats[7], (unsigned long long)cnt[3]);
+  }
 }
 
 void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,
-                    float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label) {
+                    float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label,
+                    unsigned long long* stats) {
   dim3 blk(256), grd((w + 255) / 256, h);
-  k_raycast<<<grd, blk, 0, s>>>(objs, nobj, w, h, mu, nearp, farp, eps, vx, vy, vz, nx, ny, nz, label);
+  k_raycast<<<grd, blk, 0, s>>>(objs, nobj, w, h, mu, nearp, farp, eps, vx, vy, vz, nx, ny, nz, label, stats);
 }
 
 }  // namespace mf

@@ -68,6 +68,8 @@ SIGNATURES = {
     'mf_enable_timing': (C.c_int, [P, C.c_int]),
     'mf_get_timing': (C.c_int, [P, f64p, C.c_int]),
     'mf_kernel_name': (C.c_char_p, [C.c_int]),
+    'mf_get_stats': (C.c_int, [P, u64p, C.c_int]),
+    'mf_reset_stats': (C.c_int, [P]),
     'mf_device_bytes': (C.c_long, [P]),
 }
 
@@ -236,6 +238,17 @@ class Kfusion:
         out = np.zeros(64)
         n = self.L.mf_get_timing(self.ctx, out, 32)
         return {self.L.mf_kernel_name(i).decode(): (out[2 * i], int(out[2 * i + 1])) for i in range(n)}
+
+    STAT_NAMES = ('voxel_updates', 'blocks_integrated', 'march_steps', 'trilinear_evals', 'hits', 'band_samples',
+                  'alloc_pixels', 'ray_object_pairs')
+
+    def stats(self):
+        out = np.zeros(8, np.uint64)
+        self.L.mf_get_stats(self.ctx, out, 8)
+        return dict(zip(self.STAT_NAMES, (int(v) for v in out)))
+
+    def reset_stats(self):
+        self.L.mf_reset_stats(self.ctx)
 
     def device_bytes(self):
         return self.L.mf_device_bytes(self.ctx)
