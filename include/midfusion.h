@@ -75,6 +75,15 @@ int mf_preprocess_device(mf_ctx* ctx, const uint16_t* d_depth_mm, const uint8_t*
 /* Kfusion::tracking(k, rate, frame) (kfusion.cpp:237-488).  Returns 1 if
  * tracking ran, 0 if skipped (rate / seeding frame). */
 int mf_track(mf_ctx* ctx, const float k[4], unsigned frame);
+/* The two phases of mf_track: the camera phase builds the pyramid and runs
+ * Tracking::trackLiveFrame (plus the masked re-track when objects move,
+ * kfusion.cpp:237-392); the object phase runs Tracking::trackEachObject for
+ * every moving object at the current camera pose and hands the live pyramid
+ * over as the next reference (kfusion.cpp:395-471, tracking.cpp:580-708,
+ * 1401-1414).  mf_track_objects does nothing unless a camera phase that
+ * tracked is pending. */
+int mf_track_camera(mf_ctx* ctx, const float k[4], unsigned frame);
+int mf_track_objects(mf_ctx* ctx, const float k[4], unsigned frame);
 /* Kfusion::integration(k, rate, mu, frame) (kfusion.cpp:573-677). */
 int mf_integrate(mf_ctx* ctx, const float k[4], unsigned frame);
 /* Kfusion::raycasting(k, mu, frame) (kfusion.cpp:539-569). */
@@ -112,6 +121,12 @@ int mf_get_trackresult(mf_ctx* ctx, int id, int32_t* codes);
  * level, iteration, error, inliers, flag, pose[16]}; returns the row count. */
 long mf_get_trace(mf_ctx* ctx, double* out, long cap);
 void mf_clear_trace(mf_ctx* ctx);
+/* Row dump for parity checks: when on, every tracking launch also writes its
+ * TrackData rows (commons.h:624-628; ICP rows then RGB rows, row stride W at
+ * every level) for GN state 0 (camera) and 1 (first moving object).
+ * mf_get_rows returns the rows of the last launch at that level. */
+int mf_debug_rows(mf_ctx* ctx, int on);
+int mf_get_rows(mf_ctx* ctx, int state, int level, int32_t* result, float* error, float* J);
 /* Pyramid buffers of the live frame: which = 0 depth, 1 grey, 2 vertex (3/px),
  * 3 normal (3/px), 4 grad x, 5 grad y, 6 icp covariance (3/px). */
 long mf_get_level(mf_ctx* ctx, int level, int which, float* out);

@@ -78,6 +78,8 @@ struct mf_ctx {
   std::vector<Object*> objs;
   float ray_eps = -1.f;
   bool have_ref = false;
+  bool pending_objects = false;   // camera phase done, object phase due
+  TrackRow* dbg_rows = nullptr;   // row dump [2 states][levels][2*W*H] when enabled
   // timing
   bool timing = false;
   struct Ev { hipEvent_t a, b; int cls; };
@@ -193,6 +195,7 @@ TrackLaunch base_launch(mf_ctx* c, const float k[4], int level, int object) {
   p.inv_cov_rgb = (float)(1.0 / (double)sigma_bright);
   p.live = c->live[level]; p.ref = c->ref[level];
   p.label = nullptr; p.moving = c->moving; p.n_labels = MAX_INST;
+  p.dbg = c->dbg_rows; p.n_levels = c->cfg.n_levels;
   return p;
 }
 
@@ -380,7 +383,8 @@ int mf_preprocess(mf_ctx* c, const uint16_t* depth, const uint8_t* rgb, int iw, 
   return mf_preprocess_device(c, c->in_depth, rgb ? c->in_rgb : nullptr, iw, ih, filter);
 }
 
-int mf_track(mf_ctx* c, const float k[4], unsigned frame) {
+int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
+  c->pending_objects = false;
   if (frame % c->cfg.track_rate != 0) return 0;
   hipStream_t s = c->stream;
   const int L = c->cfg.n_levels;
@@ -417,6 +421,20 @@ int mf_track(mf_ctx* c, const float k[4], unsigned frame) {
     { Timed t(c, KC_GNSMALL); launch_gn_begin_camera(s, c->gn + 0, c->mats + 0, c->mats + 1, k[0], k[1], k[2], k[3]); }
     gn_run(c, k, 0, 0, 1, c->label);
     { Timed t(c, KC_GNSMALL); launch_gn_end_camera(s, c->gn + 0, c->mats + 0); }
+  }
+  c->pending_objects = true;
+  return 1;
+}
+
+int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
+  (void)frame;
+  if (!c->pending_objects) return 0;
+  c->pending_objects = false;
+  hipStream_t s = c->stream;
+  const int L = c->cfg.n_levels;
+  std::vector<Object*> mov;
+  for (size_t i = 1; i < c->objs.size(); i++) if (c->objs[i]->moving) mov.push_back(c->objs[i]);
+  if (!mov.empty()) {
     // trackEachObject for every moving object (kfusion.cpp:395-414)
     for (size_t j = 0; j < mov.size(); j++) {
       Object* o = mov[j];
@@ -434,6 +452,12 @@ int mf_track(mf_ctx* c, const float k[4], unsigned frame) {
   }
   for (int l = 0; l < L; l++) std::swap(c->live[l], c->ref[l]);   // setRefImgFromCurr
   return 1;
+}
+
+int mf_track(mf_ctx* c, const float k[4], unsigned frame) {
+  const int r = mf_track_camera(c, k, frame);
+  if (r <= 0) return r;
+  return mf_track_objects(c, k, frame);
 }
 
 int mf_integrate(mf_ctx* c, const float k[4], unsigned frame) {
@@ -648,6 +672,29 @@ int mf_get_frame(mf_ctx* c, float* depth, float* grey, float* rgb) {
   if (depth) hipMemcpy(depth, c->depth, c->N0 * 4, hipMemcpyDeviceToHost);
   if (grey) hipMemcpy(grey, c->grey, c->N0 * 4, hipMemcpyDeviceToHost);
   if (rgb) download_planes(c, c->rgb, rgb, c->N0);
+  return 0;
+}
+
+int mf_debug_rows(mf_ctx* c, int on) {
+  if (on && !c->dbg_rows) {
+    c->dbg_rows = dalloc<TrackRow>(c, (size_t)2 * c->cfg.n_levels * 2 * c->N0);
+    if (!c->dbg_rows) return fail(c, MF_E_OOM, "row dump");
+  }
+  if (!on) c->dbg_rows = nullptr;   // buffer stays owned by the context
+  return 0;
+}
+int mf_get_rows(mf_ctx* c, int state, int level, int32_t* result, float* error, float* J) {
+  if (!c->dbg_rows || state < 0 || state > 1 || level < 0 || level >= c->cfg.n_levels)
+    return fail(c, MF_E_ARG, "row dump not enabled / bad index");
+  const size_t n = 2 * c->N0;
+  std::vector<TrackRow> r(n);
+  hipStreamSynchronize(c->stream);
+  hipMemcpy(r.data(), c->dbg_rows + ((size_t)state * c->cfg.n_levels + level) * n, n * sizeof(TrackRow),
+            hipMemcpyDeviceToHost);
+  for (size_t i = 0; i < n; i++) {
+    result[i] = r[i].result; error[i] = r[i].error;
+    for (int q = 0; q < 6; q++) J[6 * i + q] = r[i].J[q];
+  }
   return 0;
 }
 

@@ -26,6 +26,8 @@ struct TrackLaunch {
   const int32_t* label;                // camera re-track: outlier mask from raycast labels
   const uint8_t* moving;               // per-object moving flags (label-indexed)
   int n_labels;
+  TrackRow* dbg;                       // optional row dump: [state][level][2*W*H], or null
+  int n_levels;
 };
 int track_blocks(int w, int h);
 void launch_track(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,

@@ -68,7 +68,9 @@ __global__ __launch_bounds__(256) void k_bilateral(const float* __restrict__ in0
             if (p > 0.0f) {
               const float d = p - c;
               const float mod = d * d;
-              const float f = g[i + 2] * g[j + 2] * expf(-mod / e2);
+              // expf as the correctly rounded float of a double exp, so the
+              // host reference and the device agree bit for bit
+              const float f = g[i + 2] * g[j + 2] * (float)exp((double)(-mod / e2));
               t = t + f * p;
               sum = sum + f;
             }

@@ -237,20 +237,25 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
   for (int k = 0; k < 32; k++) s[k] = 0.f;
   const int n = p.w * p.h;
   const size_t N0 = (size_t)p.W * p.H;
+  TrackRow* dbg = p.dbg ? p.dbg + ((size_t)in.state * p.n_levels + p.level) * 2 * N0 : nullptr;
   for (int i = blockIdx.x * TB + tid; i < n; i += nblk * TB) {
     const int x = i % p.w, y = i / p.w;
-    float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
+    const size_t ro = (size_t)x + (size_t)y * p.W;   // row stride W at every level (tracking.cpp:49-50)
     if (ICP) {
+      float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
       int r;
       if (OBJ) r = obj_icp_row(p, in, M, x, y, (size_t)i, e, J);
       else r = icp_row(p, in, M, x, y, (size_t)i, dyn_masked(p, x, y), e, J);
       accum(s, r, e, J, p.robust, p.robust_k);
-      if (in.codes && p.level == 0) in.codes[(size_t)x + (size_t)y * p.W] = (int8_t)r;
+      if (in.codes && p.level == 0) in.codes[ro] = (int8_t)r;
+      if (dbg) { TrackRow& d = dbg[ro]; d.result = r; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
     }
     if (RGB) {
+      float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
       const int r = rgb_row<OBJ>(p, in, M, x, y, (size_t)i, e, J);
       accum(s, r, e, J, p.robust, p.robust_k);
-      if (in.codes && p.level == 0) in.codes[N0 + (size_t)x + (size_t)y * p.W] = (int8_t)r;
+      if (in.codes && p.level == 0) in.codes[N0 + ro] = (int8_t)r;
+      if (dbg) { TrackRow& d = dbg[N0 + ro]; d.result = r; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
     }
   }
   // LDS transpose reduction: 32 sums x 256 threads -> 32 sums.

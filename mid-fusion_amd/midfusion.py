@@ -48,6 +48,8 @@ SIGNATURES = {
     'mf_preprocess': (C.c_int, [P, u16p, u8p, C.c_int, C.c_int, C.c_int]),
     'mf_preprocess_device': (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_int]),
     'mf_track': (C.c_int, [P, f32p, C.c_uint]),
+    'mf_track_camera': (C.c_int, [P, f32p, C.c_uint]),
+    'mf_track_objects': (C.c_int, [P, f32p, C.c_uint]),
     'mf_integrate': (C.c_int, [P, f32p, C.c_uint]),
     'mf_raycast': (C.c_int, [P, f32p, C.c_uint]),
     'mf_sync': (C.c_int, [P]),
@@ -64,6 +66,8 @@ SIGNATURES = {
     'mf_get_trace': (C.c_long, [P, f64p, C.c_long]),
     'mf_clear_trace': (None, [P]),
     'mf_get_level': (C.c_long, [P, C.c_int, C.c_int, f32p]),
+    'mf_debug_rows': (C.c_int, [P, C.c_int]),
+    'mf_get_rows': (C.c_int, [P, C.c_int, C.c_int, i32p, f32p, f32p]),
     'mf_get_frame': (C.c_int, [P, f32p, f32p, f32p]),
     'mf_enable_timing': (C.c_int, [P, C.c_int]),
     'mf_get_timing': (C.c_int, [P, f64p, C.c_int]),
@@ -152,6 +156,12 @@ class Kfusion:
     def tracking(self, k, frame):
         return self._chk(self.L.mf_track(self.ctx, _f32(k), frame))
 
+    def track_camera(self, k, frame):
+        return self._chk(self.L.mf_track_camera(self.ctx, _f32(k), frame))
+
+    def track_objects(self, k, frame):
+        return self._chk(self.L.mf_track_objects(self.ctx, _f32(k), frame))
+
     def integration(self, k, frame):
         return self._chk(self.L.mf_integrate(self.ctx, _f32(k), frame))
 
@@ -224,6 +234,15 @@ class Kfusion:
         self._chk(self.L.mf_get_level(self.ctx, level, which, out))
         shape = ((self.h >> level), (self.w >> level))
         return out.reshape(shape + (3,)) if comp == 3 else out.reshape(shape)
+
+    def debug_rows(self, on=True):
+        self._chk(self.L.mf_debug_rows(self.ctx, int(on)))
+
+    def rows(self, state, level):
+        n = 2 * self.w * self.h
+        r, e, J = np.zeros(n, np.int32), np.zeros(n, np.float32), np.zeros(6 * n, np.float32)
+        self._chk(self.L.mf_get_rows(self.ctx, state, level, r, e, J))
+        return r, e, J.reshape(n, 6)
 
     def frame_buffers(self):
         n = self.w * self.h

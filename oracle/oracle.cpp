@@ -434,9 +434,10 @@ int orc_preprocess(void* h, const uint16_t* depth, const uint8_t* rgb, int iw, i
   return 0;
 }
 
-// Kfusion::tracking (kfusion.cpp:237-488) with the moving-object set given
-// by the caller (orc_object_set_moving) instead of the motion-residual test.
-int orc_track(void* h, const float k[4], unsigned frame) {
+// Camera phase: pyramid, trackLiveFrame, and the masked re-track when
+// objects move (kfusion.cpp:237-392).  Returns 1 if the object phase must
+// follow, 0 if the frame only seeded the reference.
+int orc_track_camera(void* h, const float k[4], unsigned frame) {
   Pipe* p = (Pipe*)h;
   Tracker& t = p->trk;
   const int W = p->c.w, H = p->c.h;
@@ -461,6 +462,16 @@ int orc_track(void* h, const float k[4], unsigned frame) {
     p->objs[0]->virtual_cam = p->camera_pose;
     memcpy(p->objs[0]->trackresult.data(), t.J[0].data(), sizeof(TrackRow) * 2 * n);
   }
+  return 1;
+}
+
+// Object phase: trackEachObject for every moving object at the current
+// camera pose, then setRefImgFromCurr (kfusion.cpp:395-471).
+int orc_track_objects(void* h, const float k[4], unsigned frame) {
+  (void)k; (void)frame;
+  Pipe* p = (Pipe*)h;
+  Tracker& t = p->trk;
+  const size_t n = (size_t)p->c.w * p->c.h;
   for (size_t i = 1; i < p->objs.size(); i++) {
     Object* o = p->objs[i];
     if (o->is_static) { o->trackresult = p->objs[0]->trackresult; continue; }
@@ -470,6 +481,13 @@ int orc_track(void* h, const float k[4], unsigned frame) {
   }
   t.set_ref_from_curr();
   return 1;
+}
+
+// Kfusion::tracking (kfusion.cpp:237-488) with the moving-object set given
+// by the caller (orc_object_set_moving) instead of the motion-residual test.
+int orc_track(void* h, const float k[4], unsigned frame) {
+  if (!orc_track_camera(h, k, frame)) return 0;
+  return orc_track_objects(h, k, frame);
 }
 
 // Kfusion::integration (kfusion.cpp:573-677), objects with a mask set.
@@ -576,6 +594,15 @@ void orc_object_trackresult(void* h, int id, int* res, float* err) {
   Pipe* p = (Pipe*)h; Object* o = p->objs[id];
   const size_t n = (size_t)2 * p->c.w * p->c.h;
   for (size_t i = 0; i < n; i++) { res[i] = o->trackresult[i].result; if (err) err[i] = (float)o->trackresult[i].error; }
+}
+// TrackData rows of the tracker's last pass at a level (2*W*H rows).
+void orc_get_rows(void* h, int level, int* res, float* err, float* J) {
+  Pipe* p = (Pipe*)h;
+  const std::vector<TrackRow>& r = p->trk.J[level];
+  for (size_t i = 0; i < r.size(); i++) {
+    res[i] = r[i].result; err[i] = (float)r[i].error;
+    for (int q = 0; q < 6; q++) J[6 * i + q] = (float)r[i].J[q];
+  }
 }
 long orc_trace(void* h, double* out, long cap) {
   Pipe* p = (Pipe*)h;

@@ -45,6 +45,8 @@ def _lib(fp64):
         'orc_object_set_mask': (C.c_int, [P, C.c_int, f32p]),
         'orc_preprocess': (C.c_int, [P, u16p, u8p, C.c_int, C.c_int, C.c_int]),
         'orc_track': (C.c_int, [P, f32p, C.c_uint]),
+        'orc_track_camera': (C.c_int, [P, f32p, C.c_uint]),
+        'orc_track_objects': (C.c_int, [P, f32p, C.c_uint]),
         'orc_integrate': (C.c_int, [P, f32p, C.c_uint]),
         'orc_raycast': (C.c_int, [P, f32p, C.c_uint]),
         'orc_get_pose': (None, [P, C.c_int, f64p]),
@@ -61,6 +63,7 @@ def _lib(fp64):
         'orc_last_keys': (C.c_long, [P, C.c_int, u64p, C.c_long]),
         'orc_object_trackresult': (None, [P, C.c_int, i32p, f32p]),
         'orc_trace': (C.c_long, [P, f64p, C.c_long]),
+        'orc_get_rows': (None, [P, C.c_int, i32p, f32p, f32p]),
         'orc_trace_clear': (None, [P]),
         'orc_tracker_level': (C.c_long, [P, C.c_int, C.c_int, f32p]),
         'orc_get_frame': (None, [P, f32p, f32p, f32p]),
@@ -154,6 +157,12 @@ class Oracle:
     def track(self, k, frame):
         return self.L.orc_track(self.h_, f32(k), frame)
 
+    def track_camera(self, k, frame):
+        return self.L.orc_track_camera(self.h_, f32(k), frame)
+
+    def track_objects(self, k, frame):
+        return self.L.orc_track_objects(self.h_, f32(k), frame)
+
     def integrate(self, k, frame):
         return self.L.orc_integrate(self.h_, f32(k), frame)
 
@@ -200,6 +209,12 @@ class Oracle:
         out = np.zeros(512 * 8)
         ok = self.L.orc_object_block(self.h_, oid, x, y, z, out)
         return out.reshape(512, 8) if ok else None
+
+    def rows(self, level):
+        n = 2 * self.w * self.h
+        r, e, J = np.zeros(n, np.int32), np.zeros(n, np.float32), np.zeros(6 * n, np.float32)
+        self.L.orc_get_rows(self.h_, level, r, e, J)
+        return r, e, J.reshape(n, 6)
 
     def trace(self):
         n = self.L.orc_trace(self.h_, np.zeros(22), 0)

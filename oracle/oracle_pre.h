@@ -70,7 +70,8 @@ static inline void bilateral(R* out, const R* in, int w, int h, R e_d, int r) {
           if (p > 0) {
             const R d = p - c;
             const R mod = d * d;
-            const R f = g[i + r] * g[j + r] * std::exp(-mod / e2);
+            // expf evaluated as the correctly rounded float of a double exp
+            const R f = g[i + r] * g[j + r] * (R)std::exp((double)(-mod / e2));
             t = t + f * p;
             sum = sum + f;
           }
