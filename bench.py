@@ -75,6 +75,8 @@ def main():
     ap.add_argument('--plain', action='store_true', help='one object per launch instead of batched launches')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline leg')
     ap.add_argument('--cpu-frames', type=int, default=3)
+    ap.add_argument('--copy-ref', type=int, default=0, metavar='MB',
+                    help='only launch a plain copy kernel of MB megabytes (traffic-counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build runs the dense path on a single MI355X '
@@ -84,6 +86,11 @@ def main():
     mf = _load('midfusion', os.path.join(PKG, 'midfusion.py'))
     scene = _load('mf_scene', os.path.join(PKG, 'scene.py'))
     hiprt = _load('mf_hiprt', os.path.join(PKG, 'hiprt.py'))
+    if args.copy_ref:
+        g = mf.Kfusion(64, 48, np.eye(4, dtype=np.float32), bg_res=64)
+        g._chk(g.L.mf_copy_reference(g.ctx, args.copy_ref << 20, 5))
+        print(json.dumps({'copy_reference_bytes': args.copy_ref << 20, 'launches': 5}))
+        return
 
     n_obj = args.objects
     sc, movers = make_scene(scene, n_obj)

@@ -14,6 +14,7 @@
  */
 #ifndef MIDFUSION_H
 #define MIDFUSION_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -145,6 +146,10 @@ const char* mf_kernel_name(int i);
  * ray-object pairs}.  Returns the number of counters. */
 int mf_get_stats(mf_ctx* ctx, uint64_t* out, int cap);
 int mf_reset_stats(mf_ctx* ctx);
+/* Launches a plain float4 device copy of `bytes` bytes `reps` times on the
+ * context stream (no counterpart in the reference: the calibration point for
+ * hardware traffic counters). */
+int mf_copy_reference(mf_ctx* ctx, size_t bytes, int reps);
 /* Device bytes held by the context. */
 long mf_device_bytes(mf_ctx* ctx);
 

@@ -723,4 +723,22 @@ int mf_reset_stats(mf_ctx* c) {
 const char* mf_kernel_name(int i) { return (i >= 0 && i < KC_COUNT) ? kNames[i] : ""; }
 long mf_device_bytes(mf_ctx* c) { return c->bytes; }
 
+// Plain device-to-device copy kernel: the reference point for scaling HBM
+// traffic counters (FETCH_SIZE / WRITE_SIZE) to bytes.
+__global__ void k_copy_ref(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+int mf_copy_reference(mf_ctx* c, size_t bytes, int reps) {
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    if (a) hipFree(a);
+    return fail(c, MF_E_OOM, "copy buffers");
+  }
+  hipMemsetAsync(a, 1, bytes, c->stream);
+  for (int r = 0; r < reps; r++) k_copy_ref<<<4096, 256, 0, c->stream>>>((const float4*)a, (float4*)b, bytes / 16);
+  hipStreamSynchronize(c->stream);
+  hipFree(a); hipFree(b);
+  return 0;
+}
+
 }  // extern "C"

@@ -75,6 +75,7 @@ SIGNATURES = {
     'mf_get_stats': (C.c_int, [P, u64p, C.c_int]),
     'mf_reset_stats': (C.c_int, [P]),
     'mf_device_bytes': (C.c_long, [P]),
+    'mf_copy_reference': (C.c_int, [P, C.c_size_t, C.c_int]),
 }
 
 

@@ -14,8 +14,10 @@ pytestmark = pytest.mark.gpu
 
 
 def rot_err(A, B):
-    R = A[:3, :3].T @ B[:3, :3]
-    return float(np.arccos(np.clip((np.trace(R) - 1) / 2, -1, 1)))
+    # angle of A^T B from atan2(|skew|, (tr-1)/2): well conditioned near 0
+    R = A[:3, :3].astype(np.float64).T @ B[:3, :3].astype(np.float64)
+    s = 0.5 * np.linalg.norm([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]])
+    return float(np.arctan2(s, (np.trace(R) - 1) / 2))
 
 
 def run_pair(W, H, K, n_frames, use_rgb, res=256, noise=True, moving=False, check=None, seed=1):
