@@ -118,8 +118,10 @@ long mf_object_node_count(mf_ctx* ctx, int id);
 /* Level-0 tracking result codes of an instance's last pass (TrackData.result
  * of Object::trackresult_, ICP rows then RGB rows: 2*W*H int32). */
 int mf_get_trackresult(mf_ctx* ctx, int id, int32_t* codes);
-/* Per-iteration GN trace since the last clear: rows of 22 doubles {instance,
- * level, iteration, error, inliers, flag, pose[16]}; returns the row count. */
+/* Per-iteration GN trace since the last clear (the first clear turns tracing
+ * on): rows of 22 doubles {GN state, level, iteration, error, inliers, flag
+ * (0 step, 1 converged, 2 step-back, 3 no inliers), pose[16]}; returns the
+ * row count. */
 long mf_get_trace(mf_ctx* ctx, double* out, long cap);
 void mf_clear_trace(mf_ctx* ctx);
 /* Row dump for parity checks: when on, every tracking launch also writes its

@@ -69,12 +69,16 @@ struct mf_ctx {
   uint8_t* moving = nullptr;     // [MAX_INST] label-indexed flags
   TraceRec* trace = nullptr;
   int* trace_n = nullptr;
+  bool trace_on = false;
+  unsigned* arrive = nullptr;    // [MAX_INST] last-workgroup counters
+  int* obj_ids = nullptr;        // [MAX_INST] ids of the moving objects being tracked
   int trace_cap = 4096;
   RayObj* rayobjs = nullptr;
   unsigned long long* stats = nullptr;   // device work counters [N_STATS]
   TrackInst inst_host[MAX_INST];   // mirrors of the device descriptor tables
   RayObj ray_host[MAX_INST];
   int ray_n = -1;
+  std::vector<int> ids_host;
   std::vector<Object*> objs;
   float ray_eps = -1.f;
   bool have_ref = false;
@@ -210,19 +214,20 @@ void render_reference(mf_ctx* c, float* const rD[MAX_LEVELS], float* const mv[3]
 }
 
 // One coarse-to-fine GN run over `n` instances sharing a launch
-// (trackLiveFrame / trackEachObject loops, tracking.cpp:242-335, 624-690).
+// (trackLiveFrame / trackEachObject loops, tracking.cpp:242-335, 624-690):
+// one fused track+reduce+finalize launch per (level, iteration).
 void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, const int32_t* label_mask) {
+  TrackSync sy;
+  sy.arrive = c->arrive; sy.trace = c->trace_on ? c->trace : nullptr; sy.trace_n = c->trace_n;
+  sy.trace_cap = c->trace_cap;
   for (int l = c->cfg.n_levels - 1; l >= 0; --l) {
-    { Timed t(c, KC_GNSMALL); launch_gn_level_begin(c->stream, c->gn, MAX_INST); }
     TrackLaunch p = base_launch(c, k, l, object);
     p.label = label_mask;
     const int nblk = track_blocks(p.w, p.h);
     for (int it = 0; it < c->cfg.iters[l]; ++it) {
       p.iter = it;
-      { Timed t(c, KC_TRACK); launch_track(c->stream, p, c->inst + first_inst, n, c->gn, c->partial, nblk); }
-      { Timed t(c, KC_FINALIZE);
-        launch_finalize(c->stream, p, n, c->inst + first_inst, c->gn, c->partial, nblk, c->trace, c->trace_n,
-                        c->trace_cap); }
+      Timed t(c, KC_TRACK);
+      launch_track(c->stream, p, c->inst + first_inst, n, c->gn, c->partial, nblk, sy);
     }
   }
 }
@@ -241,6 +246,13 @@ void set_inst(mf_ctx* c, int slot, float* const mv[3], float* const mn[3], float
     hipMemcpyAsync(c->inst + slot, &c->inst_host[slot], sizeof ti, hipMemcpyHostToDevice, c->stream);
     hipStreamSynchronize(c->stream);
   }
+}
+
+void set_ids(mf_ctx* c, const std::vector<int>& ids) {
+  if (ids == c->ids_host) return;
+  c->ids_host = ids;
+  hipMemcpyAsync(c->obj_ids, c->ids_host.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+  hipStreamSynchronize(c->stream);
 }
 
 void download_planes(mf_ctx* c, float* const src[3], float* dst_aos, size_t n) {
@@ -290,7 +302,9 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->rayobjs = dalloc<RayObj>(c, MAX_INST);
   c->gauss = dalloc<float>(c, 5);
   c->stats = dalloc<unsigned long long>(c, N_STATS);
-  ok = ok && c->stats && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
+  c->arrive = dalloc<unsigned>(c, MAX_INST);
+  c->obj_ids = dalloc<int>(c, MAX_INST);
+  ok = ok && c->stats && c->arrive && c->obj_ids && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
        c->inst && c->partial && c->moving && c->trace && c->trace_n && c->rayobjs && c->gauss;
   if (!ok) { mf_destroy(c); return nullptr; }
   // Gaussian of the bilateral filter (kfusion.cpp:80-88): expf(-(x*x)/(2*delta^2)), delta = 4
@@ -388,7 +402,6 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
   if (frame % c->cfg.track_rate != 0) return 0;
   hipStream_t s = c->stream;
   const int L = c->cfg.n_levels;
-  { Timed t(c, KC_GNSMALL); launch_copy_mat(s, c->mats + 0, c->mats + 1); }   // T_w_r = camera_pose
   // set_params_frame + buildPyramid (tracking.cpp:142-171, 776-818)
   { Timed t(c, KC_BILATERAL); launch_bilateral(s, c->grey, c->live[0].I, c->depth, c->live[0].D, c->W, c->H, 0.1f, c->gauss); }
   for (int l = 0; l < L; l++) {
@@ -404,15 +417,13 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
     return 0;
   }
   Object* bg = c->objs[0];
-  { Timed t(c, KC_GNSMALL); launch_invert_mat(s, c->mats + 1, c->mats + 2); }
   // camera: trackLiveFrame against the global model maps
   float* camD[MAX_LEVELS];
   for (int l = 0; l < MAX_LEVELS; l++) camD[l] = bg->rD[l];
   set_inst(c, 0, c->vtx, c->nrm, camD, bg->codes, 0);
+  { Timed t(c, KC_GNSMALL); launch_camera_begin(s, c->mats, c->gn, k[0], k[1], k[2], k[3]); }
   if (c->cfg.use_rgb) render_reference(c, camD, c->vtx);
-  { Timed t(c, KC_GNSMALL); launch_gn_begin_camera(s, c->gn + 0, c->mats + 0, c->mats + 1, k[0], k[1], k[2], k[3]); }
   gn_run(c, k, 0, 0, 1, nullptr);
-  { Timed t(c, KC_GNSMALL); launch_gn_end_camera(s, c->gn + 0, c->mats + 0); }
   std::vector<Object*> mov;
   for (size_t i = 1; i < c->objs.size(); i++) if (c->objs[i]->moving) mov.push_back(c->objs[i]);
   if (!mov.empty()) {
@@ -420,7 +431,6 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
     // (kfusion.cpp:359-392, trackLiveFrame mask overload)
     { Timed t(c, KC_GNSMALL); launch_gn_begin_camera(s, c->gn + 0, c->mats + 0, c->mats + 1, k[0], k[1], k[2], k[3]); }
     gn_run(c, k, 0, 0, 1, c->label);
-    { Timed t(c, KC_GNSMALL); launch_gn_end_camera(s, c->gn + 0, c->mats + 0); }
   }
   c->pending_objects = true;
   return 1;
@@ -436,19 +446,18 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
   for (size_t i = 1; i < c->objs.size(); i++) if (c->objs[i]->moving) mov.push_back(c->objs[i]);
   if (!mov.empty()) {
     // trackEachObject for every moving object (kfusion.cpp:395-414)
+    std::vector<int> ids(mov.size());
     for (size_t j = 0; j < mov.size(); j++) {
       Object* o = mov[j];
+      ids[j] = 3 + o->id;
       set_inst(c, 1 + (int)j, o->mv, o->mn, o->rD, o->codes, 1 + (int)j);
-      if (c->cfg.use_rgb) render_reference(c, o->rD, o->mv);
-      Timed t(c, KC_GNSMALL);
-      launch_gn_begin_object(s, c->gn + 1 + j, c->mats + 0, c->mats + 1, c->mats + 3 + o->id);
     }
+    set_ids(c, ids);
+    { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, 1, (int)mov.size(), c->obj_ids, c->mats, c->mats + 0, c->mats + 1); }
+    for (size_t j = 0; j < mov.size(); j++)
+      if (c->cfg.use_rgb) render_reference(c, mov[j]->rD, mov[j]->mv);
     if (c->cfg.batch_objects) gn_run(c, k, 1, 1, (int)mov.size(), nullptr);
     else for (size_t j = 0; j < mov.size(); j++) gn_run(c, k, 1, 1 + (int)j, 1, nullptr);
-    for (size_t j = 0; j < mov.size(); j++) {
-      Timed t(c, KC_GNSMALL);
-      launch_gn_end_object(s, c->gn + 1 + j, c->mats + 0, c->mats + 3 + mov[j]->id);
-    }
   }
   for (int l = 0; l < L; l++) std::swap(c->live[l], c->ref[l]);   // setRefImgFromCurr
   return 1;
@@ -646,6 +655,7 @@ long mf_get_trace(mf_ctx* c, double* out, long cap) {
 void mf_clear_trace(mf_ctx* c) {
   hipMemsetAsync(c->trace_n, 0, sizeof(int), c->stream);
   hipStreamSynchronize(c->stream);
+  c->trace_on = true;
 }
 
 long mf_get_level(mf_ctx* c, int level, int which, float* out) {

@@ -55,9 +55,11 @@ struct GNState {
   Mat4 C;          // object: inverse(T_c1_o1)
   Mat4 T_wo1;      // object: pose at the previous frame
   float prev_err;
-  int done;        // current level finished (break taken)
+  int done_level;  // pyramid level at which the GN loop broke, -1 if none
   int iters_run;
-  int pad;
+  int object;      // 1: object instance (zero-inlier break, pose composed with T_w_c2)
+  Mat4* out_pose;  // written after every update: T_w_l, or volume_pose = T_w_c2 * T_c2_o2
+  const Mat4* T_w_c2;
   double sums[32]; // last reduction (FP64)
 };
 

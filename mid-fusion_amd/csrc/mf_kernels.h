@@ -29,19 +29,23 @@ struct TrackLaunch {
   TrackRow* dbg;                       // optional row dump: [state][level][2*W*H], or null
   int n_levels;
 };
+struct TrackSync {             // last-workgroup finalize of one launch
+  unsigned* arrive;             // per GN state arrival counters (self-resetting)
+  TraceRec* trace;              // optional per-iteration trace
+  int* trace_n;
+  int trace_cap;
+};
 int track_blocks(int w, int h);
 void launch_track(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
-                  float* partial, int nblk);
-void launch_finalize(hipStream_t s, const TrackLaunch& p, int ninst, const TrackInst* inst, GNState* st,
-                     const float* partial, int nblk, TraceRec* trace, int* trace_n, int trace_cap);
-void launch_gn_begin_camera(hipStream_t s, GNState* st, const Mat4* T_w_l, const Mat4* T_w_r, float fx, float fy,
+                  float* partial, int nblk, const TrackSync& sync);
+// Frame start for the camera: T_w_r = T_w_c, inverse(T_w_r), GN state 0.
+void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy);
+// GN state of a camera pass starting from the current camera pose.
+void launch_gn_begin_camera(hipStream_t s, GNState* st, Mat4* T_w_l, const Mat4* T_w_r, float fx, float fy,
                             float cx, float cy);
-void launch_gn_begin_object(hipStream_t s, GNState* st, const Mat4* T_w_c2, const Mat4* T_w_c1, const Mat4* T_wo1);
-void launch_gn_level_begin(hipStream_t s, GNState* st, int n);
-void launch_gn_end_camera(hipStream_t s, const GNState* st, Mat4* T_w_l);
-void launch_gn_end_object(hipStream_t s, const GNState* st, const Mat4* T_w_c2, Mat4* volume_pose);
-void launch_copy_mat(hipStream_t s, const Mat4* src, Mat4* dst);
-void launch_invert_mat(hipStream_t s, const Mat4* src, Mat4* dst);
+// GN states of `n` objects: states[first+j] <- object poses poses[ids[j]].
+void launch_gn_begin_objects(hipStream_t s, GNState* st, int first, int n, const int* ids, Mat4* poses,
+                             const Mat4* T_w_c2, const Mat4* T_w_c1);
 
 // mapping (mf_map.hip)
 void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* T_w_o, float fx, float fy, float cx, float cy,

@@ -218,14 +218,21 @@ __device__ __forceinline__ bool dyn_masked(const TrackLaunch& p, int x, int y) {
   return m;
 }
 
+__device__ void gn_finalize(const TrackLaunch& p, GNState& S, int state, const float* __restrict__ partial,
+                            int nblk, int inst_idx, const TrackSync& sy);
+
 template <bool ICP, bool RGB, bool OBJ>
 __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __restrict__ insts,
-                                              GNState* __restrict__ st, float* __restrict__ partial, int nblk) {
+                                              GNState* __restrict__ st, float* __restrict__ partial, int nblk,
+                                              TrackSync sy) {
   __shared__ float red[32][TB + 1];
   __shared__ Mats Msh;
-  const TrackInst in = insts[blockIdx.y];
+  __shared__ int is_last;
+  const TrackInst& in = insts[blockIdx.y];
   GNState& S = st[in.state];
-  if (S.done) return;   // uniform per workgroup
+  // a level starts afresh (tracking.cpp:246); later iterations are skipped
+  // once this level's loop has broken (uniform per workgroup)
+  if (p.iter > 0 && S.done_level == p.level) return;
   const int tid = threadIdx.x;
   if (tid == 0) {
     Msh.T = S.T; Msh.invT = inverse(S.T); Msh.A = S.A; Msh.B = S.B; Msh.C = S.C;
@@ -271,6 +278,14 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
     a += __shfl_xor(a, 4);
     if (seg == 0) partial[((size_t)blockIdx.y * nblk + blockIdx.x) * 32 + k] = a;
   }
+  // the last workgroup to arrive finalises the iteration
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) is_last = (atomicAdd(&sy.arrive[in.state], 1u) == (unsigned)(nblk - 1));
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  gn_finalize(p, S, in.state, partial, nblk, blockIdx.y, sy);
 }
 
 // ---- FP64 6x6 solve (commons.h:672-696 semantics: TooN GR_SVD backsub with
@@ -374,18 +389,16 @@ __device__ Mat4 se3_exp(const double x[6]) {
   return M;
 }
 
-__global__ __launch_bounds__(256) void k_finalize(int object, int level, int iter, int ninst,
-                                                  const TrackInst* __restrict__ insts, GNState* __restrict__ st,
-                                                  const float* __restrict__ partial, int nblk,
-                                                  TraceRec* __restrict__ trace, int* trace_n, int trace_cap) {
+// FP64 accumulation of the workgroup partials, 6x6 solve and the GN control
+// flow of trackLiveFrame / trackEachObject (tracking.cpp:300-334, 664-688),
+// run by the last workgroup of an iteration.
+__device__ void gn_finalize(const TrackLaunch& p, GNState& S, int state, const float* __restrict__ partial,
+                            int nblk, int inst_idx, const TrackSync& sy) {
   __shared__ double acc[32][9];
-  const int inst = blockIdx.x;
-  GNState& S = st[insts[inst].state];
-  if (S.done) return;
   const int tid = threadIdx.x;
   const int k = tid >> 3, seg = tid & 7;
   double a = 0;
-  for (int b = seg; b < nblk; b += 8) a += (double)partial[((size_t)inst * nblk + b) * 32 + k];
+  for (int b = seg; b < nblk; b += 8) a += (double)__builtin_nontemporal_load(&partial[((size_t)inst_idx * nblk + b) * 32 + k]);
   acc[k][seg] = a;
   __syncthreads();
   if (tid < 32) {
@@ -395,58 +408,73 @@ __global__ __launch_bounds__(256) void k_finalize(int object, int level, int ite
   }
   __syncthreads();
   if (tid != 0) return;
+  sy.arrive[state] = 0u;   // ready for the next launch
   const float s0 = (float)S.sums[0], s28 = (float)S.sums[28];
   const float err = s0 / s28;
+  const float prev_err = (p.iter == 0) ? INFINITY : S.prev_err;
   int flag = 0;
   S.iters_run++;
-  if (object && s28 == 0) { S.done = 1; flag = 3; }
-  else if (err > S.prev_err) { S.T = S.prev; S.done = 1; flag = 2; }
+  if (S.object && s28 == 0) { S.done_level = p.level; flag = 3; }
+  else if (err > prev_err) { S.T = S.prev; S.done_level = p.level; flag = 2; }
   else {
     S.prev_err = err;
     double x[6];
     solve6(S.sums + 1, x);
     const Mat4 upd = se3_exp(x);
     const double nx = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
-    if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) { S.done = 1; flag = 1; }
+    if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) { S.done_level = p.level; flag = 1; }
     else { S.prev = S.T; S.T = mul(upd, S.prev); }
   }
-  if (trace) {
-    const int ti = atomicAdd(trace_n, 1);
-    if (ti < trace_cap) {
-      TraceRec& r = trace[ti];
-      r.inst = insts[inst].state; r.level = level; r.iter = iter; r.flag = flag; r.err = err; r.s28 = s28;
+  // camera: T_w_l; object: volume_pose_ = T_w_c2 * T_c2_o2 (tracking.cpp:695)
+  *S.out_pose = S.object ? mul(*S.T_w_c2, S.T) : S.T;
+  if (sy.trace) {
+    const int ti = atomicAdd(sy.trace_n, 1);
+    if (ti < sy.trace_cap) {
+      TraceRec& r = sy.trace[ti];
+      r.inst = state; r.level = p.level; r.iter = p.iter; r.flag = flag; r.err = err; r.s28 = s28;
       for (int q = 0; q < 16; q++) r.pose[q] = S.T.m[q / 4][q % 4];
     }
   }
 }
 
-__global__ void k_gn_begin_camera(GNState* st, const Mat4* T_w_l, const Mat4* T_w_r, float fx, float fy, float cx,
-                                  float cy) {
-  GNState& S = *st;
-  S.T = *T_w_l; S.prev = S.T; S.A = *T_w_r;
-  S.B = mul(camK(fx, fy, cx, cy), inverse(*T_w_r));
+__device__ void gn_init_camera(GNState& S, Mat4* T_w_l, const Mat4& Twr, float fx, float fy, float cx, float cy) {
+  S.T = *T_w_l; S.prev = S.T; S.A = Twr;
+  S.B = mul(camK(fx, fy, cx, cy), inverse(Twr));
   S.C = ident(); S.T_wo1 = ident();
-  S.prev_err = INFINITY; S.done = 0; S.iters_run = 0;
+  S.prev_err = INFINITY; S.done_level = -1; S.iters_run = 0; S.object = 0;
+  S.out_pose = T_w_l; S.T_w_c2 = nullptr;
 }
-__global__ void k_gn_begin_object(GNState* st, const Mat4* T_w_c2, const Mat4* T_w_c1, const Mat4* T_wo1) {
-  GNState& S = *st;
-  const Mat4 To = *T_wo1;
+// Tracking frame start: T_w_r = camera pose (kfusion.cpp:243), its inverse for
+// the rendered reference, and the camera's GN state.
+__global__ void k_camera_begin(Mat4* mats, GNState* st, float fx, float fy, float cx, float cy) {
+  const Mat4 P = mats[0];
+  mats[1] = P;
+  mats[2] = inverse(P);
+  gn_init_camera(st[0], mats + 0, P, fx, fy, cx, cy);
+}
+__global__ void k_gn_begin_camera(GNState* st, Mat4* T_w_l, const Mat4* T_w_r, float fx, float fy, float cx,
+                                  float cy) {
+  gn_init_camera(*st, T_w_l, *T_w_r, fx, fy, cx, cy);
+}
+// trackEachObject setup (tracking.cpp:587-596): T_c1_o1 and the initial
+// T_c2_o2, one thread per object.
+__global__ void k_gn_begin_objects(GNState* st, int first, int n, const int* ids, Mat4* poses, const Mat4* T_w_c2,
+                                   const Mat4* T_w_c1) {
+  const int j = threadIdx.x;
+  if (j >= n) return;
+  GNState& S = st[first + j];
+  Mat4* pose = poses + ids[j];
+  const Mat4 To = *pose;
   S.T_wo1 = To;
   S.A = mul(inverse(*T_w_c1), To);
   S.T = mul(inverse(*T_w_c2), To);
   S.prev = S.T;
   S.B = inverse(To);
   S.C = inverse(S.A);
-  S.prev_err = INFINITY; S.done = 0; S.iters_run = 0;
+  S.prev_err = INFINITY; S.done_level = -1; S.iters_run = 0; S.object = 1;
+  S.out_pose = pose; S.T_w_c2 = T_w_c2;
+  *pose = mul(*T_w_c2, S.T);
 }
-__global__ void k_gn_level_begin(GNState* st, int n) {
-  const int i = threadIdx.x;
-  if (i < n) { st[i].prev_err = INFINITY; st[i].done = 0; }
-}
-__global__ void k_gn_end_camera(const GNState* st, Mat4* T_w_l) { *T_w_l = st->T; }
-__global__ void k_gn_end_object(const GNState* st, const Mat4* T_w_c2, Mat4* pose) { *pose = mul(*T_w_c2, st->T); }
-__global__ void k_copy_mat(const Mat4* src, Mat4* dst) { *dst = *src; }
-__global__ void k_invert_mat(const Mat4* src, Mat4* dst) { *dst = inverse(*src); }
 
 // ------------------------------------------------------------ launchers
 int track_blocks(int w, int h) {
@@ -455,36 +483,28 @@ int track_blocks(int w, int h) {
   return nb < 1 ? 1 : nb;
 }
 void launch_track(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
-                  float* partial, int nblk) {
+                  float* partial, int nblk, const TrackSync& sy) {
   dim3 grd(nblk, ninst), blk(TB);
   if (p.object) {
-    if (p.use_icp && p.use_rgb) k_track<true, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
-    else if (p.use_icp) k_track<true, false, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
-    else k_track<false, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+    if (p.use_icp && p.use_rgb) k_track<true, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, sy);
+    else if (p.use_icp) k_track<true, false, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, sy);
+    else k_track<false, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, sy);
   } else {
-    if (p.use_icp && p.use_rgb) k_track<true, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
-    else if (p.use_icp) k_track<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
-    else k_track<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+    if (p.use_icp && p.use_rgb) k_track<true, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, sy);
+    else if (p.use_icp) k_track<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, sy);
+    else k_track<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, sy);
   }
 }
-void launch_finalize(hipStream_t s, const TrackLaunch& p, int ninst, const TrackInst* inst, GNState* st,
-                     const float* partial, int nblk, TraceRec* trace, int* trace_n, int trace_cap) {
-  k_finalize<<<ninst, 256, 0, s>>>(p.object, p.level, p.iter, ninst, inst, st, partial, nblk, trace, trace_n,
-                                   trace_cap);
+void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy) {
+  k_camera_begin<<<1, 1, 0, s>>>(mats, st, fx, fy, cx, cy);
 }
-void launch_gn_begin_camera(hipStream_t s, GNState* st, const Mat4* T_w_l, const Mat4* T_w_r, float fx, float fy,
+void launch_gn_begin_camera(hipStream_t s, GNState* st, Mat4* T_w_l, const Mat4* T_w_r, float fx, float fy,
                             float cx, float cy) {
   k_gn_begin_camera<<<1, 1, 0, s>>>(st, T_w_l, T_w_r, fx, fy, cx, cy);
 }
-void launch_gn_begin_object(hipStream_t s, GNState* st, const Mat4* T_w_c2, const Mat4* T_w_c1, const Mat4* T_wo1) {
-  k_gn_begin_object<<<1, 1, 0, s>>>(st, T_w_c2, T_w_c1, T_wo1);
+void launch_gn_begin_objects(hipStream_t s, GNState* st, int first, int n, const int* ids, Mat4* poses,
+                             const Mat4* T_w_c2, const Mat4* T_w_c1) {
+  k_gn_begin_objects<<<1, 64, 0, s>>>(st, first, n, ids, poses, T_w_c2, T_w_c1);
 }
-void launch_gn_level_begin(hipStream_t s, GNState* st, int n) { k_gn_level_begin<<<1, 64, 0, s>>>(st, n); }
-void launch_gn_end_camera(hipStream_t s, const GNState* st, Mat4* T_w_l) { k_gn_end_camera<<<1, 1, 0, s>>>(st, T_w_l); }
-void launch_gn_end_object(hipStream_t s, const GNState* st, const Mat4* T_w_c2, Mat4* pose) {
-  k_gn_end_object<<<1, 1, 0, s>>>(st, T_w_c2, pose);
-}
-void launch_copy_mat(hipStream_t s, const Mat4* src, Mat4* dst) { k_copy_mat<<<1, 1, 0, s>>>(src, dst); }
-void launch_invert_mat(hipStream_t s, const Mat4* src, Mat4* dst) { k_invert_mat<<<1, 1, 0, s>>>(src, dst); }
 
 }  // namespace mf
