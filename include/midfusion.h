@@ -119,9 +119,9 @@ long mf_object_node_count(mf_ctx* ctx, int id);
  * of Object::trackresult_, ICP rows then RGB rows: 2*W*H int32). */
 int mf_get_trackresult(mf_ctx* ctx, int id, int32_t* codes);
 /* Per-iteration GN trace since the last clear (the first clear turns tracing
- * on): rows of 22 doubles {GN state, level, iteration, error, inliers, flag
- * (0 step, 1 converged, 2 step-back, 3 no inliers), pose[16]}; returns the
- * row count. */
+ * on): rows of 24 doubles {GN state, level, iteration, error, inliers, flag
+ * (0 step, 1 converged, 2 step-back, 3 no inliers), pose[16], FP64
+ * accumulation us, solve + control us}; returns the row count. */
 long mf_get_trace(mf_ctx* ctx, double* out, long cap);
 void mf_clear_trace(mf_ctx* ctx);
 /* Row dump for parity checks: when on, every tracking launch also writes its
@@ -154,6 +154,15 @@ int mf_reset_stats(mf_ctx* ctx);
 int mf_copy_reference(mf_ctx* ctx, size_t bytes, int reps);
 /* Device bytes held by the context. */
 long mf_device_bytes(mf_ctx* ctx);
+
+/* Launch-shape knobs (no counterpart in the reference, whose loops run on the
+ * host CPU).  Results do not depend on them beyond FP32 summation order.
+ *   MF_TUNE_TRACK_ROWS_PER_THREAD_L0 + l: residual rows (ICP or photometric)
+ *   each tracking thread sums before the workgroup reduction at pyramid
+ *   level l, 1..64 (defaults 4, 1, 1, 1). */
+enum { MF_TUNE_TRACK_ROWS_PER_THREAD_L0 = 0 };
+int mf_set_tuning(mf_ctx* ctx, int knob, int value);
+int mf_get_tuning(mf_ctx* ctx, int knob, int* value);
 
 #ifdef __cplusplus
 }

@@ -66,11 +66,11 @@ struct mf_ctx {
   TrackInst* inst = nullptr;     // device descriptors [MAX_INST]
   float* partial = nullptr;
   int nblk0 = 0;
+  int track_rpt[MAX_LEVELS] = {4, 1, 1, 1};   // tracking rows per thread, per pyramid level
   uint8_t* moving = nullptr;     // [MAX_INST] label-indexed flags
   TraceRec* trace = nullptr;
   int* trace_n = nullptr;
   bool trace_on = false;
-  unsigned* arrive = nullptr;    // [MAX_INST] last-workgroup counters
   int* obj_ids = nullptr;        // [MAX_INST] ids of the moving objects being tracked
   int trace_cap = 4096;
   RayObj* rayobjs = nullptr;
@@ -218,12 +218,13 @@ void render_reference(mf_ctx* c, float* const rD[MAX_LEVELS], float* const mv[3]
 // one fused track+reduce+finalize launch per (level, iteration).
 void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, const int32_t* label_mask) {
   TrackSync sy;
-  sy.arrive = c->arrive; sy.trace = c->trace_on ? c->trace : nullptr; sy.trace_n = c->trace_n;
+  sy.trace = c->trace_on ? c->trace : nullptr; sy.trace_n = c->trace_n;
   sy.trace_cap = c->trace_cap;
   for (int l = c->cfg.n_levels - 1; l >= 0; --l) {
     TrackLaunch p = base_launch(c, k, l, object);
     p.label = label_mask;
-    const int nblk = track_blocks(p.w, p.h);
+    const int nrows = p.w * p.h * ((c->cfg.use_icp ? 1 : 0) + (c->cfg.use_rgb ? 1 : 0));
+    const int nblk = track_blocks(nrows, c->track_rpt[l]);
     for (int it = 0; it < c->cfg.iters[l]; ++it) {
       p.iter = it;
       Timed t(c, KC_TRACK);
@@ -294,7 +295,7 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->mats = dalloc<Mat4>(c, 3 + MAX_INST);
   c->gn = dalloc<GNState>(c, MAX_INST);
   c->inst = dalloc<TrackInst>(c, MAX_INST);
-  c->nblk0 = track_blocks(c->W, c->H);
+  c->nblk0 = track_blocks(2 * (int)c->N0, 1);
   c->partial = dalloc<float>(c, (size_t)MAX_INST * c->nblk0 * 32);
   c->moving = dalloc<uint8_t>(c, MAX_INST);
   c->trace = dalloc<TraceRec>(c, c->trace_cap);
@@ -302,9 +303,8 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->rayobjs = dalloc<RayObj>(c, MAX_INST);
   c->gauss = dalloc<float>(c, 5);
   c->stats = dalloc<unsigned long long>(c, N_STATS);
-  c->arrive = dalloc<unsigned>(c, MAX_INST);
   c->obj_ids = dalloc<int>(c, MAX_INST);
-  ok = ok && c->stats && c->arrive && c->obj_ids && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
+  ok = ok && c->stats && c->obj_ids && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
        c->inst && c->partial && c->moving && c->trace && c->trace_n && c->rayobjs && c->gauss;
   if (!ok) { mf_destroy(c); return nullptr; }
   // Gaussian of the bilateral filter (kfusion.cpp:80-88): expf(-(x*x)/(2*delta^2)), delta = 4
@@ -646,9 +646,10 @@ long mf_get_trace(mf_ctx* c, double* out, long cap) {
   std::vector<TraceRec> r(std::max(n, 1));
   hipMemcpy(r.data(), c->trace, n * sizeof(TraceRec), hipMemcpyDeviceToHost);
   for (long i = 0; i < n && i < cap; i++) {
-    double* d = out + 22 * i;
+    double* d = out + 24 * i;
     d[0] = r[i].inst; d[1] = r[i].level; d[2] = r[i].iter; d[3] = r[i].err; d[4] = r[i].s28; d[5] = r[i].flag;
     for (int j = 0; j < 16; j++) d[6 + j] = r[i].pose[j];
+    d[22] = r[i].t_accum * 0.01; d[23] = r[i].t_solve * 0.01;   // microseconds
   }
   return n;
 }
@@ -732,6 +733,24 @@ int mf_reset_stats(mf_ctx* c) {
 }
 const char* mf_kernel_name(int i) { return (i >= 0 && i < KC_COUNT) ? kNames[i] : ""; }
 long mf_device_bytes(mf_ctx* c) { return c->bytes; }
+
+int mf_set_tuning(mf_ctx* c, int knob, int value) {
+  if (!c) return MF_E_ARG;
+  if (knob >= MF_TUNE_TRACK_ROWS_PER_THREAD_L0 && knob < MF_TUNE_TRACK_ROWS_PER_THREAD_L0 + MAX_LEVELS) {
+    if (value < 1 || value > 64) return fail(c, MF_E_ARG, "rows per thread must be in [1, 64]");
+    c->track_rpt[knob - MF_TUNE_TRACK_ROWS_PER_THREAD_L0] = value;
+    return 0;
+  }
+  return fail(c, MF_E_ARG, "unknown tuning knob");
+}
+int mf_get_tuning(mf_ctx* c, int knob, int* value) {
+  if (!c || !value) return MF_E_ARG;
+  if (knob >= MF_TUNE_TRACK_ROWS_PER_THREAD_L0 && knob < MF_TUNE_TRACK_ROWS_PER_THREAD_L0 + MAX_LEVELS) {
+    *value = c->track_rpt[knob - MF_TUNE_TRACK_ROWS_PER_THREAD_L0];
+    return 0;
+  }
+  return fail(c, MF_E_ARG, "unknown tuning knob");
+}
 
 // Plain device-to-device copy kernel: the reference point for scaling HBM
 // traffic counters (FETCH_SIZE / WRITE_SIZE) to bytes.

@@ -49,6 +49,7 @@ struct ObjParams {
 // Gauss-Newton state of one tracked instance (device resident).
 struct GNState {
   Mat4 T;          // camera: T_w_l; object: T_c2_o2
+  Mat4 invT;       // inverse(T), refreshed with every update of T
   Mat4 prev;       // previous_pose (carried across levels)
   Mat4 A;          // camera: T_w_r;            object: T_c1_o1
   Mat4 B;          // camera: K * inverse(T_w_r); object: inverse(T_w_o1)
@@ -60,6 +61,8 @@ struct GNState {
   int object;      // 1: object instance (zero-inlier break, pose composed with T_w_c2)
   Mat4* out_pose;  // written after every update: T_w_l, or volume_pose = T_w_c2 * T_c2_o2
   const Mat4* T_w_c2;
+  unsigned t_accum; // 100 MHz ticks of the last FP64 accumulation
+  unsigned pad2;
   double sums[32]; // last reduction (FP64)
 };
 
@@ -79,6 +82,6 @@ struct Level {
   float *D, *I, *V[3], *N[3], *gx, *gy, *cov[3];
 };
 
-struct TraceRec { int inst, level, iter, flag; float err, s28; float pose[16]; };
+struct TraceRec { int inst, level, iter, flag; float err, s28; float pose[16]; unsigned t_accum, t_solve; };   // t_*: 100 MHz ticks
 
 }  // namespace mf

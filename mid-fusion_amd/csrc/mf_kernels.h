@@ -29,13 +29,12 @@ struct TrackLaunch {
   TrackRow* dbg;                       // optional row dump: [state][level][2*W*H], or null
   int n_levels;
 };
-struct TrackSync {             // last-workgroup finalize of one launch
-  unsigned* arrive;             // per GN state arrival counters (self-resetting)
+struct TrackSync {             // accumulation + solve of one launch
   TraceRec* trace;              // optional per-iteration trace
   int* trace_n;
   int trace_cap;
 };
-int track_blocks(int w, int h);
+int track_blocks(int nrows, int rows_per_thread);
 void launch_track(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
                   float* partial, int nblk, const TrackSync& sync);
 // Frame start for the camera: T_w_r = T_w_c, inverse(T_w_r), GN state 0.

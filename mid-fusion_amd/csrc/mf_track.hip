@@ -15,7 +15,8 @@
 
 namespace mf {
 
-static constexpr int TB = 256;   // threads per tracking workgroup
+static constexpr int TB = 256;   // threads per tracking workgroup (4 waves)
+static constexpr int NW = TB / 64;
 
 __device__ __forceinline__ f3 ld3(const float* const* p, size_t i) { return mk3(p[0][i], p[1][i], p[2][i]); }
 
@@ -70,7 +71,9 @@ __device__ __forceinline__ void robust(float e, int fn, double k, float& w, floa
   }
 }
 
-struct Mats { Mat4 T, invT, A, B, C; };
+// Row functions read the pose matrices straight from the GN state: the
+// address is uniform per workgroup, so they live in scalar registers.
+using Mats = GNState;
 
 // ICP row (camera), track_ICP_Kernel mask overload (tracking.cpp:923-1020)
 __device__ __forceinline__ int icp_row(const TrackLaunch& p, const TrackInst& in, const Mats& M, int x, int y,
@@ -218,74 +221,75 @@ __device__ __forceinline__ bool dyn_masked(const TrackLaunch& p, int x, int y) {
   return m;
 }
 
-__device__ void gn_finalize(const TrackLaunch& p, GNState& S, int state, const float* __restrict__ partial,
-                            int nblk, int inst_idx, const TrackSync& sy);
-
 template <bool ICP, bool RGB, bool OBJ>
 __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __restrict__ insts,
-                                              GNState* __restrict__ st, float* __restrict__ partial, int nblk,
-                                              TrackSync sy) {
-  __shared__ float red[32][TB + 1];
-  __shared__ Mats Msh;
-  __shared__ int is_last;
+                                              const GNState* __restrict__ st, float* __restrict__ partial,
+                                              int nblk) {
+  __shared__ float red[NW][32];
   const TrackInst& in = insts[blockIdx.y];
-  GNState& S = st[in.state];
+  const GNState& S = st[in.state];
   // a level starts afresh (tracking.cpp:246); later iterations are skipped
   // once this level's loop has broken (uniform per workgroup)
   if (p.iter > 0 && S.done_level == p.level) return;
   const int tid = threadIdx.x;
-  if (tid == 0) {
-    Msh.T = S.T; Msh.invT = inverse(S.T); Msh.A = S.A; Msh.B = S.B; Msh.C = S.C;
-  }
-  __syncthreads();
-  const Mats& M = Msh;
+  const Mats& M = S;
   float s[32];
 #pragma unroll
   for (int k = 0; k < 32; k++) s[k] = 0.f;
   const int n = p.w * p.h;
+  const int nrows = (ICP ? n : 0) + (RGB ? n : 0);
   const size_t N0 = (size_t)p.W * p.H;
   TrackRow* dbg = p.dbg ? p.dbg + ((size_t)in.state * p.n_levels + p.level) * 2 * N0 : nullptr;
-  for (int i = blockIdx.x * TB + tid; i < n; i += nblk * TB) {
+  // rows [0, n) are ICP rows, [n, 2n) photometric rows of the same pixels
+  // (the order of TrackData in the reference); n is a multiple of 64 at
+  // every level of a 640x480 or 320x240 pyramid, so waves do not mix kinds
+  for (int r = blockIdx.x * TB + tid; r < nrows; r += nblk * TB) {
+    const bool icp = ICP && (!RGB || r < n);
+    const int i = (ICP && !icp) ? r - n : r;
     const int x = i % p.w, y = i / p.w;
     const size_t ro = (size_t)x + (size_t)y * p.W;   // row stride W at every level (tracking.cpp:49-50)
-    if (ICP) {
-      float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
-      int r;
-      if (OBJ) r = obj_icp_row(p, in, M, x, y, (size_t)i, e, J);
-      else r = icp_row(p, in, M, x, y, (size_t)i, dyn_masked(p, x, y), e, J);
-      accum(s, r, e, J, p.robust, p.robust_k);
-      if (in.codes && p.level == 0) in.codes[ro] = (int8_t)r;
-      if (dbg) { TrackRow& d = dbg[ro]; d.result = r; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
+    float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
+    int res;
+    if (icp) {
+      if (OBJ) res = obj_icp_row(p, in, M, x, y, (size_t)i, e, J);
+      else res = icp_row(p, in, M, x, y, (size_t)i, dyn_masked(p, x, y), e, J);
+    } else {
+      res = rgb_row<OBJ>(p, in, M, x, y, (size_t)i, e, J);
     }
-    if (RGB) {
-      float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
-      const int r = rgb_row<OBJ>(p, in, M, x, y, (size_t)i, e, J);
-      accum(s, r, e, J, p.robust, p.robust_k);
-      if (in.codes && p.level == 0) in.codes[N0 + ro] = (int8_t)r;
-      if (dbg) { TrackRow& d = dbg[N0 + ro]; d.result = r; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
-    }
+    accum(s, res, e, J, p.robust, p.robust_k);
+    const size_t oo = icp ? ro : N0 + ro;
+    if (in.codes && p.level == 0) in.codes[oo] = (int8_t)res;
+    if (dbg) { TrackRow& d = dbg[oo]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
   }
-  // LDS transpose reduction: 32 sums x 256 threads -> 32 sums.
-#pragma unroll
-  for (int k = 0; k < 32; k++) red[k][tid] = s[k];
+  // Wave butterfly: at each step a lane keeps half of its sums and adds the
+  // partner lane's copy of that half, so after 5 exchange steps (16+8+4+2+1
+  // lane swaps) lane l holds sum l>>1 over 32 lanes, and one more swap adds
+  // the other 32.  Fixed order, so the partial is deterministic.
+  const int lane = tid & 63;
+#define MF_BFLY(HALF, OFF)                                                   \
+  {                                                                          \
+    const bool hi = (lane & OFF) != 0;                                       \
+    _Pragma("unroll") for (int j = 0; j < HALF; j++) {                       \
+      const float keep = hi ? s[j + HALF] : s[j];                            \
+      const float send = hi ? s[j] : s[j + HALF];                            \
+      s[j] = keep + __shfl_xor(send, OFF);                                   \
+    }                                                                        \
+  }
+  MF_BFLY(16, 32)
+  MF_BFLY(8, 16)
+  MF_BFLY(4, 8)
+  MF_BFLY(2, 4)
+  MF_BFLY(1, 2)
+#undef MF_BFLY
+  const float v = s[0] + __shfl_xor(s[0], 1);
+  if ((lane & 1) == 0) red[tid >> 6][lane >> 1] = v;
   __syncthreads();
-  {
-    const int k = tid >> 3, seg = tid & 7;   // 8 threads per sum
+  if (tid < 32) {
     float a = 0.f;
-    for (int j = seg * (TB / 8); j < (seg + 1) * (TB / 8); j++) a += red[k][j];
-    a += __shfl_xor(a, 1);
-    a += __shfl_xor(a, 2);
-    a += __shfl_xor(a, 4);
-    if (seg == 0) partial[((size_t)blockIdx.y * nblk + blockIdx.x) * 32 + k] = a;
+#pragma unroll
+    for (int w = 0; w < NW; w++) a += red[w][tid];
+    partial[((size_t)blockIdx.y * nblk + blockIdx.x) * 32 + tid] = a;
   }
-  // the last workgroup to arrive finalises the iteration
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) is_last = (atomicAdd(&sy.arrive[in.state], 1u) == (unsigned)(nblk - 1));
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  gn_finalize(p, S, in.state, partial, nblk, blockIdx.y, sy);
 }
 
 // ---- FP64 6x6 solve (commons.h:672-696 semantics: TooN GR_SVD backsub with
@@ -293,48 +297,66 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
 // value is truncated; otherwise a Jacobi eigendecomposition pseudo-inverse.
 __device__ void jacobi_pinv(const double C[6][6], const double b[6], double x[6]) {
   double A[6][6], V[6][6];
+  #pragma unroll
   for (int i = 0; i < 6; i++) for (int j = 0; j < 6; j++) { A[i][j] = C[i][j]; V[i][j] = (i == j); }
   for (int sweep = 0; sweep < 60; sweep++) {
     double off = 0;
+    #pragma unroll
     for (int pp = 0; pp < 6; pp++) for (int q = pp + 1; q < 6; q++) off += A[pp][q] * A[pp][q];
     if (off == 0) break;
+    #pragma unroll
     for (int pp = 0; pp < 6; pp++)
+      #pragma unroll
       for (int q = pp + 1; q < 6; q++) {
         if (A[pp][q] == 0) continue;
         const double theta = (A[q][q] - A[pp][pp]) / (2 * A[pp][q]);
         const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
         const double c = 1 / sqrt(t * t + 1), sn = t * c;
+        #pragma unroll
         for (int k = 0; k < 6; k++) { const double akp = A[k][pp], akq = A[k][q]; A[k][pp] = c * akp - sn * akq; A[k][q] = sn * akp + c * akq; }
+        #pragma unroll
         for (int k = 0; k < 6; k++) { const double apk = A[pp][k], aqk = A[q][k]; A[pp][k] = c * apk - sn * aqk; A[q][k] = sn * apk + c * aqk; }
+        #pragma unroll
         for (int k = 0; k < 6; k++) { const double vkp = V[k][pp], vkq = V[k][q]; V[k][pp] = c * vkp - sn * vkq; V[k][q] = sn * vkp + c * vkq; }
       }
   }
   double mx = 0;
+  #pragma unroll
   for (int i = 0; i < 6; i++) mx = fmax(mx, fabs(A[i][i]));
+  #pragma unroll
   for (int i = 0; i < 6; i++) x[i] = 0;
+  #pragma unroll
   for (int i = 0; i < 6; i++) {
     if (fabs(A[i][i]) * 1e6 <= mx) continue;
     double pr = 0;
+    #pragma unroll
     for (int k = 0; k < 6; k++) pr += V[k][i] * b[k];
     pr /= A[i][i];
+    #pragma unroll
     for (int k = 0; k < 6; k++) x[k] += pr * V[k][i];
   }
 }
 __device__ void solve6(const double* s27, double x[6]) {
   double b[6], C[6][6];
+  #pragma unroll
   for (int i = 0; i < 6; i++) b[i] = s27[i];
   int c = 6;
+  #pragma unroll
   for (int i = 0; i < 6; i++) for (int j = i; j < 6; j++) { C[i][j] = s27[c]; C[j][i] = s27[c]; c++; }
   // LDL^T
   double L[6][6], D[6];
   bool ok = true;
+  #pragma unroll
   for (int j = 0; j < 6 && ok; j++) {
     double d = C[j][j];
+    #pragma unroll
     for (int k = 0; k < j; k++) d -= L[j][k] * L[j][k] * D[k];
     if (!(d > 0)) { ok = false; break; }
     D[j] = d; L[j][j] = 1;
+    #pragma unroll
     for (int i = j + 1; i < 6; i++) {
       double v = C[i][j];
+      #pragma unroll
       for (int k = 0; k < j; k++) v -= L[i][k] * L[j][k] * D[k];
       L[i][j] = v / d;
     }
@@ -342,25 +364,34 @@ __device__ void solve6(const double* s27, double x[6]) {
   if (ok) {
     // tr(C^-1) = sum_j (1/D_j) sum_i (L^-1)_{ji}^2 ; Li = L^-1 (unit lower)
     double Li[6][6];
+    #pragma unroll
     for (int i = 0; i < 6; i++)
+      #pragma unroll
       for (int j = 0; j < 6; j++) {
         if (j > i) { Li[i][j] = 0; continue; }
         if (j == i) { Li[i][j] = 1; continue; }
         double v = 0;
+        #pragma unroll
         for (int k = j; k < i; k++) v -= L[i][k] * Li[k][j];
         Li[i][j] = v;
       }
     double trC = 0, trI = 0;
+    #pragma unroll
     for (int i = 0; i < 6; i++) trC += C[i][i];
+    #pragma unroll
     for (int j = 0; j < 6; j++) {
       double ss = 0;
+      #pragma unroll
       for (int i = j; i < 6; i++) ss += Li[i][j] * Li[i][j];
       trI += ss / D[j];
     }
     if (trC * trI < 1e6) {
       double z[6];
+      #pragma unroll
       for (int i = 0; i < 6; i++) { double v = b[i]; for (int k = 0; k < i; k++) v -= L[i][k] * z[k]; z[i] = v; }
+      #pragma unroll
       for (int i = 0; i < 6; i++) z[i] /= D[i];
+      #pragma unroll
       for (int i = 5; i >= 0; i--) { double v = z[i]; for (int k = i + 1; k < 6; k++) v -= L[k][i] * x[k]; x[i] = v; }
       return;
     }
@@ -378,67 +409,101 @@ __device__ Mat4 se3_exp(const double x[6]) {
   else { const double th = sqrt(th2); A = sin(th) / th; B = (1.0 - cos(th)) / th2; C = (1.0 - A) / th2; }
   const double W[3][3] = {{0, -w2, w1}, {w2, 0, -w0}, {-w1, w0, 0}};
   double W2[3][3];
+  #pragma unroll
   for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) { double s = 0; for (int k = 0; k < 3; k++) s += W[i][k] * W[k][j]; W2[i][j] = s; }
   Mat4 M = ident();
+  #pragma unroll
   for (int i = 0; i < 3; i++) {
+    #pragma unroll
     for (int j = 0; j < 3; j++) M.m[i][j] = (float)((i == j) + A * W[i][j] + B * W2[i][j]);
     double s = x[i];
+    #pragma unroll
     for (int k = 0; k < 3; k++) s += (B * W[i][k] + C * W2[i][k]) * x[k];
     M.m[i][3] = (float)s;
   }
   return M;
 }
 
-// FP64 accumulation of the workgroup partials, 6x6 solve and the GN control
-// flow of trackLiveFrame / trackEachObject (tracking.cpp:300-334, 664-688),
-// run by the last workgroup of an iteration.
-__device__ void gn_finalize(const TrackLaunch& p, GNState& S, int state, const float* __restrict__ partial,
-                            int nblk, int inst_idx, const TrackSync& sy) {
-  __shared__ double acc[32][9];
+// FP64 accumulation of one instance's workgroup partials, the 6x6 solve and
+// the GN control flow of trackLiveFrame / trackEachObject (tracking.cpp:
+// 300-334, 664-688); one workgroup per instance of the launch.  Partials are
+// summed in a fixed order (deterministic) with eight float4 loads in flight
+// per thread; thread t always reads sums 4*(t%8)..4*(t%8)+3.
+static constexpr int SB = 256;
+__global__ __launch_bounds__(SB) void k_solve(int level, int iter, const TrackInst* __restrict__ insts,
+                                              GNState* __restrict__ st, const float* __restrict__ partial,
+                                              int nblk, TrackSync sy) {
+  __shared__ double red[SB / 64][8][4];
+  __shared__ double sums[32];
+  const int j = blockIdx.x;
+  const int state = insts[j].state;
+  GNState& S = st[state];
+  if (iter > 0 && S.done_level == level) return;
   const int tid = threadIdx.x;
-  const int k = tid >> 3, seg = tid & 7;
-  double a = 0;
-  for (int b = seg; b < nblk; b += 8) a += (double)__builtin_nontemporal_load(&partial[((size_t)inst_idx * nblk + b) * 32 + k]);
-  acc[k][seg] = a;
+  const uint64_t t0 = wall_clock64();
+  const float4* P = reinterpret_cast<const float4*>(partial + (size_t)j * nblk * 32);
+  const int nq = nblk * 8;
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  for (int q0 = 0; q0 < nq; q0 += 8 * SB) {
+    float4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int q = q0 + u * SB + tid;
+      v[u] = q < nq ? P[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) { a0 += (double)v[u].x; a1 += (double)v[u].y; a2 += (double)v[u].z; a3 += (double)v[u].w; }
+  }
+  // lanes with equal (lane % 8) hold the same four sums
+#pragma unroll
+  for (int o = 8; o < 64; o <<= 1) {
+    a0 += __shfl_xor(a0, o); a1 += __shfl_xor(a1, o); a2 += __shfl_xor(a2, o); a3 += __shfl_xor(a3, o);
+  }
+  const int lane = tid & 63;
+  if (lane < 8) { red[tid >> 6][lane][0] = a0; red[tid >> 6][lane][1] = a1; red[tid >> 6][lane][2] = a2; red[tid >> 6][lane][3] = a3; }
   __syncthreads();
   if (tid < 32) {
     double t = 0;
-    for (int j = 0; j < 8; j++) t += acc[tid][j];
+#pragma unroll
+    for (int w = 0; w < SB / 64; w++) t += red[w][tid >> 2][tid & 3];
+    sums[tid] = t;
     S.sums[tid] = t;
   }
   __syncthreads();
   if (tid != 0) return;
-  sy.arrive[state] = 0u;   // ready for the next launch
-  const float s0 = (float)S.sums[0], s28 = (float)S.sums[28];
+  const uint64_t t1 = wall_clock64();
+  const float s0 = (float)sums[0], s28 = (float)sums[28];
   const float err = s0 / s28;
-  const float prev_err = (p.iter == 0) ? INFINITY : S.prev_err;
+  const float prev_err = (iter == 0) ? INFINITY : S.prev_err;
   int flag = 0;
   S.iters_run++;
-  if (S.object && s28 == 0) { S.done_level = p.level; flag = 3; }
-  else if (err > prev_err) { S.T = S.prev; S.done_level = p.level; flag = 2; }
+  if (S.object && s28 == 0) { S.done_level = level; flag = 3; }
+  else if (err > prev_err) { S.T = S.prev; S.invT = inverse(S.T); S.done_level = level; flag = 2; }
   else {
     S.prev_err = err;
     double x[6];
-    solve6(S.sums + 1, x);
+    solve6(sums + 1, x);
     const Mat4 upd = se3_exp(x);
     const double nx = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
-    if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) { S.done_level = p.level; flag = 1; }
-    else { S.prev = S.T; S.T = mul(upd, S.prev); }
+    if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) { S.done_level = level; flag = 1; }
+    else { S.prev = S.T; S.T = mul(upd, S.prev); S.invT = inverse(S.T); }
   }
   // camera: T_w_l; object: volume_pose_ = T_w_c2 * T_c2_o2 (tracking.cpp:695)
   *S.out_pose = S.object ? mul(*S.T_w_c2, S.T) : S.T;
   if (sy.trace) {
+    const uint64_t t2 = wall_clock64();
     const int ti = atomicAdd(sy.trace_n, 1);
     if (ti < sy.trace_cap) {
       TraceRec& r = sy.trace[ti];
-      r.inst = state; r.level = p.level; r.iter = p.iter; r.flag = flag; r.err = err; r.s28 = s28;
+      r.inst = state; r.level = level; r.iter = iter; r.flag = flag; r.err = err; r.s28 = s28;
       for (int q = 0; q < 16; q++) r.pose[q] = S.T.m[q / 4][q % 4];
+      r.t_accum = (unsigned)(t1 - t0); r.t_solve = (unsigned)(t2 - t1);
     }
   }
 }
 
 __device__ void gn_init_camera(GNState& S, Mat4* T_w_l, const Mat4& Twr, float fx, float fy, float cx, float cy) {
-  S.T = *T_w_l; S.prev = S.T; S.A = Twr;
+  S.T = *T_w_l; S.invT = inverse(S.T); S.prev = S.T; S.A = Twr;
   S.B = mul(camK(fx, fy, cx, cy), inverse(Twr));
   S.C = ident(); S.T_wo1 = ident();
   S.prev_err = INFINITY; S.done_level = -1; S.iters_run = 0; S.object = 0;
@@ -468,6 +533,7 @@ __global__ void k_gn_begin_objects(GNState* st, int first, int n, const int* ids
   S.T_wo1 = To;
   S.A = mul(inverse(*T_w_c1), To);
   S.T = mul(inverse(*T_w_c2), To);
+  S.invT = inverse(S.T);
   S.prev = S.T;
   S.B = inverse(To);
   S.C = inverse(S.A);
@@ -477,23 +543,24 @@ __global__ void k_gn_begin_objects(GNState* st, int first, int n, const int* ids
 }
 
 // ------------------------------------------------------------ launchers
-int track_blocks(int w, int h) {
-  const int n = w * h;
-  int nb = (n + TB * 4 - 1) / (TB * 4);   // ~4 pixels per thread
+int track_blocks(int nrows, int rows_per_thread) {
+  if (rows_per_thread < 1) rows_per_thread = 1;
+  const int nb = (nrows + TB * rows_per_thread - 1) / (TB * rows_per_thread);
   return nb < 1 ? 1 : nb;
 }
 void launch_track(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
                   float* partial, int nblk, const TrackSync& sy) {
   dim3 grd(nblk, ninst), blk(TB);
   if (p.object) {
-    if (p.use_icp && p.use_rgb) k_track<true, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, sy);
-    else if (p.use_icp) k_track<true, false, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, sy);
-    else k_track<false, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, sy);
+    if (p.use_icp && p.use_rgb) k_track<true, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+    else if (p.use_icp) k_track<true, false, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+    else k_track<false, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
   } else {
-    if (p.use_icp && p.use_rgb) k_track<true, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, sy);
-    else if (p.use_icp) k_track<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, sy);
-    else k_track<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, sy);
+    if (p.use_icp && p.use_rgb) k_track<true, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+    else if (p.use_icp) k_track<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+    else k_track<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
   }
+  k_solve<<<ninst, SB, 0, s>>>(p.level, p.iter, inst, st, partial, nblk, sy);
 }
 void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy) {
   k_camera_begin<<<1, 1, 0, s>>>(mats, st, fx, fy, cx, cy);

@@ -76,6 +76,8 @@ SIGNATURES = {
     'mf_reset_stats': (C.c_int, [P]),
     'mf_device_bytes': (C.c_long, [P]),
     'mf_copy_reference': (C.c_int, [P, C.c_size_t, C.c_int]),
+    'mf_set_tuning': (C.c_int, [P, C.c_int, C.c_int]),
+    'mf_get_tuning': (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
 }
 
 
@@ -220,10 +222,10 @@ class Kfusion:
         return out
 
     def trace(self):
-        n = self.L.mf_get_trace(self.ctx, np.zeros(22), 0)
-        out = np.zeros(22 * max(n, 1))
+        n = self.L.mf_get_trace(self.ctx, np.zeros(24), 0)
+        out = np.zeros(24 * max(n, 1))
         self.L.mf_get_trace(self.ctx, out, n)
-        return out[:22 * n].reshape(n, 22)
+        return out[:24 * n].reshape(n, 24)
 
     def trace_clear(self):
         self.L.mf_clear_trace(self.ctx)
@@ -272,3 +274,13 @@ class Kfusion:
 
     def device_bytes(self):
         return self.L.mf_device_bytes(self.ctx)
+
+    TUNE_TRACK_ROWS_PER_THREAD_L0 = 0   # + pyramid level
+
+    def set_tuning(self, knob, value):
+        self._chk(self.L.mf_set_tuning(self.ctx, knob, int(value)))
+
+    def get_tuning(self, knob):
+        v = C.c_int()
+        self._chk(self.L.mf_get_tuning(self.ctx, knob, C.byref(v)))
+        return v.value

@@ -174,6 +174,9 @@ class Oracle:
         self.L.orc_get_pose(self.h_, oid, T)
         return T.reshape(4, 4)
 
+    def set_pose(self, oid, T):
+        self.L.orc_set_pose(self.h_, oid, f32(np.asarray(T).reshape(16)))
+
     def maps(self):
         n = self.w * self.h
         v, nn, lab = np.zeros(3 * n, np.float32), np.zeros(3 * n, np.float32), np.zeros(n, np.int32)

@@ -133,13 +133,15 @@ def test_pipeline_icp_only_cfg1_like():
 
 
 def test_pipeline_icp_rgb_moving_object():
-    # Object GN is far worse conditioned than the camera's: FP32 summation
-    # order alone moves its pose by ~1e-4 (the FP32 and FP64 oracles differ by
-    # as much), so whole-pipeline object poses are held to 1e-3 here while
-    # test_gpu_rows pins every object row exactly.
+    # Against the FP32 oracle, whose 8-block sequential FP32 sums carry up to
+    # ~2e-5 m of pose error of their own with the masked camera re-track
+    # (test_gpu_fp64 holds the GPU to 1e-5 of the FP64 oracle and records the
+    # FP32 oracle's error beside it): camera within 5e-5 here.  Object GN is
+    # far worse conditioned than the camera's, so whole-pipeline object poses
+    # are held to 1e-3 while test_gpu_rows pins every object row exactly.
     rep = run_pair(320, 240, K320, 4, use_rgb=True, moving=True)
     for r in rep:
-        assert r['pose_t'] < 1e-5 and r['pose_r'] < 1e-5, r
+        assert r['pose_t'] < 5e-5 and r['pose_r'] < 5e-5, r
         assert r.get('obj1_t', 0) < 1e-3 and r.get('obj1_r', 0) < 1e-3, r
         assert r['keys0'][2] == 0 and r['keys1'][2] == 0, r
         assert r['label_mismatch'] < 1e-3, r
