@@ -128,6 +128,7 @@ def main():
     t = time.perf_counter()
     for f in range(args.warmup, n_frames):
         step(f)
+    t_enq = time.perf_counter() - t
     g.sync()
     el = time.perf_counter() - t
     fps = args.steps / el
@@ -205,6 +206,7 @@ def main():
         'cpu_baseline': cpu,
         'stage_ms_per_frame': stage,
         'kernel_ms_per_frame_total': round(tot_ms / 3, 3),
+        'host_enqueue_ms_per_frame': round(1000.0 * t_enq / args.steps, 3),
         'work_per_frame': {k: v // 3 for k, v in st.items()},
         'camera_translation_error_vs_gt_m': round(pose_err, 5),
     }

@@ -56,7 +56,7 @@ void mf_destroy(mf_ctx* ctx);
 /* Last error message of the context (static storage of the context). */
 const char* mf_last_error(mf_ctx* ctx);
 
-/* Object::Object (object/object.cpp:19-46).  Returns the instance id
+/* Object::Object (object/object.cpp:19-46); res a power of two in [16, 8192].  Returns the instance id
  * (= instance_label_), capacity = max voxel blocks (0: default). */
 int mf_object_create(mf_ctx* ctx, float size_m, int res, const float T_w_o[16], long capacity);
 /* Object::set_static_state (object.cpp:317-320): moving objects are tracked
@@ -119,9 +119,10 @@ long mf_object_node_count(mf_ctx* ctx, int id);
  * of Object::trackresult_, ICP rows then RGB rows: 2*W*H int32). */
 int mf_get_trackresult(mf_ctx* ctx, int id, int32_t* codes);
 /* Per-iteration GN trace since the last clear (the first clear turns tracing
- * on): rows of 24 doubles {GN state, level, iteration, error, inliers, flag
+ * on): rows of 25 doubles {GN state, level, iteration, error, inliers, flag
  * (0 step, 1 converged, 2 step-back, 3 no inliers), pose[16], FP64
- * accumulation us, solve + control us}; returns the row count. */
+ * accumulation us, solve + control us, solver (0 none, 1 certified
+ * Gauss-Jordan, 2 Jacobi pseudo-inverse)}; returns the row count. */
 long mf_get_trace(mf_ctx* ctx, double* out, long cap);
 void mf_clear_trace(mf_ctx* ctx);
 /* Row dump for parity checks: when on, every tracking launch also writes its

@@ -18,11 +18,11 @@ using namespace mf;
 namespace {
 
 enum KClass {
-  KC_CONVERT, KC_BILATERAL, KC_HALF, KC_LEVEL, KC_V2D, KC_TRACK, KC_FINALIZE, KC_GNSMALL, KC_PARAMS, KC_ALLOC,
+  KC_CONVERT, KC_BILATERAL, KC_HALF, KC_LEVEL, KC_V2D, KC_TRACK, KC_GNSMALL, KC_PARAMS, KC_ALLOC,
   KC_COMMIT, KC_INTEGRATE, KC_RAYCAST, KC_COUNT
 };
 const char* kNames[KC_COUNT] = {"convert", "bilateral", "half_sample", "level_maps", "vertex2depth", "track_reduce",
-                                "gn_finalize", "gn_control", "obj_params", "alloc_band", "alloc_commit",
+                                "gn_control", "obj_params", "alloc_band", "alloc_commit",
                                 "integrate", "raycast"};
 
 struct Object {
@@ -132,13 +132,23 @@ int fail(mf_ctx* c, int code, const char* msg) { c->err = msg; return code; }
 bool alloc_level(mf_ctx* c, Level& L, int w, int h) {
   const size_t n = (size_t)w * h;
   L.w = w; L.h = h;
-  L.D = dalloc<float>(c, n); L.I = dalloc<float>(c, n); L.gx = dalloc<float>(c, n); L.gy = dalloc<float>(c, n);
-  for (int k = 0; k < 3; k++) { L.V[k] = dalloc<float>(c, n); L.N[k] = dalloc<float>(c, n); L.cov[k] = dalloc<float>(c, n); }
-  return L.D && L.I && L.gx && L.gy && L.V[2] && L.N[2] && L.cov[2];
+  float* b = dalloc<float>(c, n * PL_COUNT);
+  if (!b) return false;
+  L.D = b + PL_D * n; L.I = b + PL_I * n; L.gx = b + PL_GX * n; L.gy = b + PL_GY * n;
+  for (int k = 0; k < 3; k++) { L.V[k] = b + (PL_V + k) * n; L.N[k] = b + (PL_N + k) * n; L.cov[k] = b + (PL_COV + k) * n; }
+  return true;
+}
+// Offset of pyramid level l in a rendered-depth pyramid stored back to back
+// (one guard element after each level).
+size_t rd_offset(const mf_ctx* c, int l) {
+  size_t off = 0;
+  for (int k = 0; k < l; k++) off += (c->N0 >> (2 * k)) + 1;
+  return off;
 }
 
 int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
-  if (res < 16 || (res & (res - 1)) || size <= 0) return fail(c, MF_E_ARG, "object resolution must be a power of two >= 16");
+  if (res < 16 || res > 8192 || (res & (res - 1)) || size <= 0)
+    return fail(c, MF_E_ARG, "object resolution must be a power of two in [16, 8192]");
   if ((int)c->objs.size() >= MAX_INST - 3) return fail(c, MF_E_ARG, "too many objects");
   Object* o = new Object();
   o->id = (int)c->objs.size();
@@ -168,9 +178,15 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   v.counters = dalloc<int32_t>(c, 8);
   v.active_list = dalloc<int32_t>(c, cap);
   o->mask = dalloc<float>(c, c->N0);
-  for (int k = 0; k < 3; k++) { o->mv[k] = dalloc<float>(c, c->N0); o->mn[k] = dalloc<float>(c, c->N0); }
+  {
+    float* b = dalloc<float>(c, 6 * c->N0);   // vertex planes then normal planes
+    for (int k = 0; k < 3; k++) { o->mv[k] = b ? b + k * c->N0 : nullptr; o->mn[k] = b ? b + (3 + k) * c->N0 : nullptr; }
+  }
   o->codes = dalloc<int8_t>(c, 2 * c->N0);
-  for (int l = 0; l < c->cfg.n_levels; l++) o->rD[l] = dalloc<float>(c, (c->N0 >> (2 * l)) + 1);
+  {
+    float* b = dalloc<float>(c, rd_offset(c, c->cfg.n_levels));
+    for (int l = 0; l < c->cfg.n_levels; l++) o->rD[l] = b ? b + rd_offset(c, l) : nullptr;
+  }
   o->prm = dalloc<ObjParams>(c, 1);
   if (!v.slot || !v.req || !v.nodes || !v.tsdf || !v.fg || !v.c || !v.fgw || !v.active_list || !o->mask ||
       !o->codes || !o->prm || !o->mv[2] || !o->mn[2]) {
@@ -197,7 +213,8 @@ TrackLaunch base_launch(mf_ctx* c, const float k[4], int level, int object) {
   // sigma_bright_ = sqrtf(100.0)/255.0 (tracking.cpp:764,769); inv_cov = 1.0/sigma
   const float sigma_bright = (float)(sqrtf(100.0f) / 255.0);
   p.inv_cov_rgb = (float)(1.0 / (double)sigma_bright);
-  p.live = c->live[level]; p.ref = c->ref[level];
+  p.live = c->live[level].D; p.ref = c->ref[level].D; p.lstride = c->live[level].w * c->live[level].h;
+  p.rd_off = (int)rd_offset(c, level);
   p.label = nullptr; p.moving = c->moving; p.n_labels = MAX_INST;
   p.dbg = c->dbg_rows; p.n_levels = c->cfg.n_levels;
   return p;
@@ -220,15 +237,15 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
   TrackSync sy;
   sy.trace = c->trace_on ? c->trace : nullptr; sy.trace_n = c->trace_n;
   sy.trace_cap = c->trace_cap;
+  const int kinds = (c->cfg.use_icp ? 1 : 0) + (c->cfg.use_rgb ? 1 : 0);
   for (int l = c->cfg.n_levels - 1; l >= 0; --l) {
     TrackLaunch p = base_launch(c, k, l, object);
     p.label = label_mask;
-    const int nrows = p.w * p.h * ((c->cfg.use_icp ? 1 : 0) + (c->cfg.use_rgb ? 1 : 0));
-    const int nblk = track_blocks(nrows, c->track_rpt[l]);
+    const int nblk = track_blocks(p.w * p.h * kinds, c->track_rpt[l]);
     for (int it = 0; it < c->cfg.iters[l]; ++it) {
       p.iter = it;
       Timed t(c, KC_TRACK);
-      launch_track(c->stream, p, c->inst + first_inst, n, c->gn, c->partial, nblk, sy);
+      launch_gn_iter(c->stream, p, c->inst + first_inst, n, c->gn, c->partial, nblk, sy);
     }
   }
 }
@@ -237,8 +254,9 @@ void set_inst(mf_ctx* c, int slot, float* const mv[3], float* const mn[3], float
               int8_t* codes, int state) {
   TrackInst ti;
   memset(&ti, 0, sizeof ti);
-  for (int k = 0; k < 3; k++) { ti.mv[k] = mv[k]; ti.mn[k] = mn[k]; }
-  for (int l = 0; l < MAX_LEVELS; l++) ti.rD[l] = rD[l];
+  ti.mvn = mv[0];      // planes: mv[0..2], mn[0..2] back to back
+  ti.rD = rD[0];
+  (void)mn;
   ti.codes = codes; ti.state = state;
   // descriptors only change with the object set: upload on change, ordered
   // after every pending launch that may still read the old table
@@ -290,7 +308,10 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
     ok = ok && alloc_level(c, c->live[l], c->W >> l, c->H >> l);
     ok = ok && alloc_level(c, c->ref[l], c->W >> l, c->H >> l);
   }
-  for (int k = 0; k < 3; k++) { c->vtx[k] = dalloc<float>(c, c->N0); c->nrm[k] = dalloc<float>(c, c->N0); }
+  {
+    float* b = dalloc<float>(c, 6 * c->N0);   // vertex planes then normal planes
+    for (int k = 0; k < 3; k++) { c->vtx[k] = b ? b + k * c->N0 : nullptr; c->nrm[k] = b ? b + (3 + k) * c->N0 : nullptr; }
+  }
   c->label = dalloc<int32_t>(c, c->N0);
   c->mats = dalloc<Mat4>(c, 3 + MAX_INST);
   c->gn = dalloc<GNState>(c, MAX_INST);
@@ -646,10 +667,11 @@ long mf_get_trace(mf_ctx* c, double* out, long cap) {
   std::vector<TraceRec> r(std::max(n, 1));
   hipMemcpy(r.data(), c->trace, n * sizeof(TraceRec), hipMemcpyDeviceToHost);
   for (long i = 0; i < n && i < cap; i++) {
-    double* d = out + 24 * i;
+    double* d = out + 25 * i;
     d[0] = r[i].inst; d[1] = r[i].level; d[2] = r[i].iter; d[3] = r[i].err; d[4] = r[i].s28; d[5] = r[i].flag;
     for (int j = 0; j < 16; j++) d[6 + j] = r[i].pose[j];
     d[22] = r[i].t_accum * 0.01; d[23] = r[i].t_solve * 0.01;   // microseconds
+    d[24] = r[i].solver;
   }
   return n;
 }

@@ -46,42 +46,49 @@ struct ObjParams {
   Mat4 view;                // T_w_c * getInverseCameraMatrix(k) (raycast)
 };
 
+// Per-iteration GN state.
+struct GNIter {
+  Mat4 T;          // camera: T_w_l; object: T_c2_o2
+  Mat4 invT;       // inverse(T)
+  Mat4 prev;       // previous_pose (carried across levels)
+  float prev_err;
+  int done_level;  // pyramid level at which the GN loop broke, -1 if none
+  int pend_level;  // level / iteration of the rows being solved
+  int pend_iter;
+};
+
 // Gauss-Newton state of one tracked instance (device resident).
 struct GNState {
-  Mat4 T;          // camera: T_w_l; object: T_c2_o2
-  Mat4 invT;       // inverse(T), refreshed with every update of T
-  Mat4 prev;       // previous_pose (carried across levels)
+  GNIter it[1];    // (one slot: updated only by the solve kernel)
   Mat4 A;          // camera: T_w_r;            object: T_c1_o1
   Mat4 B;          // camera: K * inverse(T_w_r); object: inverse(T_w_o1)
   Mat4 C;          // object: inverse(T_c1_o1)
   Mat4 T_wo1;      // object: pose at the previous frame
-  float prev_err;
-  int done_level;  // pyramid level at which the GN loop broke, -1 if none
   int iters_run;
   int object;      // 1: object instance (zero-inlier break, pose composed with T_w_c2)
   Mat4* out_pose;  // written after every update: T_w_l, or volume_pose = T_w_c2 * T_c2_o2
   const Mat4* T_w_c2;
-  unsigned t_accum; // 100 MHz ticks of the last FP64 accumulation
-  unsigned pad2;
   double sums[32]; // last reduction (FP64)
 };
 
 // Per-instance inputs of one tracking pass.
 struct TrackInst {
-  const float* mv[3];   // model vertex planes (world frame), full res
-  const float* mn[3];   // model normal planes
-  const float* rD[MAX_LEVELS];   // rendered reference depth pyramid
+  const float* mvn;     // model vertex (planes 0-2, world frame) and normal (3-5) maps, full res
+  const float* rD;      // rendered reference depth pyramid, levels back to back
   int8_t* codes;        // level-0 result codes out [2*W*H] (ICP, RGB) or null
   int state;            // index into GNState array
   int pad;
 };
 
-// Pyramid buffers of one frame (live or reference).
+// Pyramid buffers of one frame (live or reference) at one level: 13 planes
+// of w*h floats in one allocation, in the order of the PL_* indices.
+enum { PL_D = 0, PL_I = 1, PL_V = 2, PL_N = 5, PL_GX = 8, PL_GY = 9, PL_COV = 10, PL_COUNT = 13 };
 struct Level {
   int w, h;
   float *D, *I, *V[3], *N[3], *gx, *gy, *cov[3];
 };
 
-struct TraceRec { int inst, level, iter, flag; float err, s28; float pose[16]; unsigned t_accum, t_solve; };   // t_*: 100 MHz ticks
+// t_*: 100 MHz ticks; solver: 0 none, 1 Gauss-Jordan (certified), 2 Jacobi pseudo-inverse
+struct TraceRec { int inst, level, iter, flag; float err, s28; float pose[16]; unsigned t_accum, t_solve; int solver, pad; };
 
 }  // namespace mf

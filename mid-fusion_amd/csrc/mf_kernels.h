@@ -22,7 +22,10 @@ struct TrackLaunch {
   int robust; double robust_k;
   float fx, fy, cx, cy;                // full-resolution intrinsics
   float inv_cov_rgb;                   // 1/sigma_bright
-  Level live, ref;                     // live pyramid level, reference (previous frame) level
+  const float* live;                   // live pyramid level planes (PL_*), stride lstride
+  const float* ref;                    // reference (previous frame) level planes
+  int lstride;                         // w*h
+  int rd_off;                          // offset of this level in a rendered-depth pyramid
   const int32_t* label;                // camera re-track: outlier mask from raycast labels
   const uint8_t* moving;               // per-object moving flags (label-indexed)
   int n_labels;
@@ -35,8 +38,11 @@ struct TrackSync {             // accumulation + solve of one launch
   int trace_cap;
 };
 int track_blocks(int nrows, int rows_per_thread);
-void launch_track(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
-                  float* partial, int nblk, const TrackSync& sync);
+
+// One GN iteration of `ninst` instances: rows into `partial` (nblk per
+// instance), then one solve workgroup per instance updates its state.
+void launch_gn_iter(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                    float* partial, int nblk, const TrackSync& sy);
 // Frame start for the camera: T_w_r = T_w_c, inverse(T_w_r), GN state 0.
 void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy);
 // GN state of a camera pass starting from the current camera pose.

@@ -18,21 +18,42 @@ __device__ __forceinline__ int32_t vslot(const VolDev& v, int x, int y, int z) {
   if (x < 0 || y < 0 || z < 0 || x >= v.res || y >= v.res || z >= v.res) return -1;
   return v.slot[morton3((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3)];
 }
-__device__ __forceinline__ float tsdf_at(const VolDev& v, int x, int y, int z) {
-  const int32_t s = vslot(v, x, y, z);
-  if (s < 0) return 1.f;
-  return v.tsdf[(size_t)s * BLOCK_VOX + (x & 7) + 8 * (y & 7) + 64 * (z & 7)];
+
+// One-entry block cache: consecutive voxel reads of a ray (march steps, the
+// 8 interpolation corners, the gradient stencil) mostly stay in one 8^3
+// block, so the block-table probe is skipped and each read is one load.
+// Block coordinates are packed 10 bits each (res <= 8192, checked at
+// volume creation).
+struct VCache {
+  int key = -1;
+  int32_t slot = -1;
+};
+__device__ __forceinline__ int32_t cslot(const VolDev& v, VCache& c, int x, int y, int z) {
+  if ((unsigned)x >= (unsigned)v.res || (unsigned)y >= (unsigned)v.res || (unsigned)z >= (unsigned)v.res) return -1;
+  const int key = ((x >> 3) << 20) | ((y >> 3) << 10) | (z >> 3);
+  if (key != c.key) {
+    c.key = key;
+    c.slot = v.slot[morton3((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3)];
+  }
+  return c.slot;
 }
-__device__ __forceinline__ float fg_at(const VolDev& v, int x, int y, int z) {
-  const int32_t s = vslot(v, x, y, z);
-  if (s < 0) return 0.f;
-  return (float)v.fg[(size_t)s * BLOCK_VOX + (x & 7) + 8 * (y & 7) + 64 * (z & 7)];
+__device__ __forceinline__ size_t vox(int32_t s, int x, int y, int z) {
+  return (size_t)s * BLOCK_VOX + (x & 7) + 8 * (y & 7) + 64 * (z & 7);
+}
+// missing voxels read as voxel_traits<SDF>::empty(): tsdf 1, fg 0
+__device__ __forceinline__ float tsdf_at(const VolDev& v, VCache& c, int x, int y, int z) {
+  const int32_t s = cslot(v, c, x, y, z);
+  return s < 0 ? 1.f : v.tsdf[vox(s, x, y, z)];
+}
+__device__ __forceinline__ float fg_at(const VolDev& v, VCache& c, int x, int y, int z) {
+  const int32_t s = cslot(v, c, x, y, z);
+  return s < 0 ? 0.f : (float)v.fg[vox(s, x, y, z)];
 }
 
 // Octree::interp (octree.hpp:479-501): voxel values at integer corners, base
 // clamped at 0, fraction of the unclamped position.
 template <bool FG>
-__device__ __forceinline__ float interp(const VolDev& v, f3 pos) {
+__device__ __forceinline__ float interp(const VolDev& v, VCache& c, f3 pos) {
   const float flx = floorf(pos.x), fly = floorf(pos.y), flz = floorf(pos.z);
   const int bx = (int)flx, by = (int)fly, bz = (int)flz;
   const float fx = pos.x - flx, fy = pos.y - fly, fz = pos.z - flz;
@@ -40,14 +61,14 @@ __device__ __forceinline__ float interp(const VolDev& v, f3 pos) {
   float p[8];
 #pragma unroll
   for (int i = 0; i < 8; i++)
-    p[i] = FG ? fg_at(v, lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1))
-              : tsdf_at(v, lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1));
+    p[i] = FG ? fg_at(v, c, lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1))
+              : tsdf_at(v, c, lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1));
   return (((p[0] * (1 - fx) + p[1] * fx) * (1 - fy) + (p[2] * (1 - fx) + p[3] * fx) * fy) * (1 - fz) +
           ((p[4] * (1 - fx) + p[5] * fx) * (1 - fy) + (p[6] * (1 - fx) + p[7] * fx) * fy) * fz);
 }
 
 // Octree::grad(pos, select) (octree.hpp:593-680), tsdf field.
-__device__ f3 grad(const VolDev& v, f3 pos) {
+__device__ f3 grad(const VolDev& v, VCache& cc, f3 pos) {
   const float fl[3] = {floorf(pos.x), floorf(pos.y), floorf(pos.z)};
   const int b[3] = {(int)fl[0], (int)fl[1], (int)fl[2]};
   const float f[3] = {pos.x - fl[0], pos.y - fl[1], pos.z - fl[2]};
@@ -70,7 +91,7 @@ __device__ f3 grad(const VolDev& v, f3 pos) {
         if (c == a) { hi[c] = id[c] ? uu[c] : ul[c]; lo[c] = id[c] ? lu[c] : ll[c]; }
         else { hi[c] = lo[c] = id[c] ? ul[c] : lu[c]; }
       }
-      D[q] = tsdf_at(v, hi[0], hi[1], hi[2]) - tsdf_at(v, lo[0], lo[1], lo[2]);
+      D[q] = tsdf_at(v, cc, hi[0], hi[1], hi[2]) - tsdf_at(v, cc, lo[0], lo[1], lo[2]);
     }
     g[a] = (((D[0] * (1 - f[0]) + D[1] * f[0]) * (1 - f[1]) + (D[2] * (1 - f[0]) + D[3] * f[0]) * f[1]) * (1 - f[2]) +
             ((D[4] * (1 - f[0]) + D[5] * f[0]) * (1 - f[1]) + (D[6] * (1 - f[0]) + D[7] * f[0]) * f[1]) * f[2]);
@@ -81,15 +102,21 @@ __device__ f3 grad(const VolDev& v, f3 pos) {
 
 // Child occupancy of an octree node: level `lv` (1 = children of the root),
 // Morton prefix `pre` of the child.
-__device__ __forceinline__ bool child_exists(const VolDev& v, int lv, uint64_t pre) {
+__device__ __forceinline__ bool child_exists(const VolDev& v, int lv, uint32_t pre) {
   if (lv == v.leaf_level) return v.slot[pre] >= 0;
   return v.nodes[v.node_off[lv] + pre] != 0;
 }
 
 // ray_iterator::next() first call (octree.hpp:923-1093): entry distance, in
 // voxel units, of the first allocated leaf block along the ray; -1 if none.
-// Same float operations as the reference traversal over [1,2]^3.
-__device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, float farp, float eps) {
+// Same float operations as the reference traversal over [1,2]^3.  The
+// traversal stack lives in LDS, one column per thread (`stride` apart): entry
+// scale - min_scale - 1 holds the parent's Morton prefix (bit 31: pushed) and
+// its t_max, for the leaf_level + 1 scales a push or pop can reach.
+static constexpr int RT = 256;          // raycast threads per workgroup
+static constexpr int MAX_LG = 10;       // leaf level <= 10: res <= 8192
+__device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, float farp, float eps,
+                                 uint32_t* __restrict__ st_par, float* __restrict__ st_tmax) {
   constexpr int DEPTH = 23;
   const int lg = v.leaf_level;                   // log2(res/8)
   const int min_scale = DEPTH - lg;
@@ -115,12 +142,9 @@ __device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, fl
     if (1.5f * t_coef[a] - t_bias[a] > t_min) { idx ^= (1 << a); pos[a] = 1.5f; }
   int scale = DEPTH - 1;
   float scale_exp2 = 0.5f;
-  uint64_t parent = 0;          // Morton prefix of the parent node (root: 0)
+  uint32_t parent = 0;          // Morton prefix of the parent node (root: 0)
   bool parent_ok = true;
-  uint64_t st_parent[DEPTH + 1];
-  float st_tmax[DEPTH + 1];
-  bool st_ok[DEPTH + 1];
-  for (int i = 0; i <= DEPTH; i++) st_ok[i] = false;
+  for (int i = 0; i <= lg; i++) st_par[i * RT] = 0u;
   float t_corner[3] = {0, 0, 0}, tc_max = 0.f;
   const float vs = v.size / (float)v.res;
   for (int guard = 0; scale < DEPTH; guard++) {
@@ -129,7 +153,7 @@ __device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, fl
     tc_max = fmin_r(fmin_r(t_corner[0], t_corner[1]), t_corner[2]);
     const int cidx = idx ^ octant_mask ^ 7;
     const int clevel = DEPTH - scale;            // level of the child examined
-    const uint64_t child = (parent << 3) | (uint64_t)cidx;
+    const uint32_t child = (parent << 3) | (uint32_t)cidx;
     const bool has_child = child_exists(v, clevel, child);
     if (scale == min_scale && has_child) return t_min * v.size / vs;
     if (has_child && t_min <= t_max) {
@@ -138,7 +162,11 @@ __device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, fl
       const float half = scale_exp2 * 0.5f;
       float t_center[3];
       for (int a = 0; a < 3; a++) t_center[a] = half * t_coef[a] + t_corner[a];
-      if (tc_max < h) { st_parent[scale] = parent; st_tmax[scale] = t_max; st_ok[scale] = parent_ok; }
+      if (tc_max < h) {
+        const int e = (scale - min_scale - 1) * RT;   // scale > min_scale here
+        st_par[e] = parent | (parent_ok ? 0x80000000u : 0u);
+        st_tmax[e] = t_max;
+      }
       h = tc_max;
       parent = child;
       scale--;
@@ -159,10 +187,15 @@ __device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, fl
         if (step_mask & (1 << a)) diff |= (unsigned)(__float_as_int(pos[a]) ^ __float_as_int(pos[a] + scale_exp2));
       scale = (__float_as_int((float)diff) >> 23) - 127;
       scale_exp2 = __int_as_float((scale - DEPTH + 127) << 23);
-      if (scale < 0 || scale > DEPTH) return -1.f;
-      parent = st_parent[scale];
-      t_max = st_tmax[scale];
-      parent_ok = st_ok[scale];
+      if (scale <= min_scale || scale > DEPTH) return -1.f;
+      if (scale == DEPTH) return -1.f;            // popped past the root
+      {
+        const int e = (scale - min_scale - 1) * RT;
+        const uint32_t sp = st_par[e];
+        parent = sp & 0x7fffffffu;
+        parent_ok = (sp >> 31) != 0;
+        t_max = st_tmax[e];
+      }
       int sh[3];
       for (int a = 0; a < 3; a++) { sh[a] = __float_as_int(pos[a]) >> scale; pos[a] = __int_as_float(sh[a] << scale); }
       idx = (sh[0] & 1) | ((sh[1] & 1) << 1) | ((sh[2] & 1) << 2);
@@ -191,15 +224,16 @@ __device__ float march(const VolDev& v, f3 origin, f3 dir, float nearp, float fa
   if (!(tnear < tfar)) return 0.f;
   float t = tnear, stepsize = largestep;
   f3 p = origin + dir * t;
-  float f_t = interp<false>(v, p * inv_vs);
+  VCache c;
+  float f_t = interp<false>(v, c, p * inv_vs);
   ninterp++;
   float f_tt = 0;
   if (f_t > 0) {
     for (; t < tfar; t += stepsize) {
       const f3 q = p * inv_vs;
       nsteps++;
-      f_tt = tsdf_at(v, (int)q.x, (int)q.y, (int)q.z);
-      if (f_tt < 0.1f && f_tt >= -0.5f) { f_tt = interp<false>(v, q); ninterp++; }   // (double)f <= 0.1
+      f_tt = tsdf_at(v, c, (int)q.x, (int)q.y, (int)q.z);
+      if (f_tt < 0.1f && f_tt >= -0.5f) { f_tt = interp<false>(v, c, q); ninterp++; }   // (double)f <= 0.1
       if (f_tt < 0) break;
       stepsize = fmax_r(f_tt * mu, step);
       p = p + stepsize * dir;
@@ -214,13 +248,15 @@ __device__ float march(const VolDev& v, f3 origin, f3 dir, float nearp, float fa
   return 0.f;
 }
 
-__global__ __launch_bounds__(256) void k_raycast(const RayObj* __restrict__ objs, int nobj, int w, int h, float mu,
+__global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs, int nobj, int w, int h, float mu,
                                                  float nearp, float farp, float eps, float* __restrict__ vx,
                                                  float* __restrict__ vy, float* __restrict__ vz,
                                                  float* __restrict__ nx, float* __restrict__ ny,
                                                  float* __restrict__ nz, int32_t* __restrict__ label,
                                                  unsigned long long* __restrict__ stats) {
   __shared__ unsigned cnt[4];
+  __shared__ uint32_t st_par[MAX_LG + 1][RT];
+  __shared__ float st_tmax[MAX_LG + 1][RT];
   if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
   __syncthreads();
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
@@ -243,7 +279,7 @@ __global__ __launch_bounds__(256) void k_raycast(const RayObj* __restrict__ objs
     const f3 wd = rotate(P.view, mk3((float)x, (float)y, 1.f));
     const f3 od = normalize(rotate(P.inv_pose, wd));
     const float nfar = fmin_r(nearest, farp);
-    const float tmin = ray_first_block(v, oo, od, nearp, nfar, eps);
+    const float tmin = ray_first_block(v, oo, od, nearp, nfar, eps, &st_par[0][threadIdx.x], &st_tmax[0][threadIdx.x]);
     f3 hit = mk3(0.f, 0.f, 0.f);
     float hw = 0.f;
     npairs++;
@@ -251,7 +287,8 @@ __global__ __launch_bounds__(256) void k_raycast(const RayObj* __restrict__ objs
     const float hd = (O.label == 0) ? hw + 0.05f : hw;
     if (hd > nearest) { miss++; continue; }
     const float inv_vs = (float)v.res / v.size;
-    const double fg = (double)interp<true>(v, hit * inv_vs);
+    VCache c;
+    const double fg = (double)interp<true>(v, c, hit * inv_vs);
     ninterp++;
     if (fg <= 0.5f) { miss++; continue; }
     bool take = false;
@@ -261,7 +298,7 @@ __global__ __launch_bounds__(256) void k_raycast(const RayObj* __restrict__ objs
     high_fg = fg;
     out_l = O.label;
     out_v = affine(P.pose, hit);
-    const f3 g = grad(v, hit * inv_vs);
+    const f3 g = grad(v, c, hit * inv_vs);
     out_n = (length(g) == 0) ? mk3((float)INVALID, 0.f, 0.f) : normalize(rotate(P.pose, g));
     nhits++;
   }
@@ -287,7 +324,7 @@ __global__ __launch_bounds__(256) void k_raycast(const RayObj* __restrict__ objs
 void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,
                     float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label,
                     unsigned long long* stats) {
-  dim3 blk(256), grd((w + 255) / 256, h);
+  dim3 blk(RT), grd((w + RT - 1) / RT, h);
   k_raycast<<<grd, blk, 0, s>>>(objs, nobj, w, h, mu, nearp, farp, eps, vx, vy, vz, nx, ny, nz, label, stats);
 }
 

@@ -1,16 +1,16 @@
 // ICP + photometric Gauss-Newton tracking on gfx950.
 //
-// One fused kernel per GN iteration computes every pixel's ICP and RGB rows
+// One kernel per GN iteration computes every pixel's ICP and RGB rows
 // (Tracking::track_ICP_Kernel / trackRGB / track_Obj_ICP_Kernel /
 // track_obj_RGB_kernel, tracking.cpp:392-578, 923-1111) in registers and
 // reduces the 32 robust-weighted sums (Tracking::reduce, tracking.cpp:
-// 1208-1352) per workgroup through an LDS transpose; rows never reach HBM.
-// A one-workgroup-per-instance finalize kernel accumulates the workgroup
-// partials in FP64, solves the 6x6 normal equations in FP64 and applies the
-// reference GN control flow (step-back, break-without-update, zero-inlier
-// break; tracking.cpp:300-334, 664-688) on the device, so a whole pyramid of
-// iterations is enqueued without host synchronisation.  A launch covers any
-// number of instances (camera or objects) through its grid's y dimension.
+// 1208-1352) per workgroup with a register butterfly; rows never reach HBM.
+// A one-workgroup-per-instance kernel then accumulates the workgroup partials
+// in FP64, solves the 6x6 normal equations in FP64 and applies the reference
+// GN control flow (step-back, break-without-update, zero-inlier break;
+// tracking.cpp:300-334, 664-688), so a whole pyramid of iterations is
+// enqueued without host synchronisation.  A launch covers any number of instances (camera or objects)
+// through its grid's y dimension.
 #include "mf_kernels.h"
 
 namespace mf {
@@ -18,7 +18,21 @@ namespace mf {
 static constexpr int TB = 256;   // threads per tracking workgroup (4 waves)
 static constexpr int NW = TB / 64;
 
-__device__ __forceinline__ f3 ld3(const float* const* p, size_t i) { return mk3(p[0][i], p[1][i], p[2][i]); }
+// Level planes (live / reference pyramid level, stride lstride) and the
+// instance's model vertex (planes 0-2) and normal (planes 3-5) maps.
+__device__ __forceinline__ f3 lv3(const TrackLaunch& p, int pl, size_t i) {
+  const float* b = p.live + (size_t)pl * p.lstride + i;
+  return mk3(b[0], b[p.lstride], b[2 * (size_t)p.lstride]);
+}
+__device__ __forceinline__ f3 rf3(const TrackLaunch& p, int pl, size_t i) {
+  const float* b = p.ref + (size_t)pl * p.lstride + i;
+  return mk3(b[0], b[p.lstride], b[2 * (size_t)p.lstride]);
+}
+__device__ __forceinline__ f3 mdl3(const TrackLaunch& p, const TrackInst& in, int pl, size_t i) {
+  const size_t n = (size_t)p.W * p.H;
+  const float* b = in.mvn + pl * n + i;
+  return mk3(b[0], b[n], b[2 * n]);
+}
 
 // bilinear_interp<T> (utils/math_utils.h:149-166)
 __device__ __forceinline__ float bilin(const float* __restrict__ f, int w, float px, float py) {
@@ -29,7 +43,7 @@ __device__ __forceinline__ float bilin(const float* __restrict__ f, int w, float
   const float a = f[i0], b = f[i0 + 1], c = f[i0 + w], d = f[i0 + w + 1];
   return a * (1.f - fx) * (1.f - fy) + b * (1.f - fy) * fx + c * (1.f - fx) * fy + d * fx * fy;
 }
-__device__ __forceinline__ f3 bilin3(const float* const* f, int w, float px, float py) {
+__device__ __forceinline__ f3 bilin3(const float* f0, size_t stride, int w, float px, float py) {
   const float fpx = floorf(px), fpy = floorf(py);
   const int bx = (int)fpx, by = (int)fpy;
   const float fx = px - fpx, fy = py - fpy;
@@ -38,7 +52,7 @@ __device__ __forceinline__ f3 bilin3(const float* const* f, int w, float px, flo
   f3 r;
   float* rr = &r.x;
   for (int k = 0; k < 3; k++) {
-    const float* p = f[k];
+    const float* p = f0 + k * stride;
     rr[k] = p[i0] * w0 * w1 + p[i0 + 1] * w1 * fx + p[i0 + w] * w0 * fy + p[i0 + w + 1] * fx * fy;
   }
   return r;
@@ -71,30 +85,32 @@ __device__ __forceinline__ void robust(float e, int fn, double k, float& w, floa
   }
 }
 
-// Row functions read the pose matrices straight from the GN state: the
-// address is uniform per workgroup, so they live in scalar registers.
-using Mats = GNState;
+// Pose matrices a row reads, staged in LDS by the workgroup and re-read from
+// LDS (broadcast) inside the row loop: holding 60 uniform floats in scalar
+// registers spills them, and in vector registers costs occupancy.
+struct Mats { Mat4 T, invT, A, B, C; };
 
 // ICP row (camera), track_ICP_Kernel mask overload (tracking.cpp:923-1020)
-__device__ __forceinline__ int icp_row(const TrackLaunch& p, const TrackInst& in, const Mats& M, int x, int y,
+template <class MT>
+__device__ __forceinline__ int icp_row(const TrackLaunch& p, const TrackInst& in, const MT& M, int x, int y,
                                        size_t i, bool masked, float& e, float J[6]) {
   if (masked) return -6;
-  const f3 nl = mk3(p.live.N[0][i], p.live.N[1][i], p.live.N[2][i]);
+  const f3 nl = lv3(p, PL_N, i);
   if (nl.x == INVALID) return -1;
-  const f3 vl = mk3(p.live.V[0][i], p.live.V[1][i], p.live.V[2][i]);
+  const f3 vl = lv3(p, PL_V, i);
   const f3 pv = affine(M.T, vl);
   const f3 pp = affine(M.B, pv);
   const float ux = pp.x / pp.z, uy = pp.y / pp.z;
   if (ux < 1 || ux > (float)(p.W - 2) || uy < 1 || uy > (float)(p.H - 2) || isnan(ux) || isnan(uy)) return -2;
   const unsigned rx = (unsigned)ux, ry = (unsigned)uy;
   const size_t ri = rx + (size_t)ry * p.W;
-  const f3 rn = ld3(in.mn, ri);
+  const f3 rn = mdl3(p, in, 3, ri);
   if (rn.x == INVALID) return -3;
-  const f3 diff = ld3(in.mv, ri) - pv;
+  const f3 diff = mdl3(p, in, 0, ri) - pv;
   const f3 pn = rotate(M.T, nl);
   if (length(diff) > 0.1f) return -4;
   if (dot(pn, rn) < 0.8f) return -5;
-  const f3 P = mk3(p.live.cov[0][i], p.live.cov[1][i], p.live.cov[2][i]);
+  const f3 P = lv3(p, PL_COV, i);
   const float sig = rn.x * rn.x * P.x + rn.y * rn.y * P.y + rn.z * rn.z * P.z;
   const float w = sqrtf((float)(1.0 / (double)sig));
   e = w * dot(rn, diff);
@@ -104,11 +120,12 @@ __device__ __forceinline__ int icp_row(const TrackLaunch& p, const TrackInst& in
 }
 
 // Object ICP row, track_Obj_ICP_Kernel (tracking.cpp:392-486)
-__device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst& in, const Mats& M, int x, int y,
+template <class MT>
+__device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst& in, const MT& M, int x, int y,
                                            size_t i, float& e, float J[6]) {
-  const f3 nl = mk3(p.live.N[0][i], p.live.N[1][i], p.live.N[2][i]);
+  const f3 nl = lv3(p, PL_N, i);
   if (nl.x == INVALID) return -1;
-  const f3 c2v = mk3(p.live.V[0][i], p.live.V[1][i], p.live.V[2][i]);
+  const f3 c2v = lv3(p, PL_V, i);
   if (c2v.x == 0 && c2v.y == 0 && c2v.z == 0) return -1;
   const f3 o2v = affine(M.invT, c2v);
   const f3 c1v = affine(M.A, o2v);
@@ -116,8 +133,8 @@ __device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst
   const float ux = pj.x / pj.z, uy = pj.y / pj.z;
   if (ux < 1 || ux > (float)(p.W - 2) || uy < 1 || uy > (float)(p.H - 2) || isnan(ux) || isnan(uy)) return -2;
   const f3 o2n = rotate(M.invT, nl);
-  const f3 wrv = bilin3(in.mv, p.W, ux, uy);
-  const f3 wrn = bilin3(in.mn, p.W, ux, uy);
+  const f3 wrv = bilin3(in.mvn, (size_t)p.W * p.H, p.W, ux, uy);
+  const f3 wrn = bilin3(in.mvn + (size_t)3 * p.W * p.H, (size_t)p.W * p.H, p.W, ux, uy);
   if (wrn.x == INVALID) return -3;
   const f3 o1rv = affine(M.B, wrv);
   const f3 o1rn = rotate(M.B, wrn);
@@ -125,7 +142,7 @@ __device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst
   const float residual = dot(o1rn, diff);
   if (length(diff) > 0.1f) return -4;
   if (dot(o2n, o1rn) < 0.8f) return -5;
-  const f3 P = mk3(p.live.cov[0][i], p.live.cov[1][i], p.live.cov[2][i]);
+  const f3 P = lv3(p, PL_COV, i);
   const float sig = o1rn.x * o1rn.x * P.x + o1rn.y * o1rn.y * P.y + o1rn.z * o1rn.z * P.z;
   const float w = sqrtf((float)(1.0 / (double)sig));
   e = w * residual;
@@ -138,14 +155,14 @@ __device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst
 
 // Photometric rows: trackRGB (tracking.cpp:1023-1111) for the camera,
 // track_obj_RGB_kernel (tracking.cpp:491-578) for objects.
-template <bool OBJ>
-__device__ __forceinline__ int rgb_row(const TrackLaunch& p, const TrackInst& in, const Mats& M, int x, int y,
+template <bool OBJ, class MT>
+__device__ __forceinline__ int rgb_row(const TrackLaunch& p, const TrackInst& in, const MT& M, int x, int y,
                                        size_t i, float& e, float J[6]) {
   const float s = (float)(1 << p.level);
   const float fx = p.fx / s, fy = p.fy / s, cx = p.cx / s, cy = p.cy / s;
-  const float dr = in.rD[p.level][i];
+  const float dr = in.rD[p.rd_off + i];
   const f3 vr = dr > 0.0f ? dr * rotate(camKinv(fx, fy, cx, cy), mk3((float)x, (float)y, 1.0f)) : mk3(0.f, 0.f, 0.f);
-  const f3 vl = mk3(p.ref.V[0][i], p.ref.V[1][i], p.ref.V[2][i]);
+  const f3 vl = rf3(p, PL_V, i);
   if (vr.z <= 0.f || vr.z == INVALID) return -1;
   if (length(vr - vl) > 0.1f) { if (vl.z > 0.f) return -3; }
   f3 lv, wv;
@@ -163,9 +180,10 @@ __device__ __forceinline__ int rgb_row(const TrackLaunch& p, const TrackInst& in
   } else {
     if (ux < 1 || ux > (float)(p.w - 2) || uy < 1 || uy > (float)(p.h - 2) || isnan(ux) || isnan(uy)) return -2;
   }
-  const float li = bilin(p.live.I, p.w, ux, uy);
-  const float res = p.ref.I[i] - li;
-  const float g_x = bilin(p.live.gx, p.w, ux, uy), g_y = bilin(p.live.gy, p.w, ux, uy);
+  const float li = bilin(p.live + (size_t)PL_I * p.lstride, p.w, ux, uy);
+  const float res = p.ref[(size_t)PL_I * p.lstride + i] - li;
+  const float g_x = bilin(p.live + (size_t)PL_GX * p.lstride, p.w, ux, uy),
+              g_y = bilin(p.live + (size_t)PL_GY * p.lstride, p.w, ux, uy);
   const float sc = 1.f / lv.z;
   f3 jt = sc * mk3(g_x * fx, g_y * fy, -(g_x * lv.x * fx + g_y * lv.y * fy) / lv.z);
   f3 jr;
@@ -221,29 +239,25 @@ __device__ __forceinline__ bool dyn_masked(const TrackLaunch& p, int x, int y) {
   return m;
 }
 
-template <bool ICP, bool RGB, bool OBJ>
-__global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __restrict__ insts,
-                                              const GNState* __restrict__ st, float* __restrict__ partial,
-                                              int nblk) {
-  __shared__ float red[NW][32];
-  const TrackInst& in = insts[blockIdx.y];
-  const GNState& S = st[in.state];
-  // a level starts afresh (tracking.cpp:246); later iterations are skipped
-  // once this level's loop has broken (uniform per workgroup)
-  if (p.iter > 0 && S.done_level == p.level) return;
+// Rows of one instance at one level over the workgroups [0, nactive): FP32
+// per-thread sums, the wave butterfly, and this workgroup's 32 partial sums
+// in `part` (32 floats).
+template <bool ICP, bool RGB, bool OBJ, class MT>
+__device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst& in, const MT& M, int state,
+                                           int wg, int nactive, float* __restrict__ part, float (*red)[32]) {
   const int tid = threadIdx.x;
-  const Mats& M = S;
   float s[32];
 #pragma unroll
   for (int k = 0; k < 32; k++) s[k] = 0.f;
   const int n = p.w * p.h;
   const int nrows = (ICP ? n : 0) + (RGB ? n : 0);
   const size_t N0 = (size_t)p.W * p.H;
-  TrackRow* dbg = p.dbg ? p.dbg + ((size_t)in.state * p.n_levels + p.level) * 2 * N0 : nullptr;
+  TrackRow* dbg = p.dbg ? p.dbg + ((size_t)state * p.n_levels + p.level) * 2 * N0 : nullptr;
   // rows [0, n) are ICP rows, [n, 2n) photometric rows of the same pixels
   // (the order of TrackData in the reference); n is a multiple of 64 at
   // every level of a 640x480 or 320x240 pyramid, so waves do not mix kinds
-  for (int r = blockIdx.x * TB + tid; r < nrows; r += nblk * TB) {
+  for (int r = wg * TB + tid; r < nrows; r += nactive * TB) {
+    asm volatile("" ::: "memory");   // keep the LDS matrix reads inside the loop
     const bool icp = ICP && (!RGB || r < n);
     const int i = (ICP && !icp) ? r - n : r;
     const int x = i % p.w, y = i / p.w;
@@ -288,115 +302,102 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
     float a = 0.f;
 #pragma unroll
     for (int w = 0; w < NW; w++) a += red[w][tid];
-    partial[((size_t)blockIdx.y * nblk + blockIdx.x) * 32 + tid] = a;
+    part[tid] = a;
   }
+  __syncthreads();   // red is reused by the caller's next instance
 }
 
 // ---- FP64 6x6 solve (commons.h:672-696 semantics: TooN GR_SVD backsub with
-// condition 1e6).  LDLT when tr(C)*tr(C^-1) < 1e6 certifies that no singular
-// value is truncated; otherwise a Jacobi eigendecomposition pseudo-inverse.
-__device__ void jacobi_pinv(const double C[6][6], const double b[6], double x[6]) {
-  double A[6][6], V[6][6];
-  #pragma unroll
-  for (int i = 0; i < 6; i++) for (int j = 0; j < 6; j++) { A[i][j] = C[i][j]; V[i][j] = (i == j); }
+// condition 1e6).  C is the symmetric JtWJ of the reduction, b its JtWe.
+// Wave 0 runs Gauss-Jordan on [C | I | b]; when every pivot is positive and
+// tr(C)*tr(C^-1) < 1e6 (so lambda_max/lambda_min < 1e6 and backsub would
+// truncate nothing), x = C^-1 b.  Otherwise lane 0 computes the pseudo-inverse
+// from a cyclic Jacobi eigendecomposition with the reference's truncation
+// rule.  All work arrays live in LDS, so the solve adds few registers to the
+// kernels that embed it.
+struct SolveLDS {
+  double M[6][13];           // [C | I | b] during Gauss-Jordan
+  double A[6][6], V[6][6];   // Jacobi work arrays
+  double x[6];
+  int ok;
+};
+__device__ __forceinline__ int tri(int i, int j) {   // packed upper-triangular index of the 21 JtWJ sums
+  if (i > j) { const int t = i; i = j; j = t; }
+  return 6 + 6 * i - (i * (i - 1)) / 2 + (j - i);
+}
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+// s27 = sums[1..27] (b then the packed JtWJ); called by the 64 lanes of one wave
+__device__ void gj_solve(SolveLDS& W, const double* s27, int lane) {
+  for (int e = lane; e < 78; e += 64) {
+    const int i = e / 13, j = e % 13;
+    W.M[i][j] = j < 6 ? s27[tri(i, j)] : (j < 12 ? (j - 6 == i ? 1.0 : 0.0) : s27[i]);
+  }
+  wave_sync();
+  bool ok = true;
+  for (int k = 0; k < 6; k++) {
+    const double pk = W.M[k][k];
+    ok = ok && (pk > 0.0);
+    double nv[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int e = lane + 64 * u;
+      nv[u] = 0.0;
+      if (e < 78) {
+        const int i = e / 13, j = e % 13;
+        const double r = W.M[k][j] / pk;
+        nv[u] = (i == k) ? r : W.M[i][j] - W.M[i][k] * r;
+      }
+    }
+    wave_sync();
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int e = lane + 64 * u;
+      if (e < 78) W.M[e / 13][e % 13] = nv[u];
+    }
+    wave_sync();
+  }
+  if (lane == 0) {
+    double trC = 0, trI = 0;
+    for (int i = 0; i < 6; i++) { trC += s27[tri(i, i)]; trI += W.M[i][6 + i]; }
+    W.ok = ok && trI > 0.0 && trC * trI < 1e6;
+    for (int i = 0; i < 6; i++) W.x[i] = W.M[i][12];
+  }
+  wave_sync();
+}
+// Pseudo-inverse solve by cyclic Jacobi (one lane, LDS arrays).
+__device__ void jacobi_pinv(SolveLDS& W, const double* s27) {
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) { W.A[i][j] = s27[tri(i, j)]; W.V[i][j] = (i == j) ? 1.0 : 0.0; }
   for (int sweep = 0; sweep < 60; sweep++) {
     double off = 0;
-    #pragma unroll
-    for (int pp = 0; pp < 6; pp++) for (int q = pp + 1; q < 6; q++) off += A[pp][q] * A[pp][q];
-    if (off == 0) break;
-    #pragma unroll
     for (int pp = 0; pp < 6; pp++)
-      #pragma unroll
+      for (int q = pp + 1; q < 6; q++) off += W.A[pp][q] * W.A[pp][q];
+    if (off == 0) break;
+    for (int pp = 0; pp < 6; pp++)
       for (int q = pp + 1; q < 6; q++) {
-        if (A[pp][q] == 0) continue;
-        const double theta = (A[q][q] - A[pp][pp]) / (2 * A[pp][q]);
+        const double apq = W.A[pp][q];
+        if (apq == 0) continue;
+        const double theta = (W.A[q][q] - W.A[pp][pp]) / (2 * apq);
         const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
         const double c = 1 / sqrt(t * t + 1), sn = t * c;
-        #pragma unroll
-        for (int k = 0; k < 6; k++) { const double akp = A[k][pp], akq = A[k][q]; A[k][pp] = c * akp - sn * akq; A[k][q] = sn * akp + c * akq; }
-        #pragma unroll
-        for (int k = 0; k < 6; k++) { const double apk = A[pp][k], aqk = A[q][k]; A[pp][k] = c * apk - sn * aqk; A[q][k] = sn * apk + c * aqk; }
-        #pragma unroll
-        for (int k = 0; k < 6; k++) { const double vkp = V[k][pp], vkq = V[k][q]; V[k][pp] = c * vkp - sn * vkq; V[k][q] = sn * vkp + c * vkq; }
+        for (int k = 0; k < 6; k++) { const double akp = W.A[k][pp], akq = W.A[k][q]; W.A[k][pp] = c * akp - sn * akq; W.A[k][q] = sn * akp + c * akq; }
+        for (int k = 0; k < 6; k++) { const double apk = W.A[pp][k], aqk = W.A[q][k]; W.A[pp][k] = c * apk - sn * aqk; W.A[q][k] = sn * apk + c * aqk; }
+        for (int k = 0; k < 6; k++) { const double vkp = W.V[k][pp], vkq = W.V[k][q]; W.V[k][pp] = c * vkp - sn * vkq; W.V[k][q] = sn * vkp + c * vkq; }
       }
   }
   double mx = 0;
-  #pragma unroll
-  for (int i = 0; i < 6; i++) mx = fmax(mx, fabs(A[i][i]));
-  #pragma unroll
-  for (int i = 0; i < 6; i++) x[i] = 0;
-  #pragma unroll
+  for (int i = 0; i < 6; i++) mx = fmax(mx, fabs(W.A[i][i]));
+  for (int i = 0; i < 6; i++) W.x[i] = 0;
   for (int i = 0; i < 6; i++) {
-    if (fabs(A[i][i]) * 1e6 <= mx) continue;
+    if (fabs(W.A[i][i]) * 1e6 <= mx) continue;
     double pr = 0;
-    #pragma unroll
-    for (int k = 0; k < 6; k++) pr += V[k][i] * b[k];
-    pr /= A[i][i];
-    #pragma unroll
-    for (int k = 0; k < 6; k++) x[k] += pr * V[k][i];
+    for (int k = 0; k < 6; k++) pr += W.V[k][i] * s27[k];
+    pr /= W.A[i][i];
+    for (int k = 0; k < 6; k++) W.x[k] += pr * W.V[k][i];
   }
-}
-__device__ void solve6(const double* s27, double x[6]) {
-  double b[6], C[6][6];
-  #pragma unroll
-  for (int i = 0; i < 6; i++) b[i] = s27[i];
-  int c = 6;
-  #pragma unroll
-  for (int i = 0; i < 6; i++) for (int j = i; j < 6; j++) { C[i][j] = s27[c]; C[j][i] = s27[c]; c++; }
-  // LDL^T
-  double L[6][6], D[6];
-  bool ok = true;
-  #pragma unroll
-  for (int j = 0; j < 6 && ok; j++) {
-    double d = C[j][j];
-    #pragma unroll
-    for (int k = 0; k < j; k++) d -= L[j][k] * L[j][k] * D[k];
-    if (!(d > 0)) { ok = false; break; }
-    D[j] = d; L[j][j] = 1;
-    #pragma unroll
-    for (int i = j + 1; i < 6; i++) {
-      double v = C[i][j];
-      #pragma unroll
-      for (int k = 0; k < j; k++) v -= L[i][k] * L[j][k] * D[k];
-      L[i][j] = v / d;
-    }
-  }
-  if (ok) {
-    // tr(C^-1) = sum_j (1/D_j) sum_i (L^-1)_{ji}^2 ; Li = L^-1 (unit lower)
-    double Li[6][6];
-    #pragma unroll
-    for (int i = 0; i < 6; i++)
-      #pragma unroll
-      for (int j = 0; j < 6; j++) {
-        if (j > i) { Li[i][j] = 0; continue; }
-        if (j == i) { Li[i][j] = 1; continue; }
-        double v = 0;
-        #pragma unroll
-        for (int k = j; k < i; k++) v -= L[i][k] * Li[k][j];
-        Li[i][j] = v;
-      }
-    double trC = 0, trI = 0;
-    #pragma unroll
-    for (int i = 0; i < 6; i++) trC += C[i][i];
-    #pragma unroll
-    for (int j = 0; j < 6; j++) {
-      double ss = 0;
-      #pragma unroll
-      for (int i = j; i < 6; i++) ss += Li[i][j] * Li[i][j];
-      trI += ss / D[j];
-    }
-    if (trC * trI < 1e6) {
-      double z[6];
-      #pragma unroll
-      for (int i = 0; i < 6; i++) { double v = b[i]; for (int k = 0; k < i; k++) v -= L[i][k] * z[k]; z[i] = v; }
-      #pragma unroll
-      for (int i = 0; i < 6; i++) z[i] /= D[i];
-      #pragma unroll
-      for (int i = 5; i >= 0; i--) { double v = z[i]; for (int k = i + 1; k < 6; k++) v -= L[k][i] * x[k]; x[i] = v; }
-      return;
-    }
-  }
-  jacobi_pinv(C, b, x);
 }
 // SE3 exponential, x[0..2] translation, x[3..5] rotation (TooN SE3::exp as
 // used at tracking.cpp:1361): Rodrigues + left Jacobian in FP64.
@@ -426,29 +427,28 @@ __device__ Mat4 se3_exp(const double x[6]) {
 
 // FP64 accumulation of one instance's workgroup partials, the 6x6 solve and
 // the GN control flow of trackLiveFrame / trackEachObject (tracking.cpp:
-// 300-334, 664-688); one workgroup per instance of the launch.  Partials are
-// summed in a fixed order (deterministic) with eight float4 loads in flight
-// per thread; thread t always reads sums 4*(t%8)..4*(t%8)+3.
-static constexpr int SB = 256;
-__global__ __launch_bounds__(SB) void k_solve(int level, int iter, const TrackInst* __restrict__ insts,
-                                              GNState* __restrict__ st, const float* __restrict__ partial,
-                                              int nblk, TrackSync sy) {
-  __shared__ double red[SB / 64][8][4];
+// 300-334, 664-688) for the rows computed with X's pose at (X.pend_level,
+// X.pend_iter), by one 256-thread workgroup on its LDS copy X.  Partials
+// are summed in a fixed order with eight float4 loads in flight per thread
+// (thread t always reads sums 4*(t%8)..4*(t%8)+3), so every workgroup that
+// runs this on the same partials reaches bit-identical results.  Only the
+// `writer` workgroup publishes the per-iteration outputs (pose, trace).
+__device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, int state, const float* __restrict__ part,
+                                         int nparts, bool writer, const TrackSync& sy) {
+  __shared__ double red[TB / 64][8][4];
   __shared__ double sums[32];
-  const int j = blockIdx.x;
-  const int state = insts[j].state;
-  GNState& S = st[state];
-  if (iter > 0 && S.done_level == level) return;
+  __shared__ SolveLDS W;
+  __shared__ int mode;
   const int tid = threadIdx.x;
   const uint64_t t0 = wall_clock64();
-  const float4* P = reinterpret_cast<const float4*>(partial + (size_t)j * nblk * 32);
-  const int nq = nblk * 8;
+  const float4* P = reinterpret_cast<const float4*>(part);
+  const int nq = nparts * 8;
   double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-  for (int q0 = 0; q0 < nq; q0 += 8 * SB) {
+  for (int q0 = 0; q0 < nq; q0 += 8 * TB) {
     float4 v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const int q = q0 + u * SB + tid;
+      const int q = q0 + u * TB + tid;
       v[u] = q < nq ? P[q] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
@@ -465,48 +465,121 @@ __global__ __launch_bounds__(SB) void k_solve(int level, int iter, const TrackIn
   if (tid < 32) {
     double t = 0;
 #pragma unroll
-    for (int w = 0; w < SB / 64; w++) t += red[w][tid >> 2][tid & 3];
+    for (int w = 0; w < TB / 64; w++) t += red[w][tid >> 2][tid & 3];
     sums[tid] = t;
-    S.sums[tid] = t;
+    if (writer) S.sums[tid] = t;
   }
   __syncthreads();
-  if (tid != 0) return;
-  const uint64_t t1 = wall_clock64();
-  const float s0 = (float)sums[0], s28 = (float)sums[28];
-  const float err = s0 / s28;
-  const float prev_err = (iter == 0) ? INFINITY : S.prev_err;
-  int flag = 0;
-  S.iters_run++;
-  if (S.object && s28 == 0) { S.done_level = level; flag = 3; }
-  else if (err > prev_err) { S.T = S.prev; S.invT = inverse(S.T); S.done_level = level; flag = 2; }
-  else {
-    S.prev_err = err;
-    double x[6];
-    solve6(sums + 1, x);
-    const Mat4 upd = se3_exp(x);
-    const double nx = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
-    if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) { S.done_level = level; flag = 1; }
-    else { S.prev = S.T; S.T = mul(upd, S.prev); S.invT = inverse(S.T); }
+  const int level = X.pend_level, iter = X.pend_iter;
+  // 3 = no inliers (objects), 2 = error grew (step back), 0 = solve
+  if (tid == 0) {
+    const float s0 = (float)sums[0], s28 = (float)sums[28];
+    const float err = s0 / s28;
+    const float prev_err = (iter == 0) ? INFINITY : X.prev_err;
+    mode = (S.object && s28 == 0) ? 3 : (err > prev_err ? 2 : 0);
   }
-  // camera: T_w_l; object: volume_pose_ = T_w_c2 * T_c2_o2 (tracking.cpp:695)
-  *S.out_pose = S.object ? mul(*S.T_w_c2, S.T) : S.T;
-  if (sy.trace) {
-    const uint64_t t2 = wall_clock64();
-    const int ti = atomicAdd(sy.trace_n, 1);
-    if (ti < sy.trace_cap) {
-      TraceRec& r = sy.trace[ti];
-      r.inst = state; r.level = level; r.iter = iter; r.flag = flag; r.err = err; r.s28 = s28;
-      for (int q = 0; q < 16; q++) r.pose[q] = S.T.m[q / 4][q % 4];
-      r.t_accum = (unsigned)(t1 - t0); r.t_solve = (unsigned)(t2 - t1);
+  __syncthreads();
+  const uint64_t t1 = wall_clock64();
+  if (mode == 0 && tid < 64) {
+    gj_solve(W, sums + 1, tid);
+    if (tid == 0 && !W.ok) jacobi_pinv(W, sums + 1);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const float s0 = (float)sums[0], s28 = (float)sums[28];
+    const float err = s0 / s28;
+    int flag = mode;
+    if (mode == 3) { X.done_level = level; }
+    else if (mode == 2) { X.T = X.prev; X.invT = inverse(X.T); X.done_level = level; }
+    else {
+      X.prev_err = err;
+      double x[6];
+      for (int q = 0; q < 6; q++) x[q] = W.x[q];
+      const Mat4 upd = se3_exp(x);
+      const double nx = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+      if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) { X.done_level = level; flag = 1; }
+      else { X.prev = X.T; X.T = mul(upd, X.prev); X.invT = inverse(X.T); }
+    }
+    X.pend_level = -1;
+    if (writer) {
+      S.iters_run++;
+      // camera: T_w_l; object: volume_pose_ = T_w_c2 * T_c2_o2 (tracking.cpp:695)
+      *S.out_pose = S.object ? mul(*S.T_w_c2, X.T) : X.T;
+      if (sy.trace) {
+        const uint64_t t2 = wall_clock64();
+        const int ti = atomicAdd(sy.trace_n, 1);
+        if (ti < sy.trace_cap) {
+          TraceRec& r = sy.trace[ti];
+          r.inst = state; r.level = level; r.iter = iter; r.flag = flag; r.err = err; r.s28 = s28;
+          for (int q = 0; q < 16; q++) r.pose[q] = X.T.m[q / 4][q % 4];
+          r.t_accum = (unsigned)(t1 - t0); r.t_solve = (unsigned)(t2 - t1);
+          r.solver = (mode == 0) ? (W.ok ? 1 : 2) : 0;
+        }
+      }
     }
   }
+  __syncthreads();
 }
 
+// GN iteration rows of every instance (grid.y = instance): one partial of
+// 32 FP32 sums per workgroup.
+template <bool ICP, bool RGB, bool OBJ>
+__global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __restrict__ insts,
+                                              const GNState* __restrict__ st, float* __restrict__ partial,
+                                              int nblk) {
+  __shared__ float red[NW][32];
+  __shared__ Mats Msh;
+  __shared__ int skip;
+  const int tid = threadIdx.x;
+  const int j = blockIdx.y;
+  const int state = insts[j].state;
+  const GNState& S = st[state];
+  if (tid == 0) {
+    const GNIter& X = S.it[0];
+    // a level starts afresh (tracking.cpp:246); later iterations are skipped
+    // once this level's loop has broken (uniform for the instance)
+    skip = (p.iter > 0 && X.done_level == p.level);
+    if (!skip) { Msh.T = X.T; Msh.invT = X.invT; Msh.A = S.A; Msh.B = S.B; Msh.C = S.C; }
+  }
+  __syncthreads();
+  if (skip) return;
+  track_rows<ICP, RGB, OBJ>(p, insts[j], Msh, state, blockIdx.x, nblk,
+                            partial + ((size_t)j * nblk + blockIdx.x) * 32, red);
+}
+
+// The iteration's FP64 accumulation, solve and GN control, one workgroup per
+// instance, after k_track (the kernel boundary publishes the partials; a
+// device-scope fence in every track workgroup costs far more on gfx950).
+__global__ __launch_bounds__(TB) void k_solve(int level, int iter, const TrackInst* __restrict__ insts,
+                                              GNState* __restrict__ st, const float* __restrict__ partial,
+                                              int nblk, TrackSync sy) {
+  __shared__ GNIter X;
+  __shared__ int skip;
+  const int tid = threadIdx.x;
+  const int j = blockIdx.x;
+  const int state = insts[j].state;
+  GNState& S = st[state];
+  if (tid == 0) {
+    X = S.it[0];
+    skip = (iter > 0 && X.done_level == level);
+    X.pend_level = level; X.pend_iter = iter;
+  }
+  __syncthreads();
+  if (skip) return;
+  gn_solve(X, S, state, partial + (size_t)j * nblk * 32, nblk, true, sy);
+  if (tid == 0) S.it[0] = X;
+}
+
+__device__ void gn_iter_init(GNIter& X, const Mat4& T) {
+  X.T = T; X.invT = inverse(T); X.prev = T;
+  X.prev_err = INFINITY; X.done_level = -1; X.pend_level = -1; X.pend_iter = -1;
+}
 __device__ void gn_init_camera(GNState& S, Mat4* T_w_l, const Mat4& Twr, float fx, float fy, float cx, float cy) {
-  S.T = *T_w_l; S.invT = inverse(S.T); S.prev = S.T; S.A = Twr;
+  gn_iter_init(S.it[0], *T_w_l);
+  S.A = Twr;
   S.B = mul(camK(fx, fy, cx, cy), inverse(Twr));
   S.C = ident(); S.T_wo1 = ident();
-  S.prev_err = INFINITY; S.done_level = -1; S.iters_run = 0; S.object = 0;
+  S.iters_run = 0; S.object = 0;
   S.out_pose = T_w_l; S.T_w_c2 = nullptr;
 }
 // Tracking frame start: T_w_r = camera pose (kfusion.cpp:243), its inverse for
@@ -532,14 +605,13 @@ __global__ void k_gn_begin_objects(GNState* st, int first, int n, const int* ids
   const Mat4 To = *pose;
   S.T_wo1 = To;
   S.A = mul(inverse(*T_w_c1), To);
-  S.T = mul(inverse(*T_w_c2), To);
-  S.invT = inverse(S.T);
-  S.prev = S.T;
+  const Mat4 T = mul(inverse(*T_w_c2), To);
+  gn_iter_init(S.it[0], T);
   S.B = inverse(To);
   S.C = inverse(S.A);
-  S.prev_err = INFINITY; S.done_level = -1; S.iters_run = 0; S.object = 1;
+  S.iters_run = 0; S.object = 1;
   S.out_pose = pose; S.T_w_c2 = T_w_c2;
-  *pose = mul(*T_w_c2, S.T);
+  *pose = mul(*T_w_c2, T);
 }
 
 // ------------------------------------------------------------ launchers
@@ -548,8 +620,8 @@ int track_blocks(int nrows, int rows_per_thread) {
   const int nb = (nrows + TB * rows_per_thread - 1) / (TB * rows_per_thread);
   return nb < 1 ? 1 : nb;
 }
-void launch_track(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
-                  float* partial, int nblk, const TrackSync& sy) {
+void launch_gn_iter(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                    float* partial, int nblk, const TrackSync& sy) {
   dim3 grd(nblk, ninst), blk(TB);
   if (p.object) {
     if (p.use_icp && p.use_rgb) k_track<true, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
@@ -560,7 +632,7 @@ void launch_track(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, in
     else if (p.use_icp) k_track<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
     else k_track<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
   }
-  k_solve<<<ninst, SB, 0, s>>>(p.level, p.iter, inst, st, partial, nblk, sy);
+  k_solve<<<ninst, TB, 0, s>>>(p.level, p.iter, inst, st, partial, nblk, sy);
 }
 void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy) {
   k_camera_begin<<<1, 1, 0, s>>>(mats, st, fx, fy, cx, cy);

@@ -222,10 +222,10 @@ class Kfusion:
         return out
 
     def trace(self):
-        n = self.L.mf_get_trace(self.ctx, np.zeros(24), 0)
-        out = np.zeros(24 * max(n, 1))
+        n = self.L.mf_get_trace(self.ctx, np.zeros(25), 0)
+        out = np.zeros(25 * max(n, 1))
         self.L.mf_get_trace(self.ctx, out, n)
-        return out[:24 * n].reshape(n, 24)
+        return out[:25 * n].reshape(n, 25)
 
     def trace_clear(self):
         self.L.mf_clear_trace(self.ctx)
