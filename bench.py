@@ -75,6 +75,8 @@ def main():
     ap.add_argument('--plain', action='store_true', help='one object per launch instead of batched launches')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline leg')
     ap.add_argument('--cpu-frames', type=int, default=3)
+    ap.add_argument('--track-fused', type=str, default='', metavar='F0,F1,F2',
+                    help='1 per pyramid level: one fused launch per GN iteration (default: library default)')
     ap.add_argument('--track-rpt', type=str, default='', metavar='R0,R1,R2',
                     help='tracking rows per thread per pyramid level (default: library default)')
     ap.add_argument('--copy-ref', type=int, default=0, metavar='MB',
@@ -105,6 +107,8 @@ def main():
     g = mf.Kfusion(W, H, P0, bg_size=5.12, bg_res=512, batch_objects=not args.plain)
     for lvl, v in enumerate(x for x in args.track_rpt.split(',') if x):
         g.set_tuning(g.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl, int(v))
+    for lvl, v in enumerate(x for x in args.track_fused.split(',') if x):
+        g.set_tuning(g.TUNE_TRACK_FUSED_L0 + lvl, int(v))
     for i, m in enumerate(movers):
         g.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
         g.set_moving(i + 1, True)

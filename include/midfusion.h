@@ -143,8 +143,8 @@ int mf_get_frame(mf_ctx* ctx, float* depth, float* grey, float* rgb);
 int mf_enable_timing(mf_ctx* ctx, int on);
 int mf_get_timing(mf_ctx* ctx, double* out, int cap);
 const char* mf_kernel_name(int i);
-/* Device work counters since the last reset, for algorithmic-byte
- * accounting: {voxel updates, blocks integrated, march steps, trilinear
+/* Device work counters since the last reset, counted while kernel timing is
+ * enabled, for algorithmic-byte accounting: {voxel updates, blocks integrated, march steps, trilinear
  * evaluations, raycast hits, allocation band samples, pixels allocated from,
  * ray-object pairs}.  Returns the number of counters. */
 int mf_get_stats(mf_ctx* ctx, uint64_t* out, int cap);
@@ -161,7 +161,10 @@ long mf_device_bytes(mf_ctx* ctx);
  *   MF_TUNE_TRACK_ROWS_PER_THREAD_L0 + l: residual rows (ICP or photometric)
  *   each tracking thread sums before the workgroup reduction at pyramid
  *   level l, 1..64 (defaults 4, 1, 1, 1). */
-enum { MF_TUNE_TRACK_ROWS_PER_THREAD_L0 = 0 };
+/*   MF_TUNE_TRACK_FUSED_L0 + l: 1 runs each GN iteration at level l as one
+ *   launch whose workgroups first finish the previous iteration themselves;
+ *   0 (default) as a row launch and a one-workgroup solve launch. */
+enum { MF_TUNE_TRACK_ROWS_PER_THREAD_L0 = 0, MF_TUNE_TRACK_FUSED_L0 = 8 };
 int mf_set_tuning(mf_ctx* ctx, int knob, int value);
 int mf_get_tuning(mf_ctx* ctx, int knob, int* value);
 

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <cmath>
 #include <algorithm>
+#include <map>
 
 using namespace mf;
 
@@ -67,6 +68,7 @@ struct mf_ctx {
   float* partial = nullptr;
   int nblk0 = 0;
   int track_rpt[MAX_LEVELS] = {4, 1, 1, 1};   // tracking rows per thread, per pyramid level
+  int track_fused[MAX_LEVELS] = {0, 0, 0, 0};   // 1: one fused launch per iteration at that level
   uint8_t* moving = nullptr;     // [MAX_INST] label-indexed flags
   TraceRec* trace = nullptr;
   int* trace_n = nullptr;
@@ -83,6 +85,10 @@ struct mf_ctx {
   float ray_eps = -1.f;
   bool have_ref = false;
   bool pending_objects = false;   // camera phase done, object phase due
+  // hipGraph cache: each stage's launch sequence is captured once per
+  // distinct set of kernel arguments (key) and replayed afterwards
+  bool graphs = true;
+  std::map<std::vector<uint64_t>, hipGraphExec_t> gcache;
   TrackRow* dbg_rows = nullptr;   // row dump [2 states][levels][2*W*H] when enabled
   // timing
   bool timing = false;
@@ -238,16 +244,41 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
   sy.trace = c->trace_on ? c->trace : nullptr; sy.trace_n = c->trace_n;
   sy.trace_cap = c->trace_cap;
   const int kinds = (c->cfg.use_icp ? 1 : 0) + (c->cfg.use_rgb ? 1 : 0);
+  // launch q of the run reads state slot q&1; fused launches read partials
+  // [(q+1)&1] (iteration q-1) and write partials[q&1]
+  const size_t half = (size_t)MAX_INST * c->nblk0 * 32;
+  int q = 0, nblk_prev = 1;
+  bool pending = false;   // a fused launch left its rows' solve to the next launch
+  TrackLaunch p = base_launch(c, k, c->cfg.n_levels - 1, object);
+  auto finish = [&]() {
+    if (!pending) return;
+    Timed t(c, KC_TRACK);
+    launch_gn_fused(c->stream, p, c->inst + first_inst, n, c->gn, c->partial + ((q + 1) & 1) * half, nblk_prev,
+                    c->partial + (q & 1) * half, 1, q, 0, sy);
+    ++q;
+    pending = false;
+  };
   for (int l = c->cfg.n_levels - 1; l >= 0; --l) {
-    TrackLaunch p = base_launch(c, k, l, object);
+    p = base_launch(c, k, l, object);
     p.label = label_mask;
     const int nblk = track_blocks(p.w * p.h * kinds, c->track_rpt[l]);
-    for (int it = 0; it < c->cfg.iters[l]; ++it) {
+    const bool fused = c->track_fused[l] != 0;
+    if (!fused) finish();
+    for (int it = 0; it < c->cfg.iters[l]; ++it, ++q) {
       p.iter = it;
       Timed t(c, KC_TRACK);
-      launch_gn_iter(c->stream, p, c->inst + first_inst, n, c->gn, c->partial, nblk, sy);
+      if (fused) {
+        launch_gn_fused(c->stream, p, c->inst + first_inst, n, c->gn, c->partial + ((q + 1) & 1) * half, nblk_prev,
+                        c->partial + (q & 1) * half, nblk, q, 1, sy);
+        pending = true;
+      } else {
+        launch_gn_two(c->stream, p, c->inst + first_inst, n, c->gn, c->partial + (q & 1) * half, nblk, q, sy);
+      }
+      nblk_prev = nblk;
     }
   }
+  finish();
+  // the run's result is the state of slot q&1 (poses are already published)
 }
 
 void set_inst(mf_ctx* c, int slot, float* const mv[3], float* const mn[3], float* const rD[MAX_LEVELS],
@@ -272,6 +303,38 @@ void set_ids(mf_ctx* c, const std::vector<int>& ids) {
   c->ids_host = ids;
   hipMemcpyAsync(c->obj_ids, c->ids_host.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
   hipStreamSynchronize(c->stream);
+}
+
+// Runs `enqueue` (kernel launches only: no copies, memsets or host syncs) on
+// the context stream, through a captured graph keyed by every value its
+// kernel arguments depend on.  Kernel timing needs per-launch events, so it
+// bypasses the graphs.
+uint64_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+template <class F>
+int run_graph(mf_ctx* c, const std::vector<uint64_t>& key, F&& enqueue) {
+  if (!c->graphs || c->timing) { enqueue(); return 0; }
+  auto it = c->gcache.find(key);
+  if (it == c->gcache.end()) {
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess)
+      return fail(c, MF_E_HIP, "stream capture failed to start");
+    enqueue();
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(c->stream, &g) != hipSuccess || !g) return fail(c, MF_E_HIP, "stream capture failed");
+    hipGraphExec_t e = nullptr;
+    const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (r != hipSuccess) return fail(c, MF_E_HIP, "graph instantiation failed");
+    it = c->gcache.emplace(key, e).first;
+  }
+  if (hipGraphLaunch(it->second, c->stream) != hipSuccess) return fail(c, MF_E_HIP, "graph launch failed");
+  return 0;
+}
+std::vector<uint64_t> base_key(mf_ctx* c, int stage, const float k[4]) {
+  std::vector<uint64_t> key = {(uint64_t)stage, (uint64_t)(uintptr_t)c->live[0].D, (uint64_t)(uintptr_t)c->ref[0].D,
+                               fbits(k[0]), fbits(k[1]), fbits(k[2]), fbits(k[3]), (uint64_t)c->trace_on,
+                               (uint64_t)(uintptr_t)c->dbg_rows};
+  for (int l = 0; l < MAX_LEVELS; l++) key.push_back((uint64_t)c->track_rpt[l] | ((uint64_t)c->track_fused[l] << 32));
+  return key;
 }
 
 void download_planes(mf_ctx* c, float* const src[3], float* dst_aos, size_t n) {
@@ -317,13 +380,13 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->gn = dalloc<GNState>(c, MAX_INST);
   c->inst = dalloc<TrackInst>(c, MAX_INST);
   c->nblk0 = track_blocks(2 * (int)c->N0, 1);
-  c->partial = dalloc<float>(c, (size_t)MAX_INST * c->nblk0 * 32);
+  c->partial = dalloc<float>(c, 2 * (size_t)MAX_INST * c->nblk0 * 32);   // two launch parities
   c->moving = dalloc<uint8_t>(c, MAX_INST);
   c->trace = dalloc<TraceRec>(c, c->trace_cap);
   c->trace_n = dalloc<int>(c, 1);
   c->rayobjs = dalloc<RayObj>(c, MAX_INST);
   c->gauss = dalloc<float>(c, 5);
-  c->stats = dalloc<unsigned long long>(c, N_STATS);
+  c->stats = dalloc<unsigned long long>(c, (size_t)N_STATS * STAT_SLOTS);
   c->obj_ids = dalloc<int>(c, MAX_INST);
   ok = ok && c->stats && c->obj_ids && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
        c->inst && c->partial && c->moving && c->trace && c->trace_n && c->rayobjs && c->gauss;
@@ -352,6 +415,7 @@ void mf_destroy(mf_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (auto& e : c->evs) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
   for (auto e : c->pool) hipEventDestroy(e);
+  for (auto& g : c->gcache) hipGraphExecDestroy(g.second);
   for (void* p : c->allocs) hipFree(p);
   for (Object* o : c->objs) delete o;
   if (c->stream) hipStreamDestroy(c->stream);
@@ -424,15 +488,18 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
   hipStream_t s = c->stream;
   const int L = c->cfg.n_levels;
   // set_params_frame + buildPyramid (tracking.cpp:142-171, 776-818)
-  { Timed t(c, KC_BILATERAL); launch_bilateral(s, c->grey, c->live[0].I, c->depth, c->live[0].D, c->W, c->H, 0.1f, c->gauss); }
-  for (int l = 0; l < L; l++) {
-    if (l > 0) { Timed t(c, KC_HALF); launch_half(s, c->live[l - 1].D, c->live[l].D, c->live[l - 1].I, c->live[l].I, c->live[l - 1].w, c->live[l - 1].h, 0.1f * 3); }
-    const float sc = (float)(1 << l);
-    const Mat4 invK = camKinv(k[0] / sc, k[1] / sc, k[2] / sc, k[3] / sc);
-    Timed t(c, KC_LEVEL);
-    launch_level(s, c->live[l], invK, k[1] < 0 ? 1 : 0, k[0] / sc, 1.0f, 1.0f, 0.075f, c->cfg.use_rgb != 0, c->cfg.use_icp != 0);
-  }
+  auto pyramid = [&]() {
+    { Timed t(c, KC_BILATERAL); launch_bilateral(s, c->grey, c->live[0].I, c->depth, c->live[0].D, c->W, c->H, 0.1f, c->gauss); }
+    for (int l = 0; l < L; l++) {
+      if (l > 0) { Timed t(c, KC_HALF); launch_half(s, c->live[l - 1].D, c->live[l].D, c->live[l - 1].I, c->live[l].I, c->live[l - 1].w, c->live[l - 1].h, 0.1f * 3); }
+      const float sc = (float)(1 << l);
+      const Mat4 invK = camKinv(k[0] / sc, k[1] / sc, k[2] / sc, k[3] / sc);
+      Timed t(c, KC_LEVEL);
+      launch_level(s, c->live[l], invK, k[1] < 0 ? 1 : 0, k[0] / sc, 1.0f, 1.0f, 0.075f, c->cfg.use_rgb != 0, c->cfg.use_icp != 0);
+    }
+  };
   if ((int)frame == c->cfg.init_frame || !c->have_ref) {
+    pyramid();
     for (int l = 0; l < L; l++) std::swap(c->live[l], c->ref[l]);   // setRefImgFromCurr
     c->have_ref = true;
     return 0;
@@ -442,17 +509,23 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
   float* camD[MAX_LEVELS];
   for (int l = 0; l < MAX_LEVELS; l++) camD[l] = bg->rD[l];
   set_inst(c, 0, c->vtx, c->nrm, camD, bg->codes, 0);
-  { Timed t(c, KC_GNSMALL); launch_camera_begin(s, c->mats, c->gn, k[0], k[1], k[2], k[3]); }
-  if (c->cfg.use_rgb) render_reference(c, camD, c->vtx);
-  gn_run(c, k, 0, 0, 1, nullptr);
   std::vector<Object*> mov;
   for (size_t i = 1; i < c->objs.size(); i++) if (c->objs[i]->moving) mov.push_back(c->objs[i]);
-  if (!mov.empty()) {
-    // re-track the camera with the moving objects' raycast pixels masked
-    // (kfusion.cpp:359-392, trackLiveFrame mask overload)
-    { Timed t(c, KC_GNSMALL); launch_gn_begin_camera(s, c->gn + 0, c->mats + 0, c->mats + 1, k[0], k[1], k[2], k[3]); }
-    gn_run(c, k, 0, 0, 1, c->label);
-  }
+  std::vector<uint64_t> key = base_key(c, 1, k);
+  key.push_back(mov.size());
+  const int r = run_graph(c, key, [&]() {
+    pyramid();
+    { Timed t(c, KC_GNSMALL); launch_camera_begin(s, c->mats, c->gn, k[0], k[1], k[2], k[3]); }
+    if (c->cfg.use_rgb) render_reference(c, camD, c->vtx);
+    gn_run(c, k, 0, 0, 1, nullptr);
+    if (!mov.empty()) {
+      // re-track the camera with the moving objects' raycast pixels masked
+      // (kfusion.cpp:359-392, trackLiveFrame mask overload)
+      { Timed t(c, KC_GNSMALL); launch_gn_begin_camera(s, c->gn + 0, c->mats + 0, c->mats + 1, k[0], k[1], k[2], k[3]); }
+      gn_run(c, k, 0, 0, 1, c->label);
+    }
+  });
+  if (r < 0) return r;
   c->pending_objects = true;
   return 1;
 }
@@ -474,11 +547,17 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
       set_inst(c, 1 + (int)j, o->mv, o->mn, o->rD, o->codes, 1 + (int)j);
     }
     set_ids(c, ids);
-    { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, 1, (int)mov.size(), c->obj_ids, c->mats, c->mats + 0, c->mats + 1); }
-    for (size_t j = 0; j < mov.size(); j++)
-      if (c->cfg.use_rgb) render_reference(c, mov[j]->rD, mov[j]->mv);
-    if (c->cfg.batch_objects) gn_run(c, k, 1, 1, (int)mov.size(), nullptr);
-    else for (size_t j = 0; j < mov.size(); j++) gn_run(c, k, 1, 1 + (int)j, 1, nullptr);
+    std::vector<uint64_t> key = base_key(c, 2, k);
+    key.push_back((uint64_t)c->cfg.batch_objects);
+    for (int id : ids) key.push_back((uint64_t)id);
+    const int r = run_graph(c, key, [&]() {
+      { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, 1, (int)mov.size(), c->obj_ids, c->mats, c->mats + 0, c->mats + 1); }
+      for (size_t j = 0; j < mov.size(); j++)
+        if (c->cfg.use_rgb) render_reference(c, mov[j]->rD, mov[j]->mv);
+      if (c->cfg.batch_objects) gn_run(c, k, 1, 1, (int)mov.size(), nullptr);
+      else for (size_t j = 0; j < mov.size(); j++) gn_run(c, k, 1, 1 + (int)j, 1, nullptr);
+    });
+    if (r < 0) return r;
   }
   for (int l = 0; l < L; l++) std::swap(c->live[l], c->ref[l]);   // setRefImgFromCurr
   return 1;
@@ -494,20 +573,24 @@ int mf_integrate(mf_ctx* c, const float k[4], unsigned frame) {
   if (!((frame % c->cfg.integ_rate) == 0 || frame <= 3)) return 0;
   hipStream_t s = c->stream;
   const float band = 2.0f * c->cfg.mu;
-  for (Object* o : c->objs) {
-    if (!o->has_mask) continue;
-    // refine_mask_use_motion with the instance's last tracking rows; static
-    // objects carry the camera's rows (kfusion.cpp:398-401)
-    const int8_t* codes = (o->id == 0 || o->moving) ? o->codes : c->objs[0]->codes;
-    const int refine = frame > 1 ? 1 : 0;
-    { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats + 3 + o->id, k[0], k[1], k[2], k[3], o->prm); }
-    { Timed t(c, KC_ALLOC); launch_alloc(s, o->v, o->prm, c->depth, o->mask, codes, refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, band, c->stats); }
-    { Timed t(c, KC_COMMIT); launch_alloc_commit(s, o->v); }
-    { Timed t(c, KC_INTEGRATE);
-      launch_integrate(s, o->v, o->prm, k[0], k[1], k[2], k[3], c->depth, c->rgb[0], c->rgb[1], c->rgb[2], o->mask, codes,
-                       refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, c->cfg.mu, 100, c->stats); }
-  }
-  return 1;
+  const int refine = frame > 1 ? 1 : 0;
+  std::vector<uint64_t> key = base_key(c, 3, k);
+  key.push_back((uint64_t)refine);
+  for (Object* o : c->objs) key.push_back((uint64_t)o->has_mask | ((uint64_t)o->moving << 1));
+  return run_graph(c, key, [&]() {
+    for (Object* o : c->objs) {
+      if (!o->has_mask) continue;
+      // refine_mask_use_motion with the instance's last tracking rows; static
+      // objects carry the camera's rows (kfusion.cpp:398-401)
+      const int8_t* codes = (o->id == 0 || o->moving) ? o->codes : c->objs[0]->codes;
+      { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats + 3 + o->id, k[0], k[1], k[2], k[3], o->prm); }
+      { Timed t(c, KC_ALLOC); launch_alloc(s, o->v, o->prm, c->depth, o->mask, codes, refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, band, c->timing ? c->stats : nullptr); }
+      { Timed t(c, KC_COMMIT); launch_alloc_commit(s, o->v); }
+      { Timed t(c, KC_INTEGRATE);
+        launch_integrate(s, o->v, o->prm, k[0], k[1], k[2], k[3], c->depth, c->rgb[0], c->rgb[1], c->rgb[2], o->mask, codes,
+                         refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, c->cfg.mu, 100, c->timing ? c->stats : nullptr); }
+    }
+  }) < 0 ? -1 : 1;
 }
 
 int mf_raycast(mf_ctx* c, const float k[4], unsigned frame) {
@@ -515,10 +598,6 @@ int mf_raycast(mf_ctx* c, const float k[4], unsigned frame) {
   hipStream_t s = c->stream;
   if (c->ray_eps < 0) c->ray_eps = exp2f(-(float)c->objs[0]->v.max_level);   // ray_iterator's static epsilon
   const int n = (int)c->objs.size();
-  for (int i = 0; i < n; i++) {
-    Object* o = c->objs[i];
-    { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats + 3 + o->id, k[0], k[1], k[2], k[3], o->prm); }
-  }
   if (c->ray_n != n) {   // descriptor table changes only with the object set
     for (int i = 0; i < n; i++) {
       Object* o = c->objs[i];
@@ -531,10 +610,18 @@ int mf_raycast(mf_ctx* c, const float k[4], unsigned frame) {
     hipStreamSynchronize(s);
     c->ray_n = n;
   }
-  Timed t(c, KC_RAYCAST);
-  launch_raycast(s, c->rayobjs, n, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, c->vtx[0], c->vtx[1],
-                 c->vtx[2], c->nrm[0], c->nrm[1], c->nrm[2], c->label, c->stats);
-  return 1;
+  std::vector<uint64_t> key = base_key(c, 4, k);
+  key.push_back((uint64_t)n);
+  key.push_back(fbits(c->ray_eps));
+  return run_graph(c, key, [&]() {
+    for (int i = 0; i < n; i++) {
+      Object* o = c->objs[i];
+      { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats + 3 + o->id, k[0], k[1], k[2], k[3], o->prm); }
+    }
+    Timed t(c, KC_RAYCAST);
+    launch_raycast(s, c->rayobjs, n, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, c->vtx[0], c->vtx[1],
+                   c->vtx[2], c->nrm[0], c->nrm[1], c->nrm[2], c->label, c->timing ? c->stats : nullptr);
+  }) < 0 ? -1 : 1;
 }
 
 int mf_sync(mf_ctx* c) {
@@ -743,14 +830,18 @@ int mf_get_timing(mf_ctx* c, double* out, int cap) {
   return KC_COUNT;
 }
 int mf_get_stats(mf_ctx* c, uint64_t* out, int cap) {
-  unsigned long long v[N_STATS];
-  hipMemcpyAsync(v, c->stats, sizeof v, hipMemcpyDeviceToHost, c->stream);
+  std::vector<unsigned long long> v((size_t)N_STATS * STAT_SLOTS);
+  hipMemcpyAsync(v.data(), c->stats, v.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream);
   hipStreamSynchronize(c->stream);
-  for (int i = 0; i < N_STATS && i < cap; i++) out[i] = v[i];
+  for (int i = 0; i < N_STATS && i < cap; i++) {
+    uint64_t t = 0;
+    for (int k = 0; k < STAT_SLOTS; k++) t += v[(size_t)i * STAT_SLOTS + k];
+    out[i] = t;
+  }
   return N_STATS;
 }
 int mf_reset_stats(mf_ctx* c) {
-  hipMemsetAsync(c->stats, 0, N_STATS * sizeof(unsigned long long), c->stream);
+  hipMemsetAsync(c->stats, 0, (size_t)N_STATS * STAT_SLOTS * sizeof(unsigned long long), c->stream);
   return 0;
 }
 const char* mf_kernel_name(int i) { return (i >= 0 && i < KC_COUNT) ? kNames[i] : ""; }
@@ -758,6 +849,10 @@ long mf_device_bytes(mf_ctx* c) { return c->bytes; }
 
 int mf_set_tuning(mf_ctx* c, int knob, int value) {
   if (!c) return MF_E_ARG;
+  if (knob >= MF_TUNE_TRACK_FUSED_L0 && knob < MF_TUNE_TRACK_FUSED_L0 + MAX_LEVELS) {
+    c->track_fused[knob - MF_TUNE_TRACK_FUSED_L0] = value ? 1 : 0;
+    return 0;
+  }
   if (knob >= MF_TUNE_TRACK_ROWS_PER_THREAD_L0 && knob < MF_TUNE_TRACK_ROWS_PER_THREAD_L0 + MAX_LEVELS) {
     if (value < 1 || value > 64) return fail(c, MF_E_ARG, "rows per thread must be in [1, 64]");
     c->track_rpt[knob - MF_TUNE_TRACK_ROWS_PER_THREAD_L0] = value;
@@ -767,6 +862,10 @@ int mf_set_tuning(mf_ctx* c, int knob, int value) {
 }
 int mf_get_tuning(mf_ctx* c, int knob, int* value) {
   if (!c || !value) return MF_E_ARG;
+  if (knob >= MF_TUNE_TRACK_FUSED_L0 && knob < MF_TUNE_TRACK_FUSED_L0 + MAX_LEVELS) {
+    *value = c->track_fused[knob - MF_TUNE_TRACK_FUSED_L0];
+    return 0;
+  }
   if (knob >= MF_TUNE_TRACK_ROWS_PER_THREAD_L0 && knob < MF_TUNE_TRACK_ROWS_PER_THREAD_L0 + MAX_LEVELS) {
     *value = c->track_rpt[knob - MF_TUNE_TRACK_ROWS_PER_THREAD_L0];
     return 0;

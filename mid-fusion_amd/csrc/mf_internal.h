@@ -46,20 +46,20 @@ struct ObjParams {
   Mat4 view;                // T_w_c * getInverseCameraMatrix(k) (raycast)
 };
 
-// Per-iteration GN state.
+// Per-iteration GN state (double-buffered in GNState::it by launch parity).
 struct GNIter {
   Mat4 T;          // camera: T_w_l; object: T_c2_o2
   Mat4 invT;       // inverse(T)
   Mat4 prev;       // previous_pose (carried across levels)
   float prev_err;
   int done_level;  // pyramid level at which the GN loop broke, -1 if none
-  int pend_level;  // level / iteration of the rows being solved
+  int pend_level;  // rows computed with T whose solve is pending, -1 if none
   int pend_iter;
 };
 
 // Gauss-Newton state of one tracked instance (device resident).
 struct GNState {
-  GNIter it[1];    // (one slot: updated only by the solve kernel)
+  GNIter it[2];    // by launch parity
   Mat4 A;          // camera: T_w_r;            object: T_c1_o1
   Mat4 B;          // camera: K * inverse(T_w_r); object: inverse(T_w_o1)
   Mat4 C;          // object: inverse(T_c1_o1)

@@ -39,10 +39,16 @@ struct TrackSync {             // accumulation + solve of one launch
 };
 int track_blocks(int nrows, int rows_per_thread);
 
-// One GN iteration of `ninst` instances: rows into `partial` (nblk per
-// instance), then one solve workgroup per instance updates its state.
-void launch_gn_iter(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
-                    float* partial, int nblk, const TrackSync& sy);
+// GN iteration q of `ninst` instances, two-kernel form: rows into `partial`
+// (nblk per instance), then one solve workgroup per instance.
+void launch_gn_two(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                   float* partial, int nblk, int q, const TrackSync& sy);
+// GN iteration q, fused form: finishes iteration q-1 (part_prev, nblk_prev
+// per instance) in every workgroup, then iteration q's rows into part_cur;
+// rows == 0 only finishes iteration q-1.
+void launch_gn_fused(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                     const float* part_prev, int nblk_prev, float* part_cur, int nblk, int q, int rows,
+                     const TrackSync& sy);
 // Frame start for the camera: T_w_r = T_w_c, inverse(T_w_r), GN state 0.
 void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy);
 // GN state of a camera pass starting from the current camera pose.
@@ -60,6 +66,11 @@ void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* T_w_o, floa
 // evaluations, [4] raycast hits, [5] band samples, [6] pixels allocated from,
 // [7] ray-object pairs traversed.
 static constexpr int N_STATS = 8;
+// Work counters are spread over STAT_SLOTS addresses per counter (stats[k *
+// STAT_SLOTS + slot]) so that workgroups do not all hit one atomic address;
+// the host sums the slots.  A null stats pointer disables counting.
+static constexpr int STAT_SLOTS = 64;
+__device__ __forceinline__ int stat_slot() { return (blockIdx.x + 7 * blockIdx.y) & (STAT_SLOTS - 1); }
 void launch_alloc(hipStream_t s, const VolDev& v, const ObjParams* prm, const float* depth, const float* mask,
                   const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float band,
                   unsigned long long* stats);

@@ -59,10 +59,13 @@ __global__ void k_alloc(VolDev v, const ObjParams* __restrict__ prm, const float
                      refined_mask(mask, codes, i, (size_t)w * h, refine, use_icp, use_rgb) >= 0.f;
   const float inv_vox = 1.0f / v.voxel;
   const int num_steps = (int)ceilf(band * inv_vox);
-  const int nvalid = __syncthreads_count(valid);
-  if (threadIdx.x == 0 && nvalid) {
-    atomicAdd(&stats[6], (unsigned long long)nvalid);
-    atomicAdd(&stats[5], (unsigned long long)nvalid * num_steps);
+  if (stats) {
+    const int nvalid = __syncthreads_count(valid);
+    if (threadIdx.x == 0 && nvalid) {
+      unsigned long long* sl = stats + stat_slot();
+      atomicAdd(&sl[6 * STAT_SLOTS], (unsigned long long)nvalid);
+      atomicAdd(&sl[5 * STAT_SLOTS], (unsigned long long)nvalid * num_steps);
+    }
   }
   if (!valid) return;
   const float d = depth[i];
@@ -194,11 +197,14 @@ __global__ __launch_bounds__(512) void k_integrate(VolDev v, const ObjParams* __
       }
     }
     const int any = __syncthreads_or(vis ? 1 : 0);
-    const int nupd = __syncthreads_count(upd ? 1 : 0);
-    if (tid == 0) {
-      v.blk_active[s] = any ? 1 : 0;
-      atomicAdd(&stats[0], (unsigned long long)nupd);
-      atomicAdd(&stats[1], 1ULL);
+    if (tid == 0) v.blk_active[s] = any ? 1 : 0;
+    if (stats) {
+      const int nupd = __syncthreads_count(upd ? 1 : 0);
+      if (tid == 0) {
+        unsigned long long* sl = stats + stat_slot();
+        atomicAdd(&sl[0], (unsigned long long)nupd);
+        atomicAdd(&sl[STAT_SLOTS], 1ULL);
+      }
     }
   }
 }

@@ -313,11 +313,13 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
     O.mnx[pix] = mine ? out_n.x : (float)INVALID; O.mny[pix] = mine ? out_n.y : 0.f; O.mnz[pix] = mine ? out_n.z : 0.f;
   }
   }
+  if (!stats) return;
   atomicAdd(&cnt[0], nsteps); atomicAdd(&cnt[1], ninterp); atomicAdd(&cnt[2], nhits); atomicAdd(&cnt[3], npairs);
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicAdd(&stats[2], (unsigned long long)cnt[0]); atomicAdd(&stats[3], (unsigned long long)cnt[1]);
-    atomicAdd(&stats[4], (unsigned long long)cnt[2]); atomicAdd(&stats[7], (unsigned long long)cnt[3]);
+    unsigned long long* sl = stats + stat_slot();
+    atomicAdd(&sl[2 * STAT_SLOTS], (unsigned long long)cnt[0]); atomicAdd(&sl[3 * STAT_SLOTS], (unsigned long long)cnt[1]);
+    atomicAdd(&sl[4 * STAT_SLOTS], (unsigned long long)cnt[2]); atomicAdd(&sl[7 * STAT_SLOTS], (unsigned long long)cnt[3]);
   }
 }
 

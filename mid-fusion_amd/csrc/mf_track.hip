@@ -521,12 +521,17 @@ __device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, int state, const
   __syncthreads();
 }
 
-// GN iteration rows of every instance (grid.y = instance): one partial of
-// 32 FP32 sums per workgroup.
+// Tracking state is double-buffered by launch index q: a launch reads
+// GNState::it[q & 1] and the solve step of iteration q writes it[(q+1) & 1],
+// so no workgroup ever reads a slot that another workgroup of the same
+// launch writes.
+
+// Two-kernel form, rows: GN iteration q of every instance (grid.y =
+// instance), one partial of 32 FP32 sums per workgroup.
 template <bool ICP, bool RGB, bool OBJ>
 __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __restrict__ insts,
                                               const GNState* __restrict__ st, float* __restrict__ partial,
-                                              int nblk) {
+                                              int nblk, int q) {
   __shared__ float red[NW][32];
   __shared__ Mats Msh;
   __shared__ int skip;
@@ -535,7 +540,7 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
   const int state = insts[j].state;
   const GNState& S = st[state];
   if (tid == 0) {
-    const GNIter& X = S.it[0];
+    const GNIter& X = S.it[q & 1];
     // a level starts afresh (tracking.cpp:246); later iterations are skipped
     // once this level's loop has broken (uniform for the instance)
     skip = (p.iter > 0 && X.done_level == p.level);
@@ -547,12 +552,13 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
                             partial + ((size_t)j * nblk + blockIdx.x) * 32, red);
 }
 
-// The iteration's FP64 accumulation, solve and GN control, one workgroup per
-// instance, after k_track (the kernel boundary publishes the partials; a
-// device-scope fence in every track workgroup costs far more on gfx950).
+// Two-kernel form, solve: the iteration's FP64 accumulation, solve and GN
+// control, one workgroup per instance, after k_track (the kernel boundary
+// publishes the partials; a device-scope fence in every track workgroup costs
+// far more on gfx950).
 __global__ __launch_bounds__(TB) void k_solve(int level, int iter, const TrackInst* __restrict__ insts,
                                               GNState* __restrict__ st, const float* __restrict__ partial,
-                                              int nblk, TrackSync sy) {
+                                              int nblk, int q, TrackSync sy) {
   __shared__ GNIter X;
   __shared__ int skip;
   const int tid = threadIdx.x;
@@ -560,14 +566,55 @@ __global__ __launch_bounds__(TB) void k_solve(int level, int iter, const TrackIn
   const int state = insts[j].state;
   GNState& S = st[state];
   if (tid == 0) {
-    X = S.it[0];
+    X = S.it[q & 1];
     skip = (iter > 0 && X.done_level == level);
+    if (skip) S.it[(q + 1) & 1] = X;
     X.pend_level = level; X.pend_iter = iter;
   }
   __syncthreads();
   if (skip) return;
   gn_solve(X, S, state, partial + (size_t)j * nblk * 32, nblk, true, sy);
-  if (tid == 0) S.it[0] = X;
+  if (tid == 0) S.it[(q + 1) & 1] = X;
+}
+
+// Fused form, one launch per iteration: every workgroup first finishes
+// iteration q-1 itself from it[q & 1] (rows pending) and iteration q-1's
+// partials (the same solve on the same data in every workgroup, so all reach
+// bit-identical states), then computes iteration q's rows; workgroup 0
+// stores the state, with these rows pending, in it[(q+1) & 1].  With
+// rows == 0 the launch only finishes iteration q-1.  Each workgroup reads all
+// of the previous partials, so this form pays nblk^2 partial reads: it suits
+// levels with few workgroups.
+template <bool ICP, bool RGB, bool OBJ>
+__global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* __restrict__ insts,
+                                                GNState* __restrict__ st, const float* __restrict__ part_prev,
+                                                int nblk_prev, float* __restrict__ part_cur, int nblk, int q,
+                                                int rows, TrackSync sy) {
+  __shared__ float red[NW][32];
+  __shared__ Mats Msh;
+  __shared__ GNIter X;
+  __shared__ int skip;
+  const int tid = threadIdx.x;
+  const int j = blockIdx.y;
+  const int state = insts[j].state;
+  GNState& S = st[state];
+  const bool writer = blockIdx.x == 0;
+  if (tid == 0) X = S.it[q & 1];
+  __syncthreads();
+  if (X.pend_level >= 0)
+    gn_solve(X, S, state, part_prev + (size_t)j * nblk_prev * 32, nblk_prev, writer, sy);
+  if (tid == 0) {
+    skip = !rows || (p.iter > 0 && X.done_level == p.level);
+    if (!skip) {
+      Msh.T = X.T; Msh.invT = X.invT; Msh.A = S.A; Msh.B = S.B; Msh.C = S.C;
+      X.pend_level = p.level; X.pend_iter = p.iter;
+    }
+    if (writer) S.it[(q + 1) & 1] = X;
+  }
+  __syncthreads();
+  if (skip) return;
+  track_rows<ICP, RGB, OBJ>(p, insts[j], Msh, state, blockIdx.x, nblk, part_cur + ((size_t)j * nblk + blockIdx.x) * 32,
+                            red);
 }
 
 __device__ void gn_iter_init(GNIter& X, const Mat4& T) {
@@ -620,19 +667,33 @@ int track_blocks(int nrows, int rows_per_thread) {
   const int nb = (nrows + TB * rows_per_thread - 1) / (TB * rows_per_thread);
   return nb < 1 ? 1 : nb;
 }
-void launch_gn_iter(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
-                    float* partial, int nblk, const TrackSync& sy) {
+void launch_gn_two(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                   float* partial, int nblk, int q, const TrackSync& sy) {
   dim3 grd(nblk, ninst), blk(TB);
   if (p.object) {
-    if (p.use_icp && p.use_rgb) k_track<true, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
-    else if (p.use_icp) k_track<true, false, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
-    else k_track<false, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+    if (p.use_icp && p.use_rgb) k_track<true, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
+    else if (p.use_icp) k_track<true, false, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
+    else k_track<false, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
   } else {
-    if (p.use_icp && p.use_rgb) k_track<true, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
-    else if (p.use_icp) k_track<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
-    else k_track<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk);
+    if (p.use_icp && p.use_rgb) k_track<true, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
+    else if (p.use_icp) k_track<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
+    else k_track<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
   }
-  k_solve<<<ninst, TB, 0, s>>>(p.level, p.iter, inst, st, partial, nblk, sy);
+  k_solve<<<ninst, TB, 0, s>>>(p.level, p.iter, inst, st, partial, nblk, q, sy);
+}
+void launch_gn_fused(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                     const float* part_prev, int nblk_prev, float* part_cur, int nblk, int q, int rows,
+                     const TrackSync& sy) {
+  dim3 grd(rows ? nblk : 1, ninst), blk(TB);
+  if (p.object) {
+    if (p.use_icp && p.use_rgb) k_gn_iter<true, true, true><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
+    else if (p.use_icp) k_gn_iter<true, false, true><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
+    else k_gn_iter<false, true, true><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
+  } else {
+    if (p.use_icp && p.use_rgb) k_gn_iter<true, true, false><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
+    else if (p.use_icp) k_gn_iter<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
+    else k_gn_iter<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
+  }
 }
 void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy) {
   k_camera_begin<<<1, 1, 0, s>>>(mats, st, fx, fy, cx, cy);

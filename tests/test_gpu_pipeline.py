@@ -145,3 +145,40 @@ def test_pipeline_icp_rgb_moving_object():
         assert r.get('obj1_t', 0) < 1e-3 and r.get('obj1_r', 0) < 1e-3, r
         assert r['keys0'][2] == 0 and r['keys1'][2] == 0, r
         assert r['label_mismatch'] < 1e-3, r
+
+
+def test_fused_iteration_form_matches_two_kernel_form():
+    # Both forms sum the same workgroup partials in the same FP64 order, so
+    # the tracked poses must be bit-identical.
+    W, H = 320, 240
+    mb_T0 = np.eye(4)
+    mb_T0[:3, 3] = np.array([2.56, 2.56, 1.06]) + np.array([0.05, 0.05, 1.0]) - 0.256
+    mov = [scene.MovingBox([0.15, 0.12, 0.1], mb_T0, [0.003, -0.002, 0.002, 0.01, 0.012, -0.008], [0.8, 0.3, 0.2],
+                           0.512, 128)]
+    sc = scene.Scene(seed=4, width=W, height=H, K=tuple(K320), moving=mov)
+    P0 = sc.camera_pose(0).astype(np.float32)
+    ctx = []
+    for fused in (0, 1):
+        g = midfusion.Kfusion(W, H, P0, bg_size=5.12, bg_res=256, use_rgb=True)
+        g.add_object(mov[0].vol_size, mov[0].vol_res, mov[0].pose(0).astype(np.float32))
+        g.set_moving(1, True)
+        for lvl in range(3):
+            g.set_tuning(g.TUNE_TRACK_FUSED_L0 + lvl, fused)
+        ctx.append(g)
+    for f in range(4):
+        d, rgb, lab = sc.render(f)
+        masks = scene.masks_for_labels(lab, 1)
+        for g in ctx:
+            for i, m in enumerate(masks):
+                g.set_mask(i, m)
+            g.preprocessing(d, rgb)
+            g.tracking(K320, f)
+            g.integration(K320, f)
+            g.raycasting(K320, f)
+        a, b = ctx
+        for oid in (-1, 1):
+            Pa, Pb = a.getPose(oid), b.getPose(oid)
+            print(f, oid, np.abs(Pa - Pb).max())
+            assert np.array_equal(Pa, Pb), (f, oid)
+    for g in ctx:
+        g.close()
