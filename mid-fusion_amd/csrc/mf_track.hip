@@ -433,9 +433,10 @@ __device__ Mat4 se3_exp(const double x[6]) {
 // (thread t always reads sums 4*(t%8)..4*(t%8)+3), so every workgroup that
 // runs this on the same partials reaches bit-identical results.  Only the
 // `writer` workgroup publishes the per-iteration outputs (pose, trace).
+template <int NT>
 __device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, int state, const float* __restrict__ part,
                                          int nparts, bool writer, const TrackSync& sy) {
-  __shared__ double red[TB / 64][8][4];
+  __shared__ double red[NT / 64][8][4];
   __shared__ double sums[32];
   __shared__ SolveLDS W;
   __shared__ int mode;
@@ -444,11 +445,11 @@ __device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, int state, const
   const float4* P = reinterpret_cast<const float4*>(part);
   const int nq = nparts * 8;
   double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-  for (int q0 = 0; q0 < nq; q0 += 8 * TB) {
+  for (int q0 = 0; q0 < nq; q0 += 8 * NT) {
     float4 v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const int q = q0 + u * TB + tid;
+      const int q = q0 + u * NT + tid;
       v[u] = q < nq ? P[q] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
@@ -465,7 +466,7 @@ __device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, int state, const
   if (tid < 32) {
     double t = 0;
 #pragma unroll
-    for (int w = 0; w < TB / 64; w++) t += red[w][tid >> 2][tid & 3];
+    for (int w = 0; w < NT / 64; w++) t += red[w][tid >> 2][tid & 3];
     sums[tid] = t;
     if (writer) S.sums[tid] = t;
   }
@@ -486,32 +487,34 @@ __device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, int state, const
   }
   __syncthreads();
   if (tid == 0) {
+    GNIter Y = X;   // registers: the pose algebra below is serial
     const float s0 = (float)sums[0], s28 = (float)sums[28];
     const float err = s0 / s28;
     int flag = mode;
-    if (mode == 3) { X.done_level = level; }
-    else if (mode == 2) { X.T = X.prev; X.invT = inverse(X.T); X.done_level = level; }
+    if (mode == 3) { Y.done_level = level; }
+    else if (mode == 2) { Y.T = Y.prev; Y.invT = inverse(Y.T); Y.done_level = level; }
     else {
-      X.prev_err = err;
+      Y.prev_err = err;
       double x[6];
       for (int q = 0; q < 6; q++) x[q] = W.x[q];
       const Mat4 upd = se3_exp(x);
       const double nx = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
-      if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) { X.done_level = level; flag = 1; }
-      else { X.prev = X.T; X.T = mul(upd, X.prev); X.invT = inverse(X.T); }
+      if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) { Y.done_level = level; flag = 1; }
+      else { Y.prev = Y.T; Y.T = mul(upd, Y.prev); Y.invT = inverse(Y.T); }
     }
-    X.pend_level = -1;
+    Y.pend_level = -1;
+    X = Y;
     if (writer) {
       S.iters_run++;
       // camera: T_w_l; object: volume_pose_ = T_w_c2 * T_c2_o2 (tracking.cpp:695)
-      *S.out_pose = S.object ? mul(*S.T_w_c2, X.T) : X.T;
+      *S.out_pose = S.object ? mul(*S.T_w_c2, Y.T) : Y.T;
       if (sy.trace) {
         const uint64_t t2 = wall_clock64();
         const int ti = atomicAdd(sy.trace_n, 1);
         if (ti < sy.trace_cap) {
           TraceRec& r = sy.trace[ti];
           r.inst = state; r.level = level; r.iter = iter; r.flag = flag; r.err = err; r.s28 = s28;
-          for (int q = 0; q < 16; q++) r.pose[q] = X.T.m[q / 4][q % 4];
+          for (int q = 0; q < 16; q++) r.pose[q] = Y.T.m[q / 4][q % 4];
           r.t_accum = (unsigned)(t1 - t0); r.t_solve = (unsigned)(t2 - t1);
           r.solver = (mode == 0) ? (W.ok ? 1 : 2) : 0;
         }
@@ -556,7 +559,8 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
 // control, one workgroup per instance, after k_track (the kernel boundary
 // publishes the partials; a device-scope fence in every track workgroup costs
 // far more on gfx950).
-__global__ __launch_bounds__(TB) void k_solve(int level, int iter, const TrackInst* __restrict__ insts,
+static constexpr int SOLVE_NT = 512;    // threads of the solve workgroup (partials in flight)
+__global__ __launch_bounds__(SOLVE_NT) void k_solve(int level, int iter, const TrackInst* __restrict__ insts,
                                               GNState* __restrict__ st, const float* __restrict__ partial,
                                               int nblk, int q, TrackSync sy) {
   __shared__ GNIter X;
@@ -573,7 +577,7 @@ __global__ __launch_bounds__(TB) void k_solve(int level, int iter, const TrackIn
   }
   __syncthreads();
   if (skip) return;
-  gn_solve(X, S, state, partial + (size_t)j * nblk * 32, nblk, true, sy);
+  gn_solve<SOLVE_NT>(X, S, state, partial + (size_t)j * nblk * 32, nblk, true, sy);
   if (tid == 0) S.it[(q + 1) & 1] = X;
 }
 
@@ -602,7 +606,7 @@ __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* 
   if (tid == 0) X = S.it[q & 1];
   __syncthreads();
   if (X.pend_level >= 0)
-    gn_solve(X, S, state, part_prev + (size_t)j * nblk_prev * 32, nblk_prev, writer, sy);
+    gn_solve<TB>(X, S, state, part_prev + (size_t)j * nblk_prev * 32, nblk_prev, writer, sy);
   if (tid == 0) {
     skip = !rows || (p.iter > 0 && X.done_level == p.level);
     if (!skip) {
@@ -679,7 +683,7 @@ void launch_gn_two(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, i
     else if (p.use_icp) k_track<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
     else k_track<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
   }
-  k_solve<<<ninst, TB, 0, s>>>(p.level, p.iter, inst, st, partial, nblk, q, sy);
+  k_solve<<<ninst, SOLVE_NT, 0, s>>>(p.level, p.iter, inst, st, partial, nblk, q, sy);
 }
 void launch_gn_fused(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
                      const float* part_prev, int nblk_prev, float* part_cur, int nblk, int q, int rows,
