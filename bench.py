@@ -53,17 +53,55 @@ def render_frames(sc, n_objects, n):
     return frames
 
 
-def track_bytes(trace, use_rgb=True):
-    """Algorithmic bytes of the executed tracking iterations: per pixel of the
-    level, ICP reads live v, n, cov (36 B) + model v, n (24 B) + mask (1 B);
-    RGB reads rendered-reference depth, live/reference vertices, two
-    intensities and two gradients (41 B); plus the 32-float reduction."""
+TB = 256   # threads per tracking workgroup
+REPLAYS = 200   # back-to-back launches of the roofline kernel
+
+
+def track_bytes(trace, rpt, objects, use_rgb=True):
+    """Algorithmic bytes of the executed row launches of the camera
+    (objects=False) or of the moving objects: per pixel of the level, ICP
+    reads live v, n, cov (36 B) + model v, n (24 B) + mask (1 B); RGB reads
+    rendered-reference depth, live/reference vertices, two intensities and two
+    gradients (41 B); each workgroup writes a 128-B partial."""
     per_px = 61 + (41 if use_rgb else 0)
     tot = 0
     for r in trace:
+        if (int(r[0]) > 0) != objects:
+            continue
         lvl = int(r[1])
-        tot += (W >> lvl) * (H >> lvl) * per_px + 256
+        n = (W >> lvl) * (H >> lvl)
+        nblk = -(-(2 * n) // (TB * rpt[lvl]))
+        tot += n * per_px + 128 * nblk
     return tot
+
+
+def solve_bytes(trace, rpt):
+    """Per executed iteration: the workgroup partials read (128 B each) and the
+    GN state read and written (~1 KB)."""
+    tot = 0
+    for r in trace:
+        lvl = int(r[1])
+        n = (W >> lvl) * (H >> lvl)
+        tot += 128 * -(-(2 * n) // (TB * rpt[lvl])) + 1024
+    return tot
+
+
+# kernel class -> kernel name in rocprofv3 output (traffic counters)
+KERNEL_NAMES = {'track_rows': 'mf::k_track<true, true, false>', 'track_rows_l0': 'mf::k_track<true, true, false> level 0', 'track_rows_obj': 'mf::k_track<true, true, true>',
+                'track_solve': 'mf::k_solve', 'integrate': 'mf::k_integrate', 'raycast': 'mf::k_raycast',
+                'alloc_band': 'mf::k_alloc'}
+
+
+def traffic_per_dispatch(kernel_class):
+    """HBM bytes per dispatch of a kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, 'profiles', 'hbm_traffic_cfg2.json')
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t['kernels'][KERNEL_NAMES[kernel_class]]['bytes_per_dispatch']
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -153,23 +191,32 @@ def main():
     stage = {k: round(v[0] / 3, 4) for k, v in tm.items() if v[1]}
     # algorithmic bytes per kernel class over the 3 frames
     N0 = W * H
+    rpt = [g.get_tuning(g.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl) for lvl in range(4)]
     alg = {
-        'track_reduce': track_bytes(tr),
+        'track_rows': track_bytes(tr, rpt, False),
+        'track_rows_obj': track_bytes(tr, rpt, True),
+        'track_solve': solve_bytes(tr, rpt),
         'integrate': 20 * N0 * (1 + n_obj) + 72 * st['voxel_updates'] + 16 * st['blocks_integrated'],
         'raycast': 12 * st['march_steps'] + 32 * st['trilinear_evals'] + 128 * st['hits'] + 28 * N0 * 3,
         'alloc_band': 8 * st['alloc_pixels'] + 8 * st['band_samples'],
     }
-    dom = max(tm, key=lambda k: tm[k][0])
-    dom_ms, dom_n = tm[dom]
-    if dom in alg:
-        achieved = alg[dom] / (dom_ms / 1000.0) / 1e9
-        roof = {'kernel': dom, 'bound': 'hbm', 'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                'frac': round(achieved / HBM_PEAK_GBS, 5), 'traffic': None,
-                'avg_launch_us': round(1000.0 * dom_ms / dom_n, 3), 'launches': dom_n,
-                'alg_bytes_per_launch': int(alg[dom] / dom_n)}
-    else:
-        roof = {'kernel': dom, 'bound': 'hbm', 'achieved': None, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': None,
-                'traffic': None}
+    kern = {k: v for k, v in tm.items() if v[1] and k in alg}
+    # Dominant kernel: the camera's level-0 tracking rows (most HBM bytes and
+    # the most launches of the frame).  Per-launch event pairs inflate short
+    # kernels, so its duration is measured over a back-to-back burst of the
+    # same launch, bracketed by two events on the context stream.
+    g.enable_timing(False)
+    rep_us, rep_bytes = g.replay_track_rows(K, 0, REPLAYS)
+    achieved = rep_bytes / (rep_us * 1e-6) / 1e9
+    roof = {'kernel': KERNEL_NAMES['track_rows'] + ' (level 0, %d workgroups)' % (-(-(2 * N0) // (TB * rpt[0]))),
+            'bound': 'hbm', 'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS,
+            'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 5), 'traffic': traffic_per_dispatch('track_rows_l0'),
+            'avg_launch_us': round(rep_us, 3), 'launches': REPLAYS, 'alg_bytes_per_launch': int(rep_bytes)}
+    roof_all = {}
+    for k, (ms_k, n_k) in kern.items():
+        a = alg[k] / (ms_k / 1000.0) / 1e9
+        roof_all[k] = {'achieved_gbs': round(a, 1), 'frac': round(a / HBM_PEAK_GBS, 4),
+                       'avg_launch_us': round(1000.0 * ms_k / n_k, 2), 'launches': n_k}
 
     cpu = None
     if not args.no_cpu:
@@ -207,6 +254,7 @@ def main():
                                '%s' % (n_obj, 'one object per launch' if args.plain else 'batched objects'),
                    'frames': args.steps},
         'roofline': roof,
+        'event_timed_by_kernel': roof_all,
         'cpu_baseline': cpu,
         'stage_ms_per_frame': stage,
         'kernel_ms_per_frame_total': round(tot_ms / 3, 3),

@@ -153,6 +153,12 @@ int mf_reset_stats(mf_ctx* ctx);
  * context stream (no counterpart in the reference: the calibration point for
  * hardware traffic counters). */
 int mf_copy_reference(mf_ctx* ctx, size_t bytes, int reps);
+/* Replays the camera's tracking-row kernel of pyramid level `level` `reps`
+ * times back to back at the current camera state and returns its average
+ * launch time (HIP events on the context stream) and algorithmic bytes per
+ * launch (no counterpart in the reference: roofline measurement).  Call it
+ * between frames; it overwrites the camera's track codes. */
+int mf_replay_track_rows(mf_ctx* ctx, const float k[4], int level, int reps, double* avg_us, double* alg_bytes);
 /* Device bytes held by the context. */
 long mf_device_bytes(mf_ctx* ctx);
 

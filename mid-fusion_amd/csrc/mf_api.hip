@@ -19,11 +19,11 @@ using namespace mf;
 namespace {
 
 enum KClass {
-  KC_CONVERT, KC_BILATERAL, KC_HALF, KC_LEVEL, KC_V2D, KC_TRACK, KC_GNSMALL, KC_PARAMS, KC_ALLOC,
+  KC_CONVERT, KC_BILATERAL, KC_HALF, KC_LEVEL, KC_V2D, KC_TRACK, KC_TRACK_OBJ, KC_SOLVE, KC_FUSED, KC_GNSMALL, KC_PARAMS, KC_ALLOC,
   KC_COMMIT, KC_INTEGRATE, KC_RAYCAST, KC_COUNT
 };
-const char* kNames[KC_COUNT] = {"convert", "bilateral", "half_sample", "level_maps", "vertex2depth", "track_reduce",
-                                "gn_control", "obj_params", "alloc_band", "alloc_commit",
+const char* kNames[KC_COUNT] = {"convert", "bilateral", "half_sample", "level_maps", "vertex2depth", "track_rows",
+                                "track_rows_obj", "track_solve", "track_fused", "gn_control", "obj_params", "alloc_band", "alloc_commit",
                                 "integrate", "raycast"};
 
 struct Object {
@@ -252,7 +252,7 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
   TrackLaunch p = base_launch(c, k, c->cfg.n_levels - 1, object);
   auto finish = [&]() {
     if (!pending) return;
-    Timed t(c, KC_TRACK);
+    Timed t(c, KC_FUSED);
     launch_gn_fused(c->stream, p, c->inst + first_inst, n, c->gn, c->partial + ((q + 1) & 1) * half, nblk_prev,
                     c->partial + (q & 1) * half, 1, q, 0, sy);
     ++q;
@@ -266,13 +266,16 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
     if (!fused) finish();
     for (int it = 0; it < c->cfg.iters[l]; ++it, ++q) {
       p.iter = it;
-      Timed t(c, KC_TRACK);
       if (fused) {
+        Timed t(c, KC_FUSED);
         launch_gn_fused(c->stream, p, c->inst + first_inst, n, c->gn, c->partial + ((q + 1) & 1) * half, nblk_prev,
                         c->partial + (q & 1) * half, nblk, q, 1, sy);
         pending = true;
       } else {
-        launch_gn_two(c->stream, p, c->inst + first_inst, n, c->gn, c->partial + (q & 1) * half, nblk, q, sy);
+        { Timed t2(c, object ? KC_TRACK_OBJ : KC_TRACK);
+          launch_track_rows(c->stream, p, c->inst + first_inst, n, c->gn, c->partial + (q & 1) * half, nblk, q); }
+        { Timed t2(c, KC_SOLVE);
+          launch_track_solve(c->stream, p, c->inst + first_inst, n, c->gn, c->partial + (q & 1) * half, nblk, q, sy); }
       }
       nblk_prev = nblk;
     }
@@ -878,6 +881,36 @@ int mf_get_tuning(mf_ctx* c, int knob, int* value) {
 __global__ void k_copy_ref(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
 }
+// Replays the camera's row kernel of one pyramid level `reps` times back to
+// back (iteration 0 at the current camera state), bracketed by two events on
+// the context stream, and returns the average launch time and the
+// launch's algorithmic bytes.  Overwrites the camera's level-0 track codes and
+// the row partials; meant for measurement between frames.
+int mf_replay_track_rows(mf_ctx* c, const float k[4], int level, int reps, double* avg_us, double* alg_bytes) {
+  if (level < 0 || level >= c->cfg.n_levels || reps < 1) return fail(c, MF_E_ARG, "bad level or repetition count");
+  if (!c->have_ref) return fail(c, MF_E_ARG, "no tracked frame yet");
+  TrackLaunch p = base_launch(c, k, level, 0);
+  p.iter = 0;
+  // the reference pyramid is the last tracked frame, the live one the frame
+  // before it: after a frame's hand-over both hold valid planes
+  const int kinds = (c->cfg.use_icp ? 1 : 0) + (c->cfg.use_rgb ? 1 : 0);
+  const int nblk = track_blocks(p.w * p.h * kinds, c->track_rpt[level]);
+  hipEvent_t a = ev_get(c), b = ev_get(c);
+  launch_track_rows(c->stream, p, c->inst, 1, c->gn, c->partial, nblk, 0);   // warm
+  hipEventRecord(a, c->stream);
+  for (int r = 0; r < reps; r++) launch_track_rows(c->stream, p, c->inst, 1, c->gn, c->partial, nblk, 0);
+  hipEventRecord(b, c->stream);
+  if (hipEventSynchronize(b) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  float ms = 0;
+  hipEventElapsedTime(&ms, a, b);
+  c->pool.push_back(a); c->pool.push_back(b);
+  if (avg_us) *avg_us = 1000.0 * ms / reps;
+  // live v, n, cov + model v, n + mask (61 B) and the photometric inputs
+  // (41 B) per pixel, one 128-B partial per workgroup
+  if (alg_bytes) *alg_bytes = (double)p.w * p.h * ((c->cfg.use_icp ? 61 : 0) + (c->cfg.use_rgb ? 41 : 0)) + 128.0 * nblk;
+  return 0;
+}
+
 int mf_copy_reference(mf_ctx* c, size_t bytes, int reps) {
   void *a = nullptr, *b = nullptr;
   if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {

@@ -41,8 +41,10 @@ int track_blocks(int nrows, int rows_per_thread);
 
 // GN iteration q of `ninst` instances, two-kernel form: rows into `partial`
 // (nblk per instance), then one solve workgroup per instance.
-void launch_gn_two(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
-                   float* partial, int nblk, int q, const TrackSync& sy);
+void launch_track_rows(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                       float* partial, int nblk, int q);
+void launch_track_solve(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                        const float* partial, int nblk, int q, const TrackSync& sy);
 // GN iteration q, fused form: finishes iteration q-1 (part_prev, nblk_prev
 // per instance) in every workgroup, then iteration q's rows into part_cur;
 // rows == 0 only finishes iteration q-1.

@@ -671,8 +671,8 @@ int track_blocks(int nrows, int rows_per_thread) {
   const int nb = (nrows + TB * rows_per_thread - 1) / (TB * rows_per_thread);
   return nb < 1 ? 1 : nb;
 }
-void launch_gn_two(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
-                   float* partial, int nblk, int q, const TrackSync& sy) {
+void launch_track_rows(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                       float* partial, int nblk, int q) {
   dim3 grd(nblk, ninst), blk(TB);
   if (p.object) {
     if (p.use_icp && p.use_rgb) k_track<true, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
@@ -683,6 +683,9 @@ void launch_gn_two(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, i
     else if (p.use_icp) k_track<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
     else k_track<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
   }
+}
+void launch_track_solve(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                        const float* partial, int nblk, int q, const TrackSync& sy) {
   k_solve<<<ninst, SOLVE_NT, 0, s>>>(p.level, p.iter, inst, st, partial, nblk, q, sy);
 }
 void launch_gn_fused(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,

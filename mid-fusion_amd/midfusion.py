@@ -77,6 +77,7 @@ SIGNATURES = {
     'mf_device_bytes': (C.c_long, [P]),
     'mf_copy_reference': (C.c_int, [P, C.c_size_t, C.c_int]),
     'mf_set_tuning': (C.c_int, [P, C.c_int, C.c_int]),
+    'mf_replay_track_rows': (C.c_int, [P, f32p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     'mf_get_tuning': (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
 }
 
@@ -274,6 +275,11 @@ class Kfusion:
 
     def device_bytes(self):
         return self.L.mf_device_bytes(self.ctx)
+
+    def replay_track_rows(self, k, level, reps):
+        us, b = C.c_double(), C.c_double()
+        self._chk(self.L.mf_replay_track_rows(self.ctx, _f32(k), level, reps, C.byref(us), C.byref(b)))
+        return us.value, b.value
 
     TUNE_TRACK_ROWS_PER_THREAD_L0 = 0   # + pyramid level
     TUNE_TRACK_FUSED_L0 = 8             # + pyramid level
