@@ -34,14 +34,20 @@ K = np.array([525.0, 525.0, 319.5, 239.5], np.float32)
 
 
 def make_scene(scene, n_objects):
-    movers = []
-    rng = np.random.default_rng(2)
-    for i in range(n_objects):
+    """One object: the textured 0.2x0.15x0.1 m box in a 128^3 volume at 2 mm.
+    K > 1 objects: seeded boxes and ellipsoids of 0.05-0.4 m, each in a
+    64^3-256^3 volume at 2-5 mm, on constant twists."""
+    if n_objects == 1:
+        rng = np.random.default_rng(2)
         T0 = np.eye(4)
-        T0[:3, 3] = [2.30 + 0.25 * (i % 4) - 0.128, 2.45 + 0.2 * (i // 4) - 0.128, 2.45]
+        T0[:3, 3] = [2.30 - 0.128, 2.45 - 0.128, 2.45]
         tw = np.r_[rng.uniform(-0.004, 0.004, 3), rng.uniform(-0.015, 0.015, 3)]
-        movers.append(scene.MovingBox([0.1, 0.075, 0.05], T0, tw, rng.uniform(0.3, 0.9, 3), 0.256, 128))
-    return scene.Scene(seed=2, width=W, height=H, K=tuple(K), moving=movers), movers
+        movers = [scene.MovingBox([0.1, 0.075, 0.05], T0, tw, rng.uniform(0.3, 0.9, 3), 0.256, 128)]
+        seed = 2
+    else:
+        movers = scene.batch_objects(n_objects, 4000 + n_objects)
+        seed = 4
+    return scene.Scene(seed=seed, width=W, height=H, K=tuple(K), moving=movers), movers
 
 
 def render_frames(sc, n_objects, n):
@@ -250,9 +256,12 @@ def main():
         'value': round(fps, 3), 'unit': 'frames/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': round(ms, 3), 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f32', 'data': 'synthetic',
-        'config': {'workload': 'cfg2: 640x480 RGB-D, bg 512^3 @1cm + %d moving box(es) 128^3 @2mm, ICP+RGB {10,5,4}, '
-                               '%s' % (n_obj, 'one object per launch' if args.plain else 'batched objects'),
-                   'frames': args.steps},
+        'config': {'workload': ('640x480 RGB-D, bg 512^3 @1cm + 1 moving box 0.2x0.15x0.1 m in 128^3 @2mm'
+                                if n_obj == 1 else
+                                '640x480 RGB-D, bg 512^3 @1cm + %d moving boxes/ellipsoids 0.05-0.4 m in 64^3-256^3 '
+                                '@2-5mm' % n_obj) + ', ICP+RGB {10,5,4}, %s' % (
+                                    'one object per launch' if args.plain else 'objects batched per launch'),
+                   'objects': n_obj, 'frames': args.steps},
         'roofline': roof,
         'event_timed_by_kernel': roof_all,
         'cpu_baseline': cpu,

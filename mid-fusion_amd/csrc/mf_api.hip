@@ -76,6 +76,10 @@ struct mf_ctx {
   int* obj_ids = nullptr;        // [MAX_INST] ids of the moving objects being tracked
   int trace_cap = 4096;
   RayObj* rayobjs = nullptr;
+  MapObj* mapobjs = nullptr;     // device [2*MAX_INST]: integration set, then every object
+  MapWork work;                  // mapping work lists (sized for every object's pool)
+  long work_cap = 0;
+  std::vector<MapObj> mapobjs_host = std::vector<MapObj>(2 * MAX_INST);
   unsigned long long* stats = nullptr;   // device work counters [N_STATS]
   TrackInst inst_host[MAX_INST];   // mirrors of the device descriptor tables
   RayObj ray_host[MAX_INST];
@@ -182,7 +186,6 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   v.r = dalloc<float>(c, nv); v.g = dalloc<float>(c, nv); v.b = dalloc<float>(c, nv);
   v.c = dalloc<int32_t>(c, nv); v.fgw = dalloc<int32_t>(c, nv); v.fg = dalloc<double>(c, nv);
   v.counters = dalloc<int32_t>(c, 8);
-  v.active_list = dalloc<int32_t>(c, cap);
   o->mask = dalloc<float>(c, c->N0);
   {
     float* b = dalloc<float>(c, 6 * c->N0);   // vertex planes then normal planes
@@ -194,7 +197,7 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
     for (int l = 0; l < c->cfg.n_levels; l++) o->rD[l] = b ? b + rd_offset(c, l) : nullptr;
   }
   o->prm = dalloc<ObjParams>(c, 1);
-  if (!v.slot || !v.req || !v.nodes || !v.tsdf || !v.fg || !v.c || !v.fgw || !v.active_list || !o->mask ||
+  if (!v.slot || !v.req || !v.nodes || !v.tsdf || !v.fg || !v.c || !v.fgw || !o->mask ||
       !o->codes || !o->prm || !o->mv[2] || !o->mn[2]) {
     delete o;
     return fail(c, MF_E_OOM, "device allocation failed");
@@ -205,6 +208,20 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   const Mat4 M = m4(T);
   hipMemcpyAsync(c->mats + 3 + o->id, &M, sizeof(Mat4), hipMemcpyHostToDevice, c->stream);
   hipStreamSynchronize(c->stream);
+  // mapping work lists cover every object's pool (grown with each object;
+  // the previous lists stay allocated until the context is destroyed)
+  if (!c->work.pool_pre) {
+    c->work.pool_pre = dalloc<long>(c, MAX_INST + 1);
+    c->work.new_count = dalloc<int>(c, 1);
+    c->work.act_count = dalloc<int>(c, 1);
+  }
+  c->work_cap += cap;
+  c->work.new_list = dalloc<int2>(c, c->work_cap);
+  c->work.act_list = dalloc<int2>(c, c->work_cap);
+  if (!c->work.pool_pre || !c->work.new_count || !c->work.act_count || !c->work.new_list || !c->work.act_list) {
+    delete o;
+    return fail(c, MF_E_OOM, "device allocation failed");
+  }
   c->objs.push_back(o);
   return o->id;
 }
@@ -226,14 +243,14 @@ TrackLaunch base_launch(mf_ctx* c, const float k[4], int level, int object) {
   return p;
 }
 
-// Rendered reference depth pyramid of one instance (tracking.cpp:218-238):
-// vertex2depth at T_w_r (inverse in mats[2]) then robust half-sampling.
-void render_reference(mf_ctx* c, float* const rD[MAX_LEVELS], float* const mv[3]) {
-  { Timed t(c, KC_V2D); launch_vertex2depth(c->stream, mv, (int)c->N0, c->mats + 2, rD[0]); }
-  for (int l = 1; l < c->cfg.n_levels; l++) {
-    Timed t(c, KC_HALF);
-    launch_half(c->stream, rD[l - 1], rD[l], nullptr, nullptr, c->live[l - 1].w, c->live[l - 1].h, 0.1f * 3);
-  }
+// Rendered reference depth pyramids (tracking.cpp:218-238): vertex2depth at
+// T_w_r (inverse in mats[2]) then robust half-sampling, for the tracking
+// instances [first, first + n) of the descriptor table c->inst.
+void render_reference(mf_ctx* c, int first, int n) {
+  int off[MAX_LEVELS];
+  for (int l = 0; l < MAX_LEVELS; l++) off[l] = (int)rd_offset(c, l);
+  Timed t(c, KC_V2D);
+  launch_render_reference(c->stream, c->inst + first, n, c->W, c->H, c->cfg.n_levels, c->mats + 2, off);
 }
 
 // One coarse-to-fine GN run over `n` instances sharing a launch
@@ -340,6 +357,34 @@ std::vector<uint64_t> base_key(mf_ctx* c, int stage, const float k[4]) {
   return key;
 }
 
+// Mapping descriptor tables: entries [0, n_int) are the objects integrated
+// this frame (those with a mask), [MAX_INST, MAX_INST + n_all) every object.
+// Uploaded only when they change (ordered after pending launches).
+void set_mapobjs(mf_ctx* c, int& n_int, int& n_all, long& sum_nb3, long& sum_cap) {
+  std::vector<MapObj> t(2 * MAX_INST);
+  memset(t.data(), 0, t.size() * sizeof(MapObj));
+  n_int = 0; n_all = 0; sum_nb3 = 0; sum_cap = 0;
+  for (Object* o : c->objs) {
+    MapObj m;
+    memset(&m, 0, sizeof m);
+    m.v = o->v; m.prm = o->prm; m.mask = o->mask; m.id = o->id;
+    // refine_mask_use_motion with the instance's last tracking rows; static
+    // objects carry the camera's rows (kfusion.cpp:398-401)
+    m.codes = (o->id == 0 || o->moving) ? o->codes : c->objs[0]->codes;
+    t[MAX_INST + n_all++] = m;
+    if (o->has_mask) {
+      t[n_int++] = m;
+      sum_nb3 += (long)o->v.nb * o->v.nb * o->v.nb;
+      sum_cap += (long)o->v.cap;
+    }
+  }
+  if (memcmp(t.data(), c->mapobjs_host.data(), t.size() * sizeof(MapObj)) != 0) {
+    c->mapobjs_host = t;
+    hipMemcpyAsync(c->mapobjs, c->mapobjs_host.data(), t.size() * sizeof(MapObj), hipMemcpyHostToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
+  }
+}
+
 void download_planes(mf_ctx* c, float* const src[3], float* dst_aos, size_t n) {
   std::vector<float> tmp(3 * n);
   for (int k = 0; k < 3; k++) hipMemcpyAsync(tmp.data() + k * n, src[k], n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
@@ -388,11 +433,12 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->trace = dalloc<TraceRec>(c, c->trace_cap);
   c->trace_n = dalloc<int>(c, 1);
   c->rayobjs = dalloc<RayObj>(c, MAX_INST);
+  c->mapobjs = dalloc<MapObj>(c, 2 * MAX_INST);
   c->gauss = dalloc<float>(c, 5);
   c->stats = dalloc<unsigned long long>(c, (size_t)N_STATS * STAT_SLOTS);
   c->obj_ids = dalloc<int>(c, MAX_INST);
   ok = ok && c->stats && c->obj_ids && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
-       c->inst && c->partial && c->moving && c->trace && c->trace_n && c->rayobjs && c->gauss;
+       c->inst && c->partial && c->moving && c->trace && c->trace_n && c->rayobjs && c->mapobjs && c->gauss;
   if (!ok) { mf_destroy(c); return nullptr; }
   // Gaussian of the bilateral filter (kfusion.cpp:80-88): expf(-(x*x)/(2*delta^2)), delta = 4
   float g[5];
@@ -519,7 +565,7 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
   const int r = run_graph(c, key, [&]() {
     pyramid();
     { Timed t(c, KC_GNSMALL); launch_camera_begin(s, c->mats, c->gn, k[0], k[1], k[2], k[3]); }
-    if (c->cfg.use_rgb) render_reference(c, camD, c->vtx);
+    if (c->cfg.use_rgb) render_reference(c, 0, 1);
     gn_run(c, k, 0, 0, 1, nullptr);
     if (!mov.empty()) {
       // re-track the camera with the moving objects' raycast pixels masked
@@ -555,8 +601,10 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
     for (int id : ids) key.push_back((uint64_t)id);
     const int r = run_graph(c, key, [&]() {
       { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, 1, (int)mov.size(), c->obj_ids, c->mats, c->mats + 0, c->mats + 1); }
-      for (size_t j = 0; j < mov.size(); j++)
-        if (c->cfg.use_rgb) render_reference(c, mov[j]->rD, mov[j]->mv);
+      if (c->cfg.use_rgb) {
+        if (c->cfg.batch_objects) render_reference(c, 1, (int)mov.size());
+        else for (size_t j = 0; j < mov.size(); j++) render_reference(c, 1 + (int)j, 1);
+      }
       if (c->cfg.batch_objects) gn_run(c, k, 1, 1, (int)mov.size(), nullptr);
       else for (size_t j = 0; j < mov.size(); j++) gn_run(c, k, 1, 1 + (int)j, 1, nullptr);
     });
@@ -577,21 +625,34 @@ int mf_integrate(mf_ctx* c, const float k[4], unsigned frame) {
   hipStream_t s = c->stream;
   const float band = 2.0f * c->cfg.mu;
   const int refine = frame > 1 ? 1 : 0;
+  int n_int, n_all;
+  long sum_nb3, sum_cap;
+  set_mapobjs(c, n_int, n_all, sum_nb3, sum_cap);
+  if (n_int == 0) return 1;
   std::vector<uint64_t> key = base_key(c, 3, k);
   key.push_back((uint64_t)refine);
+  key.push_back((uint64_t)c->cfg.batch_objects);
   for (Object* o : c->objs) key.push_back((uint64_t)o->has_mask | ((uint64_t)o->moving << 1));
+  unsigned long long* stats = c->timing ? c->stats : nullptr;
+  // all integrated objects per launch, or one object per launch
+  const int per = c->cfg.batch_objects ? n_int : 1;
   return run_graph(c, key, [&]() {
-    for (Object* o : c->objs) {
-      if (!o->has_mask) continue;
-      // refine_mask_use_motion with the instance's last tracking rows; static
-      // objects carry the camera's rows (kfusion.cpp:398-401)
-      const int8_t* codes = (o->id == 0 || o->moving) ? o->codes : c->objs[0]->codes;
-      { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats + 3 + o->id, k[0], k[1], k[2], k[3], o->prm); }
-      { Timed t(c, KC_ALLOC); launch_alloc(s, o->v, o->prm, c->depth, o->mask, codes, refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, band, c->timing ? c->stats : nullptr); }
-      { Timed t(c, KC_COMMIT); launch_alloc_commit(s, o->v); }
-      { Timed t(c, KC_INTEGRATE);
-        launch_integrate(s, o->v, o->prm, k[0], k[1], k[2], k[3], c->depth, c->rgb[0], c->rgb[1], c->rgb[2], o->mask, codes,
-                         refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, c->cfg.mu, 100, c->timing ? c->stats : nullptr); }
+    for (int j0 = 0; j0 < n_int; j0 += per) {
+      const MapObj* t = c->mapobjs + j0;
+      MapWork wl = c->work;
+      long cap = 0;
+      wl.table_pre[0] = 0;
+      for (int j = 0; j < per; j++) {
+        const VolDev& v = c->mapobjs_host[j0 + j].v;
+        wl.table_pre[j + 1] = wl.table_pre[j] + (long)v.nb * v.nb * v.nb;
+        cap += v.cap;
+      }
+      { Timed tt(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats, t, per, k[0], k[1], k[2], k[3], 1, wl); }
+      { Timed tt(c, KC_ALLOC); launch_alloc(s, t, per, c->depth, refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, band, stats); }
+      { Timed tt(c, KC_COMMIT); launch_alloc_commit(s, t, per, wl); }
+      { Timed tt(c, KC_INTEGRATE);
+        launch_integrate(s, t, per, cap, k[0], k[1], k[2], k[3], c->depth, c->rgb[0], c->rgb[1], c->rgb[2],
+                         refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, c->cfg.mu, 100, wl, stats); }
     }
   }) < 0 ? -1 : 1;
 }
@@ -616,11 +677,11 @@ int mf_raycast(mf_ctx* c, const float k[4], unsigned frame) {
   std::vector<uint64_t> key = base_key(c, 4, k);
   key.push_back((uint64_t)n);
   key.push_back(fbits(c->ray_eps));
+  int n_int, n_all;
+  long sum_nb3, sum_cap;
+  set_mapobjs(c, n_int, n_all, sum_nb3, sum_cap);
   return run_graph(c, key, [&]() {
-    for (int i = 0; i < n; i++) {
-      Object* o = c->objs[i];
-      { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats + 3 + o->id, k[0], k[1], k[2], k[3], o->prm); }
-    }
+    { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats, c->mapobjs + MAX_INST, n_all, k[0], k[1], k[2], k[3], 0, c->work); }
     Timed t(c, KC_RAYCAST);
     launch_raycast(s, c->rayobjs, n, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, c->vtx[0], c->vtx[1],
                    c->vtx[2], c->nrm[0], c->nrm[1], c->nrm[2], c->label, c->timing ? c->stats : nullptr);

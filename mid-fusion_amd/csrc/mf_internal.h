@@ -33,8 +33,7 @@ struct VolDev {
   float *tsdf, *wt, *r, *g, *b;
   int32_t *c, *fgw;
   double* fg;
-  int32_t* counters;    // [0] allocated blocks, [1] overflow flag, [2] active-list size, [3] new this frame
-  int32_t* active_list; // [cap]
+  int32_t* counters;    // [0] allocated blocks, [1] overflow flag, [3] allocated blocks at frame start
 };
 
 // Matrices derived from the poses, computed on the device once per stage.

@@ -14,6 +14,11 @@ void launch_half(hipStream_t s, const float* in0, float* out0, const float* in1,
 void launch_level(hipStream_t s, const Level& L, const Mat4& invK, int negY, float focal, float sxy, float sdisp,
                   float baseline, bool grads, bool cov);
 void launch_vertex2depth(hipStream_t s, const float* const v[3], int n, const Mat4* invT, float* out);
+// Rendered reference depth pyramids (vertex2depth + robust half-sampling) of
+// `n` tracking instances in one launch per level; rd_off[l] is the offset of
+// level l in each instance's pyramid.
+void launch_render_reference(hipStream_t s, const TrackInst* insts, int n, int w, int h, int n_levels,
+                             const Mat4* invT, const int* rd_off);
 
 // tracking (mf_track.hip)
 struct TrackLaunch {
@@ -61,26 +66,44 @@ void launch_gn_begin_objects(hipStream_t s, GNState* st, int first, int n, const
                              const Mat4* T_w_c2, const Mat4* T_w_c1);
 
 // mapping (mf_map.hip)
-void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* T_w_o, float fx, float fy, float cx, float cy,
-                       ObjParams* out);
+// One object of a mapping launch (device table, indexed by the launch's
+// object dimension).
+struct MapObj {
+  VolDev v;
+  ObjParams* prm;
+  const float* mask;      // generalised integration mask
+  const int8_t* codes;    // the instance's last level-0 track codes (refine_mask_use_motion)
+  int id;                 // object id: pose in mats[3 + id]
+  int pad;
+};
+// Per-launch work lists of the mapping kernels.
+struct MapWork {
+  long table_pre[MAX_INST + 1];   // prefix of the objects' Morton-table sizes (host computed)
+  long* pool_pre;                 // [MAX_INST + 1] prefix of allocated-block counts (device)
+  int* new_count;                 // new blocks this frame
+  int2* new_list;                 // (object, slot) of each new block
+  int* act_count;                 // active blocks this frame
+  int2* act_list;                 // (object, slot) of each active block
+};
+void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx,
+                       float fy, float cx, float cy, int alloc_reset, const MapWork& wl);
 // Work counters accumulated on the device (algorithmic-byte accounting):
 // [0] voxel updates, [1] blocks integrated, [2] march steps, [3] trilinear
-// evaluations, [4] raycast hits, [5] band samples, [6] pixels allocated from,
-// [7] ray-object pairs traversed.
+// evaluations, [4] raycast hits, [5] allocation band samples, [6] pixels
+// allocated from, [7] ray-object pairs.
 static constexpr int N_STATS = 8;
 // Work counters are spread over STAT_SLOTS addresses per counter (stats[k *
 // STAT_SLOTS + slot]) so that workgroups do not all hit one atomic address;
 // the host sums the slots.  A null stats pointer disables counting.
 static constexpr int STAT_SLOTS = 64;
-__device__ __forceinline__ int stat_slot() { return (blockIdx.x + 7 * blockIdx.y) & (STAT_SLOTS - 1); }
-void launch_alloc(hipStream_t s, const VolDev& v, const ObjParams* prm, const float* depth, const float* mask,
-                  const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float band,
-                  unsigned long long* stats);
-void launch_alloc_commit(hipStream_t s, const VolDev& v);
-void launch_integrate(hipStream_t s, const VolDev& v, const ObjParams* prm, float fx, float fy, float cx, float cy,
-                      const float* depth, const float* r, const float* g, const float* b, const float* mask,
-                      const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float mu,
-                      int maxweight, unsigned long long* stats);
+__device__ __forceinline__ int stat_slot() { return (blockIdx.x + 7 * blockIdx.y + 13 * blockIdx.z) & (STAT_SLOTS - 1); }
+void launch_alloc(hipStream_t s, const MapObj* objs, int n, const float* depth, int refine, int use_icp,
+                  int use_rgb, int w, int h, float band, unsigned long long* stats);
+void launch_alloc_commit(hipStream_t s, const MapObj* objs, int n, const MapWork& wl);
+void launch_integrate(hipStream_t s, const MapObj* objs, int n, long sum_cap, float fx, float fy, float cx,
+                      float cy, const float* depth, const float* r, const float* g, const float* b, int refine,
+                      int use_icp, int use_rgb, int w, int h, float mu, int maxweight, const MapWork& wl,
+                      unsigned long long* stats);
 
 // raycasting (mf_raycast.hip)
 struct RayObj {

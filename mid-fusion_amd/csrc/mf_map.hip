@@ -27,9 +27,7 @@ __device__ __forceinline__ float refined_mask(const float* mask, const int8_t* c
   return m;
 }
 
-__global__ void k_obj_params(const Mat4* T_w_c, const Mat4* T_w_o, float fx, float fy, float cx, float cy,
-                             ObjParams* out) {
-  const Mat4 Twc = *T_w_c, Two = *T_w_o;
+__device__ void obj_params(const Mat4& Twc, const Mat4& Two, float fx, float fy, float cx, float cy, ObjParams* out) {
   const Mat4 K = camK(fx, fy, cx, cy);
   ObjParams p;
   p.pose = Two;
@@ -41,18 +39,28 @@ __global__ void k_obj_params(const Mat4* T_w_c, const Mat4* T_w_o, float fx, flo
   p.view = mul(Twc, camKinv(fx, fy, cx, cy));
   *out = p;
 }
-
-__global__ void k_alloc_reset(VolDev v) {
-  v.counters[2] = 0;
-  v.counters[3] = v.counters[0];
+// Matrices of every object of a stage (one thread each), and the start of
+// this frame's allocation bookkeeping when `alloc_reset` is set.
+__global__ void k_obj_params(const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx, float fy,
+                             float cx, float cy, int alloc_reset, int* new_count) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (alloc_reset && j == 0) *new_count = 0;
+  if (j >= n) return;
+  const MapObj& o = objs[j];
+  obj_params(*T_w_c, mats[3 + o.id], fx, fy, cx, cy, o.prm);
+  if (alloc_reset) o.v.counters[3] = o.v.counters[0];
 }
 
 // Band ray-march of buildVolumeAllocationList: every sample in a missing
 // block raises that block's request flag; a sample in an allocated block sets
 // the block active.
-__global__ void k_alloc(VolDev v, const ObjParams* __restrict__ prm, const float* __restrict__ depth,
-                        const float* __restrict__ mask, const int8_t* __restrict__ codes, int refine, int use_icp,
+__global__ void k_alloc(const MapObj* __restrict__ objs, const float* __restrict__ depth, int refine, int use_icp,
                         int use_rgb, int w, int h, float band, unsigned long long* __restrict__ stats) {
+  const MapObj& O = objs[blockIdx.z];
+  const VolDev& v = O.v;
+  const ObjParams* __restrict__ prm = O.prm;
+  const float* __restrict__ mask = O.mask;
+  const int8_t* __restrict__ codes = O.codes;
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   const size_t i = (size_t)x + (size_t)y * w;
   const bool valid = x < w && y < h && depth[i] != 0.0f &&
@@ -92,25 +100,59 @@ __global__ void k_alloc(VolDev v, const ObjParams* __restrict__ prm, const float
 // Commit: requested blocks receive pool slots (Octree::updateLevel: new leaf
 // blocks start active, octree.hpp:846-851); their ancestors are marked in the
 // per-level node-occupancy bytes.
-__global__ void k_alloc_commit(VolDev v, long nb3) {
-  for (long m = blockIdx.x * (long)blockDim.x + threadIdx.x; m < nb3; m += (long)gridDim.x * blockDim.x) {
-    if (!v.req[m]) continue;
-    v.req[m] = 0;
-    const int s = atomicAdd(&v.counters[0], 1);
-    if (s >= v.cap) { v.counters[1] = 1; atomicSub(&v.counters[0], 1); continue; }
-    v.slot[m] = s;
-    v.blk_morton[s] = (int32_t)m;
-    v.blk_active[s] = 1;
-    for (int l = v.leaf_level - 1; l >= 1; l--) v.nodes[v.node_off[l] + (m >> (3 * (v.leaf_level - l)))] = 1;
+// Wave-aggregated append: one atomic per wave on `counter`, ranks by ballot.
+// Must be reached by every lane of the wave.
+__device__ __forceinline__ int wave_append(int* counter, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  if (!m) return -1;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, leader);
+  return pred ? base + __popcll(m & ((1ull << lane) - 1)) : -1;
+}
+
+// Octree::alloc_update commit over the concatenated Morton tables of the
+// launch's objects (prefix `wl.table_pre`): every requested block gets a pool
+// slot, its path's internal-node occupancy is set, and (object, slot) joins
+// this frame's new-block list.
+__global__ void k_alloc_commit(const MapObj* __restrict__ objs, int n, MapWork wl) {
+  const long total = wl.table_pre[n];
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long a0 = (long)blockIdx.x * blockDim.x; a0 < total; a0 += stride) {
+    const long a = a0 + threadIdx.x;
+    int j = 0;
+    while (j + 1 < n && wl.table_pre[j + 1] <= a) j++;
+    const VolDev& v = objs[j].v;
+    const long m = a - wl.table_pre[j];
+    bool fresh = false;
+    int s = -1;
+    if (a < total && v.req[m]) {
+      v.req[m] = 0;
+      s = atomicAdd(&v.counters[0], 1);
+      if (s >= v.cap) { v.counters[1] = 1; atomicSub(&v.counters[0], 1); }
+      else fresh = true;
+    }
+    const int slot = wave_append(wl.new_count, fresh);
+    if (fresh) {
+      v.slot[m] = s;
+      v.blk_morton[s] = (int32_t)m;
+      v.blk_active[s] = 1;
+      for (int l = v.leaf_level - 1; l >= 1; l--) v.nodes[v.node_off[l] + (m >> (3 * (v.leaf_level - l)))] = 1;
+      wl.new_list[slot] = make_int2(j, s);
+    }
   }
 }
 
 // Voxel initialisation of this frame's new blocks (voxel_traits<SDF>::initValue,
-// continuous/volume_traits.hpp:38).
-__global__ void k_init_new(VolDev v) {
-  const int first = v.counters[3], last = min(v.counters[0], (int)v.cap);
-  for (long i = (long)first * BLOCK_VOX + blockIdx.x * (long)blockDim.x + threadIdx.x; i < (long)last * BLOCK_VOX;
-       i += (long)gridDim.x * blockDim.x) {
+// continuous/volume_traits.hpp:38), one 512-thread workgroup per block.
+__global__ __launch_bounds__(512) void k_init_new(const MapObj* __restrict__ objs, MapWork wl) {
+  const int n_new = *wl.new_count;
+  for (int a = blockIdx.x; a < n_new; a += gridDim.x) {
+    const int2 e = wl.new_list[a];
+    const VolDev& v = objs[e.x].v;
+    const size_t i = (size_t)e.y * BLOCK_VOX + threadIdx.x;
     v.tsdf[i] = 1.f; v.wt[i] = 0.f; v.r[i] = 0.f; v.g[i] = 0.f; v.b[i] = 0.f; v.c[i] = 0; v.fg[i] = 0.0; v.fgw[i] = 0;
   }
 }
@@ -119,44 +161,73 @@ __device__ __forceinline__ void block_coords(int32_t m, int& x, int& y, int& z) 
   x = (int)compact21((uint64_t)m) * 8; y = (int)compact21((uint64_t)m >> 1) * 8; z = (int)compact21((uint64_t)m >> 2) * 8;
 }
 
-// projective_functor::build_active_list: active || in_frustum(block corner).
-__global__ void k_active_list(VolDev v, const ObjParams* __restrict__ prm, int w, int h) {
-  const int n = min(v.counters[0], (int)v.cap);
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  bool keep = v.blk_active[s] != 0;
-  if (!keep) {
-    int bx, by, bz;
-    block_coords(v.blk_morton[s], bx, by, bz);
-    const f3 bc = mk3((float)bx * v.voxel, (float)by * v.voxel, (float)bz * v.voxel);
-    const f3 vc = affine(prm->KT, bc);
-    const int px = (int)(vc.x / vc.z), py = (int)(vc.y / vc.z);
-    keep = px >= 0 && px < w && py >= 0 && py < h;
+// projective_functor::build_active_list: active || in_frustum(block corner),
+// over the concatenated block pools of the launch's objects (prefix of their
+// allocated counts in `wl.pool_pre`, filled by k_pool_prefix); entries
+// (object, slot) of one list for the whole launch.
+__global__ void k_pool_prefix(const MapObj* __restrict__ objs, int n, MapWork wl) {
+  if (threadIdx.x != 0) return;
+  long t = 0;
+  wl.pool_pre[0] = 0;
+  for (int j = 0; j < n; j++) { t += min(objs[j].v.counters[0], (int)objs[j].v.cap); wl.pool_pre[j + 1] = t; }
+  *wl.act_count = 0;
+}
+__global__ void k_active_list(const MapObj* __restrict__ objs, int n, int w, int h, MapWork wl) {
+  __shared__ long pre[MAX_INST + 1];
+  if ((int)threadIdx.x <= n) pre[threadIdx.x] = wl.pool_pre[threadIdx.x];
+  __syncthreads();
+  const long total = pre[n];
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long a0 = (long)blockIdx.x * blockDim.x; a0 < total; a0 += stride) {
+    const long a = a0 + threadIdx.x;
+    int j = 0;
+    while (j + 1 < n && pre[j + 1] <= a) j++;
+    bool keep = false;
+    const int s = (int)(a - pre[j]);
+    if (a < total) {
+      const VolDev& v = objs[j].v;
+      keep = v.blk_active[s] != 0;
+      if (!keep) {
+        int bx, by, bz;
+        block_coords(v.blk_morton[s], bx, by, bz);
+        const f3 bc = mk3((float)bx * v.voxel, (float)by * v.voxel, (float)bz * v.voxel);
+        const f3 vc = affine(objs[j].prm->KT, bc);
+        const int px = (int)(vc.x / vc.z), py = (int)(vc.y / vc.z);
+        keep = px >= 0 && px < w && py >= 0 && py < h;
+      }
+    }
+    const int slot = wave_append(wl.act_count, keep);
+    if (keep) wl.act_list[slot] = make_int2(j, s);
   }
-  if (keep) v.active_list[atomicAdd(&v.counters[2], 1)] = s;
 }
 
 // update_block + sdf_update: one 512-thread workgroup per active block, one
 // voxel per thread (x fastest, the block's storage order).  The row start
 // and x-increment are formed exactly as the reference forms them.
-__global__ __launch_bounds__(512) void k_integrate(VolDev v, const ObjParams* __restrict__ prm, float fx, float fy,
+__global__ __launch_bounds__(512) void k_integrate(const MapObj* __restrict__ objs, float fx, float fy,
                                                    float cx, float cy, const float* __restrict__ depth,
                                                    const float* __restrict__ rr, const float* __restrict__ gg,
-                                                   const float* __restrict__ bb, const float* __restrict__ mask,
-                                                   const int8_t* __restrict__ codes, int refine, int use_icp,
-                                                   int use_rgb, int w, int h, float mu, int maxweight,
+                                                   const float* __restrict__ bb, int refine, int use_icp, int use_rgb,
+                                                   int w, int h, float mu, int maxweight, MapWork wl,
                                                    unsigned long long* __restrict__ stats) {
-  const int nact = v.counters[2];
   const int tid = threadIdx.x;
   const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
-  const Mat4 Tco = prm->Tco;
   const Mat4 K = camK(fx, fy, cx, cy);
-  const float vs = v.voxel;
-  const f3 delta = rotate(Tco, mk3(vs, 0.f, 0.f));
-  const f3 cdelta = rotate(K, delta);
   const size_t npx = (size_t)w * h;
+  const int nact = *wl.act_count;
   for (int a = blockIdx.x; a < nact; a += gridDim.x) {
-    const int s = v.active_list[a];
+    const int2 e = wl.act_list[a];
+    const int j = e.x;
+    const MapObj& O = objs[j];
+    const VolDev& v = O.v;
+    const ObjParams* __restrict__ prm = O.prm;
+    const float* __restrict__ mask = O.mask;
+    const int8_t* __restrict__ codes = O.codes;
+    const Mat4 Tco = prm->Tco;
+    const float vs = v.voxel;
+    const f3 delta = rotate(Tco, mk3(vs, 0.f, 0.f));
+    const f3 cdelta = rotate(K, delta);
+    const int s = e.y;
     int bx, by, bz;
     block_coords(v.blk_morton[s], bx, by, bz);
     const int y = by + ly, z = bz + lz;
@@ -210,32 +281,33 @@ __global__ __launch_bounds__(512) void k_integrate(VolDev v, const ObjParams* __
 }
 
 // ------------------------------------------------------------ launchers
-void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* T_w_o, float fx, float fy, float cx, float cy,
-                       ObjParams* out) {
-  k_obj_params<<<1, 1, 0, s>>>(T_w_c, T_w_o, fx, fy, cx, cy, out);
+// Every launcher covers the `n` objects of `objs` (device table) in one
+// launch; a one-object-per-launch caller passes n = 1 and a table offset.
+void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx,
+                       float fy, float cx, float cy, int alloc_reset, const MapWork& wl) {
+  k_obj_params<<<(n + 63) / 64, 64, 0, s>>>(T_w_c, mats, objs, n, fx, fy, cx, cy, alloc_reset, wl.new_count);
 }
-void launch_alloc(hipStream_t s, const VolDev& v, const ObjParams* prm, const float* depth, const float* mask,
-                  const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float band,
-                  unsigned long long* stats) {
-  k_alloc_reset<<<1, 1, 0, s>>>(v);
-  dim3 blk(256), grd((w + 255) / 256, h);
-  k_alloc<<<grd, blk, 0, s>>>(v, prm, depth, mask, codes, refine, use_icp, use_rgb, w, h, band, stats);
+void launch_alloc(hipStream_t s, const MapObj* objs, int n, const float* depth, int refine, int use_icp,
+                  int use_rgb, int w, int h, float band, unsigned long long* stats) {
+  dim3 blk(256), grd((w + 255) / 256, h, n);
+  k_alloc<<<grd, blk, 0, s>>>(objs, depth, refine, use_icp, use_rgb, w, h, band, stats);
 }
-void launch_alloc_commit(hipStream_t s, const VolDev& v) {
-  const long nb3 = (long)v.nb * v.nb * v.nb;
-  long g = (nb3 + 255) / 256;
+void launch_alloc_commit(hipStream_t s, const MapObj* objs, int n, const MapWork& wl) {
+  long g = (wl.table_pre[n] + 255) / 256;
   if (g > 4096) g = 4096;
-  k_alloc_commit<<<(int)g, 256, 0, s>>>(v, nb3);
-  k_init_new<<<2048, 256, 0, s>>>(v);
+  k_alloc_commit<<<(unsigned)g, 256, 0, s>>>(objs, n, wl);
+  k_init_new<<<1024, 512, 0, s>>>(objs, wl);
 }
-void launch_integrate(hipStream_t s, const VolDev& v, const ObjParams* prm, float fx, float fy, float cx, float cy,
-                      const float* depth, const float* r, const float* g, const float* b, const float* mask,
-                      const int8_t* codes, int refine, int use_icp, int use_rgb, int w, int h, float mu,
-                      int maxweight, unsigned long long* stats) {
-  const long cap = v.cap;
-  k_active_list<<<(int)((cap + 255) / 256), 256, 0, s>>>(v, prm, w, h);
-  k_integrate<<<4096, 512, 0, s>>>(v, prm, fx, fy, cx, cy, depth, r, g, b, mask, codes, refine, use_icp, use_rgb, w,
-                                   h, mu, maxweight, stats);
+void launch_integrate(hipStream_t s, const MapObj* objs, int n, long sum_cap, float fx, float fy, float cx,
+                      float cy, const float* depth, const float* r, const float* g, const float* b, int refine,
+                      int use_icp, int use_rgb, int w, int h, float mu, int maxweight, const MapWork& wl,
+                      unsigned long long* stats) {
+  long ga = (sum_cap + 255) / 256;
+  if (ga > 2048) ga = 2048;
+  k_pool_prefix<<<1, 64, 0, s>>>(objs, n, wl);
+  k_active_list<<<(unsigned)ga, 256, 0, s>>>(objs, n, w, h, wl);
+  k_integrate<<<4096, 512, 0, s>>>(objs, fx, fy, cx, cy, depth, r, g, b, refine, use_icp, use_rgb, w, h, mu,
+                                   maxweight, wl, stats);
 }
 
 }  // namespace mf

@@ -168,7 +168,46 @@ __global__ void k_vertex2depth(const float* __restrict__ vx, const float* __rest
   out[i] = (v.z > 0.0f) ? affine(M, v).z : 0.0f;
 }
 
+// Rendered reference depth pyramids of `gridDim.y` tracking instances in one
+// pass per level: level 0 is vertex2depth of the instance's model vertex map,
+// level l the robust half-sample of level l-1 (tracking.cpp:218-238).
+__global__ void k_render_ref0(const TrackInst* __restrict__ insts, int n0, const Mat4* __restrict__ invT) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n0) return;
+  const TrackInst& in = insts[blockIdx.y];
+  const Mat4 M = *invT;
+  const f3 v = mk3(in.mvn[i], in.mvn[(size_t)n0 + i], in.mvn[2 * (size_t)n0 + i]);
+  const_cast<float*>(in.rD)[i] = (v.z > 0.0f) ? affine(M, v).z : 0.0f;
+}
+__global__ void k_render_half(const TrackInst* __restrict__ insts, int off_in, int off_out, int iw, int ih, float e_d) {
+  const int ow = iw / 2, oh = ih / 2;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.z;
+  if (x >= ow || y >= oh) return;
+  const TrackInst& ti = insts[blockIdx.y];
+  const float* in = ti.rD + off_in;
+  float* out = const_cast<float*>(ti.rD) + off_out;
+  const int cx = 2 * x, cy = 2 * y;
+  const float c = in[cx + (size_t)cy * iw];
+  float sum = 0.0f, t = 0.0f;
+  for (int i = 0; i <= 1; ++i)
+    for (int j = 0; j <= 1; ++j) {
+      const int ux = min(cx + j, 2 * ow - 1), uy = min(cy + i, 2 * oh - 1);
+      const float cur = in[ux + (size_t)uy * iw];
+      if (fabsf(cur - c) < e_d) { sum = sum + 1.0f; t = t + cur; }
+    }
+  out[x + (size_t)y * ow] = t / sum;
+}
+
 // ------------------------------------------------------------ launchers
+void launch_render_reference(hipStream_t s, const TrackInst* insts, int n, int w, int h, int n_levels,
+                             const Mat4* invT, const int* rd_off) {
+  const int n0 = w * h;
+  k_render_ref0<<<dim3((n0 + 255) / 256, n), 256, 0, s>>>(insts, n0, invT);
+  for (int l = 1; l < n_levels; l++) {
+    const int iw = w >> (l - 1), ih = h >> (l - 1);
+    k_render_half<<<dim3((iw / 2 + 255) / 256, n, ih / 2), 256, 0, s>>>(insts, rd_off[l - 1], rd_off[l], iw, ih, 0.1f * 3);
+  }
+}
 void launch_convert(hipStream_t s, const uint16_t* din, const uint8_t* rgb, int iw, int ratio, int ow, int oh,
                     float* depth, float* grey, float* r, float* g, float* b) {
   dim3 blk(256), grd((ow + 255) / 256, oh);

@@ -38,10 +38,12 @@ def _rot(axis_angle):
 
 
 class MovingBox:
-    """Rigid box moving with a constant body twist per frame.  ``T0`` is the
-    pose of its volume's origin corner; the box is centred in the volume."""
+    """Rigid box (or ellipsoid, ``shape='ellipsoid'``, with semi-axes
+    ``half``) moving with a constant body twist per frame.  ``T0`` is the
+    pose of its volume's origin corner; the shape is centred in the volume."""
 
-    def __init__(self, half, T0, twist, albedo, vol_size, vol_res):
+    def __init__(self, half, T0, twist, albedo, vol_size, vol_res, shape='box'):
+        self.shape = shape
         self.half = np.asarray(half, float)
         self.T0 = np.asarray(T0, float)
         self.twist = np.asarray(twist, float)
@@ -160,13 +162,28 @@ class Scene:
             oo = Tow[:3, :3] @ o + Tow[:3, 3]
             do = d @ Tow[:3, :3].T
             ctr = np.full(3, mb.vol_size / 2)
-            tmin, tmx, amax, _, _, _ = self._slab(oo[None], do, ctr - mb.half, ctr + mb.half)
-            t = np.where((tmin < tmx) & (tmin > 1e-6), tmin, np.inf)
-            hit = t < best_t
-            nn = np.zeros((hit.sum(), 3))
-            ii = np.arange(hit.sum())
-            ax = amax[hit]
-            nn[ii, ax] = np.where(do[hit][ii, ax] > 0, -1.0, 1.0)
+            if mb.shape == 'ellipsoid':
+                # unit sphere in coordinates scaled by the semi-axes
+                os_ = (oo - ctr) / mb.half
+                ds = do / mb.half
+                a = np.einsum('ij,ij->i', ds, ds)
+                b = ds @ os_
+                cc = os_ @ os_ - 1.0
+                disc = b * b - a * cc
+                t = np.where(disc > 0, (-b - np.sqrt(np.maximum(disc, 0))) / a, np.inf)
+                t = np.where(t > 1e-6, t, np.inf)
+                hit = t < best_t
+                ph = oo + do[hit] * t[hit, None] - ctr
+                nn = ph / (mb.half * mb.half)
+                nn /= np.linalg.norm(nn, axis=1, keepdims=True)
+            else:
+                tmin, tmx, amax, _, _, _ = self._slab(oo[None], do, ctr - mb.half, ctr + mb.half)
+                t = np.where((tmin < tmx) & (tmin > 1e-6), tmin, np.inf)
+                hit = t < best_t
+                nn = np.zeros((hit.sum(), 3))
+                ii = np.arange(hit.sum())
+                ax = amax[hit]
+                nn[ii, ax] = np.where(do[hit][ii, ax] > 0, -1.0, 1.0)
             best_n[hit] = nn @ Two[:3, :3].T
             best_t = np.where(hit, t, best_t)
             # texture in the object frame so it moves with the object
@@ -191,6 +208,34 @@ class Scene:
             depth[rng.uniform(0, 1, n) < self.invalid_frac] = 0
         return (depth.reshape(self.h, self.w).astype(np.uint16), rgb.reshape(self.h, self.w, 3),
                 best_lab.reshape(self.h, self.w))
+
+
+def batch_objects(k, seed, world=5.12):
+    """K rigid objects for the multi-object workload: boxes and ellipsoids of
+    seeded extent 0.05-0.4 m, each in its own volume of 64^3-256^3 voxels at
+    2-5 mm, on constant twists, laid out in front of the camera's start."""
+    rng = np.random.Generator(np.random.MT19937(seed))
+    c = world / 2
+    cols = int(np.ceil(np.sqrt(k)))
+    out = []
+    for i in range(k):
+        ext = rng.uniform(0.05, 0.4)
+        half = 0.5 * ext * rng.uniform(0.6, 1.0, 3)
+        voxel = rng.choice([0.002, 0.003, 0.004, 0.005])
+        res = 64
+        while res * voxel < 1.3 * ext and res < 256:
+            res *= 2
+        voxel = max(voxel, 1.3 * ext / res)
+        vs = res * voxel
+        row, col = divmod(i, cols)
+        centre = np.array([c - 0.45 + 0.9 * (col + 0.5) / cols, c - 0.35 + 0.7 * (row + 0.5) / cols,
+                           c - 0.2 + rng.uniform(-0.2, 0.2)])
+        T0 = np.eye(4)
+        T0[:3, 3] = centre - vs / 2
+        tw = np.r_[rng.uniform(-0.004, 0.004, 3), rng.uniform(-0.015, 0.015, 3)]
+        shape = 'ellipsoid' if i % 2 else 'box'
+        out.append(MovingBox(half, T0, tw, rng.uniform(0.3, 0.9, 3), vs, res, shape=shape))
+    return out
 
 
 # OpenCV getStructuringElement(MORPH_ELLIPSE, (5, 5))
