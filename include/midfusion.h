@@ -137,6 +137,19 @@ long mf_get_level(mf_ctx* ctx, int level, int which, float* out);
 /* Preprocessed frame: depth (m), grey, rgb (3/px). */
 int mf_get_frame(mf_ctx* ctx, float* depth, float* grey, float* rgb);
 
+/* Forward/backward of the ICP + photometric tracking residual at a given
+ * pose (the residual kernels and Tracking::reduceKernel, tracking.cpp:
+ * 392-578, 923-1111, 1191-1352).  For the camera (id < 0) T = T_w_l; for
+ * moving object id > 0, T = T_c2_o2 with the object's pose at the frame's
+ * start.  out = the 32 FP64 sums at pyramid level `level`: [0] the robust
+ * loss L (forward), [1..6] J^T W e and [7..27] the upper triangle of J^T W J,
+ * [28] inliers, [29..31] outlier counts.  The rows' J is minus the derivative
+ * of the residual for a left perturbation T <- exp(xi) T, xi = (v, w), so
+ * dL/dxi = -out[1..6] (backward) wherever the rows' correspondences are
+ * locally fixed.  Valid between mf_track_camera and mf_track_objects of a
+ * frame; the robust function is the context's. */
+int mf_residual_eval(mf_ctx* ctx, int id, const float k[4], int level, const float T[16], double out[32]);
+
 /* Kernel timing: when enabled every launch is bracketed by HIP events on the
  * context stream.  out[i] = {total_ms, launches} for each kernel class in
  * mf_kernel_names order; returns the number of classes. */

@@ -942,6 +942,36 @@ int mf_get_tuning(mf_ctx* c, int knob, int* value) {
 __global__ void k_copy_ref(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
 }
+// Forward/backward of the tracking residual at a given pose (see the header).
+int mf_residual_eval(mf_ctx* c, int id, const float k[4], int level, const float T16[16], double out[32]) {
+  if (!c->pending_objects) return fail(c, MF_E_ARG, "call between mf_track_camera and mf_track_objects");
+  if (level < 0 || level >= c->cfg.n_levels) return fail(c, MF_E_ARG, "bad level");
+  if (id == 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad instance id");
+  hipStream_t s = c->stream;
+  const int object = id > 0 ? 1 : 0;
+  int slot = 0, state = 0;
+  if (object) {
+    // a spare descriptor / state slot: the object's GN constants at the
+    // frame's start (T_c1_o1 etc.), without publishing a pose
+    Object* o = c->objs[id];
+    slot = state = MAX_INST - 1;
+    set_inst(c, slot, o->mv, o->mn, o->rD, nullptr, state);
+    const int oid = 3 + id;
+    hipMemcpyAsync(c->obj_ids + MAX_INST - 1, &oid, sizeof oid, hipMemcpyHostToDevice, s);
+    hipStreamSynchronize(s);
+    launch_gn_begin_objects(s, c->gn, state, 1, c->obj_ids + MAX_INST - 1, c->mats, c->mats + 0, c->mats + 1, 0);
+    if (c->cfg.use_rgb) render_reference(c, slot, 1);
+  }
+  TrackLaunch p = base_launch(c, k, level, object);
+  p.iter = 0;
+  const int kinds = (c->cfg.use_icp ? 1 : 0) + (c->cfg.use_rgb ? 1 : 0);
+  const int nblk = track_blocks(p.w * p.h * kinds, c->track_rpt[level]);
+  launch_eval(s, p, c->inst + slot, c->gn, state, m4(T16), c->partial, nblk);
+  hipMemcpyAsync(out, &c->gn[state].sums[0], 32 * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
 // Replays the camera's row kernel of one pyramid level `reps` times back to
 // back (iteration 0 at the current camera state), bracketed by two events on
 // the context stream, and returns the average launch time and the

@@ -56,6 +56,10 @@ void launch_track_solve(hipStream_t s, const TrackLaunch& p, const TrackInst* in
 void launch_gn_fused(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
                      const float* part_prev, int nblk_prev, float* part_cur, int nblk, int q, int rows,
                      const TrackSync& sy);
+// Residual sums of one instance (`inst` points at its descriptor) at pose T:
+// GN state `state` gets T, then rows and the FP64 sums (into GNState::sums).
+void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNState* st, int state, const Mat4& T,
+                 float* partial, int nblk);
 // Frame start for the camera: T_w_r = T_w_c, inverse(T_w_r), GN state 0.
 void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy);
 // GN state of a camera pass starting from the current camera pose.
@@ -63,7 +67,7 @@ void launch_gn_begin_camera(hipStream_t s, GNState* st, Mat4* T_w_l, const Mat4*
                             float cx, float cy);
 // GN states of `n` objects: states[first+j] <- object poses poses[ids[j]].
 void launch_gn_begin_objects(hipStream_t s, GNState* st, int first, int n, const int* ids, Mat4* poses,
-                             const Mat4* T_w_c2, const Mat4* T_w_c1);
+                             const Mat4* T_w_c2, const Mat4* T_w_c1, int publish = 1);
 
 // mapping (mf_map.hip)
 // One object of a mapping launch (device table, indexed by the launch's

@@ -433,15 +433,15 @@ __device__ Mat4 se3_exp(const double x[6]) {
 // (thread t always reads sums 4*(t%8)..4*(t%8)+3), so every workgroup that
 // runs this on the same partials reaches bit-identical results.  Only the
 // `writer` workgroup publishes the per-iteration outputs (pose, trace).
+// FP64 sum of `nparts` workgroup partials (32 floats each) into sums (LDS) and,
+// when `out` is set, into out (device), by an NT-thread workgroup: eight
+// float4 loads in flight per thread, fixed order (deterministic); thread t
+// always reads sums 4*(t%8)..4*(t%8)+3.
 template <int NT>
-__device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, int state, const float* __restrict__ part,
-                                         int nparts, bool writer, const TrackSync& sy) {
+__device__ __forceinline__ void accumulate_sums(const float* __restrict__ part, int nparts, double* sums,
+                                                double* out) {
   __shared__ double red[NT / 64][8][4];
-  __shared__ double sums[32];
-  __shared__ SolveLDS W;
-  __shared__ int mode;
   const int tid = threadIdx.x;
-  const uint64_t t0 = wall_clock64();
   const float4* P = reinterpret_cast<const float4*>(part);
   const int nq = nparts * 8;
   double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
@@ -468,9 +468,20 @@ __device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, int state, const
 #pragma unroll
     for (int w = 0; w < NT / 64; w++) t += red[w][tid >> 2][tid & 3];
     sums[tid] = t;
-    if (writer) S.sums[tid] = t;
+    if (out) out[tid] = t;
   }
   __syncthreads();
+}
+
+template <int NT>
+__device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, int state, const float* __restrict__ part,
+                                         int nparts, bool writer, const TrackSync& sy) {
+  __shared__ double sums[32];
+  __shared__ SolveLDS W;
+  __shared__ int mode;
+  const int tid = threadIdx.x;
+  const uint64_t t0 = wall_clock64();
+  accumulate_sums<NT>(part, nparts, sums, writer ? S.sums : nullptr);
   const int level = X.pend_level, iter = X.pend_iter;
   // 3 = no inliers (objects), 2 = error grew (step back), 0 = solve
   if (tid == 0) {
@@ -621,6 +632,20 @@ __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* 
                             red);
 }
 
+// Residual evaluation at a given pose (forward/backward of the tracking
+// residual): the state's pose is set to T, rows are computed by k_track, and
+// k_eval_sums writes the instance's 32 FP64 sums to S.sums.
+__global__ void k_eval_set(GNState* st, int state, Mat4 T) {
+  GNIter& X = st[state].it[0];
+  X.T = T; X.invT = inverse(T); X.prev = T;
+  X.done_level = -1; X.pend_level = -1; X.pend_iter = -1;
+}
+__global__ __launch_bounds__(SOLVE_NT) void k_eval_sums(GNState* st, int state, const float* __restrict__ partial,
+                                                        int nblk) {
+  __shared__ double sums[32];
+  accumulate_sums<SOLVE_NT>(partial, nblk, sums, st[state].sums);
+}
+
 __device__ void gn_iter_init(GNIter& X, const Mat4& T) {
   X.T = T; X.invT = inverse(T); X.prev = T;
   X.prev_err = INFINITY; X.done_level = -1; X.pend_level = -1; X.pend_iter = -1;
@@ -648,7 +673,7 @@ __global__ void k_gn_begin_camera(GNState* st, Mat4* T_w_l, const Mat4* T_w_r, f
 // trackEachObject setup (tracking.cpp:587-596): T_c1_o1 and the initial
 // T_c2_o2, one thread per object.
 __global__ void k_gn_begin_objects(GNState* st, int first, int n, const int* ids, Mat4* poses, const Mat4* T_w_c2,
-                                   const Mat4* T_w_c1) {
+                                   const Mat4* T_w_c1, int publish) {
   const int j = threadIdx.x;
   if (j >= n) return;
   GNState& S = st[first + j];
@@ -662,7 +687,7 @@ __global__ void k_gn_begin_objects(GNState* st, int first, int n, const int* ids
   S.C = inverse(S.A);
   S.iters_run = 0; S.object = 1;
   S.out_pose = pose; S.T_w_c2 = T_w_c2;
-  *pose = mul(*T_w_c2, T);
+  if (publish) *pose = mul(*T_w_c2, T);
 }
 
 // ------------------------------------------------------------ launchers
@@ -702,6 +727,12 @@ void launch_gn_fused(hipStream_t s, const TrackLaunch& p, const TrackInst* inst,
     else k_gn_iter<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
   }
 }
+void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNState* st, int state, const Mat4& T,
+                 float* partial, int nblk) {
+  k_eval_set<<<1, 1, 0, s>>>(st, state, T);
+  launch_track_rows(s, p, inst, 1, st, partial, nblk, 0);
+  k_eval_sums<<<1, SOLVE_NT, 0, s>>>(st, state, partial, nblk);
+}
 void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy) {
   k_camera_begin<<<1, 1, 0, s>>>(mats, st, fx, fy, cx, cy);
 }
@@ -710,8 +741,8 @@ void launch_gn_begin_camera(hipStream_t s, GNState* st, Mat4* T_w_l, const Mat4*
   k_gn_begin_camera<<<1, 1, 0, s>>>(st, T_w_l, T_w_r, fx, fy, cx, cy);
 }
 void launch_gn_begin_objects(hipStream_t s, GNState* st, int first, int n, const int* ids, Mat4* poses,
-                             const Mat4* T_w_c2, const Mat4* T_w_c1) {
-  k_gn_begin_objects<<<1, 64, 0, s>>>(st, first, n, ids, poses, T_w_c2, T_w_c1);
+                             const Mat4* T_w_c2, const Mat4* T_w_c1, int publish) {
+  k_gn_begin_objects<<<1, 64, 0, s>>>(st, first, n, ids, poses, T_w_c2, T_w_c1, publish);
 }
 
 }  // namespace mf

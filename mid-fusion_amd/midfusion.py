@@ -77,6 +77,7 @@ SIGNATURES = {
     'mf_device_bytes': (C.c_long, [P]),
     'mf_copy_reference': (C.c_int, [P, C.c_size_t, C.c_int]),
     'mf_set_tuning': (C.c_int, [P, C.c_int, C.c_int]),
+    'mf_residual_eval': (C.c_int, [P, C.c_int, f32p, C.c_int, f32p, np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS')]),
     'mf_replay_track_rows': (C.c_int, [P, f32p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     'mf_get_tuning': (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
 }
@@ -275,6 +276,14 @@ class Kfusion:
 
     def device_bytes(self):
         return self.L.mf_device_bytes(self.ctx)
+
+    def residual_eval(self, oid, k, level, T):
+        """32 FP64 sums of the tracking residual of the camera (oid < 0, T =
+        T_w_l) or object oid (T = T_c2_o2) at `level`: loss, J^T W e,
+        J^T W J, counts.  Call between track_camera and track_objects."""
+        out = np.zeros(32)
+        self._chk(self.L.mf_residual_eval(self.ctx, oid, _f32(k), level, _f32(np.asarray(T).reshape(16)), out))
+        return out
 
     def replay_track_rows(self, k, level, reps):
         us, b = C.c_double(), C.c_double()

@@ -153,29 +153,35 @@ struct Tracker {
     }
     return out;
   }
+  // One level's camera rows at T_w_l (track_ICP_Kernel / trackRGB,
+  // tracking.cpp:923-1111) into J[l].
+  void rows_camera(int l, const M4& T_w_l, const M4& T_w_r, const M4& view, const V3* mvert, const V3* mnorm,
+                   const uint8_t* lmask) {
+    const M4 inv_T = inverse(T_w_l);
+    const R s = (R)(1 << l);
+    const R inv_cov = (R)(1.0 / (double)sigma_bright);
+#pragma omp parallel for
+    for (int y = 0; y < lh[l]; y++)
+      for (int x = 0; x < lw[l]; x++) {
+        if (c.use_icp)
+          icp_row(J[l][x + y * W], x, y, l_V[l].data(), l_N[l].data(), lw[l], mvert, mnorm, W, H, T_w_l, view,
+                  R(0.1), R(0.8), cov[l].data(), lmask);
+        if (c.use_rgb)
+          rgb_row(J[l][(size_t)W * H + x + y * W], x, y, r_V_render[l].data(), r_V_live[l].data(), r_I[l].data(),
+                  lw[l], l_I[l].data(), lw[l], lh[l], gx[l].data(), gy[l].data(), T_w_r, T_w_l, inv_T, fx / s,
+                  fy / s, cx / s, cy / s, inv_cov, R(0.1), nullptr);
+      }
+  }
   // Tracking::trackLiveFrame (tracking.cpp:208-357; mask overload :1750-1878)
   void track_live(M4& T_w_l, const M4& T_w_r, const V3* mvert, const V3* mnorm, const uint8_t* mask) {
     if (c.use_rgb) render_reference(mvert, T_w_r);
     auto masks = level_masks(mask);
     M4 prev = T_w_l;
     const M4 view = mul(camK(fx, fy, cx, cy), inverse(T_w_r));
-    const R inv_cov = (R)(1.0 / (double)sigma_bright);
     for (int l = c.nlev - 1; l >= 0; --l) {
       R prev_err = (R)INFINITY;
       for (int it = 0; it < c.iters[l]; ++it) {
-        const M4 inv_T = inverse(T_w_l);
-        const R s = (R)(1 << l);
-#pragma omp parallel for
-        for (int y = 0; y < lh[l]; y++)
-          for (int x = 0; x < lw[l]; x++) {
-            if (c.use_icp)
-              icp_row(J[l][x + y * W], x, y, l_V[l].data(), l_N[l].data(), lw[l], mvert, mnorm, W, H, T_w_l, view,
-                      R(0.1), R(0.8), cov[l].data(), mask ? masks[l].data() : nullptr);
-            if (c.use_rgb)
-              rgb_row(J[l][(size_t)W * H + x + y * W], x, y, r_V_render[l].data(), r_V_live[l].data(), r_I[l].data(),
-                      lw[l], l_I[l].data(), lw[l], lh[l], gx[l].data(), gy[l].data(), T_w_r, T_w_l, inv_T, fx / s,
-                      fy / s, cx / s, cy / s, inv_cov, R(0.1), nullptr);
-          }
+        rows_camera(l, T_w_l, T_w_r, view, mvert, mnorm, mask ? masks[l].data() : nullptr);
         double sums[32];
         reduce_level(l, sums);
         const R err = (R)sums[0] / (R)sums[28];
@@ -189,6 +195,34 @@ struct Tracker {
       }
     }
   }
+  // The 32 reduction sums of the camera residual at level l and pose T_w_l
+  // (no mask), the forward/backward pair of the tracking residual.
+  void eval_camera(int l, const M4& T_w_l, const M4& T_w_r, const V3* mvert, const V3* mnorm, double out[32]) {
+    if (c.use_rgb) render_reference(mvert, T_w_r);
+    const M4 view = mul(camK(fx, fy, cx, cy), inverse(T_w_r));
+    rows_camera(l, T_w_l, T_w_r, view, mvert, mnorm, nullptr);
+    reduce_level(l, out);
+  }
+  // One level's object rows at T_c2_o2 (track_Obj_ICP_Kernel /
+  // track_obj_RGB_kernel, tracking.cpp:392-578) into J[l].
+  void rows_object(int l, const M4& T_c2_o2, const M4& T_c1_o1, const M4& inv_T_w_o1, const M4& inv_T_c1_o1,
+                   const V3* mvert, const V3* mnorm) {
+    const M4 inv_T = inverse(T_c2_o2);
+    const M4 K0 = camK(fx, fy, cx, cy);
+    const R inv_cov = (R)(1.0 / (double)sigma_bright);
+    const R s = (R)(1 << l);
+#pragma omp parallel for
+    for (int y = 0; y < lh[l]; y++)
+      for (int x = 0; x < lw[l]; x++) {
+        if (c.use_icp)
+          obj_icp_row(J[l][x + y * W], x, y, l_V[l].data(), l_N[l].data(), lw[l], mvert, mnorm, W, H, T_c2_o2,
+                      inv_T, T_c1_o1, inv_T_w_o1, K0, R(0.1), R(0.8), cov[l].data(), nullptr);
+        if (c.use_rgb)
+          obj_rgb_row(J[l][(size_t)W * H + x + y * W], x, y, r_V_render[l].data(), r_V_live[l].data(),
+                      r_I[l].data(), lw[l], l_I[l].data(), lw[l], lh[l], gx[l].data(), gy[l].data(), T_c2_o2,
+                      inv_T_c1_o1, fx / s, fy / s, cx / s, cy / s, inv_cov, R(0.1), nullptr);
+      }
+  }
   // Tracking::trackEachObject (tracking.cpp:580-708).  Returns T_c2_o2.
   void track_object(M4& volume_pose, M4& virtual_cam, const V3* mvert, const V3* mnorm, const M4& T_w_c2,
                     const M4& T_w_c1) {
@@ -199,24 +233,10 @@ struct Tracker {
     M4 prev = T_c2_o2;
     const M4 inv_T_w_o1 = inverse(T_wo1);
     const M4 inv_T_c1_o1 = inverse(T_c1_o1);
-    const M4 K0 = camK(fx, fy, cx, cy);
-    const R inv_cov = (R)(1.0 / (double)sigma_bright);
     for (int l = c.nlev - 1; l >= 0; --l) {
       R prev_err = (R)INFINITY;
       for (int it = 0; it < c.iters[l]; ++it) {
-        const M4 inv_T = inverse(T_c2_o2);
-        const R s = (R)(1 << l);
-#pragma omp parallel for
-        for (int y = 0; y < lh[l]; y++)
-          for (int x = 0; x < lw[l]; x++) {
-            if (c.use_icp)
-              obj_icp_row(J[l][x + y * W], x, y, l_V[l].data(), l_N[l].data(), lw[l], mvert, mnorm, W, H, T_c2_o2,
-                          inv_T, T_c1_o1, inv_T_w_o1, K0, R(0.1), R(0.8), cov[l].data(), nullptr);
-            if (c.use_rgb)
-              obj_rgb_row(J[l][(size_t)W * H + x + y * W], x, y, r_V_render[l].data(), r_V_live[l].data(),
-                          r_I[l].data(), lw[l], l_I[l].data(), lw[l], lh[l], gx[l].data(), gy[l].data(), T_c2_o2,
-                          inv_T_c1_o1, fx / s, fy / s, cx / s, cy / s, inv_cov, R(0.1), nullptr);
-          }
+        rows_object(l, T_c2_o2, T_c1_o1, inv_T_w_o1, inv_T_c1_o1, mvert, mnorm);
         double sums[32];
         reduce_level(l, sums);
         const R err = (R)sums[0] / (R)sums[28];
@@ -232,6 +252,15 @@ struct Tracker {
     }
     volume_pose = mul(T_w_c2, T_c2_o2);
     virtual_cam = mul(T_wo1, inverse(T_c2_o2));
+  }
+  // The 32 reduction sums of an object's residual at level l and pose
+  // T_c2_o2, with the object at T_wo1 and the reference camera at T_w_c1.
+  void eval_object(int l, const M4& T_c2_o2, const M4& T_wo1, const M4& T_w_c1, const V3* mvert, const V3* mnorm,
+                   double out[32]) {
+    const M4 T_c1_o1 = mul(inverse(T_w_c1), T_wo1);
+    if (c.use_rgb) render_reference(mvert, T_w_c1);
+    rows_object(l, T_c2_o2, T_c1_o1, inverse(T_wo1), inverse(T_c1_o1), mvert, mnorm);
+    reduce_level(l, out);
   }
 };
 
@@ -481,6 +510,26 @@ int orc_track_objects(void* h, const float k[4], unsigned frame) {
   }
   t.set_ref_from_curr();
   return 1;
+}
+
+// Forward/backward of the tracking residual at a given pose: the 32 FP64
+// reduction sums (loss, JtWe, JtWJ, counts) of the camera (id < 0) at
+// T = T_w_l, or of object id at T = T_c2_o2, at pyramid level `level`.
+// Valid after orc_track_camera and before orc_track_objects of a frame.
+int orc_eval(void* h, int id, int level, const double T16[16], double out[32]) {
+  Pipe* p = (Pipe*)h;
+  Tracker& t = p->trk;
+  if (level < 0 || level >= p->c.nlev) return -1;
+  M4 T;
+  for (int i = 0; i < 16; i++) T.m[i / 4][i % 4] = (R)T16[i];   // FP64 poses reach the FP64 oracle unrounded
+  if (id < 0) {
+    t.eval_camera(level, T, p->T_w_r, p->vertex.data(), p->normal.data(), out);
+  } else {
+    if (id < 1 || id >= (int)p->objs.size()) return -1;
+    Object* o = p->objs[id];
+    t.eval_object(level, T, o->pose, p->T_w_r, o->m_vertex.data(), o->m_normal.data(), out);
+  }
+  return 0;
 }
 
 // Kfusion::tracking (kfusion.cpp:237-488) with the moving-object set given

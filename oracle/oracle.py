@@ -51,6 +51,7 @@ def _lib(fp64):
         'orc_raycast': (C.c_int, [P, f32p, C.c_uint]),
         'orc_get_pose': (None, [P, C.c_int, f64p]),
         'orc_set_pose': (None, [P, C.c_int, f32p]),
+        'orc_eval': (C.c_int, [P, C.c_int, C.c_int, f64p, f64p]),
         'orc_get_maps': (None, [P, f32p, f32p, i32p]),
         'orc_get_object_maps': (None, [P, C.c_int, f32p, f32p]),
         'orc_set_maps': (None, [P, f32p, f32p, i32p]),
@@ -173,6 +174,15 @@ class Oracle:
         T = np.zeros(16)
         self.L.orc_get_pose(self.h_, oid, T)
         return T.reshape(4, 4)
+
+    def eval(self, oid, level, T):
+        """32 FP64 reduction sums of the residual of the camera (oid < 0, T =
+        T_w_l) or object oid (T = T_c2_o2) at `level`; call between
+        track_camera and track_objects."""
+        out = np.zeros(32)
+        r = self.L.orc_eval(self.h_, oid, level, np.ascontiguousarray(np.asarray(T, np.float64).reshape(16)), out)
+        assert r == 0
+        return out
 
     def set_pose(self, oid, T):
         self.L.orc_set_pose(self.h_, oid, f32(np.asarray(T).reshape(16)))

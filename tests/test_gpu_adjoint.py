@@ -1,0 +1,123 @@
+"""GPU forward/backward of the tracking residual (mf_residual_eval) against
+the FP64 oracle's reduction sums and FP64 central differences.
+
+Both contexts see the same inputs: the GPU takes the FP64 oracle's model maps
+and poses before frame 1, so the residual at a given pose differs only by
+FP32 rows against FP64 rows.  Huber weight, delta = 0.1.  The finite
+differences use the FP64 oracle (pinned on its own in test_oracle_adjoint);
+camera ICP with h = 1e-9 (truncated correspondences stay put), photometric
+rows on an 8-bit column ramp with h = 1e-6.
+"""
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from conftest import scene, midfusion, oracle, K320
+
+pytestmark = pytest.mark.gpu
+K160 = K320 / 2
+
+
+def ramp(w, h):
+    # 8-bit column ramp, one grey level per column (linear up to 255 columns)
+    return np.repeat(np.tile(np.arange(w, dtype=np.uint8)[None, :, None], (h, 1, 1)), 3, axis=2)
+
+
+def twist(x):
+    v, w = x[:3], x[3:]
+    M = np.zeros((4, 4))
+    M[:3, 3] = v
+    M[:3, :3] = [[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]]
+    return expm(M)
+
+
+def big_box():
+    c = 2.56
+    T0 = np.eye(4)
+    T0[:3, 3] = np.array([c, c, c - 1.5]) + np.array([0.05, 0.05, 1.0]) - 0.256
+    return scene.MovingBox([0.15, 0.12, 0.1], T0, [0.003, -0.002, 0.002, 0.01, 0.012, -0.008], [0.8, 0.3, 0.2],
+                           0.512, 128)
+
+
+def pair(use_icp, use_rgb, obj, use_ramp, seed, W, H, K):
+    mov = [big_box()] if obj else []
+    sc = scene.Scene(seed=seed, width=W, height=H, K=tuple(K), moving=mov)
+    P0 = sc.camera_pose(0).astype(np.float32)
+    kw = dict(use_icp=use_icp, use_rgb=use_rgb, robust='huber', robust_k=0.1)
+    g = midfusion.Kfusion(W, H, P0, bg_res=256, **kw)
+    o = oracle.Oracle(W, H, P0, fp64=True, **kw)
+    o.add_object(5.12, 256, np.eye(4))
+    for m in mov:
+        for x in (g, o):
+            x.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
+            x.set_moving(1, True)
+    for f in range(2):
+        d, rgb, lab = sc.render(f)
+        if use_ramp:
+            rgb = ramp(W, H)
+        for x in (g, o):
+            for i, m in enumerate(scene.masks_for_labels(lab, len(mov))):
+                x.set_mask(i, m)
+        if f == 1:
+            vtx, nrm, lb = o.maps()
+            g.set_maps(vtx, nrm, lb)
+            g.setPose(-1, o.pose(-1).astype(np.float32))
+            if obj:
+                ov, on = o.object_maps(1)
+                g.set_object_maps(1, ov, on)
+                g.setPose(1, o.pose(1).astype(np.float32))
+        g.preprocessing(d, rgb)
+        o.preprocess(d, rgb)
+        g.track_camera(K, f)
+        o.track_camera(K, f)
+        if f == 1:
+            break
+        g.track_objects(K, f)
+        o.track_objects(K, f)
+        g.integration(K, f)
+        o.integrate(K, f)
+        g.raycasting(K, f)
+        o.raycast(K, f)
+    g.sync()
+    oid = 1 if obj else -1
+    T = np.linalg.inv(o.pose(-1)) @ o.pose(1) if obj else o.pose(-1)
+    rng = np.random.default_rng(seed + 2)
+    T = twist(np.r_[rng.normal(0, 1e-3, 3), rng.normal(0, 1e-3, 3)]) @ T
+    return g, o, oid, T.astype(np.float32).astype(np.float64)
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-12))
+
+
+@pytest.mark.parametrize('seed', [1, 2, 3])
+@pytest.mark.parametrize('obj', [False, True])
+def test_sums_match_fp64_oracle(obj, seed):
+    g, o, oid, T = pair(True, True, obj, False, seed, 320, 240, K320)
+    for level in (2, 1, 0):
+        sg, so = g.residual_eval(oid, K320, level, T), o.eval(oid, level, T)
+        r = {'loss': rel(sg[0], so[0]), 'JtWe': rel(sg[1:7], so[1:7]), 'JtWJ': rel(sg[7:28], so[7:28]),
+             'inliers': (sg[28], so[28])}
+        print('obj' if obj else 'camera', 'seed', seed, 'level', level, r)
+        assert so[28] > 100
+        assert abs(sg[28] - so[28]) <= max(3, 2e-3 * so[28])
+        assert r['loss'] < 2e-3 and r['JtWe'] < 2e-3 and r['JtWJ'] < 2e-3, r
+    g.close()
+
+
+@pytest.mark.parametrize('use_icp,use_rgb,obj,ramp,h,tol', [
+    (True, False, False, False, 1e-9, 2e-3),    # camera ICP
+    (False, True, False, True, 1e-6, 1e-2),     # camera photometric (border strip: 1%)
+    (False, True, True, True, 1e-6, 2e-3),      # object photometric
+])
+def test_gpu_gradient_matches_fp64_central_differences(use_icp, use_rgb, obj, ramp, h, tol):
+    g, o, oid, T = pair(use_icp, use_rgb, obj, ramp, 1, 160, 120, K160)
+    grad = -g.residual_eval(oid, K160, 0, T)[1:7]
+    fd = np.zeros(6)
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = h
+        fd[k] = (o.eval(oid, 0, twist(e) @ T)[0] - o.eval(oid, 0, twist(-e) @ T)[0]) / (2 * h)
+    print('gpu grad', grad, 'fd', fd, rel(grad, fd))
+    assert rel(grad, fd) < tol
+    g.close()
