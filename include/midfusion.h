@@ -62,6 +62,30 @@ int mf_object_create(mf_ctx* ctx, float size_m, int res, const float T_w_o[16], 
 /* Object::set_static_state (object.cpp:317-320): moving objects are tracked
  * by trackEachObject and masked out of the camera re-track. */
 int mf_object_set_moving(mf_ctx* ctx, int id, int moving);
+/* Motion detection (Kfusion::tracking, kfusion.cpp:286-354): on, after the
+ * camera's first pass, the motion residuals (Tracking::compute_residuals,
+ * tracking.cpp:1535-1747) against the maps before fusion classify every
+ * object in view by its inlier ratio (moving iff 0.05 < ratio < 0.8,
+ * check_static_state rendering.cpp:907-947), which replaces the caller's
+ * moving flags for those objects; one host readback per tracked frame.  Off
+ * (default), the moving set of a frame is the caller's flags. */
+int mf_set_motion_detection(mf_ctx* ctx, int on);
+/* Last motion check, per object id < cap: {pixels in view, inliers} and the
+ * current moving flag.  Returns the object count. */
+int mf_get_motion(mf_ctx* ctx, int64_t* pixels, int64_t* inliers, int32_t* moving, int cap);
+/* Gating state of the last tracking call: gate_rendered = {more than one
+ * instance in the last raycast, instances in the last raycast before fusion};
+ * in_view = objects_in_view_ flags and move_set = move_obj_set flags, by id. */
+int mf_get_motion_state(mf_ctx* ctx, int32_t gate_rendered[2], int32_t* in_view, int32_t* move_set, int cap);
+/* Residual codes of the last motion check (2*W*H int32, ICP rows then RGB
+ * rows, the TrackData.result values of compute_residuals). */
+int mf_get_motion_codes(mf_ctx* ctx, int32_t* codes);
+/* Tracking::compute_residuals (tracking.cpp:1535-1747) at pose T_w_l against
+ * the given model vertex and normal maps and vertex map before fusion (AoS
+ * float3, W*H each), with this frame's live and reference images and T_w_r:
+ * 2*W*H result codes.  Valid between mf_track_camera and mf_track_objects. */
+int mf_motion_residuals(mf_ctx* ctx, const float k[4], const float T_w_l[16], const float* vertex,
+                        const float* normal, const float* vertex_bf, int32_t* codes);
 /* Generalised integration mask of an instance (instance_seg::generalize_label
  * output, segmentation.cpp:10-47): fg value >= 0 inside, -1 outside.  NULL
  * clears it (the instance is then not integrated). */
@@ -118,6 +142,9 @@ long mf_object_node_count(mf_ctx* ctx, int id);
 /* Level-0 tracking result codes of an instance's last pass (TrackData.result
  * of Object::trackresult_, ICP rows then RGB rows: 2*W*H int32). */
 int mf_get_trackresult(mf_ctx* ctx, int id, int32_t* codes);
+/* Overwrite an instance's trackresult_ codes (2*W*H int32), e.g. to integrate
+ * from codes computed elsewhere. */
+int mf_set_trackresult(mf_ctx* ctx, int id, const int32_t* codes);
 /* Per-iteration GN trace since the last clear (the first clear turns tracing
  * on): rows of 25 doubles {GN state, level, iteration, error, inliers, flag
  * (0 step, 1 converged, 2 step-back, 3 no inliers), pose[16], FP64

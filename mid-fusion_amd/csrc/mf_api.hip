@@ -20,11 +20,11 @@ namespace {
 
 enum KClass {
   KC_CONVERT, KC_BILATERAL, KC_HALF, KC_LEVEL, KC_V2D, KC_TRACK, KC_TRACK_OBJ, KC_SOLVE, KC_FUSED, KC_GNSMALL, KC_PARAMS, KC_ALLOC,
-  KC_COMMIT, KC_INTEGRATE, KC_RAYCAST, KC_COUNT
+  KC_COMMIT, KC_INTEGRATE, KC_RAYCAST, KC_MOTION, KC_COUNT
 };
 const char* kNames[KC_COUNT] = {"convert", "bilateral", "half_sample", "level_maps", "vertex2depth", "track_rows",
                                 "track_rows_obj", "track_solve", "track_fused", "gn_control", "obj_params", "alloc_band", "alloc_commit",
-                                "integrate", "raycast"};
+                                "integrate", "raycast", "motion"};
 
 struct Object {
   int id = 0;
@@ -37,6 +37,7 @@ struct Object {
   float* mv[3] = {nullptr, nullptr, nullptr};
   float* mn[3] = {nullptr, nullptr, nullptr};
   int8_t* codes = nullptr;       // level-0 result codes of this instance's last pass
+  float* mvbf[3] = {nullptr, nullptr, nullptr};   // vertex map of the last raycast before fusion
   float* rD[MAX_LEVELS] = {};    // rendered reference depth pyramid
   ObjParams* prm = nullptr;
 };
@@ -84,6 +85,18 @@ struct mf_ctx {
   TrackInst inst_host[MAX_INST];   // mirrors of the device descriptor tables
   RayObj ray_host[MAX_INST];
   int ray_n = -1;
+  // motion detection and gating (kfusion.cpp:286-471)
+  bool motion_auto = false;
+  MotionDev* md = nullptr;
+  float *vbf[3] = {}, *nbf[3] = {};   // raycast before fusion: vertex and normal planes
+  int32_t* label_bf = nullptr;
+  int8_t* motion_codes = nullptr;     // [2*N0] residual codes of the last motion check
+  Mat4* vcam = nullptr;               // [MAX_INST] virtual camera poses by object id
+  RayObj* rayobjs_bf = nullptr;       // raycast table writing the objects' maps before fusion
+  RayObj ray_bf_host[MAX_INST];
+  float* eval_maps = nullptr;         // mf_motion_residuals inputs (9 planes), allocated on first use
+  Mat4* eval_T = nullptr;
+  int8_t* eval_codes = nullptr;
   std::vector<int> ids_host;
   std::vector<Object*> objs;
   float ray_eps = -1.f;
@@ -193,12 +206,16 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   }
   o->codes = dalloc<int8_t>(c, 2 * c->N0);
   {
+    float* b = dalloc<float>(c, 3 * c->N0);
+    for (int k = 0; k < 3; k++) o->mvbf[k] = b ? b + k * c->N0 : nullptr;
+  }
+  {
     float* b = dalloc<float>(c, rd_offset(c, c->cfg.n_levels));
     for (int l = 0; l < c->cfg.n_levels; l++) o->rD[l] = b ? b + rd_offset(c, l) : nullptr;
   }
   o->prm = dalloc<ObjParams>(c, 1);
   if (!v.slot || !v.req || !v.nodes || !v.tsdf || !v.fg || !v.c || !v.fgw || !o->mask ||
-      !o->codes || !o->prm || !o->mv[2] || !o->mn[2]) {
+      !o->codes || !o->prm || !o->mv[2] || !o->mn[2] || !o->mvbf[2]) {
     delete o;
     return fail(c, MF_E_OOM, "device allocation failed");
   }
@@ -207,6 +224,11 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   hipMemcpyAsync(o->mn[0], inv.data(), c->N0 * sizeof(float), hipMemcpyHostToDevice, c->stream);
   const Mat4 M = m4(T);
   hipMemcpyAsync(c->mats + 3 + o->id, &M, sizeof(Mat4), hipMemcpyHostToDevice, c->stream);
+  // virtual_camera_pose_ starts at the current camera pose
+  // (kfusion.cpp:1058-1061); a new object is in view (kfusion.cpp:122, 1079)
+  hipMemcpyAsync(c->vcam + o->id, c->mats + 0, sizeof(Mat4), hipMemcpyDeviceToDevice, c->stream);
+  const int one = 1;
+  hipMemcpyAsync(&c->md->in_view[o->id], &one, sizeof one, hipMemcpyHostToDevice, c->stream);
   hipStreamSynchronize(c->stream);
   // mapping work lists cover every object's pool (grown with each object;
   // the previous lists stay allocated until the context is destroyed)
@@ -256,7 +278,8 @@ void render_reference(mf_ctx* c, int first, int n) {
 // One coarse-to-fine GN run over `n` instances sharing a launch
 // (trackLiveFrame / trackEachObject loops, tracking.cpp:242-335, 624-690):
 // one fused track+reduce+finalize launch per (level, iteration).
-void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, const int32_t* label_mask) {
+void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, const int32_t* label_mask,
+            const uint8_t* mask_labels = nullptr) {
   TrackSync sy;
   sy.trace = c->trace_on ? c->trace : nullptr; sy.trace_n = c->trace_n;
   sy.trace_cap = c->trace_cap;
@@ -278,6 +301,7 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
   for (int l = c->cfg.n_levels - 1; l >= 0; --l) {
     p = base_launch(c, k, l, object);
     p.label = label_mask;
+    if (mask_labels) p.moving = mask_labels;
     const int nblk = track_blocks(p.w * p.h * kinds, c->track_rpt[l]);
     const bool fused = c->track_fused[l] != 0;
     if (!fused) finish();
@@ -368,9 +392,9 @@ void set_mapobjs(mf_ctx* c, int& n_int, int& n_all, long& sum_nb3, long& sum_cap
     MapObj m;
     memset(&m, 0, sizeof m);
     m.v = o->v; m.prm = o->prm; m.mask = o->mask; m.id = o->id;
-    // refine_mask_use_motion with the instance's last tracking rows; static
-    // objects carry the camera's rows (kfusion.cpp:398-401)
-    m.codes = (o->id == 0 || o->moving) ? o->codes : c->objs[0]->codes;
+    // refine_mask_use_motion with the instance's trackresult_ (static objects
+    // hold copies of the camera's rows, kfusion.cpp:399-402, 450-454)
+    m.codes = o->codes;
     t[MAX_INST + n_all++] = m;
     if (o->has_mask) {
       t[n_int++] = m;
@@ -407,6 +431,7 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   mf_ctx* c = new mf_ctx();
   memset(c->inst_host, 0, sizeof c->inst_host);
   memset(c->ray_host, 0, sizeof c->ray_host);
+  memset(c->ray_bf_host, 0, sizeof c->ray_bf_host);
   c->cfg = *cfg;
   if (c->cfg.track_rate <= 0) c->cfg.track_rate = 1;
   if (c->cfg.integ_rate <= 0) c->cfg.integ_rate = 1;
@@ -424,6 +449,15 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
     for (int k = 0; k < 3; k++) { c->vtx[k] = b ? b + k * c->N0 : nullptr; c->nrm[k] = b ? b + (3 + k) * c->N0 : nullptr; }
   }
   c->label = dalloc<int32_t>(c, c->N0);
+  {
+    float* b = dalloc<float>(c, 6 * c->N0);
+    for (int k = 0; k < 3; k++) { c->vbf[k] = b ? b + k * c->N0 : nullptr; c->nbf[k] = b ? b + (3 + k) * c->N0 : nullptr; }
+  }
+  c->label_bf = dalloc<int32_t>(c, c->N0);
+  c->motion_codes = dalloc<int8_t>(c, 2 * c->N0);
+  c->md = dalloc<MotionDev>(c, 1);
+  c->vcam = dalloc<Mat4>(c, MAX_INST);
+  c->rayobjs_bf = dalloc<RayObj>(c, MAX_INST);
   c->mats = dalloc<Mat4>(c, 3 + MAX_INST);
   c->gn = dalloc<GNState>(c, MAX_INST);
   c->inst = dalloc<TrackInst>(c, MAX_INST);
@@ -437,6 +471,7 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->gauss = dalloc<float>(c, 5);
   c->stats = dalloc<unsigned long long>(c, (size_t)N_STATS * STAT_SLOTS);
   c->obj_ids = dalloc<int>(c, MAX_INST);
+  ok = ok && c->vbf[2] && c->label_bf && c->motion_codes && c->md && c->vcam && c->rayobjs_bf;
   ok = ok && c->stats && c->obj_ids && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
        c->inst && c->partial && c->moving && c->trace && c->trace_n && c->rayobjs && c->mapobjs && c->gauss;
   if (!ok) { mf_destroy(c); return nullptr; }
@@ -486,6 +521,44 @@ int mf_object_set_moving(mf_ctx* c, int id, int moving) {
   return 0;
 }
 
+int mf_set_motion_detection(mf_ctx* c, int on) {
+  c->motion_auto = on != 0;
+  return 0;
+}
+int mf_get_motion(mf_ctx* c, int64_t* pixels, int64_t* inliers, int32_t* moving, int cap) {
+  MotionDev m;
+  uint8_t mv[MAX_INST];
+  hipMemcpyAsync(&m, c->md, sizeof m, hipMemcpyDeviceToHost, c->stream);
+  hipMemcpyAsync(mv, c->moving, MAX_INST, hipMemcpyDeviceToHost, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  const int n = (int)c->objs.size();
+  for (int j = 0; j < n && j < cap; j++) {
+    if (pixels) pixels[j] = m.counts[2 * j];
+    if (inliers) inliers[j] = m.counts[2 * j + 1];
+    if (moving) moving[j] = mv[j];
+  }
+  return n;
+}
+int mf_get_motion_state(mf_ctx* c, int32_t gate_rendered[2], int32_t* in_view, int32_t* move_set, int cap) {
+  MotionDev m;
+  hipMemcpyAsync(&m, c->md, sizeof m, hipMemcpyDeviceToHost, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  const int n = (int)c->objs.size();
+  if (gate_rendered) { gate_rendered[0] = m.gate; gate_rendered[1] = m.rendered_n; }
+  for (int j = 0; j < n && j < cap; j++) {
+    if (in_view) in_view[j] = m.in_view[j];
+    if (move_set) move_set[j] = m.move_set[j];
+  }
+  return n;
+}
+int mf_get_motion_codes(mf_ctx* c, int32_t* codes) {
+  std::vector<int8_t> t(2 * c->N0);
+  hipMemcpyAsync(t.data(), c->motion_codes, t.size(), hipMemcpyDeviceToHost, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  for (size_t i = 0; i < t.size(); i++) codes[i] = t[i];
+  return 0;
+}
+
 int mf_object_set_mask(mf_ctx* c, int id, const float* mask) {
   if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
   Object* o = c->objs[id];
@@ -531,6 +604,59 @@ int mf_preprocess(mf_ctx* c, const uint16_t* depth, const uint8_t* rgb, int iw, 
   return mf_preprocess_device(c, c->in_depth, rgb ? c->in_rgb : nullptr, iw, ih, filter);
 }
 
+// Raycast descriptor tables: the model-map table (each object's maps) and the
+// table of the raycasts before fusion (the objects' vertex maps before fusion
+// only).  They change only with the object set.
+void upload_ray_tables(mf_ctx* c) {
+  const int n = (int)c->objs.size();
+  if (c->ray_n == n) return;
+  for (int i = 0; i < n; i++) {
+    Object* o = c->objs[i];
+    RayObj& r = c->ray_host[i];
+    r.v = o->v; r.prm = o->prm; r.label = o->id;
+    r.mvx = o->mv[0]; r.mvy = o->mv[1]; r.mvz = o->mv[2];
+    r.mnx = o->mn[0]; r.mny = o->mn[1]; r.mnz = o->mn[2];
+    RayObj& b = c->ray_bf_host[i];
+    b = r;
+    b.mnx = b.mny = b.mnz = nullptr;
+    if (i == 0) { b.mvx = b.mvy = b.mvz = nullptr; }
+    else { b.mvx = o->mvbf[0]; b.mvy = o->mvbf[1]; b.mvz = o->mvbf[2]; }
+  }
+  hipMemcpyAsync(c->rayobjs, c->ray_host, n * sizeof(RayObj), hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(c->rayobjs_bf, c->ray_bf_host, n * sizeof(RayObj), hipMemcpyHostToDevice, c->stream);
+  hipStreamSynchronize(c->stream);
+  c->ray_n = n;
+}
+
+// Raycast of every object at the current camera pose and the objects' current
+// poses into the maps before fusion (kfusion.cpp:295-297, 432-434).
+void raycast_before_fusion(mf_ctx* c, const float k[4], const int* in_view, const int* gate) {
+  hipStream_t s = c->stream;
+  const int n = (int)c->objs.size();
+  { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats, c->mapobjs + MAX_INST, n, k[0], k[1], k[2], k[3], 0, c->work); }
+  Timed t(c, KC_RAYCAST);
+  launch_raycast(s, c->rayobjs_bf, n, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, c->vbf[0], c->vbf[1], c->vbf[2],
+                 c->nbf[0], c->nbf[1], c->nbf[2], c->label_bf, c->timing ? c->stats : nullptr, in_view, gate);
+}
+
+// compute_residuals launch of one instance against model maps `mvn` (6
+// planes) and the vertex map before fusion `vbf` (3 planes).
+MotionLaunch motion_launch(mf_ctx* c, const float k[4], const Mat4* T_w_l, const float* mvn, const float* vbf,
+                           int8_t* codes, const int* gate) {
+  MotionLaunch m;
+  memset(&m, 0, sizeof m);
+  m.W = c->W; m.H = c->H;
+  m.live = c->live[0].D; m.ref = c->ref[0].D;
+  m.mvn = mvn; m.vbf = vbf;
+  m.T_w_l = T_w_l; m.inv_T_w_r = c->mats + 2;
+  m.fx = k[0]; m.fy = k[1]; m.cx = k[2]; m.cy = k[3];
+  m.use_icp = c->cfg.use_icp; m.use_rgb = c->cfg.use_rgb;
+  m.sigma_bright = (float)(sqrtf(100.0f) / 255.0);   // tracking.cpp:764, 769
+  m.threshold = 1.5f;                                 // residual_threshold_ (kernels.h:95)
+  m.codes = codes; m.gate = gate;
+  return m;
+}
+
 int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
   c->pending_objects = false;
   if (frame % c->cfg.track_rate != 0) return 0;
@@ -558,23 +684,63 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
   float* camD[MAX_LEVELS];
   for (int l = 0; l < MAX_LEVELS; l++) camD[l] = bg->rD[l];
   set_inst(c, 0, c->vtx, c->nrm, camD, bg->codes, 0);
-  std::vector<Object*> mov;
-  for (size_t i = 1; i < c->objs.size(); i++) if (c->objs[i]->moving) mov.push_back(c->objs[i]);
-  std::vector<uint64_t> key = base_key(c, 1, k);
-  key.push_back(mov.size());
-  const int r = run_graph(c, key, [&]() {
+  const int nobj = (int)c->objs.size();
+  const bool multi = nobj >= 2;
+  const bool autom = multi && c->motion_auto;
+  if (multi) {
+    if (c->ray_eps < 0) c->ray_eps = exp2f(-(float)c->objs[0]->v.max_level);
+    upload_ray_tables(c);
+    int n_int, n_all;
+    long sum_nb3, sum_cap;
+    set_mapobjs(c, n_int, n_all, sum_nb3, sum_cap);
+  }
+  MotionDev* md = c->md;
+  // first pass, then (with objects) the raycast before fusion, the in-view
+  // set and the moving set (kfusion.cpp:254-354)
+  auto first_pass = [&]() {
     pyramid();
     { Timed t(c, KC_GNSMALL); launch_camera_begin(s, c->mats, c->gn, k[0], k[1], k[2], k[3]); }
     if (c->cfg.use_rgb) render_reference(c, 0, 1);
     gn_run(c, k, 0, 0, 1, nullptr);
-    if (!mov.empty()) {
-      // re-track the camera with the moving objects' raycast pixels masked
-      // (kfusion.cpp:359-392, trackLiveFrame mask overload)
-      { Timed t(c, KC_GNSMALL); launch_gn_begin_camera(s, c->gn + 0, c->mats + 0, c->mats + 1, k[0], k[1], k[2], k[3]); }
-      gn_run(c, k, 0, 0, 1, c->label);
+    if (!multi) return;
+    { Timed t(c, KC_MOTION); launch_motion_gate(s, md, c->label, (int)c->N0, nobj); }
+    raycast_before_fusion(c, k, nullptr, &md->gate);
+    { Timed t(c, KC_MOTION); launch_motion_inview(s, md, c->label_bf, (int)c->N0, nobj, c->moving, autom ? 0 : 1); }
+    if (autom) {
+      MotionLaunch m = motion_launch(c, k, c->mats + 0, c->vtx[0], c->vbf[0], c->motion_codes, &md->gate);
+      m.label_bf = c->label_bf; m.counts = md->counts;
+      { Timed t(c, KC_MOTION); launch_motion_rows(s, m); }
+      { Timed t(c, KC_MOTION); launch_motion_classify(s, md, nobj, c->moving); }
     }
-  });
-  if (r < 0) return r;
+  };
+  // re-track the camera with the moving objects' raycast pixels masked
+  // (kfusion.cpp:357-392, trackLiveFrame mask overload)
+  auto retrack = [&]() {
+    { Timed t(c, KC_GNSMALL); launch_gn_begin_camera(s, c->gn + 0, c->mats + 0, c->mats + 1, &md->retrack, k[0], k[1], k[2], k[3]); }
+    gn_run(c, k, 0, 0, 1, c->label, md->move_set);
+  };
+  std::vector<uint64_t> key = base_key(c, 1, k);
+  key.push_back((uint64_t)nobj | ((uint64_t)autom << 32));
+  if (!autom) {
+    bool any = false;
+    for (int i = 1; i < nobj; i++) any = any || c->objs[i]->moving;
+    key.push_back(any ? 1 : 0);
+    const int r = run_graph(c, key, [&]() { first_pass(); if (any) retrack(); });
+    if (r < 0) return r;
+  } else {
+    if (run_graph(c, key, first_pass) < 0) return -1;
+    // the moving decision selects the launches that follow: read it back
+    uint8_t mv[MAX_INST], ms[MAX_INST];
+    hipMemcpyAsync(mv, c->moving, MAX_INST, hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(ms, md->move_set, MAX_INST, hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+    bool any = false;
+    for (int i = 1; i < nobj; i++) { c->objs[i]->moving = mv[i] != 0; any = any || ms[i] != 0; }
+    if (any) {
+      std::vector<uint64_t> k2 = base_key(c, 5, k);
+      if (run_graph(c, k2, retrack) < 0) return -1;
+    }
+  }
   c->pending_objects = true;
   return 1;
 }
@@ -585,28 +751,57 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
   c->pending_objects = false;
   hipStream_t s = c->stream;
   const int L = c->cfg.n_levels;
-  std::vector<Object*> mov;
-  for (size_t i = 1; i < c->objs.size(); i++) if (c->objs[i]->moving) mov.push_back(c->objs[i]);
-  if (!mov.empty()) {
-    // trackEachObject for every moving object (kfusion.cpp:395-414)
+  const int nobj = (int)c->objs.size();
+  std::vector<Object*> mov, stat;
+  for (int i = 1; i < nobj; i++) (c->objs[i]->moving ? mov : stat).push_back(c->objs[i]);
+  if (nobj >= 2) {
+    MotionDev* md = c->md;
+    Object* bg = c->objs[0];
     std::vector<int> ids(mov.size());
     for (size_t j = 0; j < mov.size(); j++) {
       Object* o = mov[j];
       ids[j] = 3 + o->id;
       set_inst(c, 1 + (int)j, o->mv, o->mn, o->rD, o->codes, 1 + (int)j);
     }
-    set_ids(c, ids);
+    if (!mov.empty()) set_ids(c, ids);
     std::vector<uint64_t> key = base_key(c, 2, k);
     key.push_back((uint64_t)c->cfg.batch_objects);
+    key.push_back((uint64_t)nobj);
     for (int id : ids) key.push_back((uint64_t)id);
     const int r = run_graph(c, key, [&]() {
-      { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, 1, (int)mov.size(), c->obj_ids, c->mats, c->mats + 0, c->mats + 1); }
-      if (c->cfg.use_rgb) {
-        if (c->cfg.batch_objects) render_reference(c, 1, (int)mov.size());
-        else for (size_t j = 0; j < mov.size(); j++) render_reference(c, 1 + (int)j, 1);
+      if (!mov.empty()) {
+        // trackEachObject for every moving object when the last raycast shows
+        // more than one instance (kfusion.cpp:292, 395-414)
+        { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, 1, (int)mov.size(), c->obj_ids, c->mats, c->mats + 0, c->mats + 1, 1, &md->gate); }
+        if (c->cfg.use_rgb) {
+          if (c->cfg.batch_objects) render_reference(c, 1, (int)mov.size());
+          else for (size_t j = 0; j < mov.size(); j++) render_reference(c, 1 + (int)j, 1);
+        }
+        if (c->cfg.batch_objects) gn_run(c, k, 1, 1, (int)mov.size(), nullptr);
+        else for (size_t j = 0; j < mov.size(); j++) gn_run(c, k, 1, 1 + (int)j, 1, nullptr);
+        { Timed t(c, KC_GNSMALL); launch_virtual_cams(s, c->gn, 1, (int)mov.size(), c->obj_ids, c->vcam); }
+        // objects moving this frame: the camera's rows become motion residuals
+        // against the maps before fusion, and a second raycast before fusion
+        // renders the objects at their new poses (kfusion.cpp:416-441)
+        { Timed t(c, KC_MOTION);
+          launch_motion_rows(s, motion_launch(c, k, c->mats + 0, c->vtx[0], c->vbf[0], bg->codes, &md->move_any)); }
+        raycast_before_fusion(c, k, md->in_view, &md->move_any);
+        { Timed t(c, KC_MOTION); launch_motion_after(s, md, c->label_bf, (int)c->N0, nobj, c->cfg.use_rgb); }
+        // each moving object's rows: motion residuals at its virtual camera
+        // pose (kfusion.cpp:445-467)
+        for (Object* o : mov) {
+          Timed t(c, KC_MOTION);
+          launch_motion_rows(s, motion_launch(c, k, c->vcam + o->id, o->mv[0], o->mvbf[0], o->codes, &md->cond2));
+        }
+      } else {
+        Timed t(c, KC_MOTION);
+        launch_motion_after(s, md, nullptr, (int)c->N0, nobj, c->cfg.use_rgb);
       }
-      if (c->cfg.batch_objects) gn_run(c, k, 1, 1, (int)mov.size(), nullptr);
-      else for (size_t j = 0; j < mov.size(); j++) gn_run(c, k, 1, 1 + (int)j, 1, nullptr);
+      // static objects carry the camera's rows (kfusion.cpp:399-402, 450-454)
+      for (Object* o : stat) {
+        Timed t(c, KC_MOTION);
+        launch_copy_codes(s, bg->codes, o->codes, 2 * c->N0, &md->copy_static);
+      }
     });
     if (r < 0) return r;
   }
@@ -662,18 +857,7 @@ int mf_raycast(mf_ctx* c, const float k[4], unsigned frame) {
   hipStream_t s = c->stream;
   if (c->ray_eps < 0) c->ray_eps = exp2f(-(float)c->objs[0]->v.max_level);   // ray_iterator's static epsilon
   const int n = (int)c->objs.size();
-  if (c->ray_n != n) {   // descriptor table changes only with the object set
-    for (int i = 0; i < n; i++) {
-      Object* o = c->objs[i];
-      RayObj& r = c->ray_host[i];
-      r.v = o->v; r.prm = o->prm; r.label = o->id;
-      r.mvx = o->mv[0]; r.mvy = o->mv[1]; r.mvz = o->mv[2];
-      r.mnx = o->mn[0]; r.mny = o->mn[1]; r.mnz = o->mn[2];
-    }
-    hipMemcpyAsync(c->rayobjs, c->ray_host, n * sizeof(RayObj), hipMemcpyHostToDevice, s);
-    hipStreamSynchronize(s);
-    c->ray_n = n;
-  }
+  upload_ray_tables(c);
   std::vector<uint64_t> key = base_key(c, 4, k);
   key.push_back((uint64_t)n);
   key.push_back(fbits(c->ray_eps));
@@ -683,8 +867,10 @@ int mf_raycast(mf_ctx* c, const float k[4], unsigned frame) {
   return run_graph(c, key, [&]() {
     { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats, c->mapobjs + MAX_INST, n_all, k[0], k[1], k[2], k[3], 0, c->work); }
     Timed t(c, KC_RAYCAST);
+    // objects in view only (kfusion.cpp:557-558)
     launch_raycast(s, c->rayobjs, n, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, c->vtx[0], c->vtx[1],
-                   c->vtx[2], c->nrm[0], c->nrm[1], c->nrm[2], c->label, c->timing ? c->stats : nullptr);
+                   c->vtx[2], c->nrm[0], c->nrm[1], c->nrm[2], c->label, c->timing ? c->stats : nullptr,
+                   c->md->in_view, nullptr);
   }) < 0 ? -1 : 1;
 }
 
@@ -802,10 +988,45 @@ long mf_object_node_count(mf_ctx* c, int id) {
 int mf_get_trackresult(mf_ctx* c, int id, int32_t* codes) {
   if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
   Object* o = c->objs[id];
-  const int8_t* src = (id == 0 || o->moving) ? o->codes : c->objs[0]->codes;
+  const int8_t* src = o->codes;
   std::vector<int8_t> t(2 * c->N0);
   hipMemcpyAsync(t.data(), src, t.size(), hipMemcpyDeviceToHost, c->stream);
   hipStreamSynchronize(c->stream);
+  for (size_t i = 0; i < t.size(); i++) codes[i] = t[i];
+  return 0;
+}
+
+int mf_set_trackresult(mf_ctx* c, int id, const int32_t* codes) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  std::vector<int8_t> t(2 * c->N0);
+  for (size_t i = 0; i < t.size(); i++) t[i] = (int8_t)codes[i];
+  hipMemcpyAsync(c->objs[id]->codes, t.data(), t.size(), hipMemcpyHostToDevice, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
+int mf_motion_residuals(mf_ctx* c, const float k[4], const float T16[16], const float* vertex, const float* normal,
+                        const float* vertex_bf, int32_t* codes) {
+  if (!c->pending_objects) return fail(c, MF_E_ARG, "call between mf_track_camera and mf_track_objects");
+  if (!c->eval_maps) {
+    c->eval_maps = dalloc<float>(c, 9 * c->N0);
+    c->eval_T = dalloc<Mat4>(c, 1);
+    c->eval_codes = dalloc<int8_t>(c, 2 * c->N0);
+    if (!c->eval_maps || !c->eval_T || !c->eval_codes) return fail(c, MF_E_OOM, "device allocation failed");
+  }
+  float* mv[3]; float* mn[3]; float* vb[3];
+  for (int k2 = 0; k2 < 3; k2++) {
+    mv[k2] = c->eval_maps + k2 * c->N0; mn[k2] = c->eval_maps + (3 + k2) * c->N0; vb[k2] = c->eval_maps + (6 + k2) * c->N0;
+  }
+  upload_planes(c, mv, vertex, c->N0);
+  upload_planes(c, mn, normal, c->N0);
+  upload_planes(c, vb, vertex_bf, c->N0);
+  const Mat4 T = m4(T16);
+  hipMemcpyAsync(c->eval_T, &T, sizeof T, hipMemcpyHostToDevice, c->stream);
+  launch_motion_rows(c->stream, motion_launch(c, k, c->eval_T, c->eval_maps, vb[0], c->eval_codes, nullptr));
+  std::vector<int8_t> t(2 * c->N0);
+  hipMemcpyAsync(t.data(), c->eval_codes, t.size(), hipMemcpyDeviceToHost, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
   for (size_t i = 0; i < t.size(); i++) codes[i] = t[i];
   return 0;
 }
@@ -959,7 +1180,8 @@ int mf_residual_eval(mf_ctx* c, int id, const float k[4], int level, const float
     const int oid = 3 + id;
     hipMemcpyAsync(c->obj_ids + MAX_INST - 1, &oid, sizeof oid, hipMemcpyHostToDevice, s);
     hipStreamSynchronize(s);
-    launch_gn_begin_objects(s, c->gn, state, 1, c->obj_ids + MAX_INST - 1, c->mats, c->mats + 0, c->mats + 1, 0);
+    launch_gn_begin_objects(s, c->gn, state, 1, c->obj_ids + MAX_INST - 1, c->mats, c->mats + 0, c->mats + 1, 0,
+                            nullptr);
     if (c->cfg.use_rgb) render_reference(c, slot, 1);
   }
   TrackLaunch p = base_launch(c, k, level, object);

@@ -67,6 +67,8 @@ struct GNState {
   int object;      // 1: object instance (zero-inlier break, pose composed with T_w_c2)
   Mat4* out_pose;  // written after every update: T_w_l, or volume_pose = T_w_c2 * T_c2_o2
   const Mat4* T_w_c2;
+  Mat4 T_final;    // T after the last solve (object: T_c2_o2)
+  int off;         // 1: this run is skipped entirely (no rows, no solves)
   double sums[32]; // last reduction (FP64)
 };
 

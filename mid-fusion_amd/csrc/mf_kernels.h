@@ -60,14 +60,67 @@ void launch_gn_fused(hipStream_t s, const TrackLaunch& p, const TrackInst* inst,
 // GN state `state` gets T, then rows and the FP64 sums (into GNState::sums).
 void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNState* st, int state, const Mat4& T,
                  float* partial, int nblk);
+// Motion gating state on the device (Kfusion::tracking, kfusion.cpp:286-471),
+// so that every decision of a frame stays stream-ordered.
+struct MotionDev {
+  int gate;          // the last raycast shows more than one instance (kfusion.cpp:292)
+  int move_any;      // the moving set of this frame is not empty
+  int retrack;       // move_any and the raycast before fusion shows an instance (kfusion.cpp:358-359)
+  int cond2;         // moving objects take motion rows (kfusion.cpp:445-446)
+  int copy_static;   // static objects take the camera's rows (kfusion.cpp:399-402, 450-454)
+  int rendered_n;    // instances in the last raycast before fusion (persists across frames)
+  int pad[2];
+  int present[MAX_INST];       // scratch: labels present in an image
+  int in_view[MAX_INST];       // objects_in_view_ by label
+  int counts[2 * MAX_INST];    // {pixels, inliers} by label of the motion check
+  uint8_t move_set[MAX_INST];  // move_obj_set by label
+};
+// Motion residuals of one instance over the full-resolution live frame
+// (Tracking::compute_residuals, tracking.cpp:1535-1747).
+struct MotionLaunch {
+  int W, H;
+  const float* live;        // live level-0 planes (PL_*)
+  const float* ref;         // reference level-0 planes (PL_I read)
+  const float* mvn;         // model vertex (0-2) and normal (3-5) planes
+  const float* vbf;         // vertex planes of a raycast before fusion
+  const Mat4* T_w_l;        // pose of the residual
+  const Mat4* inv_T_w_r;    // inverse(T_w_r)
+  float fx, fy, cx, cy;
+  int use_icp, use_rgb;
+  float sigma_bright, threshold;
+  int8_t* codes;            // out: ICP codes then RGB codes [2*W*H]
+  const int* gate;          // run only if *gate != 0 (null: always)
+  const int32_t* label_bf;  // optional: labels of the raycast before fusion, for
+  int* counts;              //   per-label {pixels, inliers} counts [2*MAX_INST]
+};
+void launch_motion_rows(hipStream_t s, const MotionLaunch& m);
+// Frame-start clear of the motion state and the presence of the last
+// raycast's labels, then the gate (and the in-view set cleared under it).
+void launch_motion_gate(hipStream_t s, MotionDev* md, const int32_t* label, int n, int nobj);
+// After the raycast before fusion: in-view set and rendered count; with
+// `manual`, the moving set is the caller's moving flags.
+void launch_motion_inview(hipStream_t s, MotionDev* md, const int32_t* label_bf, int n, int nobj,
+                          const uint8_t* moving, int manual);
+// check_static_state decision from the counts (kfusion.cpp:314-354): updates
+// the moving flags and the moving set.
+void launch_motion_classify(hipStream_t s, MotionDev* md, int nobj, uint8_t* moving);
+// Object phase: rendered count after the second raycast before fusion (when
+// objects move), then cond2 and copy_static.
+void launch_motion_after(hipStream_t s, MotionDev* md, const int32_t* label_bf, int n, int nobj, int use_rgb);
+// virtual_camera_pose_ of tracked objects: vcam[ids[j] - 3] for instances
+// [first, first + n), unless the run was skipped.
+void launch_virtual_cams(hipStream_t s, const GNState* st, int first, int n, const int* ids, Mat4* vcam);
+// dst <- src (2*n codes) when *gate != 0
+void launch_copy_codes(hipStream_t s, const int8_t* src, int8_t* dst, size_t n2, const int* gate);
 // Frame start for the camera: T_w_r = T_w_c, inverse(T_w_r), GN state 0.
 void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy);
 // GN state of a camera pass starting from the current camera pose.
-void launch_gn_begin_camera(hipStream_t s, GNState* st, Mat4* T_w_l, const Mat4* T_w_r, float fx, float fy,
+void launch_gn_begin_camera(hipStream_t s, GNState* st, Mat4* T_w_l, const Mat4* T_w_r, const int* gate, float fx, float fy,
                             float cx, float cy);
 // GN states of `n` objects: states[first+j] <- object poses poses[ids[j]].
+// With a gate, the runs are skipped unless *gate != 0.
 void launch_gn_begin_objects(hipStream_t s, GNState* st, int first, int n, const int* ids, Mat4* poses,
-                             const Mat4* T_w_c2, const Mat4* T_w_c1, int publish = 1);
+                             const Mat4* T_w_c2, const Mat4* T_w_c1, int publish = 1, const int* gate = nullptr);
 
 // mapping (mf_map.hip)
 // One object of a mapping launch (device table, indexed by the launch's
@@ -116,8 +169,12 @@ struct RayObj {
   int label;
   float* mvx; float* mvy; float* mvz; float* mnx; float* mny; float* mnz;   // per-object output maps
 };
+// raycastObjectList over `objs`: objects whose label is not flagged in
+// `in_view` are skipped unless no object is flagged (null: every object); the
+// per-object maps are written where the object's pointers are set; with a
+// gate, nothing runs unless *gate != 0.
 void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,
                     float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label,
-                    unsigned long long* stats);
+                    unsigned long long* stats, const int* in_view, const int* gate);
 
 }  // namespace mf

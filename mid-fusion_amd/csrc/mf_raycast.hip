@@ -253,7 +253,9 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
                                                  float* __restrict__ vy, float* __restrict__ vz,
                                                  float* __restrict__ nx, float* __restrict__ ny,
                                                  float* __restrict__ nz, int32_t* __restrict__ label,
-                                                 unsigned long long* __restrict__ stats) {
+                                                 unsigned long long* __restrict__ stats,
+                                                 const int* __restrict__ in_view, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   __shared__ unsigned cnt[4];
   __shared__ uint32_t st_par[MAX_LG + 1][RT];
   __shared__ float st_tmax[MAX_LG + 1][RT];
@@ -268,8 +270,13 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
   int miss = 0;
   f3 out_v = mk3(0.f, 0.f, 0.f), out_n = mk3((float)INVALID, 0.f, 0.f);
   int out_l = INVALID;
+  // should_raycast_all when the in-view set is empty (rendering.cpp:16, 27-33)
+  bool all = true;
+  if (in_view)
+    for (int oi = 0; oi < nobj; oi++) if (in_view[objs[oi].label]) all = false;
   for (int oi = 0; oi < nobj; oi++) {
     const RayObj& O = objs[oi];
+    if (!all && !in_view[O.label]) { miss++; continue; }
     const VolDev& v = O.v;
     const ObjParams& P = *O.prm;
     const float step = v.size / (float)v.res;
@@ -309,8 +316,8 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
   for (int oi = 0; oi < nobj; oi++) {
     const RayObj& O = objs[oi];
     const bool mine = (out_l == O.label);
-    O.mvx[pix] = mine ? out_v.x : 0.f; O.mvy[pix] = mine ? out_v.y : 0.f; O.mvz[pix] = mine ? out_v.z : 0.f;
-    O.mnx[pix] = mine ? out_n.x : (float)INVALID; O.mny[pix] = mine ? out_n.y : 0.f; O.mnz[pix] = mine ? out_n.z : 0.f;
+    if (O.mvx) { O.mvx[pix] = mine ? out_v.x : 0.f; O.mvy[pix] = mine ? out_v.y : 0.f; O.mvz[pix] = mine ? out_v.z : 0.f; }
+    if (O.mnx) { O.mnx[pix] = mine ? out_n.x : (float)INVALID; O.mny[pix] = mine ? out_n.y : 0.f; O.mnz[pix] = mine ? out_n.z : 0.f; }
   }
   }
   if (!stats) return;
@@ -325,9 +332,10 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
 
 void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,
                     float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label,
-                    unsigned long long* stats) {
+                    unsigned long long* stats, const int* in_view, const int* gate) {
   dim3 blk(RT), grd((w + RT - 1) / RT, h);
-  k_raycast<<<grd, blk, 0, s>>>(objs, nobj, w, h, mu, nearp, farp, eps, vx, vy, vz, nx, ny, nz, label, stats);
+  k_raycast<<<grd, blk, 0, s>>>(objs, nobj, w, h, mu, nearp, farp, eps, vx, vy, vz, nx, ny, nz, label, stats,
+                                in_view, gate);
 }
 
 }  // namespace mf

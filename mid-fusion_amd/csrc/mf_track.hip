@@ -517,6 +517,7 @@ __device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, int state, const
     X = Y;
     if (writer) {
       S.iters_run++;
+      S.T_final = Y.T;
       // camera: T_w_l; object: volume_pose_ = T_w_c2 * T_c2_o2 (tracking.cpp:695)
       *S.out_pose = S.object ? mul(*S.T_w_c2, Y.T) : Y.T;
       if (sy.trace) {
@@ -557,7 +558,7 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
     const GNIter& X = S.it[q & 1];
     // a level starts afresh (tracking.cpp:246); later iterations are skipped
     // once this level's loop has broken (uniform for the instance)
-    skip = (p.iter > 0 && X.done_level == p.level);
+    skip = S.off || (p.iter > 0 && X.done_level == p.level);
     if (!skip) { Msh.T = X.T; Msh.invT = X.invT; Msh.A = S.A; Msh.B = S.B; Msh.C = S.C; }
   }
   __syncthreads();
@@ -582,7 +583,7 @@ __global__ __launch_bounds__(SOLVE_NT) void k_solve(int level, int iter, const T
   GNState& S = st[state];
   if (tid == 0) {
     X = S.it[q & 1];
-    skip = (iter > 0 && X.done_level == level);
+    skip = S.off || (iter > 0 && X.done_level == level);
     if (skip) S.it[(q + 1) & 1] = X;
     X.pend_level = level; X.pend_iter = iter;
   }
@@ -619,7 +620,7 @@ __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* 
   if (X.pend_level >= 0)
     gn_solve<TB>(X, S, state, part_prev + (size_t)j * nblk_prev * 32, nblk_prev, writer, sy);
   if (tid == 0) {
-    skip = !rows || (p.iter > 0 && X.done_level == p.level);
+    skip = S.off || !rows || (p.iter > 0 && X.done_level == p.level);
     if (!skip) {
       Msh.T = X.T; Msh.invT = X.invT; Msh.A = S.A; Msh.B = S.B; Msh.C = S.C;
       X.pend_level = p.level; X.pend_iter = p.iter;
@@ -636,6 +637,7 @@ __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* 
 // residual): the state's pose is set to T, rows are computed by k_track, and
 // k_eval_sums writes the instance's 32 FP64 sums to S.sums.
 __global__ void k_eval_set(GNState* st, int state, Mat4 T) {
+  st[state].off = 0;
   GNIter& X = st[state].it[0];
   X.T = T; X.invT = inverse(T); X.prev = T;
   X.done_level = -1; X.pend_level = -1; X.pend_iter = -1;
@@ -652,10 +654,11 @@ __device__ void gn_iter_init(GNIter& X, const Mat4& T) {
 }
 __device__ void gn_init_camera(GNState& S, Mat4* T_w_l, const Mat4& Twr, float fx, float fy, float cx, float cy) {
   gn_iter_init(S.it[0], *T_w_l);
+  S.T_final = *T_w_l;
   S.A = Twr;
   S.B = mul(camK(fx, fy, cx, cy), inverse(Twr));
   S.C = ident(); S.T_wo1 = ident();
-  S.iters_run = 0; S.object = 0;
+  S.iters_run = 0; S.object = 0; S.off = 0;
   S.out_pose = T_w_l; S.T_w_c2 = nullptr;
 }
 // Tracking frame start: T_w_r = camera pose (kfusion.cpp:243), its inverse for
@@ -666,14 +669,15 @@ __global__ void k_camera_begin(Mat4* mats, GNState* st, float fx, float fy, floa
   mats[2] = inverse(P);
   gn_init_camera(st[0], mats + 0, P, fx, fy, cx, cy);
 }
-__global__ void k_gn_begin_camera(GNState* st, Mat4* T_w_l, const Mat4* T_w_r, float fx, float fy, float cx,
-                                  float cy) {
+__global__ void k_gn_begin_camera(GNState* st, Mat4* T_w_l, const Mat4* T_w_r, const int* gate, float fx, float fy,
+                                  float cx, float cy) {
   gn_init_camera(*st, T_w_l, *T_w_r, fx, fy, cx, cy);
+  st->off = (gate && *gate == 0) ? 1 : 0;
 }
 // trackEachObject setup (tracking.cpp:587-596): T_c1_o1 and the initial
 // T_c2_o2, one thread per object.
 __global__ void k_gn_begin_objects(GNState* st, int first, int n, const int* ids, Mat4* poses, const Mat4* T_w_c2,
-                                   const Mat4* T_w_c1, int publish) {
+                                   const Mat4* T_w_c1, int publish, const int* gate) {
   const int j = threadIdx.x;
   if (j >= n) return;
   GNState& S = st[first + j];
@@ -683,11 +687,196 @@ __global__ void k_gn_begin_objects(GNState* st, int first, int n, const int* ids
   S.A = mul(inverse(*T_w_c1), To);
   const Mat4 T = mul(inverse(*T_w_c2), To);
   gn_iter_init(S.it[0], T);
+  S.T_final = T;
   S.B = inverse(To);
   S.C = inverse(S.A);
   S.iters_run = 0; S.object = 1;
+  S.off = (gate && *gate == 0) ? 1 : 0;
   S.out_pose = pose; S.T_w_c2 = T_w_c2;
-  if (publish) *pose = mul(*T_w_c2, T);
+  if (publish && !S.off) *pose = mul(*T_w_c2, T);
+}
+
+// ---- motion residuals (Tracking::compute_residuals, tracking.cpp:1535-1747),
+// the per-object inlier counts of check_static_state (rendering.cpp:907-947)
+// and the moving decision and gating of Kfusion::tracking (kfusion.cpp:
+// 286-471).
+
+// Labels present in an image: flags[l] = 1 for l in [0, MAX_INST) (the
+// split_labels instance set, kfusion.cpp:1527-1553).  Same-value stores, so
+// the order of the workgroups does not matter.
+__device__ void presence(const int32_t* __restrict__ label, int n, int* flags) {
+  __shared__ int f[MAX_INST];
+  for (int j = threadIdx.x; j < MAX_INST; j += blockDim.x) f[j] = 0;
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int l = label[i];
+    if (l >= 0 && l < MAX_INST) f[l] = 1;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < MAX_INST; j += blockDim.x)
+    if (f[j]) flags[j] = 1;
+}
+__device__ int count_flags(const int* f, int nobj) {
+  int c = 0;
+  for (int j = 0; j < nobj; j++) c += f[j] != 0;
+  return c;
+}
+__global__ void k_motion_clear(MotionDev* md) {
+  for (int j = threadIdx.x; j < MAX_INST; j += blockDim.x) {
+    md->present[j] = 0; md->move_set[j] = 0; md->counts[2 * j] = 0; md->counts[2 * j + 1] = 0;
+  }
+  if (threadIdx.x == 0) { md->gate = 0; md->move_any = 0; md->retrack = 0; }
+}
+__global__ void k_presence(const int32_t* __restrict__ label, int n, int* flags, const int* gate) {
+  if (gate && *gate == 0) return;
+  presence(label, n, flags);
+}
+// gate = more than one instance in the last raycast (kfusion.cpp:292); under
+// it the in-view set is cleared before the raycast of every object (:294)
+__global__ void k_motion_gate(MotionDev* md, int nobj) {
+  const int g = count_flags(md->present, nobj) > 1 ? 1 : 0;
+  if (threadIdx.x == 0) md->gate = g;
+  if (g)
+    for (int j = threadIdx.x; j < MAX_INST; j += blockDim.x) md->in_view[j] = 0;
+}
+__global__ void k_motion_inview(MotionDev* md, int nobj, const uint8_t* moving, int manual) {
+  if (!md->gate) return;
+  const int rn = count_flags(md->in_view, nobj);
+  md->rendered_n = rn;
+  if (!manual) return;
+  int any = 0;
+  for (int j = 1; j < nobj; j++) { md->move_set[j] = moving[j]; any |= moving[j]; }
+  md->move_any = any;
+  md->retrack = (any && rn != 0) ? 1 : 0;
+}
+__global__ void k_motion_classify(MotionDev* md, int nobj, uint8_t* moving) {
+  if (!md->gate) return;
+  int any = 0;
+  for (int j = 1; j < nobj; j++) {
+    const int px = md->counts[2 * j], inl = md->counts[2 * j + 1];
+    if (px == 0) continue;
+    const float ratio = (float)inl / (float)px;
+    const uint8_t mv = (ratio < 0.8f && ratio > 0.05f) ? 1 : 0;
+    moving[j] = mv; md->move_set[j] = mv; any |= mv;
+  }
+  md->move_any = any;
+  md->retrack = (any && md->rendered_n != 0) ? 1 : 0;
+}
+__global__ void k_motion_clear_present(MotionDev* md) {
+  if (!md->move_any) return;
+  for (int j = threadIdx.x; j < MAX_INST; j += blockDim.x) md->present[j] = 0;
+}
+__global__ void k_motion_after(MotionDev* md, int nobj, int use_rgb) {
+  if (md->move_any) md->rendered_n = count_flags(md->present, nobj);
+  md->cond2 = ((md->rendered_n > 1 && use_rgb) || md->move_any) ? 1 : 0;
+  md->copy_static = (md->gate || md->cond2) ? 1 : 0;
+}
+// virtual_camera_pose_ = T_wo1 * inverse(T_c2_o2) (tracking.cpp:696)
+__global__ void k_virtual_cams(const GNState* st, int first, int n, const int* ids, Mat4* vcam) {
+  const int j = threadIdx.x;
+  if (j >= n) return;
+  const GNState& S = st[first + j];
+  if (S.off) return;
+  vcam[ids[j] - 3] = mul(S.T_wo1, inverse(S.T_final));
+}
+__global__ void k_copy_codes(const int8_t* __restrict__ src, int8_t* __restrict__ dst, size_t n, const int* gate) {
+  if (!*gate) return;
+  const size_t n4 = n / 4;
+  const int* s4 = reinterpret_cast<const int*>(src);
+  int* d4 = reinterpret_cast<int*>(dst);
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  for (size_t i = t; i < n4; i += (size_t)gridDim.x * blockDim.x) d4[i] = s4[i];
+  if (t < n - 4 * n4) dst[4 * n4 + t] = src[4 * n4 + t];
+}
+__global__ __launch_bounds__(256) void k_motion_rows(MotionLaunch m) {
+  if (m.gate && *m.gate == 0) return;
+  __shared__ int hist[2 * MAX_INST];
+  const bool count = m.label_bf != nullptr;
+  if (count) {
+    for (int j = threadIdx.x; j < 2 * MAX_INST; j += blockDim.x) hist[j] = 0;
+    __syncthreads();
+  }
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x < m.W && y < m.H) {
+    const size_t n = (size_t)m.W * m.H;
+    const size_t i = (size_t)x + (size_t)y * m.W;
+    const Mat4 Tl = *m.T_w_l, iTr = *m.inv_T_w_r;
+    const Mat4 K = camK(m.fx, m.fy, m.cx, m.cy);
+    const float* L = m.live;
+    int ri = 0, gi = 0;
+    if (m.use_icp) {
+      // compute_icp_residual_kernel (tracking.cpp:1617-1673)
+      const f3 nl = mk3(L[(size_t)PL_N * n + i], L[(size_t)(PL_N + 1) * n + i], L[(size_t)(PL_N + 2) * n + i]);
+      if (nl.x == INVALID) ri = -1;
+      else {
+        const f3 vl = mk3(L[(size_t)PL_V * n + i], L[(size_t)(PL_V + 1) * n + i], L[(size_t)(PL_V + 2) * n + i]);
+        const f3 wv = affine(Tl, vl);
+        const f3 rv = affine(iTr, wv);
+        const f3 pj = rotate(K, rv);
+        const float ux = pj.x / pj.z, uy = pj.y / pj.z;
+        if (ux < 1 || ux > (float)(m.W - 2) || uy < 1 || uy > (float)(m.H - 2) || isnan(ux) || isnan(uy)) ri = -2;
+        else {
+          const size_t q = (unsigned)ux + (size_t)((unsigned)uy) * m.W;
+          const f3 rn = mk3(m.mvn[3 * n + q], m.mvn[4 * n + q], m.mvn[5 * n + q]);
+          if (rn.x == INVALID) ri = -3;
+          else {
+            const f3 diff = mk3(m.mvn[q], m.mvn[n + q], m.mvn[2 * n + q]) - wv;
+            const f3 wn = rotate(Tl, nl);
+            if (length(diff) > 0.1f) ri = -4;
+            else if (dot(wn, rn) < 0.8f) ri = -5;
+            else {
+              const f3 P = mk3(L[(size_t)PL_COV * n + i], L[(size_t)(PL_COV + 1) * n + i], L[(size_t)(PL_COV + 2) * n + i]);
+              const double sig = (double)(rn.x * rn.x * P.x + rn.y * rn.y * P.y + rn.z * rn.z * P.z);
+              const double err = sqrt(1.0 / sig) * (double)dot(rn, diff);
+              ri = fabs(err) > (double)m.threshold ? -4 : 1;
+            }
+          }
+        }
+      }
+    }
+    if (m.use_rgb) {
+      // compute_rgb_residual_kernel (tracking.cpp:1676-1747); the reference
+      // vertex is vertex2depth of the map before fusion at T_w_r,
+      // back-projected (tracking.cpp:1546-1547)
+      const f3 vb = mk3(m.vbf[i], m.vbf[n + i], m.vbf[2 * n + i]);
+      const float d = vb.z > 0.0f ? affine(iTr, vb).z : 0.0f;
+      const f3 vr = d > 0.0f ? d * rotate(camKinv(m.fx, m.fy, m.cx, m.cy), mk3((float)x, (float)y, 1.0f))
+                             : mk3(0.f, 0.f, 0.f);
+      const f3 vl = mk3(L[(size_t)PL_V * n + i], L[(size_t)(PL_V + 1) * n + i], L[(size_t)(PL_V + 2) * n + i]);
+      if (vr.z <= 0.f || vr.z == INVALID) gi = -1;
+      else if (length(vr - vl) > 0.1f && vl.z > 0.f) gi = -5;
+      else {
+        const f3 wv = affine(Tl, vr);
+        const f3 rv = affine(iTr, wv);
+        const f3 pj = rotate(K, rv);
+        const float ux = pj.x / pj.z, uy = pj.y / pj.z;
+        if (ux < 1 || ux > (float)(m.W - 2) || uy < 1 || uy > (float)(m.H - 2) || isnan(ux) || isnan(uy)) gi = -2;
+        else {
+          const float residual = L[(size_t)PL_I * n + i] - bilin(m.ref + (size_t)PL_I * n, m.W, ux, uy);
+          const double err = (1.0 / (double)m.sigma_bright) * (double)residual;
+          gi = fabs(err) > (double)m.threshold ? -4 : 1;
+        }
+      }
+    }
+    m.codes[i] = (int8_t)ri;
+    m.codes[n + i] = (int8_t)gi;
+    if (count) {
+      const int l = m.label_bf[i];
+      if (l > 0 && l < MAX_INST) {
+        bool inl;
+        if (m.use_icp && m.use_rgb) inl = ri > -3 && gi > -3;
+        else if (m.use_icp) inl = ri > -3;
+        else inl = gi > -3;
+        atomicAdd(&hist[2 * l], 1);
+        if (inl) atomicAdd(&hist[2 * l + 1], 1);
+      }
+    }
+  }
+  if (count) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < 2 * MAX_INST; j += blockDim.x)
+      if (hist[j]) atomicAdd(&m.counts[j], hist[j]);
+  }
 }
 
 // ------------------------------------------------------------ launchers
@@ -733,16 +922,46 @@ void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNS
   launch_track_rows(s, p, inst, 1, st, partial, nblk, 0);
   k_eval_sums<<<1, SOLVE_NT, 0, s>>>(st, state, partial, nblk);
 }
+void launch_motion_rows(hipStream_t s, const MotionLaunch& m) {
+  k_motion_rows<<<dim3((m.W + 255) / 256, m.H), 256, 0, s>>>(m);
+}
+static unsigned presence_blocks(int n) { return (unsigned)std::min(256, (n + 1023) / 1024); }
+void launch_motion_gate(hipStream_t s, MotionDev* md, const int32_t* label, int n, int nobj) {
+  k_motion_clear<<<1, 64, 0, s>>>(md);
+  k_presence<<<presence_blocks(n), 256, 0, s>>>(label, n, md->present, nullptr);
+  k_motion_gate<<<1, 64, 0, s>>>(md, nobj);
+}
+void launch_motion_inview(hipStream_t s, MotionDev* md, const int32_t* label_bf, int n, int nobj,
+                          const uint8_t* moving, int manual) {
+  k_presence<<<presence_blocks(n), 256, 0, s>>>(label_bf, n, md->in_view, &md->gate);
+  k_motion_inview<<<1, 1, 0, s>>>(md, nobj, moving, manual);
+}
+void launch_motion_classify(hipStream_t s, MotionDev* md, int nobj, uint8_t* moving) {
+  k_motion_classify<<<1, 1, 0, s>>>(md, nobj, moving);
+}
+void launch_motion_after(hipStream_t s, MotionDev* md, const int32_t* label_bf, int n, int nobj, int use_rgb) {
+  if (label_bf) {
+    k_motion_clear_present<<<1, 64, 0, s>>>(md);
+    k_presence<<<presence_blocks(n), 256, 0, s>>>(label_bf, n, md->present, &md->move_any);
+  }
+  k_motion_after<<<1, 1, 0, s>>>(md, nobj, use_rgb);
+}
+void launch_virtual_cams(hipStream_t s, const GNState* st, int first, int n, const int* ids, Mat4* vcam) {
+  k_virtual_cams<<<1, 64, 0, s>>>(st, first, n, ids, vcam);
+}
+void launch_copy_codes(hipStream_t s, const int8_t* src, int8_t* dst, size_t n2, const int* gate) {
+  k_copy_codes<<<(unsigned)std::min<size_t>(1024, (n2 / 4 + 256) / 256), 256, 0, s>>>(src, dst, n2, gate);
+}
 void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy) {
   k_camera_begin<<<1, 1, 0, s>>>(mats, st, fx, fy, cx, cy);
 }
-void launch_gn_begin_camera(hipStream_t s, GNState* st, Mat4* T_w_l, const Mat4* T_w_r, float fx, float fy,
-                            float cx, float cy) {
-  k_gn_begin_camera<<<1, 1, 0, s>>>(st, T_w_l, T_w_r, fx, fy, cx, cy);
+void launch_gn_begin_camera(hipStream_t s, GNState* st, Mat4* T_w_l, const Mat4* T_w_r, const int* gate, float fx,
+                            float fy, float cx, float cy) {
+  k_gn_begin_camera<<<1, 1, 0, s>>>(st, T_w_l, T_w_r, gate, fx, fy, cx, cy);
 }
 void launch_gn_begin_objects(hipStream_t s, GNState* st, int first, int n, const int* ids, Mat4* poses,
-                             const Mat4* T_w_c2, const Mat4* T_w_c1, int publish) {
-  k_gn_begin_objects<<<1, 64, 0, s>>>(st, first, n, ids, poses, T_w_c2, T_w_c1, publish);
+                             const Mat4* T_w_c2, const Mat4* T_w_c1, int publish, const int* gate) {
+  k_gn_begin_objects<<<1, 64, 0, s>>>(st, first, n, ids, poses, T_w_c2, T_w_c1, publish, gate);
 }
 
 }  // namespace mf

@@ -35,6 +35,7 @@ f32p = np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS')
 f64p = np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS')
 u64p = np.ctypeslib.ndpointer(np.uint64, flags='C_CONTIGUOUS')
 i32p = np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS')
+i64p = np.ctypeslib.ndpointer(np.int64, flags='C_CONTIGUOUS')
 u16p = np.ctypeslib.ndpointer(np.uint16, flags='C_CONTIGUOUS')
 u8p = np.ctypeslib.ndpointer(np.uint8, flags='C_CONTIGUOUS')
 SIGNATURES = {
@@ -43,6 +44,12 @@ SIGNATURES = {
     'mf_last_error': (C.c_char_p, [P]),
     'mf_object_create': (C.c_int, [P, C.c_float, C.c_int, f32p, C.c_long]),
     'mf_object_set_moving': (C.c_int, [P, C.c_int, C.c_int]),
+    'mf_set_motion_detection': (C.c_int, [P, C.c_int]),
+    'mf_get_motion': (C.c_int, [P, i64p, i64p, i32p, C.c_int]),
+    'mf_get_motion_state': (C.c_int, [P, i32p, i32p, i32p, C.c_int]),
+    'mf_get_motion_codes': (C.c_int, [P, i32p]),
+    'mf_motion_residuals': (C.c_int, [P, f32p, f32p, f32p, f32p, f32p, i32p]),
+    'mf_set_trackresult': (C.c_int, [P, C.c_int, i32p]),
     'mf_object_set_mask': (C.c_int, [P, C.c_int, f32p]),
     'mf_object_set_mask_device': (C.c_int, [P, C.c_int, P]),
     'mf_preprocess': (C.c_int, [P, u16p, u8p, C.c_int, C.c_int, C.c_int]),
@@ -144,6 +151,44 @@ class Kfusion:
 
     def set_moving(self, oid, moving):
         self._chk(self.L.mf_object_set_moving(self.ctx, oid, int(moving)))
+
+    def set_motion_detection(self, on):
+        """Motion detection on/off (off: the moving set is the caller's flags)."""
+        self._chk(self.L.mf_set_motion_detection(self.ctx, int(on)))
+
+    def motion_result(self):
+        """Last motion check: per object (pixels in view, inliers, moving flag)."""
+        cap = 64
+        px, inl, mv = np.zeros(cap, np.int64), np.zeros(cap, np.int64), np.zeros(cap, np.int32)
+        n = self.L.mf_get_motion(self.ctx, px, inl, mv, cap)
+        if n < 0:
+            self._chk(n)
+        return px[:n], inl[:n], mv[:n]
+
+    def motion_state(self):
+        """(gate, rendered_n, in-view flags, moving set) of the last tracking call."""
+        cap = 64
+        g, iv, ms = np.zeros(2, np.int32), np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        n = self.L.mf_get_motion_state(self.ctx, g, iv, ms, cap)
+        if n < 0:
+            self._chk(n)
+        return int(g[0]), int(g[1]), iv[:n], ms[:n]
+
+    def motion_codes(self):
+        """Residual codes of the last motion check (2*W*H: ICP then RGB)."""
+        r = np.zeros(2 * self.w * self.h, np.int32)
+        self._chk(self.L.mf_get_motion_codes(self.ctx, r))
+        return r
+
+    def motion_residuals(self, k, T, vtx, nrm, vbf):
+        """compute_residuals codes (2*W*H) at pose T against the given maps."""
+        out = np.zeros(2 * self.w * self.h, np.int32)
+        f = lambda a: np.ascontiguousarray(a, np.float32).reshape(-1)
+        self._chk(self.L.mf_motion_residuals(self.ctx, f(k), f(T), f(vtx), f(nrm), f(vbf), out))
+        return out
+
+    def set_trackresult(self, oid, codes):
+        self._chk(self.L.mf_set_trackresult(self.ctx, oid, np.ascontiguousarray(codes, np.int32).reshape(-1)))
 
     def set_mask(self, oid, mask):
         self._chk(self.L.mf_object_set_mask(self.ctx, oid, _f32(mask).reshape(-1)))

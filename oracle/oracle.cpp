@@ -288,6 +288,17 @@ struct Pipe {
   std::vector<int> label;
   M4 camera_pose, T_w_r;
   Tracker trk;
+  // motion detection (kfusion.cpp:286-354): raycast at the current camera
+  // pose before fusion, the residual codes and the per-object inlier counts
+  bool motion_auto = false;
+  std::vector<V3> vertex_bf, normal_bf;
+  std::vector<int> label_bf;
+  std::vector<TrackRow> motion_rows;            // ICP rows then RGB rows, full resolution
+  std::vector<long> motion_pixels, motion_inliers;
+  std::vector<char> in_view;    // objects_in_view_ by label (kfusion.cpp:122, 294-299, 1079)
+  int rendered_n = 0;           // rendered_mask_.pair_instance_seg_.size() (persists across frames)
+  bool gate = false;            // raycast_mask_.pair_instance_seg_.size() > 1 at this frame's tracking
+  std::vector<char> move_set;   // move_obj_set (kfusion.cpp:289, 331)
   std::vector<Object*> objs;
   std::vector<TraceRec> trace;
   float ray_eps = -1.f;   // ray_iterator's static epsilon: set by the first raycast volume
@@ -333,7 +344,8 @@ static void integrate_object(Pipe& p, Object& o, const std::vector<R>& mask, con
 
 // raycastObjectList (rendering.cpp:9-181), all objects, has_integrated selects
 // the per-object output maps.
-static void raycast_all(Pipe& p, const float k[4], bool has_integrated) {
+static void raycast_all(Pipe& p, const float k[4], bool has_integrated, V3* out_v, V3* out_n, int* out_l,
+                        const std::vector<char>* in_view) {
   const int W = p.c.w, H = p.c.h;
   const M4 view = mul(p.camera_pose, camKinv(k[0], k[1], k[2], k[3]));
   const R nearp = R(0.1), farp = R(4.0);
@@ -343,6 +355,9 @@ static void raycast_all(Pipe& p, const float k[4], bool has_integrated) {
   }
   std::vector<M4> inv_pose(p.objs.size());
   for (size_t i = 0; i < p.objs.size(); i++) inv_pose[i] = inverse(p.objs[i]->pose);
+  // should_raycast_all when the in-view set is empty (rendering.cpp:16, 27-33)
+  bool all = true;
+  if (in_view) for (char f : *in_view) if (f) all = false;
 #pragma omp parallel for schedule(dynamic, 4)
   for (int y = 0; y < H; y++)
     for (int x = 0; x < W; x++) {
@@ -352,6 +367,7 @@ static void raycast_all(Pipe& p, const float k[4], bool has_integrated) {
       size_t miss = 0;
       for (size_t oi = 0; oi < p.objs.size(); oi++) {
         Object& o = *p.objs[oi];
+        if (!all && !(*in_view)[o.label]) { miss++; continue; }
         const R step = std::min(o.size, o.size) / (R)o.res;
         const R largestep = step * 8;
         const V3 wo = translation(view);
@@ -373,16 +389,16 @@ static void raycast_all(Pipe& p, const float k[4], bool has_integrated) {
         else if (hd < nearest) { nearest = hd; take = true; }
         if (!take) { miss++; continue; }
         high_fg = fg;
-        p.label[pix] = o.label;
-        p.vertex[pix] = affine(o.pose, hit);
+        out_l[pix] = o.label;
+        out_v[pix] = affine(o.pose, hit);
         const V3 g = grad(o.vol, hit * ((R)o.res / o.size));
-        if (length(g) == 0) p.normal[pix] = v3(R(INVALID), 0, 0);
-        else p.normal[pix] = normalize(rotate(o.pose, g));
+        if (length(g) == 0) out_n[pix] = v3(R(INVALID), 0, 0);
+        else out_n[pix] = normalize(rotate(o.pose, g));
       }
-      if (miss == p.objs.size()) { p.vertex[pix] = v3(0, 0, 0); p.normal[pix] = v3(R(INVALID), 0, 0); p.label[pix] = INVALID; }
+      if (miss == p.objs.size()) { out_v[pix] = v3(0, 0, 0); out_n[pix] = v3(R(INVALID), 0, 0); out_l[pix] = INVALID; }
       for (Object* o : p.objs) {
         V3 ov = v3(0, 0, 0), on = v3(R(INVALID), 0, 0);
-        if (p.label[pix] == o->label) { ov = p.vertex[pix]; on = p.normal[pix]; }
+        if (out_l[pix] == o->label) { ov = out_v[pix]; on = out_n[pix]; }
         if (has_integrated) { o->m_vertex[pix] = ov; o->m_normal[pix] = on; }
         else { o->m_vertex_bef[pix] = ov; o->m_normal_bef[pix] = on; }
       }
@@ -429,13 +445,14 @@ int orc_object_create(void* h, float size_m, int res, const float T_w_o[16]) {
   o->label = (int)p->objs.size();
   o->size = size_m; o->res = res;
   o->vol.init(res, size_m);
-  o->pose = m4_from(T_w_o); o->virtual_cam = o->pose;
+  o->pose = m4_from(T_w_o); o->virtual_cam = p->camera_pose;   // virtual_T_w_c = camera_pose (kfusion.cpp:1058-1061)
   const size_t n = (size_t)p->c.w * p->c.h;
   o->m_vertex.assign(n, v3(0, 0, 0)); o->m_normal.assign(n, v3(0, 0, 0));
   o->m_vertex_bef.assign(n, v3(0, 0, 0)); o->m_normal_bef.assign(n, v3(0, 0, 0));
   TrackRow z; memset(&z, 0, sizeof z);
   o->trackresult.assign(2 * n, z);
   p->objs.push_back(o);
+  p->in_view.push_back(1);   // objects_in_view_.insert (kfusion.cpp:122, 1079)
   return o->label;
 }
 int orc_object_set_moving(void* h, int id, int moving) { ((Pipe*)h)->objs[id]->is_static = !moving; return 0; }
@@ -463,6 +480,148 @@ int orc_preprocess(void* h, const uint16_t* depth, const uint8_t* rgb, int iw, i
   return 0;
 }
 
+// Tracking::compute_icp_residual_kernel (tracking.cpp:1617-1673): residual
+// of the live level-0 vertex against the model maps at T_w_r, gated, with no
+// Jacobian.
+static void motion_icp_row(TrackRow& row, size_t i, int W, int H, const M4& T_w_l, const M4& inv_T_w_r, const M4& K,
+                           const V3* l_vertex, const V3* l_normal, const V3* mvert, const V3* mnorm, const V3* cov,
+                           R threshold) {
+  row.error = 0;
+  if (l_normal[i].x == INVALID) { row.result = -1; return; }
+  const V3 w_vertex = affine(T_w_l, l_vertex[i]);   // warp (tracking.cpp:1114-1121)
+  const V3 r_vertex = affine(inv_T_w_r, w_vertex);
+  const V3 pj = rotate(K, r_vertex);
+  const R ux = pj.x / pj.z, uy = pj.y / pj.z;
+  if (ux < 1 || ux > (R)(W - 2) || uy < 1 || uy > (R)(H - 2) || std::isnan(ux) || std::isnan(uy)) {
+    row.result = -2;   // NaN projections read no pixel (the reference indexes with them)
+    return;
+  }
+  const size_t ri = (unsigned)ux + (size_t)((unsigned)uy) * W;
+  const V3 rn = mnorm[ri];
+  if (rn.x == INVALID) { row.result = -3; return; }
+  const V3 diff = mvert[ri] - w_vertex;
+  const V3 wn = rotate(T_w_l, l_normal[i]);
+  if (length(diff) > R(0.1)) { row.result = -4; return; }
+  if (dot(wn, rn) < R(0.8)) { row.result = -5; return; }
+  const V3 P = cov[i];
+  const double sig = (double)(rn.x * rn.x * P.x + rn.y * rn.y * P.y + rn.z * rn.z * P.z);
+  const double inv_cov = std::sqrt(1.0 / sig);
+  const double err = inv_cov * (double)dot(rn, diff);
+  if (std::fabs(err) > (double)threshold) { row.result = -4; return; }
+  row.error = (R)err;
+  row.result = 1;
+}
+// Tracking::compute_rgb_residual_kernel (tracking.cpp:1676-1747),
+// fill_in_missing_depth_rgb_residual_checking_ = false (tracking.h:407).
+static void motion_rgb_row(TrackRow& row, int x, int y, int W, int H, const M4& T_w_l, const M4& inv_T_w_r,
+                           const M4& K, const R* r_image, const R* l_image, const V3* l_vertex_ref,
+                           const V3* l_vertex, R sigma_bright, R threshold) {
+  row.error = 0;
+  const size_t i = x + (size_t)y * W;
+  const V3 vr = l_vertex_ref[i], vl = l_vertex[i];
+  if (vr.z <= 0 || vr.z == INVALID) { row.result = -1; return; }
+  if (length(vr - vl) > R(0.1) && vl.z > 0) { row.result = -5; return; }
+  const V3 w_vertex = affine(T_w_l, vr);
+  const V3 r_vertex = affine(inv_T_w_r, w_vertex);
+  const V3 pj = rotate(K, r_vertex);
+  const R ux = pj.x / pj.z, uy = pj.y / pj.z;
+  if (ux < 1 || ux > (R)(W - 2) || uy < 1 || uy > (R)(H - 2) || std::isnan(ux) || std::isnan(uy)) {
+    row.result = -2;   // NaN projections read no pixel (the reference indexes with them)
+    return;
+  }
+  // rgb_residual(l_image, pixel, r_image, proj) (tracking.cpp:1180-1186)
+  const R residual = l_image[i] - bilin(r_image, W, ux, uy);
+  const double err = (1.0 / (double)sigma_bright) * (double)residual;
+  if (std::fabs(err) > (double)threshold) { row.result = -4; return; }
+  row.error = (R)err;
+  row.result = 1;
+}
+// Tracking::compute_residuals (tracking.cpp:1535-1554): the reference depth
+// of `vert_bf` (a raycast before fusion) at T_w_r, back-projected at full
+// resolution, then the ICP and photometric residual rows of every pixel of
+// the live level 0 against the model maps (mvert, mnorm) at T_w_l.
+static void compute_residuals(Pipe& p, const float k[4], const M4& T_w_l, const V3* mvert, const V3* mnorm,
+                              const V3* vert_bf, std::vector<TrackRow>& rows) {
+  const int W = p.c.w, H = p.c.h;
+  const size_t n = (size_t)W * H;
+  Tracker& t = p.trk;
+  std::vector<R> d_ref(n);
+  std::vector<V3> v_ref(n);
+  vertex2depth(d_ref.data(), vert_bf, W, H, p.T_w_r);
+  depth2vertex(v_ref.data(), d_ref.data(), W, H, camKinv(k[0], k[1], k[2], k[3]));
+  const M4 K = camK(k[0], k[1], k[2], k[3]);
+  const M4 inv_T_w_r = inverse(p.T_w_r);
+  const R thr = R(1.5);   // residual_threshold_ (kernels.h:95)
+  rows.assign(2 * n, TrackRow{});
+#pragma omp parallel for
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      const size_t i = x + (size_t)y * W;
+      if (p.c.use_icp)
+        motion_icp_row(rows[i], i, W, H, T_w_l, inv_T_w_r, K, t.l_V[0].data(), t.l_N[0].data(), mvert, mnorm,
+                       t.cov[0].data(), thr);
+      if (p.c.use_rgb)
+        motion_rgb_row(rows[n + i], x, y, W, H, T_w_l, inv_T_w_r, K, t.r_I[0].data(), t.l_I[0].data(),
+                       v_ref.data(), t.l_V[0].data(), t.sigma_bright, thr);
+    }
+}
+// Labels of objects present in a label image (split_labels,
+// kfusion.cpp:1527-1553): the count, and per-label flags if asked.
+static int present_labels(const std::vector<int>& lab, size_t no, std::vector<char>* flags) {
+  std::vector<char> f(no, 0);
+  for (int l : lab) if (l >= 0 && l < (int)no) f[l] = 1;
+  int cnt = 0;
+  for (char x : f) cnt += x;
+  if (flags) *flags = f;
+  return cnt;
+}
+// After the first camera pass (kfusion.cpp:286-354): when the last raycast
+// shows more than one instance, raycast every object at the current camera
+// pose and the objects' last poses (maps "before fusion"); the objects in
+// that view become the in-view set.  With motion detection on, the residuals
+// against the model maps, check_static_state (rendering.cpp:907-947) per
+// object in view, and the moving decision with inlier ratios in (0.05, 0.8)
+// (kernels.h:95-97); off, the moving set is the caller's moving flags.
+static void motion_check(Pipe& p, const float k[4]) {
+  const int W = p.c.w, H = p.c.h;
+  const size_t n = (size_t)W * H;
+  const size_t no = p.objs.size();
+  p.motion_pixels.assign(no, 0);
+  p.motion_inliers.assign(no, 0);
+  p.motion_rows.clear();
+  p.move_set.assign(no, 0);
+  p.gate = no >= 2 && present_labels(p.label, no, nullptr) > 1;
+  if (!p.gate) return;
+  p.vertex_bf.assign(n, v3(0, 0, 0));
+  p.normal_bf.assign(n, v3(R(INVALID), 0, 0));
+  p.label_bf.assign(n, INVALID);
+  raycast_all(p, k, false, p.vertex_bf.data(), p.normal_bf.data(), p.label_bf.data(), nullptr);
+  p.rendered_n = present_labels(p.label_bf, no, &p.in_view);
+  if (!p.motion_auto) {
+    for (size_t j = 1; j < no; j++) p.move_set[j] = p.objs[j]->is_static ? 0 : 1;
+    return;
+  }
+  compute_residuals(p, k, p.camera_pose, p.vertex.data(), p.normal.data(), p.vertex_bf.data(), p.motion_rows);
+  for (size_t i = 0; i < n; i++) {
+    const int l = p.label_bf[i];
+    if (l <= 0 || l >= (int)no) continue;
+    const int ri = p.motion_rows[i].result, gi = p.motion_rows[n + i].result;
+    bool inl;
+    if (p.c.use_icp && p.c.use_rgb) inl = ri > -3 && gi > -3;
+    else if (p.c.use_icp) inl = ri > -3;
+    else inl = gi > -3;
+    p.motion_pixels[l]++;
+    if (inl) p.motion_inliers[l]++;
+  }
+  for (size_t j = 1; j < no; j++) {
+    if (p.motion_pixels[j] == 0) continue;
+    const float ratio = (float)p.motion_inliers[j] / (float)p.motion_pixels[j];
+    const bool moving = ratio < 0.8f && ratio > 0.05f;
+    p.objs[j]->is_static = !moving;
+    p.move_set[j] = moving ? 1 : 0;
+  }
+}
+
 // Camera phase: pyramid, trackLiveFrame, and the masked re-track when
 // objects move (kfusion.cpp:237-392).  Returns 1 if the object phase must
 // follow, 0 if the frame only seeded the reference.
@@ -479,13 +638,15 @@ int orc_track_camera(void* h, const float k[4], unsigned frame) {
   t.track_live(p->camera_pose, p->T_w_r, p->vertex.data(), p->normal.data(), nullptr);
   p->objs[0]->virtual_cam = p->camera_pose;
   memcpy(p->objs[0]->trackresult.data(), t.J[0].data(), sizeof(TrackRow) * 2 * n);
+  motion_check(*p, k);
   std::vector<uint8_t> dyn(n, 0);
   bool any_moving = false;
-  for (size_t i = 1; i < p->objs.size(); i++) if (!p->objs[i]->is_static) any_moving = true;
-  if (any_moving) {
+  for (char f : p->move_set) if (f) any_moving = true;
+  if (any_moving && p->rendered_n != 0) {
+    // dynamic-object mask from the last raycast's labels (kfusion.cpp:357-392)
     for (size_t i = 0; i < n; i++) {
       int l = p->label[i];
-      if (l > 0 && l < (int)p->objs.size() && !p->objs[l]->is_static) dyn[i] = 255;
+      if (l > 0 && l < (int)p->objs.size() && p->move_set[l]) dyn[i] = 255;
     }
     t.track_live(p->camera_pose, p->T_w_r, p->vertex.data(), p->normal.data(), dyn.data());
     p->objs[0]->virtual_cam = p->camera_pose;
@@ -494,19 +655,44 @@ int orc_track_camera(void* h, const float k[4], unsigned frame) {
   return 1;
 }
 
-// Object phase: trackEachObject for every moving object at the current
-// camera pose, then setRefImgFromCurr (kfusion.cpp:395-471).
+// Object phase (kfusion.cpp:395-471): trackEachObject for every moving object
+// when the last raycast showed more than one instance; with objects moving
+// this frame, the camera's integration rows become motion residuals against
+// the maps before fusion and a second raycast (objects in view, new poses)
+// renders the objects' maps before fusion; then each moving object's rows
+// become motion residuals at its virtual camera pose; setRefImgFromCurr.
 int orc_track_objects(void* h, const float k[4], unsigned frame) {
-  (void)k; (void)frame;
+  (void)frame;
   Pipe* p = (Pipe*)h;
   Tracker& t = p->trk;
   const size_t n = (size_t)p->c.w * p->c.h;
-  for (size_t i = 1; i < p->objs.size(); i++) {
-    Object* o = p->objs[i];
-    if (o->is_static) { o->trackresult = p->objs[0]->trackresult; continue; }
-    t.trace_inst = (int)i;
-    t.track_object(o->pose, o->virtual_cam, o->m_vertex.data(), o->m_normal.data(), p->camera_pose, p->T_w_r);
-    memcpy(o->trackresult.data(), t.J[0].data(), sizeof(TrackRow) * 2 * n);
+  const size_t no = p->objs.size();
+  if (p->gate) {
+    for (size_t i = 1; i < no; i++) {
+      Object* o = p->objs[i];
+      if (o->is_static) { o->trackresult = p->objs[0]->trackresult; continue; }
+      t.trace_inst = (int)i;
+      t.track_object(o->pose, o->virtual_cam, o->m_vertex.data(), o->m_normal.data(), p->camera_pose, p->T_w_r);
+      memcpy(o->trackresult.data(), t.J[0].data(), sizeof(TrackRow) * 2 * n);
+    }
+  }
+  bool any_moving = false;
+  for (char f : p->move_set) if (f) any_moving = true;
+  if (any_moving) {
+    std::vector<TrackRow> rows;
+    compute_residuals(*p, k, p->camera_pose, p->vertex.data(), p->normal.data(), p->vertex_bf.data(), rows);
+    p->objs[0]->trackresult = rows;
+    raycast_all(*p, k, false, p->vertex_bf.data(), p->normal_bf.data(), p->label_bf.data(), &p->in_view);
+    p->rendered_n = present_labels(p->label_bf, no, nullptr);
+  }
+  if ((p->rendered_n > 1 && p->c.use_rgb) || any_moving) {
+    std::vector<TrackRow> rows;
+    for (size_t i = 1; i < no; i++) {
+      Object* o = p->objs[i];
+      if (o->is_static) { o->trackresult = p->objs[0]->trackresult; continue; }
+      compute_residuals(*p, k, o->virtual_cam, o->m_vertex.data(), o->m_normal.data(), o->m_vertex_bef.data(), rows);
+      o->trackresult = rows;
+    }
   }
   t.set_ref_from_curr();
   return 1;
@@ -532,6 +718,42 @@ int orc_eval(void* h, int id, int level, const double T16[16], double out[32]) {
   return 0;
 }
 
+// Motion detection on/off (off: moving flags come from orc_object_set_moving).
+void orc_set_motion_auto(void* h, int on) { ((Pipe*)h)->motion_auto = on != 0; }
+// Last motion check: per object {pixels in view, inliers} and the moving flag.
+int orc_motion_result(void* h, long* pixels, long* inliers, int* moving, int cap) {
+  Pipe* p = (Pipe*)h;
+  const int no = (int)p->objs.size();
+  for (int j = 0; j < no && j < cap; j++) {
+    pixels[j] = j < (int)p->motion_pixels.size() ? p->motion_pixels[j] : 0;
+    inliers[j] = j < (int)p->motion_inliers.size() ? p->motion_inliers[j] : 0;
+    moving[j] = p->objs[j]->is_static ? 0 : 1;
+  }
+  return no;
+}
+// Motion-gating state of the last tracking call: {gate, rendered_n}, then the
+// in-view flags and the moving set by label.
+int orc_motion_state(void* h, int* gate_rendered, int* in_view, int* move_set, int cap) {
+  Pipe* p = (Pipe*)h;
+  gate_rendered[0] = p->gate ? 1 : 0;
+  gate_rendered[1] = p->rendered_n;
+  const int no = (int)p->objs.size();
+  for (int j = 0; j < no && j < cap; j++) {
+    in_view[j] = j < (int)p->in_view.size() ? p->in_view[j] : 0;
+    move_set[j] = j < (int)p->move_set.size() ? p->move_set[j] : 0;
+  }
+  return no;
+}
+// Motion residual rows of the last check (2*W*H: ICP rows, then RGB rows).
+int orc_motion_rows(void* h, int* result, float* error) {
+  Pipe* p = (Pipe*)h;
+  for (size_t i = 0; i < p->motion_rows.size(); i++) {
+    result[i] = p->motion_rows[i].result;
+    error[i] = (float)p->motion_rows[i].error;
+  }
+  return (int)p->motion_rows.size();
+}
+
 // Kfusion::tracking (kfusion.cpp:237-488) with the moving-object set given
 // by the caller (orc_object_set_moving) instead of the motion-residual test.
 int orc_track(void* h, const float k[4], unsigned frame) {
@@ -555,7 +777,8 @@ int orc_integrate(void* h, const float k[4], unsigned frame) {
 // Kfusion::raycasting (kfusion.cpp:539-569), every object.
 int orc_raycast(void* h, const float k[4], unsigned frame) {
   (void)frame;
-  raycast_all(*(Pipe*)h, k, true);
+  Pipe* p = (Pipe*)h;
+  raycast_all(*p, k, true, p->vertex.data(), p->normal.data(), p->label.data(), &p->in_view);
   return 1;
 }
 
@@ -644,6 +867,43 @@ void orc_object_trackresult(void* h, int id, int* res, float* err) {
   const size_t n = (size_t)2 * p->c.w * p->c.h;
   for (size_t i = 0; i < n; i++) { res[i] = o->trackresult[i].result; if (err) err[i] = (float)o->trackresult[i].error; }
 }
+// compute_residuals (tracking.cpp:1535-1747) at pose T_w_l against the given
+// model maps and vertex map before fusion (AoS float3), with this frame's live
+// and reference images and T_w_r: the 2*W*H result codes.
+int orc_motion_residuals(void* h, const float k[4], const float T16[16], const float* vtx, const float* nrm,
+                         const float* vbf, int* out) {
+  Pipe* p = (Pipe*)h;
+  const size_t n = (size_t)p->c.w * p->c.h;
+  std::vector<V3> mv(n), mn(n), vb(n);
+  for (size_t i = 0; i < n; i++) {
+    mv[i] = v3((R)vtx[3 * i], (R)vtx[3 * i + 1], (R)vtx[3 * i + 2]);
+    mn[i] = v3((R)nrm[3 * i], (R)nrm[3 * i + 1], (R)nrm[3 * i + 2]);
+    vb[i] = v3((R)vbf[3 * i], (R)vbf[3 * i + 1], (R)vbf[3 * i + 2]);
+  }
+  std::vector<TrackRow> rows;
+  compute_residuals(*p, k, m4_from(T16), mv.data(), mn.data(), vb.data(), rows);
+  for (size_t i = 0; i < 2 * n; i++) out[i] = rows[i].result;
+  return 0;
+}
+// Maps of the last raycast before fusion (AoS float3) and its labels.
+void orc_get_maps_bf(void* h, float* vtx, float* nrm, int* lab) {
+  Pipe* p = (Pipe*)h;
+  const size_t n = (size_t)p->c.w * p->c.h;
+  for (size_t i = 0; i < n && i < p->vertex_bf.size(); i++) {
+    if (vtx) { vtx[3 * i] = (float)p->vertex_bf[i].x; vtx[3 * i + 1] = (float)p->vertex_bf[i].y; vtx[3 * i + 2] = (float)p->vertex_bf[i].z; }
+    if (nrm) { nrm[3 * i] = (float)p->normal_bf[i].x; nrm[3 * i + 1] = (float)p->normal_bf[i].y; nrm[3 * i + 2] = (float)p->normal_bf[i].z; }
+    if (lab) lab[i] = p->label_bf[i];
+  }
+}
+// An object's vertex map before fusion (m_vertex_bef_integ, AoS float3).
+void orc_object_vertex_bef(void* h, int id, float* vtx) {
+  Pipe* p = (Pipe*)h; Object* o = p->objs[id];
+  for (size_t i = 0; i < o->m_vertex_bef.size(); i++) {
+    vtx[3 * i] = (float)o->m_vertex_bef[i].x; vtx[3 * i + 1] = (float)o->m_vertex_bef[i].y; vtx[3 * i + 2] = (float)o->m_vertex_bef[i].z;
+  }
+}
+// An object's virtual_camera_pose_.
+void orc_virtual_cam(void* h, int id, double T[16]) { m4_to(((Pipe*)h)->objs[id]->virtual_cam, T); }
 // TrackData rows of the tracker's last pass at a level (2*W*H rows).
 void orc_get_rows(void* h, int level, int* res, float* err, float* J) {
   Pipe* p = (Pipe*)h;

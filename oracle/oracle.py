@@ -52,6 +52,15 @@ def _lib(fp64):
         'orc_get_pose': (None, [P, C.c_int, f64p]),
         'orc_set_pose': (None, [P, C.c_int, f32p]),
         'orc_eval': (C.c_int, [P, C.c_int, C.c_int, f64p, f64p]),
+        'orc_set_motion_auto': (None, [P, C.c_int]),
+        'orc_motion_result': (C.c_int, [P, np.ctypeslib.ndpointer(np.int64, flags='C_CONTIGUOUS'),
+                                        np.ctypeslib.ndpointer(np.int64, flags='C_CONTIGUOUS'), i32p, C.c_int]),
+        'orc_motion_rows': (C.c_int, [P, i32p, f32p]),
+        'orc_motion_state': (C.c_int, [P, i32p, i32p, i32p, C.c_int]),
+        'orc_motion_residuals': (C.c_int, [P, f32p, f32p, f32p, f32p, f32p, i32p]),
+        'orc_get_maps_bf': (None, [P, f32p, f32p, i32p]),
+        'orc_object_vertex_bef': (None, [P, C.c_int, f32p]),
+        'orc_virtual_cam': (None, [P, C.c_int, f64p]),
         'orc_get_maps': (None, [P, f32p, f32p, i32p]),
         'orc_get_object_maps': (None, [P, C.c_int, f32p, f32p]),
         'orc_set_maps': (None, [P, f32p, f32p, i32p]),
@@ -183,6 +192,51 @@ class Oracle:
         r = self.L.orc_eval(self.h_, oid, level, np.ascontiguousarray(np.asarray(T, np.float64).reshape(16)), out)
         assert r == 0
         return out
+
+    def set_motion_auto(self, on):
+        self.L.orc_set_motion_auto(self.h_, int(on))
+
+    def motion_result(self):
+        cap = 64
+        px, inl, mv = np.zeros(cap, np.int64), np.zeros(cap, np.int64), np.zeros(cap, np.int32)
+        n = self.L.orc_motion_result(self.h_, px, inl, mv, cap)
+        return px[:n], inl[:n], mv[:n]
+
+    def motion_state(self):
+        """(gate, rendered_n, in_view flags, moving set) of the last tracking call."""
+        cap = 64
+        g, iv, ms = np.zeros(2, np.int32), np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        n = self.L.orc_motion_state(self.h_, g, iv, ms, cap)
+        return int(g[0]), int(g[1]), iv[:n], ms[:n]
+
+    def motion_residuals(self, k, T, vtx, nrm, vbf):
+        """compute_residuals codes (2*W*H) at pose T against the given maps."""
+        out = np.zeros(2 * self.w * self.h, np.int32)
+        self.L.orc_motion_residuals(self.h_, f32(k), f32(np.asarray(T).reshape(16)), f32(vtx).reshape(-1),
+                                    f32(nrm).reshape(-1), f32(vbf).reshape(-1), out)
+        return out
+
+    def maps_bf(self):
+        n = self.w * self.h
+        v, nn, lab = np.zeros(3 * n, np.float32), np.zeros(3 * n, np.float32), np.zeros(n, np.int32)
+        self.L.orc_get_maps_bf(self.h_, v, nn, lab)
+        return v.reshape(self.h, self.w, 3), nn.reshape(self.h, self.w, 3), lab.reshape(self.h, self.w)
+
+    def object_vertex_bef(self, oid):
+        v = np.zeros(3 * self.w * self.h, np.float32)
+        self.L.orc_object_vertex_bef(self.h_, oid, v)
+        return v.reshape(self.h, self.w, 3)
+
+    def virtual_cam(self, oid):
+        T = np.zeros(16)
+        self.L.orc_virtual_cam(self.h_, oid, T)
+        return T.reshape(4, 4)
+
+    def motion_rows(self):
+        n = 2 * self.w * self.h
+        r, e = np.zeros(n, np.int32), np.zeros(n, np.float32)
+        self.L.orc_motion_rows(self.h_, r, e)
+        return r, e
 
     def set_pose(self, oid, T):
         self.L.orc_set_pose(self.h_, oid, f32(np.asarray(T).reshape(16)))

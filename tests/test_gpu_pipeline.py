@@ -65,6 +65,13 @@ def run_pair(W, H, K, n_frames, use_rgb, res=256, noise=True, moving=False, chec
             r['obj%d_r' % i] = rot_err(Qg, Qo)
             gpu.setPose(i, Qo.astype(np.float32))
         gpu.setPose(-1, Po.astype(np.float32))
+        # integration rows: the GPU's own agree except where FP32 pose
+        # differences move a correspondence; the oracle's are copied in so
+        # that integration runs on identical inputs
+        for i in range(1 + len(mov)):
+            cg, (co, _) = gpu.trackresult(i), orc.trackresult(i)
+            r['rows%d_mismatch' % i] = float(np.mean(cg != co))
+            gpu.set_trackresult(i, co)
         if check:
             check('track', f, gpu, orc, r)
         gpu.integration(K, f)
@@ -126,6 +133,7 @@ def test_stage_preprocess_and_pyramid():
 def test_pipeline_icp_only_cfg1_like():
     rep = run_pair(320, 240, K320, 4, use_rgb=False)
     for r in rep:
+        assert r['rows0_mismatch'] < 5e-3, r
         assert r['pose_t'] < 1e-5 and r['pose_r'] < 1e-5, r
         assert r['keys0'][2] == 0, r
         assert r['vtx_max'] < 1e-4 or r['vtx_p999'] < 1e-4, r
@@ -141,6 +149,7 @@ def test_pipeline_icp_rgb_moving_object():
     # are held to 1e-3 while test_gpu_rows pins every object row exactly.
     rep = run_pair(320, 240, K320, 4, use_rgb=True, moving=True)
     for r in rep:
+        assert r['rows0_mismatch'] < 5e-3 and r['rows1_mismatch'] < 5e-3, r
         assert r['pose_t'] < 5e-5 and r['pose_r'] < 5e-5, r
         assert r.get('obj1_t', 0) < 1e-3 and r.get('obj1_r', 0) < 1e-3, r
         assert r['keys0'][2] == 0 and r['keys1'][2] == 0, r
