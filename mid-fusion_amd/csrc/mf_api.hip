@@ -88,7 +88,7 @@ struct mf_ctx {
   // motion detection and gating (kfusion.cpp:286-471)
   bool motion_auto = false;
   MotionDev* md = nullptr;
-  float *vbf[3] = {}, *nbf[3] = {};   // raycast before fusion: vertex and normal planes
+  float* vbf[3] = {};                 // raycast before fusion: vertex planes
   int32_t* label_bf = nullptr;
   int8_t* motion_codes = nullptr;     // [2*N0] residual codes of the last motion check
   Mat4* vcam = nullptr;               // [MAX_INST] virtual camera poses by object id
@@ -450,8 +450,8 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   }
   c->label = dalloc<int32_t>(c, c->N0);
   {
-    float* b = dalloc<float>(c, 6 * c->N0);
-    for (int k = 0; k < 3; k++) { c->vbf[k] = b ? b + k * c->N0 : nullptr; c->nbf[k] = b ? b + (3 + k) * c->N0 : nullptr; }
+    float* b = dalloc<float>(c, 3 * c->N0);
+    for (int k = 0; k < 3; k++) c->vbf[k] = b ? b + k * c->N0 : nullptr;
   }
   c->label_bf = dalloc<int32_t>(c, c->N0);
   c->motion_codes = dalloc<int8_t>(c, 2 * c->N0);
@@ -629,14 +629,16 @@ void upload_ray_tables(mf_ctx* c) {
 }
 
 // Raycast of every object at the current camera pose and the objects' current
-// poses into the maps before fusion (kfusion.cpp:295-297, 432-434).
+// poses into the maps before fusion (kfusion.cpp:295-297, 432-434).  Only the
+// vertices and labels of these maps are read downstream (compute_residuals,
+// split_labels), so no normals are computed.
 void raycast_before_fusion(mf_ctx* c, const float k[4], const int* in_view, const int* gate) {
   hipStream_t s = c->stream;
   const int n = (int)c->objs.size();
   { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats, c->mapobjs + MAX_INST, n, k[0], k[1], k[2], k[3], 0, c->work); }
   Timed t(c, KC_RAYCAST);
   launch_raycast(s, c->rayobjs_bf, n, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, c->vbf[0], c->vbf[1], c->vbf[2],
-                 c->nbf[0], c->nbf[1], c->nbf[2], c->label_bf, c->timing ? c->stats : nullptr, in_view, gate);
+                 nullptr, nullptr, nullptr, c->label_bf, c->timing ? c->stats : nullptr, in_view, gate);
 }
 
 // compute_residuals launch of one instance against model maps `mvn` (6

@@ -171,8 +171,9 @@ struct RayObj {
 };
 // raycastObjectList over `objs`: objects whose label is not flagged in
 // `in_view` are skipped unless no object is flagged (null: every object); the
-// per-object maps are written where the object's pointers are set; with a
-// gate, nothing runs unless *gate != 0.
+// per-object maps are written where the object's pointers are set; with
+// nx == nullptr no normal is computed; with a gate, nothing runs unless
+// *gate != 0.
 void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,
                     float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label,
                     unsigned long long* stats, const int* in_view, const int* gate);

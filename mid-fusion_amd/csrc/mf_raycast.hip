@@ -114,6 +114,7 @@ __device__ __forceinline__ bool child_exists(const VolDev& v, int lv, uint32_t p
 // scale - min_scale - 1 holds the parent's Morton prefix (bit 31: pushed) and
 // its t_max, for the leaf_level + 1 scales a push or pop can reach.
 static constexpr int RT = 256;          // raycast threads per workgroup
+static constexpr int RT_TILE = 16;      // pixel tile side per workgroup (RT = RT_TILE^2)
 static constexpr int MAX_LG = 10;       // leaf level <= 10: res <= 8192
 __device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, float farp, float eps,
                                  uint32_t* __restrict__ st_par, float* __restrict__ st_tmax) {
@@ -207,6 +208,7 @@ __device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, fl
 
 // raycast(volume, origin, dir, tnear, tfar, mu, step, largestep)
 // (kfusion/rendering_impl.hpp:80-140); returns t (0: no hit).
+template <bool STATS>
 __device__ float march(const VolDev& v, f3 origin, f3 dir, float nearp, float farp, float mu, float step,
                        float largestep, f3& hit, unsigned& nsteps, unsigned& ninterp) {
   const float size = v.size;
@@ -226,14 +228,14 @@ __device__ float march(const VolDev& v, f3 origin, f3 dir, float nearp, float fa
   f3 p = origin + dir * t;
   VCache c;
   float f_t = interp<false>(v, c, p * inv_vs);
-  ninterp++;
+  if (STATS) ninterp++;
   float f_tt = 0;
   if (f_t > 0) {
     for (; t < tfar; t += stepsize) {
       const f3 q = p * inv_vs;
-      nsteps++;
+      if (STATS) nsteps++;
       f_tt = tsdf_at(v, c, (int)q.x, (int)q.y, (int)q.z);
-      if (f_tt < 0.1f && f_tt >= -0.5f) { f_tt = interp<false>(v, c, q); ninterp++; }   // (double)f <= 0.1
+      if (f_tt < 0.1f && f_tt >= -0.5f) { f_tt = interp<false>(v, c, q); if (STATS) ninterp++; }   // (double)f <= 0.1
       if (f_tt < 0) break;
       stepsize = fmax_r(f_tt * mu, step);
       p = p + stepsize * dir;
@@ -248,6 +250,7 @@ __device__ float march(const VolDev& v, f3 origin, f3 dir, float nearp, float fa
   return 0.f;
 }
 
+template <bool STATS>
 __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs, int nobj, int w, int h, float mu,
                                                  float nearp, float farp, float eps, float* __restrict__ vx,
                                                  float* __restrict__ vy, float* __restrict__ vz,
@@ -261,22 +264,28 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
   __shared__ float st_tmax[MAX_LG + 1][RT];
   if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  // 16x16 pixel tile per workgroup, 8x8 per wave: neighbouring rays visit
+  // the same blocks, so a wave's gathers share cache lines and its lanes
+  // march similar distances
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int x = blockIdx.x * RT_TILE + (wv & 1) * 8 + (lane & 7);
+  const int y = blockIdx.y * RT_TILE + (wv >> 1) * 8 + (lane >> 3);
   unsigned nsteps = 0, ninterp = 0, nhits = 0, npairs = 0;
   if (x < w && y < h) {
   const size_t pix = (size_t)x + (size_t)y * w;
   float nearest = 1.0e3f;
-  double high_fg = 0;
-  int miss = 0;
-  f3 out_v = mk3(0.f, 0.f, 0.f), out_n = mk3((float)INVALID, 0.f, 0.f);
-  int out_l = INVALID;
+  float high_fg = 0;   // fg values are floats widened to double: float compares are exact
+  // every object either takes the pixel or counts as a miss, so "missed by
+  // every object" is "no take": win < 0
+  int win = -1;                  // object of the last take, and its hit (object frame, metres)
+  f3 win_hit = mk3(0.f, 0.f, 0.f);
   // should_raycast_all when the in-view set is empty (rendering.cpp:16, 27-33)
   bool all = true;
   if (in_view)
     for (int oi = 0; oi < nobj; oi++) if (in_view[objs[oi].label]) all = false;
   for (int oi = 0; oi < nobj; oi++) {
     const RayObj& O = objs[oi];
-    if (!all && !in_view[O.label]) { miss++; continue; }
+    if (!all && !in_view[O.label]) continue;
     const VolDev& v = O.v;
     const ObjParams& P = *O.prm;
     const float step = v.size / (float)v.res;
@@ -289,29 +298,42 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
     const float tmin = ray_first_block(v, oo, od, nearp, nfar, eps, &st_par[0][threadIdx.x], &st_tmax[0][threadIdx.x]);
     f3 hit = mk3(0.f, 0.f, 0.f);
     float hw = 0.f;
-    npairs++;
-    if (tmin > 0.f) hw = march(v, oo, od, tmin * v.size / (float)v.res, nfar, mu, step, largestep, hit, nsteps, ninterp);
+    if (STATS) npairs++;
+    if (tmin > 0.f) hw = march<STATS>(v, oo, od, tmin * v.size / (float)v.res, nfar, mu, step, largestep, hit, nsteps, ninterp);
     const float hd = (O.label == 0) ? hw + 0.05f : hw;
-    if (hd > nearest) { miss++; continue; }
+    if (hd > nearest) continue;
     const float inv_vs = (float)v.res / v.size;
     VCache c;
-    const double fg = (double)interp<true>(v, c, hit * inv_vs);
-    ninterp++;
-    if (fg <= 0.5f) { miss++; continue; }
+    const float fg = interp<true>(v, c, hit * inv_vs);
+    if (STATS) ninterp++;
+    if (fg <= 0.5f) continue;
     bool take = false;
     if (hd == nearest) { if (fg > high_fg) take = true; }
     else if (hd < nearest) { nearest = hd; take = true; }
-    if (!take) { miss++; continue; }
+    if (!take) continue;
     high_fg = fg;
-    out_l = O.label;
-    out_v = affine(P.pose, hit);
-    const f3 g = grad(v, c, hit * inv_vs);
-    out_n = (length(g) == 0) ? mk3((float)INVALID, 0.f, 0.f) : normalize(rotate(P.pose, g));
-    nhits++;
+    win = oi;
+    win_hit = hit;
+    if (STATS) nhits++;
   }
-  if (miss == nobj) { out_v = mk3(0.f, 0.f, 0.f); out_n = mk3((float)INVALID, 0.f, 0.f); out_l = INVALID; }
+  f3 out_v = mk3(0.f, 0.f, 0.f), out_n = mk3((float)INVALID, 0.f, 0.f);
+  int out_l = INVALID;
+  if (win >= 0) {
+    // vertex and normal of the last take (the values a later take would
+    // overwrite are never needed, so the gradient runs once per pixel)
+    const RayObj& O = objs[win];
+    const VolDev& v = O.v;
+    const ObjParams& P = *O.prm;
+    out_l = O.label;
+    out_v = affine(P.pose, win_hit);
+    if (nx) {
+      VCache c;
+      const f3 g = grad(v, c, win_hit * ((float)v.res / v.size));
+      out_n = (length(g) == 0) ? mk3((float)INVALID, 0.f, 0.f) : normalize(rotate(P.pose, g));
+    }
+  }
   vx[pix] = out_v.x; vy[pix] = out_v.y; vz[pix] = out_v.z;
-  nx[pix] = out_n.x; ny[pix] = out_n.y; nz[pix] = out_n.z;
+  if (nx) { nx[pix] = out_n.x; ny[pix] = out_n.y; nz[pix] = out_n.z; }
   label[pix] = out_l;
   for (int oi = 0; oi < nobj; oi++) {
     const RayObj& O = objs[oi];
@@ -320,7 +342,7 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
     if (O.mnx) { O.mnx[pix] = mine ? out_n.x : (float)INVALID; O.mny[pix] = mine ? out_n.y : 0.f; O.mnz[pix] = mine ? out_n.z : 0.f; }
   }
   }
-  if (!stats) return;
+  if (!STATS) return;
   atomicAdd(&cnt[0], nsteps); atomicAdd(&cnt[1], ninterp); atomicAdd(&cnt[2], nhits); atomicAdd(&cnt[3], npairs);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -333,9 +355,13 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
 void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,
                     float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label,
                     unsigned long long* stats, const int* in_view, const int* gate) {
-  dim3 blk(RT), grd((w + RT - 1) / RT, h);
-  k_raycast<<<grd, blk, 0, s>>>(objs, nobj, w, h, mu, nearp, farp, eps, vx, vy, vz, nx, ny, nz, label, stats,
-                                in_view, gate);
+  dim3 blk(RT), grd((w + RT_TILE - 1) / RT_TILE, (h + RT_TILE - 1) / RT_TILE);
+  if (stats)
+    k_raycast<true><<<grd, blk, 0, s>>>(objs, nobj, w, h, mu, nearp, farp, eps, vx, vy, vz, nx, ny, nz, label, stats,
+                                        in_view, gate);
+  else
+    k_raycast<false><<<grd, blk, 0, s>>>(objs, nobj, w, h, mu, nearp, farp, eps, vx, vy, vz, nx, ny, nz, label,
+                                         nullptr, in_view, gate);
 }
 
 }  // namespace mf
