@@ -186,7 +186,8 @@ const char* mf_kernel_name(int i);
 /* Device work counters since the last reset, counted while kernel timing is
  * enabled, for algorithmic-byte accounting: {voxel updates, blocks integrated, march steps, trilinear
  * evaluations, raycast hits, allocation band samples, pixels allocated from,
- * ray-object pairs}.  Returns the number of counters. */
+ * ray-object pairs, octree nodes visited by the raycast's first-block
+ * traversal}.  Returns the number of counters. */
 int mf_get_stats(mf_ctx* ctx, uint64_t* out, int cap);
 int mf_reset_stats(mf_ctx* ctx);
 /* Launches a plain float4 device copy of `bytes` bytes `reps` times on the

@@ -147,8 +147,9 @@ void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* mats, const
 // Work counters accumulated on the device (algorithmic-byte accounting):
 // [0] voxel updates, [1] blocks integrated, [2] march steps, [3] trilinear
 // evaluations, [4] raycast hits, [5] allocation band samples, [6] pixels
-// allocated from, [7] ray-object pairs.
-static constexpr int N_STATS = 8;
+// allocated from, [7] ray-object pairs, [8] octree nodes visited by the
+// first-block traversal.
+static constexpr int N_STATS = 9;
 // Work counters are spread over STAT_SLOTS addresses per counter (stats[k *
 // STAT_SLOTS + slot]) so that workgroups do not all hit one atomic address;
 // the host sums the slots.  A null stats pointer disables counting.

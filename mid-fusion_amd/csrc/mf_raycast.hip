@@ -116,8 +116,9 @@ __device__ __forceinline__ bool child_exists(const VolDev& v, int lv, uint32_t p
 static constexpr int RT = 256;          // raycast threads per workgroup
 static constexpr int RT_TILE = 16;      // pixel tile side per workgroup (RT = RT_TILE^2)
 static constexpr int MAX_LG = 10;       // leaf level <= 10: res <= 8192
+template <bool STATS>
 __device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, float farp, float eps,
-                                 uint32_t* __restrict__ st_par, float* __restrict__ st_tmax) {
+                                 uint32_t* __restrict__ st_par, float* __restrict__ st_tmax, unsigned& nnodes) {
   constexpr int DEPTH = 23;
   const int lg = v.leaf_level;                   // log2(res/8)
   const int min_scale = DEPTH - lg;
@@ -150,6 +151,7 @@ __device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, fl
   const float vs = v.size / (float)v.res;
   for (int guard = 0; scale < DEPTH; guard++) {
     if (!parent_ok || guard >= 8192) return -1.f;   // bounded walk
+    if (STATS) nnodes++;
     for (int a = 0; a < 3; a++) t_corner[a] = pos[a] * t_coef[a] - t_bias[a];
     tc_max = fmin_r(fmin_r(t_corner[0], t_corner[1]), t_corner[2]);
     const int cidx = idx ^ octant_mask ^ 7;
@@ -259,10 +261,10 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
                                                  unsigned long long* __restrict__ stats,
                                                  const int* __restrict__ in_view, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  __shared__ unsigned cnt[4];
+  __shared__ unsigned cnt[5];
   __shared__ uint32_t st_par[MAX_LG + 1][RT];
   __shared__ float st_tmax[MAX_LG + 1][RT];
-  if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 5) cnt[threadIdx.x] = 0;
   __syncthreads();
   // 16x16 pixel tile per workgroup, 8x8 per wave: neighbouring rays visit
   // the same blocks, so a wave's gathers share cache lines and its lanes
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int x = blockIdx.x * RT_TILE + (wv & 1) * 8 + (lane & 7);
   const int y = blockIdx.y * RT_TILE + (wv >> 1) * 8 + (lane >> 3);
-  unsigned nsteps = 0, ninterp = 0, nhits = 0, npairs = 0;
+  unsigned nsteps = 0, ninterp = 0, nhits = 0, npairs = 0, nnodes = 0;
   if (x < w && y < h) {
   const size_t pix = (size_t)x + (size_t)y * w;
   float nearest = 1.0e3f;
@@ -295,7 +297,8 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
     const f3 wd = rotate(P.view, mk3((float)x, (float)y, 1.f));
     const f3 od = normalize(rotate(P.inv_pose, wd));
     const float nfar = fmin_r(nearest, farp);
-    const float tmin = ray_first_block(v, oo, od, nearp, nfar, eps, &st_par[0][threadIdx.x], &st_tmax[0][threadIdx.x]);
+    const float tmin = ray_first_block<STATS>(v, oo, od, nearp, nfar, eps, &st_par[0][threadIdx.x],
+                                              &st_tmax[0][threadIdx.x], nnodes);
     f3 hit = mk3(0.f, 0.f, 0.f);
     float hw = 0.f;
     if (STATS) npairs++;
@@ -344,11 +347,13 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
   }
   if (!STATS) return;
   atomicAdd(&cnt[0], nsteps); atomicAdd(&cnt[1], ninterp); atomicAdd(&cnt[2], nhits); atomicAdd(&cnt[3], npairs);
+  atomicAdd(&cnt[4], nnodes);
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long* sl = stats + stat_slot();
     atomicAdd(&sl[2 * STAT_SLOTS], (unsigned long long)cnt[0]); atomicAdd(&sl[3 * STAT_SLOTS], (unsigned long long)cnt[1]);
     atomicAdd(&sl[4 * STAT_SLOTS], (unsigned long long)cnt[2]); atomicAdd(&sl[7 * STAT_SLOTS], (unsigned long long)cnt[3]);
+    atomicAdd(&sl[8 * STAT_SLOTS], (unsigned long long)cnt[4]);
   }
 }
 

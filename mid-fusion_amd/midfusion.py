@@ -309,11 +309,11 @@ class Kfusion:
         return {self.L.mf_kernel_name(i).decode(): (out[2 * i], int(out[2 * i + 1])) for i in range(n)}
 
     STAT_NAMES = ('voxel_updates', 'blocks_integrated', 'march_steps', 'trilinear_evals', 'hits', 'band_samples',
-                  'alloc_pixels', 'ray_object_pairs')
+                  'alloc_pixels', 'ray_object_pairs', 'octree_nodes')
 
     def stats(self):
-        out = np.zeros(8, np.uint64)
-        self.L.mf_get_stats(self.ctx, out, 8)
+        out = np.zeros(len(self.STAT_NAMES), np.uint64)
+        self.L.mf_get_stats(self.ctx, out, len(self.STAT_NAMES))
         return dict(zip(self.STAT_NAMES, (int(v) for v in out)))
 
     def reset_stats(self):
