@@ -118,6 +118,9 @@ def main():
     ap.add_argument('--objects', type=int, default=1)
     ap.add_argument('--plain', action='store_true', help='one object per launch instead of batched launches')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline leg')
+    ap.add_argument('--raw-stream', action='store_true',
+                    help='also time the same frames read from a .raw sequence file through pinned staging '
+                         '(disk + PCIe inclusive; never the headline value)')
     ap.add_argument('--cpu-frames', type=int, default=3)
     ap.add_argument('--track-fused', type=str, default='', metavar='F0,F1,F2',
                     help='1 per pyramid level: one fused launch per GN iteration (default: library default)')
@@ -251,6 +254,42 @@ def main():
                'sample': 'oracle (OpenMP C++ restatement, -O2) on frames 1..%d of the same workload, frame 0 '
                          'as warm-up' % (nc - 1)}
 
+    raw_leg = None
+    if args.raw_stream:
+        import struct
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, 'scene.raw')
+            with open(path, 'wb') as fh:   # the SLAMBench frame layout (interface.h:254-256)
+                for d, rgb, _ in frames:
+                    fh.write(struct.pack('<II', W, H) + np.ascontiguousarray(d, '<u2').tobytes())
+                    fh.write(struct.pack('<II', W, H) + np.ascontiguousarray(rgb, np.uint8).tobytes())
+            raw = mf.RawSequence(path)
+            g2 = mf.Kfusion(W, H, P0, bg_size=5.12, bg_res=512, batch_objects=not args.plain)
+            for i, m in enumerate(movers):
+                g2.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
+                g2.set_moving(i + 1, True)
+
+            def step_raw(f):
+                for i, m in enumerate(dev[f][2]):
+                    g2.L.mf_object_set_mask_device(g2.ctx, i, m.ptr)
+                g2.preprocessing_raw(raw, f)
+                g2.tracking(K, f)
+                g2.integration(K, f)
+                g2.raycasting(K, f)
+            for f in range(args.warmup):
+                step_raw(f)
+            g2.sync()
+            t = time.perf_counter()
+            for f in range(args.warmup, n_frames):
+                step_raw(f)
+            g2.sync()
+            el2 = time.perf_counter() - t
+            raw_leg = {'value': round(args.steps / el2, 3), 'unit': 'frames/s',
+                       'path': '.raw file (page cache) -> pinned double-buffered staging -> PCIe -> preprocess'}
+            raw.close()
+            g2.close()
+
     out = {
         'metric': 'frames/s (track+integrate+raycast, 640x480, cam+K objs)',
         'value': round(fps, 3), 'unit': 'frames/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
@@ -271,6 +310,8 @@ def main():
         'work_per_frame': {k: v // 3 for k, v in st.items()},
         'camera_translation_error_vs_gt_m': round(pose_err, 5),
     }
+    if raw_leg:
+        out['host_input_rate'] = raw_leg
     print(json.dumps(out))
 
 

@@ -26,6 +26,7 @@ extern "C" {
 #define MF_E_OOM (-2)       /* device allocation failed */
 #define MF_E_HIP (-3)       /* HIP runtime error */
 #define MF_E_CAPACITY (-4)  /* block pool of a volume is full */
+#define MF_E_EOF (-5)       /* end of a .raw sequence, or a frame that does not parse (interface.h:297-303) */
 
 #define MF_ROBUST_NONE 0
 #define MF_ROBUST_HUBER 1
@@ -214,6 +215,47 @@ long mf_device_bytes(mf_ctx* ctx);
 enum { MF_TUNE_TRACK_ROWS_PER_THREAD_L0 = 0, MF_TUNE_TRACK_FUSED_L0 = 8 };
 int mf_set_tuning(mf_ctx* ctx, int knob, int value);
 int mf_get_tuning(mf_ctx* ctx, int knob, int* value);
+
+/* ---- On-disk formats on either side of the path ---------------------- */
+typedef struct mf_raw mf_raw;
+/* RawDepthReader (interface.h:207-333): a SLAMBench .raw sequence, frames of
+ * {uint32 w, uint32 h, uint16 depth[w*h], uint32 w, uint32 h, uint8
+ * rgb[3*w*h]}.  NULL if the file cannot be opened or its first frame header
+ * read ("Invalid Raw file"). */
+mf_raw* mf_raw_open(const char* path);
+/* Frame size, whole frames in the file, and RawDepthReader::getK's default
+ * intrinsics (531.15, 531.15, 320, 240) (interface.h:327-329). */
+int mf_raw_info(mf_raw* raw, int* width, int* height, long* frames, float k[4]);
+/* readNextDepthFrame (interface.h:236-304) for frame `frame`: depth (w*h
+ * uint16, mm) and rgb (3*w*h bytes), either may be NULL.  MF_E_EOF past the
+ * end or on a frame whose headers do not repeat frame 0's size. */
+int mf_raw_read(mf_raw* raw, long frame, uint16_t* depth, uint8_t* rgb);
+void mf_raw_close(mf_raw* raw);
+/* mf_preprocess of frame `frame` of a .raw sequence: the file is read into
+ * pinned host staging (two buffers, so the read of one frame overlaps the
+ * copy of the previous one) and copied on the context stream. */
+int mf_preprocess_raw(mf_ctx* ctx, mf_raw* raw, long frame, int filter);
+/* parseGTFile (vtk-io.h:304-330): TUM ground truth; every line of exactly 8
+ * space-separated fields "t tx ty tz qx qy qz qw" gives a row-major 4x4 pose
+ * (toMatrix4(quat, trans), math_utils.h:137-146) in poses[16*i] (up to cap).
+ * Returns the number of poses, MF_E_ARG if the file cannot be opened or a
+ * field does not parse. */
+long mf_parse_gt(const char* path, float* poses, long cap);
+/* Kfusion::tracking with ground-truth poses (kfusion.cpp:262-268): from now
+ * on the camera pose of frame f is poses[f] with its translation moved by
+ * (initial pose - poses[0]) translation, instead of the camera's GN run.
+ * n = 0 turns it off.  mf_track_camera fails with MF_E_ARG for f >= n. */
+int mf_set_gt_poses(mf_ctx* ctx, const float* poses, long n);
+/* Kfusion::save_poses (kfusion.cpp:1591-1626): appends the TUM line of the
+ * camera to "<prefix>_camera" and of each object to "<prefix>_<id>"; the
+ * first call for a file truncates it.  Each line carries the instance's own
+ * translation (the reference writes the camera's for every instance,
+ * kfusion.cpp:1617). */
+int mf_save_poses(mf_ctx* ctx, const char* prefix, int frame);
+/* One TUM line "frame\ttx\tty\ttz\tqx\tqy\tqz\tqw\t\n" for pose T, with
+ * getQuaternion's rotation quaternion (preprocessing.cpp:312-320).  Returns
+ * the line length. */
+int mf_format_tum(const float T[16], int frame, char* out, int cap);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,9 @@
 // no host synchronisation until mf_sync / a getter.
 #include "mf_kernels.h"
 #include "../../include/midfusion.h"
+#include "mf_io.h"
 #include <cstdio>
+#include <string>
 #include <cstring>
 #include <cmath>
 #include <algorithm>
@@ -94,6 +96,21 @@ struct mf_ctx {
   Mat4* vcam = nullptr;               // [MAX_INST] virtual camera poses by object id
   RayObj* rayobjs_bf = nullptr;       // raycast table writing the objects' maps before fusion
   RayObj ray_bf_host[MAX_INST];
+  // .raw input: two pinned host staging buffers, each released by an event
+  // once its copy has run
+  uint8_t* raw_pin[2] = {nullptr, nullptr};
+  hipEvent_t raw_ev[2] = {nullptr, nullptr};
+  bool raw_ev_used[2] = {false, false};
+  size_t raw_cap = 0;
+  int raw_next = 0;
+  // ground-truth camera poses (re-based to the initial position), device copy
+  // and the device frame index the graph reads
+  std::vector<Mat4> gt;
+  Mat4* gt_dev = nullptr;
+  long gt_cap = 0;
+  int* gt_frame = nullptr;
+  float init_t[3] = {0.f, 0.f, 0.f};
+  std::vector<bool> pose_saved;     // per instance: its pose file was started
   float* eval_maps = nullptr;         // mf_motion_residuals inputs (9 planes), allocated on first use
   Mat4* eval_T = nullptr;
   int8_t* eval_codes = nullptr;
@@ -480,6 +497,7 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   for (int i = 0; i < 5; i++) { int x = i - 2; g[i] = expf((float)(-(x * x)) / (2.0f * 4.0f * 4.0f)); }
   hipMemcpyAsync(c->gauss, g, sizeof g, hipMemcpyHostToDevice, c->stream);
   const Mat4 P = m4(init_pose);
+  c->init_t[0] = init_pose[3]; c->init_t[1] = init_pose[7]; c->init_t[2] = init_pose[11];
   hipMemcpyAsync(c->mats, &P, sizeof P, hipMemcpyHostToDevice, c->stream);
   hipMemcpyAsync(c->mats + 1, &P, sizeof P, hipMemcpyHostToDevice, c->stream);
   // model normals INVALID until the first raycast
@@ -501,6 +519,10 @@ void mf_destroy(mf_ctx* c) {
   for (auto e : c->pool) hipEventDestroy(e);
   for (auto& g : c->gcache) hipGraphExecDestroy(g.second);
   for (void* p : c->allocs) hipFree(p);
+  for (int b = 0; b < 2; b++) {
+    if (c->raw_pin[b]) hipHostFree(c->raw_pin[b]);
+    if (c->raw_ev[b]) hipEventDestroy(c->raw_ev[b]);
+  }
   for (Object* o : c->objs) delete o;
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -591,17 +613,98 @@ int mf_preprocess_device(mf_ctx* c, const uint16_t* d_depth, const uint8_t* d_rg
   return 0;
 }
 
+// Device input buffers for host frames of n pixels.
+bool ensure_inputs(mf_ctx* c, size_t n) {
+  if (n <= c->in_cap) return true;
+  c->in_depth = dalloc<uint16_t>(c, n);
+  c->in_rgb = dalloc<uint8_t>(c, 3 * n);
+  if (!c->in_depth || !c->in_rgb) return false;
+  c->in_cap = n;
+  return true;
+}
+
 int mf_preprocess(mf_ctx* c, const uint16_t* depth, const uint8_t* rgb, int iw, int ih, int filter) {
   const size_t n = (size_t)iw * ih;
-  if (n > c->in_cap) {
-    c->in_depth = dalloc<uint16_t>(c, n);
-    c->in_rgb = dalloc<uint8_t>(c, 3 * n);
-    if (!c->in_depth || !c->in_rgb) return fail(c, MF_E_OOM, "input staging");
-    c->in_cap = n;
-  }
+  if (!ensure_inputs(c, n)) return fail(c, MF_E_OOM, "input staging");
   hipMemcpyAsync(c->in_depth, depth, n * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream);
   if (rgb) hipMemcpyAsync(c->in_rgb, rgb, 3 * n, hipMemcpyHostToDevice, c->stream);
   return mf_preprocess_device(c, c->in_depth, rgb ? c->in_rgb : nullptr, iw, ih, filter);
+}
+
+int mf_preprocess_raw(mf_ctx* c, mf_raw* raw, long frame, int filter) {
+  int w = 0, h = 0;
+  long frames = 0;
+  if (!raw || mf_raw_info(raw, &w, &h, &frames, nullptr) != MF_OK) return fail(c, MF_E_ARG, "no raw sequence");
+  const size_t n = (size_t)w * h;
+  if (!ensure_inputs(c, n)) return fail(c, MF_E_OOM, "input staging");
+  if (5 * n > c->raw_cap) {
+    for (int b = 0; b < 2; b++) {
+      if (c->raw_ev_used[b]) hipEventSynchronize(c->raw_ev[b]);
+      if (c->raw_pin[b]) hipHostFree(c->raw_pin[b]);
+      c->raw_pin[b] = nullptr;
+      c->raw_ev_used[b] = false;
+      if (hipHostMalloc((void**)&c->raw_pin[b], 5 * n, hipHostMallocDefault) != hipSuccess)
+        return fail(c, MF_E_OOM, "pinned staging");
+      if (!c->raw_ev[b] && hipEventCreateWithFlags(&c->raw_ev[b], hipEventDisableTiming) != hipSuccess)
+        return fail(c, MF_E_HIP, "staging event");
+    }
+    c->raw_cap = 5 * n;
+  }
+  const int b = c->raw_next;
+  c->raw_next ^= 1;
+  if (c->raw_ev_used[b]) hipEventSynchronize(c->raw_ev[b]);   // the copy that last read this buffer has run
+  uint8_t* p = c->raw_pin[b];
+  if (mf_raw_read(raw, frame, reinterpret_cast<uint16_t*>(p), p + 2 * n) != MF_OK)
+    return fail(c, MF_E_EOF, "end of the raw sequence");
+  hipMemcpyAsync(c->in_depth, p, 2 * n, hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(c->in_rgb, p + 2 * n, 3 * n, hipMemcpyHostToDevice, c->stream);
+  hipEventRecord(c->raw_ev[b], c->stream);
+  c->raw_ev_used[b] = true;
+  return mf_preprocess_device(c, c->in_depth, c->in_rgb, w, h, filter);
+}
+
+int mf_set_gt_poses(mf_ctx* c, const float* poses, long n) {
+  if (n < 0 || (n > 0 && !poses)) return fail(c, MF_E_ARG, "bad ground-truth poses");
+  c->gt.clear();
+  for (long f = 0; f < n; f++) {
+    // camera_pose = poses[frame] with its translation re-based to the
+    // initial position (kfusion.cpp:263-266), in float
+    Mat4 P = m4(poses + 16 * f);
+    for (int r = 0; r < 3; r++) P.m[r][3] = (P.m[r][3] - poses[4 * r + 3]) + c->init_t[r];
+    c->gt.push_back(P);
+  }
+  if (n > c->gt_cap) {
+    c->gt_dev = dalloc<Mat4>(c, n);
+    if (!c->gt_frame) c->gt_frame = dalloc<int>(c, 1);
+    if (!c->gt_dev || !c->gt_frame) return fail(c, MF_E_OOM, "ground-truth poses");
+    c->gt_cap = n;
+  }
+  if (n > 0) hipMemcpyAsync(c->gt_dev, c->gt.data(), n * sizeof(Mat4), hipMemcpyHostToDevice, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
+int mf_save_poses(mf_ctx* c, const char* prefix, int frame) {
+  if (!prefix) return fail(c, MF_E_ARG, "no file prefix");
+  const int n = (int)c->objs.size();
+  std::vector<Mat4> M(3 + n);
+  hipMemcpyAsync(M.data(), c->mats, M.size() * sizeof(Mat4), hipMemcpyDeviceToHost, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  if ((int)c->pose_saved.size() < n) c->pose_saved.resize(n, false);
+  for (int id = 0; id < n; id++) {
+    const Mat4& P = (id == 0) ? M[0] : M[3 + id];
+    float T[16];
+    for (int i = 0; i < 16; i++) T[i] = P.m[i / 4][i % 4];
+    char line[256];
+    if (mf::tum_line(line, sizeof line, frame, T) < 0) return fail(c, MF_E_ARG, "pose line");
+    const std::string path = std::string(prefix) + (id == 0 ? std::string("_camera") : "_" + std::to_string(id));
+    FILE* f = fopen(path.c_str(), c->pose_saved[id] ? "a" : "w");
+    if (!f) return fail(c, MF_E_ARG, "cannot open pose file");
+    fputs(line, f);
+    fclose(f);
+    c->pose_saved[id] = true;
+  }
+  return 0;
 }
 
 // Raycast descriptor tables: the model-map table (each object's maps) and the
@@ -675,7 +778,10 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
       launch_level(s, c->live[l], invK, k[1] < 0 ? 1 : 0, k[0] / sc, 1.0f, 1.0f, 0.075f, c->cfg.use_rgb != 0, c->cfg.use_icp != 0);
     }
   };
+  const bool gt = !c->gt.empty();
+  if (gt && (long)frame >= (long)c->gt.size()) return fail(c, MF_E_ARG, "no ground-truth pose for this frame");
   if ((int)frame == c->cfg.init_frame || !c->have_ref) {
+    if (gt) hipMemcpyAsync(c->mats + 0, &c->gt[frame], sizeof(Mat4), hipMemcpyHostToDevice, s);
     pyramid();
     for (int l = 0; l < L; l++) std::swap(c->live[l], c->ref[l]);   // setRefImgFromCurr
     c->have_ref = true;
@@ -702,8 +808,14 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
   auto first_pass = [&]() {
     pyramid();
     { Timed t(c, KC_GNSMALL); launch_camera_begin(s, c->mats, c->gn, k[0], k[1], k[2], k[3]); }
-    if (c->cfg.use_rgb) render_reference(c, 0, 1);
-    gn_run(c, k, 0, 0, 1, nullptr);
+    if (gt) {
+      // the ground-truth pose replaces the camera's GN run (kfusion.cpp:262-268)
+      Timed t(c, KC_GNSMALL);
+      launch_pose_from_table(s, c->mats + 0, c->gt_dev, c->gt_frame);
+    } else {
+      if (c->cfg.use_rgb) render_reference(c, 0, 1);
+      gn_run(c, k, 0, 0, 1, nullptr);
+    }
     if (!multi) return;
     { Timed t(c, KC_MOTION); launch_motion_gate(s, md, c->label, (int)c->N0, nobj); }
     raycast_before_fusion(c, k, nullptr, &md->gate);
@@ -721,8 +833,9 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
     { Timed t(c, KC_GNSMALL); launch_gn_begin_camera(s, c->gn + 0, c->mats + 0, c->mats + 1, &md->retrack, k[0], k[1], k[2], k[3]); }
     gn_run(c, k, 0, 0, 1, c->label, md->move_set);
   };
+  if (gt) launch_set_int(s, c->gt_frame, (int)frame);   // by value, ahead of the stage's graph
   std::vector<uint64_t> key = base_key(c, 1, k);
-  key.push_back((uint64_t)nobj | ((uint64_t)autom << 32));
+  key.push_back((uint64_t)nobj | ((uint64_t)autom << 32) | ((uint64_t)gt << 33));
   if (!autom) {
     bool any = false;
     for (int i = 1; i < nobj; i++) any = any || c->objs[i]->moving;

@@ -94,6 +94,9 @@ struct MotionLaunch {
   int* counts;              //   per-label {pixels, inliers} counts [2*MAX_INST]
 };
 void launch_motion_rows(hipStream_t s, const MotionLaunch& m);
+// *dst = table[*index] (ground-truth camera pose of the frame)
+void launch_pose_from_table(hipStream_t s, Mat4* dst, const Mat4* table, const int* index);
+void launch_set_int(hipStream_t s, int* dst, int v);
 // Frame-start clear of the motion state and the presence of the last
 // raycast's labels, then the gate (and the in-view set cleared under it).
 void launch_motion_gate(hipStream_t s, MotionDev* md, const int32_t* label, int n, int nobj);

@@ -922,6 +922,12 @@ void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNS
   launch_track_rows(s, p, inst, 1, st, partial, nblk, 0);
   k_eval_sums<<<1, SOLVE_NT, 0, s>>>(st, state, partial, nblk);
 }
+__global__ void k_pose_from_table(Mat4* dst, const Mat4* table, const int* index) { *dst = table[*index]; }
+__global__ void k_set_int(int* dst, int v) { *dst = v; }
+void launch_set_int(hipStream_t s, int* dst, int v) { k_set_int<<<1, 1, 0, s>>>(dst, v); }
+void launch_pose_from_table(hipStream_t s, Mat4* dst, const Mat4* table, const int* index) {
+  k_pose_from_table<<<1, 1, 0, s>>>(dst, table, index);
+}
 void launch_motion_rows(hipStream_t s, const MotionLaunch& m) {
   k_motion_rows<<<dim3((m.W + 255) / 256, m.H), 256, 0, s>>>(m);
 }

@@ -50,6 +50,15 @@ SIGNATURES = {
     'mf_get_motion_codes': (C.c_int, [P, i32p]),
     'mf_motion_residuals': (C.c_int, [P, f32p, f32p, f32p, f32p, f32p, i32p]),
     'mf_set_trackresult': (C.c_int, [P, C.c_int, i32p]),
+    'mf_raw_open': (P, [C.c_char_p]),
+    'mf_raw_info': (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_long), f32p]),
+    'mf_raw_read': (C.c_int, [P, C.c_long, P, P]),
+    'mf_raw_close': (None, [P]),
+    'mf_preprocess_raw': (C.c_int, [P, P, C.c_long, C.c_int]),
+    'mf_parse_gt': (C.c_long, [C.c_char_p, P, C.c_long]),
+    'mf_set_gt_poses': (C.c_int, [P, P, C.c_long]),
+    'mf_save_poses': (C.c_int, [P, C.c_char_p, C.c_int]),
+    'mf_format_tum': (C.c_int, [f32p, C.c_int, C.c_char_p, C.c_int]),
     'mf_object_set_mask': (C.c_int, [P, C.c_int, f32p]),
     'mf_object_set_mask_device': (C.c_int, [P, C.c_int, P]),
     'mf_preprocess': (C.c_int, [P, u16p, u8p, C.c_int, C.c_int, C.c_int]),
@@ -108,6 +117,57 @@ def load(path=LIB_PATH):
 
 def _f32(a):
     return np.ascontiguousarray(a, np.float32)
+
+
+class RawSequence:
+    """A SLAMBench .raw sequence (RawDepthReader, interface.h:207-333)."""
+
+    def __init__(self, path):
+        self.L = load()
+        self.h_ = self.L.mf_raw_open(path.encode())
+        if not self.h_:
+            raise MFError('Invalid Raw file: %s' % path)
+        w, h, n = C.c_int(), C.c_int(), C.c_long()
+        k = np.zeros(4, np.float32)
+        self.L.mf_raw_info(self.h_, C.byref(w), C.byref(h), C.byref(n), k)
+        self.width, self.height, self.frames, self.k = w.value, h.value, n.value, k
+
+    def read(self, frame):
+        """(depth uint16 [h, w], rgb uint8 [h, w, 3]); MFError past the end."""
+        d = np.zeros((self.height, self.width), np.uint16)
+        r = np.zeros((self.height, self.width, 3), np.uint8)
+        rc = self.L.mf_raw_read(self.h_, frame, d.ctypes.data, r.ctypes.data)
+        if rc != 0:
+            raise MFError('end of the raw sequence (frame %d)' % frame)
+        return d, r
+
+    def close(self):
+        if getattr(self, 'h_', None):
+            self.L.mf_raw_close(self.h_)
+            self.h_ = None
+
+    def __del__(self):
+        self.close()
+
+
+def parse_gt(path):
+    """TUM ground truth (parseGTFile, vtk-io.h:304-330): poses [n, 4, 4]."""
+    L = load()
+    n = L.mf_parse_gt(path.encode(), None, 0)
+    if n < 0:
+        raise MFError('cannot read ground truth %s' % path)
+    out = np.zeros((n, 4, 4), np.float32)
+    L.mf_parse_gt(path.encode(), out.ctypes.data, n)
+    return out
+
+
+def format_tum(T, frame):
+    """One Kfusion::save_poses line for pose T."""
+    buf = C.create_string_buffer(256)
+    n = load().mf_format_tum(_f32(np.asarray(T).reshape(16)), frame, buf, 256)
+    if n < 0:
+        raise MFError('pose line')
+    return buf.value.decode()
 
 
 class Kfusion:
@@ -199,6 +259,18 @@ class Kfusion:
         r = np.ascontiguousarray(rgb, np.uint8)
         return self._chk(self.L.mf_preprocess(self.ctx, d.reshape(-1), r.reshape(-1), d.shape[1], d.shape[0],
                                               int(filt)))
+
+    def preprocessing_raw(self, raw, frame, filt=False):
+        """Frame `frame` of a RawSequence through pinned staging."""
+        return self._chk(self.L.mf_preprocess_raw(self.ctx, raw.h_, frame, int(filt)))
+
+    def set_gt_poses(self, poses):
+        """Ground-truth camera poses [n, 4, 4] (n = 0: off)."""
+        p = np.ascontiguousarray(np.asarray(poses, np.float32).reshape(-1, 16))
+        return self._chk(self.L.mf_set_gt_poses(self.ctx, p.ctypes.data if len(p) else None, len(p)))
+
+    def save_poses(self, prefix, frame):
+        return self._chk(self.L.mf_save_poses(self.ctx, prefix.encode(), frame))
 
     def preprocessing_device(self, d_depth, d_rgb, in_w, in_h, filt=False):
         return self._chk(self.L.mf_preprocess_device(self.ctx, d_depth, d_rgb, in_w, in_h, int(filt)))

@@ -299,6 +299,8 @@ struct Pipe {
   int rendered_n = 0;           // rendered_mask_.pair_instance_seg_.size() (persists across frames)
   bool gate = false;            // raycast_mask_.pair_instance_seg_.size() > 1 at this frame's tracking
   std::vector<char> move_set;   // move_obj_set (kfusion.cpp:289, 331)
+  std::vector<M4> gt;           // ground-truth camera poses, re-based (kfusion.cpp:262-268)
+  R init_t[3] = {0, 0, 0};      // _init_camera_Pose
   std::vector<Object*> objs;
   std::vector<TraceRec> trace;
   float ray_eps = -1.f;   // ray_iterator's static epsilon: set by the first raycast volume
@@ -432,6 +434,7 @@ void* orc_create(const orc_config* cfg, const float init_pose[16]) {
   p->vertex.assign(n, v3(0, 0, 0)); p->normal.assign(n, v3(0, 0, 0)); p->label.assign(n, INVALID);
   p->camera_pose = m4_from(init_pose);
   p->T_w_r = p->camera_pose;
+  for (int r = 0; r < 3; r++) p->init_t[r] = (R)init_pose[4 * r + 3];
   p->trk.init(c);
   p->trk.trace = &p->trace;
   return p;
@@ -633,11 +636,19 @@ int orc_track_camera(void* h, const float k[4], unsigned frame) {
   if (frame % p->c.track_rate != 0) return 0;
   p->T_w_r = p->camera_pose;
   t.set_params_frame(k, p->grey.data(), p->depth.data());
-  if ((int)frame == p->c.init_frame) { t.set_ref_from_curr(); return 0; }
-  t.trace_inst = 0;
-  t.track_live(p->camera_pose, p->T_w_r, p->vertex.data(), p->normal.data(), nullptr);
-  p->objs[0]->virtual_cam = p->camera_pose;
-  memcpy(p->objs[0]->trackresult.data(), t.J[0].data(), sizeof(TrackRow) * 2 * n);
+  if (!p->gt.empty()) {
+    // ground-truth camera (kfusion.cpp:262-268): no GN run, the camera's
+    // trackresult_ is left as it was
+    if (frame >= p->gt.size()) return -1;
+    p->camera_pose = p->gt[frame];
+    if ((int)frame == p->c.init_frame) { t.set_ref_from_curr(); return 0; }
+  } else {
+    if ((int)frame == p->c.init_frame) { t.set_ref_from_curr(); return 0; }
+    t.trace_inst = 0;
+    t.track_live(p->camera_pose, p->T_w_r, p->vertex.data(), p->normal.data(), nullptr);
+    p->objs[0]->virtual_cam = p->camera_pose;
+    memcpy(p->objs[0]->trackresult.data(), t.J[0].data(), sizeof(TrackRow) * 2 * n);
+  }
   motion_check(*p, k);
   std::vector<uint8_t> dyn(n, 0);
   bool any_moving = false;
@@ -696,6 +707,19 @@ int orc_track_objects(void* h, const float k[4], unsigned frame) {
   }
   t.set_ref_from_curr();
   return 1;
+}
+
+// Ground-truth camera poses (row-major 4x4 each); n = 0 turns the mode off.
+// The translation of poses[f] is moved by (initial position - poses[0]
+// translation), as kfusion.cpp:263-266 does, in R arithmetic.
+void orc_set_gt_poses(void* h, const float* poses, long n) {
+  Pipe* p = (Pipe*)h;
+  p->gt.clear();
+  for (long f = 0; f < n; f++) {
+    M4 P = m4_from(poses + 16 * f);
+    for (int r = 0; r < 3; r++) P.m[r][3] = (P.m[r][3] - (R)poses[4 * r + 3]) + p->init_t[r];
+    p->gt.push_back(P);
+  }
 }
 
 // Forward/backward of the tracking residual at a given pose: the 32 FP64

@@ -57,6 +57,7 @@ def _lib(fp64):
                                         np.ctypeslib.ndpointer(np.int64, flags='C_CONTIGUOUS'), i32p, C.c_int]),
         'orc_motion_rows': (C.c_int, [P, i32p, f32p]),
         'orc_motion_state': (C.c_int, [P, i32p, i32p, i32p, C.c_int]),
+        'orc_set_gt_poses': (None, [P, f32p, C.c_long]),
         'orc_motion_residuals': (C.c_int, [P, f32p, f32p, f32p, f32p, f32p, i32p]),
         'orc_get_maps_bf': (None, [P, f32p, f32p, i32p]),
         'orc_object_vertex_bef': (None, [P, C.c_int, f32p]),
@@ -201,6 +202,11 @@ class Oracle:
         px, inl, mv = np.zeros(cap, np.int64), np.zeros(cap, np.int64), np.zeros(cap, np.int32)
         n = self.L.orc_motion_result(self.h_, px, inl, mv, cap)
         return px[:n], inl[:n], mv[:n]
+
+    def set_gt_poses(self, poses):
+        """Ground-truth camera poses (n x 4 x 4); an empty array turns the mode off."""
+        p = f32(np.asarray(poses, np.float32).reshape(-1))
+        self.L.orc_set_gt_poses(self.h_, p if p.size else np.zeros(16, np.float32), len(p) // 16)
 
     def motion_state(self):
         """(gate, rendered_n, in_view flags, moving set) of the last tracking call."""
