@@ -118,6 +118,8 @@ def main():
     ap.add_argument('--objects', type=int, default=1)
     ap.add_argument('--plain', action='store_true', help='one object per launch instead of batched launches')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline leg')
+    ap.add_argument('--render', action='store_true',
+                    help='also time the display renderers and volume2mask (ms per call, image on the host)')
     ap.add_argument('--raw-stream', action='store_true',
                     help='also time the same frames read from a .raw sequence file through pinned staging '
                          '(disk + PCIe inclusive; never the headline value)')
@@ -254,6 +256,24 @@ def main():
                'sample': 'oracle (OpenMP C++ restatement, -O2) on frames 1..%d of the same workload, frame 0 '
                          'as warm-up' % (nc - 1)}
 
+    render = None
+    if args.render:
+        P = g.getPose()
+        Pv = P.copy()
+        Pv[:3, 3] += np.array([0.05, -0.03, 0.02], np.float32)
+        calls = {'render_volume_maps': lambda: g.render_volume(K, P, largestep=0.075),
+                 'render_volume_raycast': lambda: g.render_volume(K, Pv, largestep=0.075),
+                 'volume2mask': lambda: g.volume2mask(K),
+                 'render_track_rgbd': lambda: g.render_track(0, 2),
+                 'render_depth': lambda: g.render_depth()}
+        render = {}
+        for name, fn in calls.items():
+            fn()
+            t = time.perf_counter()
+            for _ in range(10):
+                fn()
+            render[name] = round(1000.0 * (time.perf_counter() - t) / 10, 3)
+
     raw_leg = None
     if args.raw_stream:
         import struct
@@ -312,6 +332,8 @@ def main():
     }
     if raw_leg:
         out['host_input_rate'] = raw_leg
+    if render:
+        out['render_ms_per_call'] = render
     print(json.dumps(out))
 
 

@@ -257,6 +257,39 @@ int mf_save_poses(mf_ctx* ctx, const char* prefix, int frame);
  * the line length. */
 int mf_format_tum(const float T[16], int frame, char* out, int cap);
 
+/* ---- Display rendering and label projection ----------------------------
+ * Images are W*H (or w*h) uchar4 (x, y, z, w bytes); every call blocks until
+ * the image is on the host. */
+/* Kfusion::renderTrack (kfusion.cpp:1332-1425) of instance id's
+ * trackresult_: mode 0 renderTrackKernel (ICP rows), 1 renderRGBTrackKernel
+ * (photometric rows), 2 render_RGBD_TrackKernel (rendering.cpp:183-237,
+ * 395-487). */
+int mf_render_track(mf_ctx* ctx, int id, int mode, uint8_t* out_rgba);
+/* renderDepthKernel of the frame depth, near 0.1 m, far 4 m
+ * (rendering.cpp:368-394, kfusion.cpp:1435-1441). */
+int mf_render_depth(mf_ctx* ctx, uint8_t* out_rgba);
+/* renderIntensityKernel of the frame intensity (rendering.cpp:648-664). */
+int mf_render_intensity(mf_ctx* ctx, uint8_t* out_rgba);
+/* volume2MaskKernel (rendering.cpp:240-343; Kfusion::volume2mask,
+ * kfusion.cpp:679-686): instance labels of every object raycast at the
+ * camera pose (INVALID where every object misses), W*H int32. */
+int mf_volume2mask(mf_ctx* ctx, const float k[4], int32_t* labels);
+/* Kfusion::renderVolume (kfusion.cpp:1284-1328, renderVolume_many_Kernel
+ * rendering.cpp:489-646): shaded w*h image at view_pose.  The last raycast's
+ * maps are shaded when view_pose is that raycast's pose and w, h the
+ * computation size; otherwise every object is raycast again (far plane 8 m,
+ * march step `largestep`, 0.75*mu in the benchmark).  render_color: voxel
+ * colours, else palette colours by label (rgba bytes, n_palette <= 256; the
+ * reference draws them at random, rendering.cpp:666-676) and white for the
+ * background.  Returns 1 if it raycast, 0 if it used the maps. */
+int mf_render_volume(mf_ctx* ctx, const float k[4], const float view_pose[16], int w, int h, float largestep,
+                     int render_color, const uint8_t* palette_rgba, int n_palette, uint8_t* out_rgba);
+/* Photometric row errors (TrackData.error of the rows in trackresult_ from W*H
+ * on) of instance id: read, or overwrite (e.g. to render rows computed
+ * elsewhere). */
+int mf_get_track_errors(mf_ctx* ctx, int id, float* rgb_err);
+int mf_set_track_errors(mf_ctx* ctx, int id, const float* rgb_err);
+
 #ifdef __cplusplus
 }
 #endif

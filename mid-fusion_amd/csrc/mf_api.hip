@@ -39,6 +39,7 @@ struct Object {
   float* mv[3] = {nullptr, nullptr, nullptr};
   float* mn[3] = {nullptr, nullptr, nullptr};
   int8_t* codes = nullptr;       // level-0 result codes of this instance's last pass
+  float* rgb_err = nullptr;      // level-0 photometric row errors of that pass
   float* mvbf[3] = {nullptr, nullptr, nullptr};   // vertex map of the last raycast before fusion
   float* rD[MAX_LEVELS] = {};    // rendered reference depth pyramid
   ObjParams* prm = nullptr;
@@ -111,6 +112,16 @@ struct mf_ctx {
   int* gt_frame = nullptr;
   float init_t[3] = {0.f, 0.f, 0.f};
   std::vector<bool> pose_saved;     // per instance: its pose file was started
+  // display rendering: the pose of the last raycast (raycastPose), the view
+  // pose and palette of a render call, and the output image
+  Mat4* ray_pose = nullptr;
+  Mat4* view_dev = nullptr;
+  uint32_t* palette_dev = nullptr;
+  uint32_t* render_out = nullptr;
+  int32_t* label_scratch = nullptr;
+  size_t render_cap = 0;
+  RayObj* rayobjs_lab = nullptr;      // raycast table writing no per-object maps
+  RayObj ray_lab_host[MAX_INST];
   float* eval_maps = nullptr;         // mf_motion_residuals inputs (9 planes), allocated on first use
   Mat4* eval_T = nullptr;
   int8_t* eval_codes = nullptr;
@@ -222,6 +233,7 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
     for (int k = 0; k < 3; k++) { o->mv[k] = b ? b + k * c->N0 : nullptr; o->mn[k] = b ? b + (3 + k) * c->N0 : nullptr; }
   }
   o->codes = dalloc<int8_t>(c, 2 * c->N0);
+  o->rgb_err = dalloc<float>(c, c->N0);
   {
     float* b = dalloc<float>(c, 3 * c->N0);
     for (int k = 0; k < 3; k++) o->mvbf[k] = b ? b + k * c->N0 : nullptr;
@@ -232,7 +244,7 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   }
   o->prm = dalloc<ObjParams>(c, 1);
   if (!v.slot || !v.req || !v.nodes || !v.tsdf || !v.fg || !v.c || !v.fgw || !o->mask ||
-      !o->codes || !o->prm || !o->mv[2] || !o->mn[2] || !o->mvbf[2]) {
+      !o->codes || !o->rgb_err || !o->prm || !o->mv[2] || !o->mn[2] || !o->mvbf[2]) {
     delete o;
     return fail(c, MF_E_OOM, "device allocation failed");
   }
@@ -343,13 +355,13 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
 }
 
 void set_inst(mf_ctx* c, int slot, float* const mv[3], float* const mn[3], float* const rD[MAX_LEVELS],
-              int8_t* codes, int state) {
+              int8_t* codes, float* rgb_err, int state) {
   TrackInst ti;
   memset(&ti, 0, sizeof ti);
   ti.mvn = mv[0];      // planes: mv[0..2], mn[0..2] back to back
   ti.rD = rD[0];
   (void)mn;
-  ti.codes = codes; ti.state = state;
+  ti.codes = codes; ti.rgb_err = rgb_err; ti.state = state;
   // descriptors only change with the object set: upload on change, ordered
   // after every pending launch that may still read the old table
   if (memcmp(&ti, &c->inst_host[slot], sizeof ti) != 0) {
@@ -449,6 +461,7 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   memset(c->inst_host, 0, sizeof c->inst_host);
   memset(c->ray_host, 0, sizeof c->ray_host);
   memset(c->ray_bf_host, 0, sizeof c->ray_bf_host);
+  memset(c->ray_lab_host, 0, sizeof c->ray_lab_host);
   c->cfg = *cfg;
   if (c->cfg.track_rate <= 0) c->cfg.track_rate = 1;
   if (c->cfg.integ_rate <= 0) c->cfg.integ_rate = 1;
@@ -475,6 +488,10 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->md = dalloc<MotionDev>(c, 1);
   c->vcam = dalloc<Mat4>(c, MAX_INST);
   c->rayobjs_bf = dalloc<RayObj>(c, MAX_INST);
+  c->rayobjs_lab = dalloc<RayObj>(c, MAX_INST);
+  c->ray_pose = dalloc<Mat4>(c, 1);
+  c->view_dev = dalloc<Mat4>(c, 1);
+  c->palette_dev = dalloc<uint32_t>(c, 256);
   c->mats = dalloc<Mat4>(c, 3 + MAX_INST);
   c->gn = dalloc<GNState>(c, MAX_INST);
   c->inst = dalloc<TrackInst>(c, MAX_INST);
@@ -488,7 +505,8 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->gauss = dalloc<float>(c, 5);
   c->stats = dalloc<unsigned long long>(c, (size_t)N_STATS * STAT_SLOTS);
   c->obj_ids = dalloc<int>(c, MAX_INST);
-  ok = ok && c->vbf[2] && c->label_bf && c->motion_codes && c->md && c->vcam && c->rayobjs_bf;
+  ok = ok && c->vbf[2] && c->label_bf && c->motion_codes && c->md && c->vcam && c->rayobjs_bf && c->rayobjs_lab &&
+       c->ray_pose && c->view_dev && c->palette_dev;
   ok = ok && c->stats && c->obj_ids && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
        c->inst && c->partial && c->moving && c->trace && c->trace_n && c->rayobjs && c->mapobjs && c->gauss;
   if (!ok) { mf_destroy(c); return nullptr; }
@@ -719,6 +737,9 @@ void upload_ray_tables(mf_ctx* c) {
     r.v = o->v; r.prm = o->prm; r.label = o->id;
     r.mvx = o->mv[0]; r.mvy = o->mv[1]; r.mvz = o->mv[2];
     r.mnx = o->mn[0]; r.mny = o->mn[1]; r.mnz = o->mn[2];
+    RayObj& lb = c->ray_lab_host[i];
+    lb = r;
+    lb.mvx = lb.mvy = lb.mvz = lb.mnx = lb.mny = lb.mnz = nullptr;
     RayObj& b = c->ray_bf_host[i];
     b = r;
     b.mnx = b.mny = b.mnz = nullptr;
@@ -727,6 +748,7 @@ void upload_ray_tables(mf_ctx* c) {
   }
   hipMemcpyAsync(c->rayobjs, c->ray_host, n * sizeof(RayObj), hipMemcpyHostToDevice, c->stream);
   hipMemcpyAsync(c->rayobjs_bf, c->ray_bf_host, n * sizeof(RayObj), hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(c->rayobjs_lab, c->ray_lab_host, n * sizeof(RayObj), hipMemcpyHostToDevice, c->stream);
   hipStreamSynchronize(c->stream);
   c->ray_n = n;
 }
@@ -791,7 +813,7 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
   // camera: trackLiveFrame against the global model maps
   float* camD[MAX_LEVELS];
   for (int l = 0; l < MAX_LEVELS; l++) camD[l] = bg->rD[l];
-  set_inst(c, 0, c->vtx, c->nrm, camD, bg->codes, 0);
+  set_inst(c, 0, c->vtx, c->nrm, camD, bg->codes, bg->rgb_err, 0);
   const int nobj = (int)c->objs.size();
   const bool multi = nobj >= 2;
   const bool autom = multi && c->motion_auto;
@@ -876,7 +898,7 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
     for (size_t j = 0; j < mov.size(); j++) {
       Object* o = mov[j];
       ids[j] = 3 + o->id;
-      set_inst(c, 1 + (int)j, o->mv, o->mn, o->rD, o->codes, 1 + (int)j);
+      set_inst(c, 1 + (int)j, o->mv, o->mn, o->rD, o->codes, o->rgb_err, 1 + (int)j);
     }
     if (!mov.empty()) set_ids(c, ids);
     std::vector<uint64_t> key = base_key(c, 2, k);
@@ -899,14 +921,18 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
         // against the maps before fusion, and a second raycast before fusion
         // renders the objects at their new poses (kfusion.cpp:416-441)
         { Timed t(c, KC_MOTION);
-          launch_motion_rows(s, motion_launch(c, k, c->mats + 0, c->vtx[0], c->vbf[0], bg->codes, &md->move_any)); }
+          MotionLaunch m = motion_launch(c, k, c->mats + 0, c->vtx[0], c->vbf[0], bg->codes, &md->move_any);
+          m.rgb_err = bg->rgb_err;
+          launch_motion_rows(s, m); }
         raycast_before_fusion(c, k, md->in_view, &md->move_any);
         { Timed t(c, KC_MOTION); launch_motion_after(s, md, c->label_bf, (int)c->N0, nobj, c->cfg.use_rgb); }
         // each moving object's rows: motion residuals at its virtual camera
         // pose (kfusion.cpp:445-467)
         for (Object* o : mov) {
           Timed t(c, KC_MOTION);
-          launch_motion_rows(s, motion_launch(c, k, c->vcam + o->id, o->mv[0], o->mvbf[0], o->codes, &md->cond2));
+          MotionLaunch m = motion_launch(c, k, c->vcam + o->id, o->mv[0], o->mvbf[0], o->codes, &md->cond2);
+          m.rgb_err = o->rgb_err;
+          launch_motion_rows(s, m);
         }
       } else {
         Timed t(c, KC_MOTION);
@@ -915,7 +941,7 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
       // static objects carry the camera's rows (kfusion.cpp:399-402, 450-454)
       for (Object* o : stat) {
         Timed t(c, KC_MOTION);
-        launch_copy_codes(s, bg->codes, o->codes, 2 * c->N0, &md->copy_static);
+        launch_copy_codes(s, bg->codes, o->codes, 2 * c->N0, bg->rgb_err, o->rgb_err, c->N0, &md->copy_static);
       }
     });
     if (r < 0) return r;
@@ -986,7 +1012,110 @@ int mf_raycast(mf_ctx* c, const float k[4], unsigned frame) {
     launch_raycast(s, c->rayobjs, n, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, c->vtx[0], c->vtx[1],
                    c->vtx[2], c->nrm[0], c->nrm[1], c->nrm[2], c->label, c->timing ? c->stats : nullptr,
                    c->md->in_view, nullptr);
+    launch_copy_mat(s, c->ray_pose, c->mats + 0);   // raycastPose (kfusion.cpp:554)
   }) < 0 ? -1 : 1;
+}
+
+// ---- display rendering (rendering/rendering.cpp, kfusion.cpp:1284-1475)
+
+bool ensure_render(mf_ctx* c, size_t n) {
+  if (n <= c->render_cap) return true;
+  c->render_out = dalloc<uint32_t>(c, n);
+  c->label_scratch = dalloc<int32_t>(c, n);
+  if (!c->render_out || !c->label_scratch) return false;
+  c->render_cap = n;
+  return true;
+}
+int download_image(mf_ctx* c, void* out, size_t bytes, const void* src) {
+  hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
+int mf_render_track(mf_ctx* c, int id, int mode, uint8_t* out) {
+  if (id < 0 || id >= (int)c->objs.size() || mode < 0 || mode > 2 || !out) return fail(c, MF_E_ARG, "bad render request");
+  if (!ensure_render(c, c->N0)) return fail(c, MF_E_OOM, "render buffer");
+  const Object* o = c->objs[id];
+  launch_render_track(c->stream, o->codes, o->rgb_err, (int)c->N0, mode, c->render_out);
+  return download_image(c, out, 4 * c->N0, c->render_out);
+}
+
+int mf_render_depth(mf_ctx* c, uint8_t* out) {
+  if (!out) return fail(c, MF_E_ARG, "no output");
+  if (!ensure_render(c, c->N0)) return fail(c, MF_E_OOM, "render buffer");
+  launch_render_depth(c->stream, c->depth, (int)c->N0, 0.1f, 4.0f, c->render_out);   // nearPlane, farPlane
+  return download_image(c, out, 4 * c->N0, c->render_out);
+}
+
+int mf_render_intensity(mf_ctx* c, uint8_t* out) {
+  if (!out) return fail(c, MF_E_ARG, "no output");
+  if (!ensure_render(c, c->N0)) return fail(c, MF_E_OOM, "render buffer");
+  launch_render_intensity(c->stream, c->grey, (int)c->N0, c->render_out);
+  return download_image(c, out, 4 * c->N0, c->render_out);
+}
+
+int mf_volume2mask(mf_ctx* c, const float k[4], int32_t* labels) {
+  if (!labels) return fail(c, MF_E_ARG, "no output");
+  if (!ensure_render(c, c->N0)) return fail(c, MF_E_OOM, "render buffer");
+  if (c->ray_eps < 0) c->ray_eps = exp2f(-(float)c->objs[0]->v.max_level);
+  upload_ray_tables(c);
+  int n_int, n_all;
+  long sum_nb3, sum_cap;
+  set_mapobjs(c, n_int, n_all, sum_nb3, sum_cap);
+  hipStream_t s = c->stream;
+  launch_obj_params(s, c->mats + 0, c->mats, c->mapobjs + MAX_INST, n_all, k[0], k[1], k[2], k[3], 0, c->work);
+  // every object, labels only, at the camera pose (rendering.cpp:240-343)
+  launch_raycast(s, c->rayobjs_lab, n_all, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, nullptr, nullptr, nullptr,
+                 nullptr, nullptr, nullptr, c->label_scratch, nullptr, nullptr, nullptr);
+  return download_image(c, labels, 4 * c->N0, c->label_scratch);
+}
+
+int mf_render_volume(mf_ctx* c, const float k[4], const float view_pose[16], int w, int h, float largestep,
+                     int render_color, const uint8_t* palette, int n_palette, uint8_t* out) {
+  if (!k || !view_pose || !out || w <= 0 || h <= 0 || n_palette < 0 || n_palette > 256 || (n_palette && !palette))
+    return fail(c, MF_E_ARG, "bad render request");
+  const size_t n = (size_t)w * h;
+  if (!ensure_render(c, n)) return fail(c, MF_E_OOM, "render buffer");
+  hipStream_t s = c->stream;
+  if (c->ray_eps < 0) c->ray_eps = exp2f(-(float)c->objs[0]->v.max_level);
+  // raycast again unless the view is the last raycast's pose at the
+  // computation size (kfusion.cpp:1305-1307)
+  Mat4 rp;
+  hipMemcpyAsync(&rp, c->ray_pose, sizeof rp, hipMemcpyDeviceToHost, s);
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  const Mat4 V = m4(view_pose);
+  bool same = true;
+  for (int i = 0; i < 16; i++) same = same && (V.m[i / 4][i % 4] == rp.m[i / 4][i % 4]);
+  const bool doray = !same || w != c->W || h != c->H;
+  upload_ray_tables(c);
+  int n_int, n_all;
+  long sum_nb3, sum_cap;
+  set_mapobjs(c, n_int, n_all, sum_nb3, sum_cap);
+  std::vector<uint32_t> pal(256, 0u);
+  for (int i = 0; i < n_palette; i++)
+    pal[i] = palette[4 * i] | (palette[4 * i + 1] << 8) | (palette[4 * i + 2] << 16) | ((uint32_t)palette[4 * i + 3] << 24);
+  hipMemcpyAsync(c->palette_dev, pal.data(), pal.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(c->view_dev, &V, sizeof V, hipMemcpyHostToDevice, s);
+  launch_obj_params(s, c->view_dev, c->mats, c->mapobjs + MAX_INST, n_all, k[0], k[1], k[2], k[3], 0, c->work);
+  if (doray)
+    launch_render_volume_ray(s, c->rayobjs_lab, n_all, w, h, c->view_dev, c->cfg.mu, 0.1f, 2.0f * 4.0f, largestep,
+                             c->ray_eps, render_color, c->palette_dev, n_palette, c->render_out);
+  else
+    launch_render_volume_maps(s, c->rayobjs_lab, n_all, (int)n, c->view_dev, c->vtx, c->nrm, c->label, render_color,
+                              c->palette_dev, n_palette, c->render_out);
+  const int r = download_image(c, out, 4 * n, c->render_out);
+  return r < 0 ? r : (doray ? 1 : 0);
+}
+
+int mf_get_track_errors(mf_ctx* c, int id, float* rgb_err) {
+  if (id < 0 || id >= (int)c->objs.size() || !rgb_err) return fail(c, MF_E_ARG, "bad object id");
+  return download_image(c, rgb_err, sizeof(float) * c->N0, c->objs[id]->rgb_err);
+}
+int mf_set_track_errors(mf_ctx* c, int id, const float* rgb_err) {
+  if (id < 0 || id >= (int)c->objs.size() || !rgb_err) return fail(c, MF_E_ARG, "bad object id");
+  hipMemcpyAsync(c->objs[id]->rgb_err, rgb_err, sizeof(float) * c->N0, hipMemcpyHostToDevice, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  return 0;
 }
 
 int mf_sync(mf_ctx* c) {
@@ -1291,7 +1420,7 @@ int mf_residual_eval(mf_ctx* c, int id, const float k[4], int level, const float
     // frame's start (T_c1_o1 etc.), without publishing a pose
     Object* o = c->objs[id];
     slot = state = MAX_INST - 1;
-    set_inst(c, slot, o->mv, o->mn, o->rD, nullptr, state);
+    set_inst(c, slot, o->mv, o->mn, o->rD, nullptr, nullptr, state);
     const int oid = 3 + id;
     hipMemcpyAsync(c->obj_ids + MAX_INST - 1, &oid, sizeof oid, hipMemcpyHostToDevice, s);
     hipStreamSynchronize(s);

@@ -77,6 +77,7 @@ struct TrackInst {
   const float* mvn;     // model vertex (planes 0-2, world frame) and normal (3-5) maps, full res
   const float* rD;      // rendered reference depth pyramid, levels back to back
   int8_t* codes;        // level-0 result codes out [2*W*H] (ICP, RGB) or null
+  float* rgb_err;       // level-0 photometric row errors out [W*H] or null
   int state;            // index into GNState array
   int pad;
 };

@@ -89,14 +89,20 @@ struct MotionLaunch {
   int use_icp, use_rgb;
   float sigma_bright, threshold;
   int8_t* codes;            // out: ICP codes then RGB codes [2*W*H]
+  float* rgb_err;           // out (optional): photometric row errors [W*H]
   const int* gate;          // run only if *gate != 0 (null: always)
   const int32_t* label_bf;  // optional: labels of the raycast before fusion, for
   int* counts;              //   per-label {pixels, inliers} counts [2*MAX_INST]
 };
 void launch_motion_rows(hipStream_t s, const MotionLaunch& m);
+// Display images (uchar4 as one 32-bit word per pixel).
+void launch_render_track(hipStream_t s, const int8_t* codes, const float* err, int n, int mode, uint32_t* out);
+void launch_render_depth(hipStream_t s, const float* depth, int n, float nearp, float farp, uint32_t* out);
+void launch_render_intensity(hipStream_t s, const float* grey, int n, uint32_t* out);
 // *dst = table[*index] (ground-truth camera pose of the frame)
 void launch_pose_from_table(hipStream_t s, Mat4* dst, const Mat4* table, const int* index);
 void launch_set_int(hipStream_t s, int* dst, int v);
+void launch_copy_mat(hipStream_t s, Mat4* dst, const Mat4* src);
 // Frame-start clear of the motion state and the presence of the last
 // raycast's labels, then the gate (and the in-view set cleared under it).
 void launch_motion_gate(hipStream_t s, MotionDev* md, const int32_t* label, int n, int nobj);
@@ -113,8 +119,9 @@ void launch_motion_after(hipStream_t s, MotionDev* md, const int32_t* label_bf, 
 // virtual_camera_pose_ of tracked objects: vcam[ids[j] - 3] for instances
 // [first, first + n), unless the run was skipped.
 void launch_virtual_cams(hipStream_t s, const GNState* st, int first, int n, const int* ids, Mat4* vcam);
-// dst <- src (2*n codes) when *gate != 0
-void launch_copy_codes(hipStream_t s, const int8_t* src, int8_t* dst, size_t n2, const int* gate);
+// dst <- src (2*n codes) and edst <- esrc (ne photometric errors) when *gate != 0
+void launch_copy_codes(hipStream_t s, const int8_t* src, int8_t* dst, size_t n2, const float* esrc, float* edst,
+                       size_t ne, const int* gate);
 // Frame start for the camera: T_w_r = T_w_c, inverse(T_w_r), GN state 0.
 void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy);
 // GN state of a camera pass starting from the current camera pose.
@@ -178,6 +185,14 @@ struct RayObj {
 // per-object maps are written where the object's pointers are set; with
 // nx == nullptr no normal is computed; with a gate, nothing runs unless
 // *gate != 0.
+// renderVolume_many_Kernel from the last raycast's maps, or by raycasting at
+// the view pose T_w_c (prm of every object computed for that view).
+void launch_render_volume_maps(hipStream_t s, const RayObj* objs, int nobj, int n, const Mat4* T_w_c,
+                               float* const v[3], float* const nrm[3], const int32_t* label, int render_color,
+                               const uint32_t* palette, int npal, uint32_t* out);
+void launch_render_volume_ray(hipStream_t s, const RayObj* objs, int nobj, int w, int h, const Mat4* T_w_c, float mu,
+                              float nearp, float farp, float largestep, float eps, int render_color,
+                              const uint32_t* palette, int npal, uint32_t* out);
 void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,
                     float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label,
                     unsigned long long* stats, const int* in_view, const int* gate);

@@ -335,7 +335,7 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
       out_n = (length(g) == 0) ? mk3((float)INVALID, 0.f, 0.f) : normalize(rotate(P.pose, g));
     }
   }
-  vx[pix] = out_v.x; vy[pix] = out_v.y; vz[pix] = out_v.z;
+  if (vx) { vx[pix] = out_v.x; vy[pix] = out_v.y; vz[pix] = out_v.z; }
   if (nx) { nx[pix] = out_n.x; ny[pix] = out_n.y; nz[pix] = out_n.z; }
   label[pix] = out_l;
   for (int oi = 0; oi < nobj; oi++) {
@@ -355,6 +355,148 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
     atomicAdd(&sl[4 * STAT_SLOTS], (unsigned long long)cnt[2]); atomicAdd(&sl[7 * STAT_SLOTS], (unsigned long long)cnt[3]);
     atomicAdd(&sl[8 * STAT_SLOTS], (unsigned long long)cnt[4]);
   }
+}
+
+// ---- renderVolume_many_Kernel (rendering/rendering.cpp:489-646)
+
+// Interpolated colour channel at `pos` (voxel units); missing voxels are
+// voxel_traits<SDF>::empty(), colour 0 (volume_traits.hpp:36).
+__device__ __forceinline__ float interp_col(const VolDev& v, VCache& c, f3 pos, const float* __restrict__ field) {
+  const float flx = floorf(pos.x), fly = floorf(pos.y), flz = floorf(pos.z);
+  const int bx = (int)flx, by = (int)fly, bz = (int)flz;
+  const float fx = pos.x - flx, fy = pos.y - fly, fz = pos.z - flz;
+  const int lx = max(bx, 0), ly = max(by, 0), lz = max(bz, 0);
+  float p[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int X = lx + (i & 1), Y = ly + ((i >> 1) & 1), Z = lz + ((i >> 2) & 1);
+    const int32_t s = cslot(v, c, X, Y, Z);
+    p[i] = s < 0 ? 0.f : field[vox(s, X, Y, Z)];
+  }
+  return (((p[0] * (1 - fx) + p[1] * fx) * (1 - fy) + (p[2] * (1 - fx) + p[3] * fx) * fy) * (1 - fz) +
+          ((p[4] * (1 - fx) + p[5] * fx) * (1 - fy) + (p[6] * (1 - fx) + p[7] * fx) * fy) * fz);
+}
+__device__ __forceinline__ float clamp01(float f) { return fmaxf(0.f, fminf(f, 1.f)); }
+__device__ __forceinline__ uint32_t col_u8(float v) { return (uint32_t)(int)v & 0xffu; }
+
+// Shading of one pixel from a world hit, its normal and label: Lambert term
+// towards the camera plus ambient, times the interpolated voxel colour (or the
+// label's palette colour, white for the background).
+__device__ uint32_t shade(const RayObj* __restrict__ objs, int nobj, f3 w_origin, f3 w_hit, f3 w_n, int w_label,
+                          int render_color, const uint32_t* __restrict__ palette, int npal) {
+  const float amb = 0.1f;   // ambient (constant_parameters.h:24)
+  f3 col = mk3(0.f, 0.f, 0.f);
+  for (int oi = 0; oi < nobj; oi++) {
+    const RayObj& O = objs[oi];
+    if (w_label != O.label) continue;
+    const VolDev& v = O.v;
+    const ObjParams& P = *O.prm;
+    const f3 o_hit = affine(P.inv_pose, w_hit);
+    const f3 diff = normalize(w_origin - w_hit);
+    const float dir = fmaxf(dot(normalize(w_n), diff), 0.f);
+    if (render_color) {
+      const f3 q = o_hit * ((float)v.res / v.size);
+      VCache c;
+      const float r = interp_col(v, c, q, v.r), g = interp_col(v, c, q, v.g), b = interp_col(v, c, q, v.b);
+      col = mk3(clamp01(dir * r + amb), clamp01(dir * g + amb), clamp01(dir * b + amb)) * 255.f;
+    } else if (w_label == 0) {
+      col = mk3(clamp01(dir + amb), clamp01(dir + amb), clamp01(dir + amb)) * 255.f;
+    } else {
+      const uint32_t pc = (w_label >= 0 && w_label < npal) ? palette[w_label] : 0u;
+      const f3 rgb = mk3((float)(pc & 0xffu) / 255, (float)((pc >> 8) & 0xffu) / 255, (float)((pc >> 16) & 0xffu) / 255);
+      col = mk3(clamp01(dir * rgb.x + amb), clamp01(dir * rgb.y + amb), clamp01(dir * rgb.z + amb)) * 255.f;
+    }
+  }
+  return col_u8(col.x) | (col_u8(col.y) << 8) | (col_u8(col.z) << 16);
+}
+
+// From the last raycast's maps (the view is the raycast pose, same size).
+__global__ void k_render_volume_maps(const RayObj* __restrict__ objs, int nobj, int n, const Mat4* __restrict__ T_w_c,
+                                     const float* __restrict__ vx, const float* __restrict__ vy,
+                                     const float* __restrict__ vz, const float* __restrict__ nx,
+                                     const float* __restrict__ ny, const float* __restrict__ nz,
+                                     const int32_t* __restrict__ label, int render_color,
+                                     const uint32_t* __restrict__ palette, int npal, uint32_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = shade(objs, nobj, translation(*T_w_c), mk3(vx[i], vy[i], vz[i]), mk3(nx[i], ny[i], nz[i]), label[i],
+                 render_color, palette, npal);
+}
+
+// Raycasting at another view or size: every object, the take rule of
+// renderVolume_many_Kernel (a later hit at the same distance needs a higher
+// foreground probability; a take needs hit.w > 0 and sets the label only
+// where the gradient is non-zero), then shading.
+__global__ __launch_bounds__(RT) void k_render_volume_ray(const RayObj* __restrict__ objs, int nobj, int w, int h,
+                                                          const Mat4* __restrict__ T_w_c, float mu, float nearp,
+                                                          float farp, float largestep, float eps, int render_color,
+                                                          const uint32_t* __restrict__ palette, int npal,
+                                                          uint32_t* __restrict__ out) {
+  __shared__ uint32_t st_par[MAX_LG + 1][RT];
+  __shared__ float st_tmax[MAX_LG + 1][RT];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int x = blockIdx.x * RT_TILE + (wv & 1) * 8 + (lane & 7);
+  const int y = blockIdx.y * RT_TILE + (wv >> 1) * 8 + (lane >> 3);
+  if (x >= w || y >= h) return;
+  const f3 w_origin = translation(*T_w_c);
+  float nearest = 1.0e10f;
+  float high_fg = 0.f;   // floats widened to double in the reference: float compares are exact
+  int miss = 0;
+  f3 w_hit = mk3(0.f, 0.f, 0.f), w_n = mk3(0.f, 0.f, 0.f);
+  int w_label = 0;
+  unsigned dummy = 0;
+  for (int oi = 0; oi < nobj; oi++) {
+    const RayObj& O = objs[oi];
+    const VolDev& v = O.v;
+    const ObjParams& P = *O.prm;
+    const float step = v.size / (float)v.res;
+    const f3 oo = affine(P.inv_pose, w_origin);
+    const f3 wd = rotate(P.view, mk3((float)x, (float)y, 1.f));
+    const f3 od = normalize(rotate(P.inv_pose, wd));
+    const float nfar = fminf(nearest, farp);
+    const float tmin = ray_first_block<false>(v, oo, od, nearp, nfar, eps, &st_par[0][threadIdx.x],
+                                              &st_tmax[0][threadIdx.x], dummy);
+    f3 hit = mk3(0.f, 0.f, 0.f);
+    float hw = 0.f;
+    if (tmin > 0.f) hw = march<false>(v, oo, od, tmin * v.size / (float)v.res, nfar, mu, step, largestep, hit, dummy, dummy);
+    const float hd = (O.label == 0) ? hw + 0.05f : hw;
+    if (hd > nearest) { miss++; continue; }
+    const float inv_vs = (float)v.res / v.size;
+    VCache c;
+    const float fg = interp<true>(v, c, hit * inv_vs);
+    if (fg <= 0.5f) { miss++; continue; }
+    if (hd == nearest) {
+      if (fg > high_fg) high_fg = fg;
+      else { miss++; continue; }
+    }
+    if (hw > 0.f) {
+      nearest = hd;
+      const f3 g = grad(v, c, hit * inv_vs);
+      if (length(g) > 0.f) {
+        w_n = rotate(P.pose, g);
+        w_hit = affine(P.pose, hit);
+        w_label = O.label;
+      }
+    } else {
+      miss++;
+    }
+  }
+  const size_t pix = (size_t)x + (size_t)y * w;
+  out[pix] = (miss == nobj) ? 0u : shade(objs, nobj, w_origin, w_hit, w_n, w_label, render_color, palette, npal);
+}
+
+void launch_render_volume_maps(hipStream_t s, const RayObj* objs, int nobj, int n, const Mat4* T_w_c,
+                               float* const v[3], float* const nrm[3], const int32_t* label, int render_color,
+                               const uint32_t* palette, int npal, uint32_t* out) {
+  k_render_volume_maps<<<(n + 255) / 256, 256, 0, s>>>(objs, nobj, n, T_w_c, v[0], v[1], v[2], nrm[0], nrm[1], nrm[2],
+                                                       label, render_color, palette, npal, out);
+}
+void launch_render_volume_ray(hipStream_t s, const RayObj* objs, int nobj, int w, int h, const Mat4* T_w_c, float mu,
+                              float nearp, float farp, float largestep, float eps, int render_color,
+                              const uint32_t* palette, int npal, uint32_t* out) {
+  dim3 blk(RT), grd((w + RT_TILE - 1) / RT_TILE, (h + RT_TILE - 1) / RT_TILE);
+  k_render_volume_ray<<<grd, blk, 0, s>>>(objs, nobj, w, h, T_w_c, mu, nearp, farp, largestep, eps, render_color,
+                                          palette, npal, out);
 }
 
 void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,

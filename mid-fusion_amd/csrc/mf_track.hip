@@ -273,6 +273,7 @@ __device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst
     accum(s, res, e, J, p.robust, p.robust_k);
     const size_t oo = icp ? ro : N0 + ro;
     if (in.codes && p.level == 0) in.codes[oo] = (int8_t)res;
+    if (!icp && in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
     if (dbg) { TrackRow& d = dbg[oo]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
   }
   // Wave butterfly: at each step a lane keeps half of its sums and adds the
@@ -779,14 +780,17 @@ __global__ void k_virtual_cams(const GNState* st, int first, int n, const int* i
   if (S.off) return;
   vcam[ids[j] - 3] = mul(S.T_wo1, inverse(S.T_final));
 }
-__global__ void k_copy_codes(const int8_t* __restrict__ src, int8_t* __restrict__ dst, size_t n, const int* gate) {
+__global__ void k_copy_codes(const int8_t* __restrict__ src, int8_t* __restrict__ dst, size_t n,
+                             const float* __restrict__ esrc, float* __restrict__ edst, size_t ne, const int* gate) {
   if (!*gate) return;
   const size_t n4 = n / 4;
   const int* s4 = reinterpret_cast<const int*>(src);
   int* d4 = reinterpret_cast<int*>(dst);
   const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  for (size_t i = t; i < n4; i += (size_t)gridDim.x * blockDim.x) d4[i] = s4[i];
+  const size_t st = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = t; i < n4; i += st) d4[i] = s4[i];
   if (t < n - 4 * n4) dst[4 * n4 + t] = src[4 * n4 + t];
+  for (size_t i = t; i < ne; i += st) edst[i] = esrc[i];
 }
 __global__ __launch_bounds__(256) void k_motion_rows(MotionLaunch m) {
   if (m.gate && *m.gate == 0) return;
@@ -804,6 +808,7 @@ __global__ __launch_bounds__(256) void k_motion_rows(MotionLaunch m) {
     const Mat4 K = camK(m.fx, m.fy, m.cx, m.cy);
     const float* L = m.live;
     int ri = 0, gi = 0;
+    float ge = 0.f;   // photometric row error (TrackData.error of a result-1 row)
     if (m.use_icp) {
       // compute_icp_residual_kernel (tracking.cpp:1617-1673)
       const f3 nl = mk3(L[(size_t)PL_N * n + i], L[(size_t)(PL_N + 1) * n + i], L[(size_t)(PL_N + 2) * n + i]);
@@ -855,11 +860,13 @@ __global__ __launch_bounds__(256) void k_motion_rows(MotionLaunch m) {
           const float residual = L[(size_t)PL_I * n + i] - bilin(m.ref + (size_t)PL_I * n, m.W, ux, uy);
           const double err = (1.0 / (double)m.sigma_bright) * (double)residual;
           gi = fabs(err) > (double)m.threshold ? -4 : 1;
+          ge = gi == 1 ? (float)err : 0.f;
         }
       }
     }
     m.codes[i] = (int8_t)ri;
     m.codes[n + i] = (int8_t)gi;
+    if (m.rgb_err) m.rgb_err[i] = ge;
     if (count) {
       const int l = m.label_bf[i];
       if (l > 0 && l < MAX_INST) {
@@ -924,6 +931,8 @@ void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNS
 }
 __global__ void k_pose_from_table(Mat4* dst, const Mat4* table, const int* index) { *dst = table[*index]; }
 __global__ void k_set_int(int* dst, int v) { *dst = v; }
+__global__ void k_copy_mat(Mat4* dst, const Mat4* src) { *dst = *src; }
+void launch_copy_mat(hipStream_t s, Mat4* dst, const Mat4* src) { k_copy_mat<<<1, 1, 0, s>>>(dst, src); }
 void launch_set_int(hipStream_t s, int* dst, int v) { k_set_int<<<1, 1, 0, s>>>(dst, v); }
 void launch_pose_from_table(hipStream_t s, Mat4* dst, const Mat4* table, const int* index) {
   k_pose_from_table<<<1, 1, 0, s>>>(dst, table, index);
@@ -955,8 +964,10 @@ void launch_motion_after(hipStream_t s, MotionDev* md, const int32_t* label_bf, 
 void launch_virtual_cams(hipStream_t s, const GNState* st, int first, int n, const int* ids, Mat4* vcam) {
   k_virtual_cams<<<1, 64, 0, s>>>(st, first, n, ids, vcam);
 }
-void launch_copy_codes(hipStream_t s, const int8_t* src, int8_t* dst, size_t n2, const int* gate) {
-  k_copy_codes<<<(unsigned)std::min<size_t>(1024, (n2 / 4 + 256) / 256), 256, 0, s>>>(src, dst, n2, gate);
+void launch_copy_codes(hipStream_t s, const int8_t* src, int8_t* dst, size_t n2, const float* esrc, float* edst,
+                       size_t ne, const int* gate) {
+  k_copy_codes<<<(unsigned)std::min<size_t>(1024, (n2 / 4 + 256) / 256), 256, 0, s>>>(src, dst, n2, esrc, edst, ne,
+                                                                                       gate);
 }
 void launch_camera_begin(hipStream_t s, Mat4* mats, GNState* st, float fx, float fy, float cx, float cy) {
   k_camera_begin<<<1, 1, 0, s>>>(mats, st, fx, fy, cx, cy);

@@ -59,6 +59,13 @@ SIGNATURES = {
     'mf_set_gt_poses': (C.c_int, [P, P, C.c_long]),
     'mf_save_poses': (C.c_int, [P, C.c_char_p, C.c_int]),
     'mf_format_tum': (C.c_int, [f32p, C.c_int, C.c_char_p, C.c_int]),
+    'mf_render_track': (C.c_int, [P, C.c_int, C.c_int, u8p]),
+    'mf_render_depth': (C.c_int, [P, u8p]),
+    'mf_render_intensity': (C.c_int, [P, u8p]),
+    'mf_volume2mask': (C.c_int, [P, f32p, i32p]),
+    'mf_render_volume': (C.c_int, [P, f32p, f32p, C.c_int, C.c_int, C.c_float, C.c_int, P, C.c_int, u8p]),
+    'mf_get_track_errors': (C.c_int, [P, C.c_int, f32p]),
+    'mf_set_track_errors': (C.c_int, [P, C.c_int, f32p]),
     'mf_object_set_mask': (C.c_int, [P, C.c_int, f32p]),
     'mf_object_set_mask_device': (C.c_int, [P, C.c_int, P]),
     'mf_preprocess': (C.c_int, [P, u16p, u8p, C.c_int, C.c_int, C.c_int]),
@@ -246,6 +253,44 @@ class Kfusion:
         f = lambda a: np.ascontiguousarray(a, np.float32).reshape(-1)
         self._chk(self.L.mf_motion_residuals(self.ctx, f(k), f(T), f(vtx), f(nrm), f(vbf), out))
         return out
+
+    def render_track(self, oid, mode):
+        out = np.zeros((self.h, self.w, 4), np.uint8)
+        self._chk(self.L.mf_render_track(self.ctx, oid, mode, out.reshape(-1)))
+        return out
+
+    def render_depth(self):
+        out = np.zeros((self.h, self.w, 4), np.uint8)
+        self._chk(self.L.mf_render_depth(self.ctx, out.reshape(-1)))
+        return out
+
+    def render_intensity(self):
+        out = np.zeros((self.h, self.w, 4), np.uint8)
+        self._chk(self.L.mf_render_intensity(self.ctx, out.reshape(-1)))
+        return out
+
+    def volume2mask(self, k):
+        out = np.zeros((self.h, self.w), np.int32)
+        self._chk(self.L.mf_volume2mask(self.ctx, _f32(k), out.reshape(-1)))
+        return out
+
+    def render_volume(self, k, view_pose, w=None, h=None, largestep=0.075, render_color=True, palette=None):
+        """(image [h, w, 4] uint8, raycast?) of renderVolume at view_pose."""
+        w, h = w or self.w, h or self.h
+        out = np.zeros((h, w, 4), np.uint8)
+        pal = np.zeros((0, 4), np.uint8) if palette is None else np.ascontiguousarray(palette, np.uint8)
+        r = self._chk(self.L.mf_render_volume(self.ctx, _f32(k), _f32(np.asarray(view_pose).reshape(16)), w, h,
+                                              largestep, int(render_color), pal.ctypes.data if len(pal) else None,
+                                              len(pal), out.reshape(-1)))
+        return out, bool(r)
+
+    def track_errors(self, oid):
+        out = np.zeros(self.w * self.h, np.float32)
+        self._chk(self.L.mf_get_track_errors(self.ctx, oid, out))
+        return out
+
+    def set_track_errors(self, oid, err):
+        self._chk(self.L.mf_set_track_errors(self.ctx, oid, _f32(err).reshape(-1)))
 
     def set_trackresult(self, oid, codes):
         self._chk(self.L.mf_set_trackresult(self.ctx, oid, np.ascontiguousarray(codes, np.int32).reshape(-1)))

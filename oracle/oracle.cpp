@@ -304,6 +304,8 @@ struct Pipe {
   std::vector<Object*> objs;
   std::vector<TraceRec> trace;
   float ray_eps = -1.f;   // ray_iterator's static epsilon: set by the first raycast volume
+  M4 ray_pose;            // raycastPose: camera pose of the last raycast (kfusion.cpp:554)
+  bool ray_done = false;
 
   ~Pipe() { for (Object* o : objs) { o->vol.release(); delete o; } }
 };
@@ -347,7 +349,7 @@ static void integrate_object(Pipe& p, Object& o, const std::vector<R>& mask, con
 // raycastObjectList (rendering.cpp:9-181), all objects, has_integrated selects
 // the per-object output maps.
 static void raycast_all(Pipe& p, const float k[4], bool has_integrated, V3* out_v, V3* out_n, int* out_l,
-                        const std::vector<char>* in_view) {
+                        const std::vector<char>* in_view, bool write_objects = true) {
   const int W = p.c.w, H = p.c.h;
   const M4 view = mul(p.camera_pose, camKinv(k[0], k[1], k[2], k[3]));
   const R nearp = R(0.1), farp = R(4.0);
@@ -398,6 +400,7 @@ static void raycast_all(Pipe& p, const float k[4], bool has_integrated, V3* out_
         else out_n[pix] = normalize(rotate(o.pose, g));
       }
       if (miss == p.objs.size()) { out_v[pix] = v3(0, 0, 0); out_n[pix] = v3(R(INVALID), 0, 0); out_l[pix] = INVALID; }
+      if (!write_objects) continue;
       for (Object* o : p.objs) {
         V3 ov = v3(0, 0, 0), on = v3(R(INVALID), 0, 0);
         if (out_l[pix] == o->label) { ov = out_v[pix]; on = out_n[pix]; }
@@ -709,6 +712,202 @@ int orc_track_objects(void* h, const float k[4], unsigned frame) {
   return 1;
 }
 
+// ------------------------------------------------------ display rendering
+// (rendering/rendering.cpp)
+
+// float -> unsigned char by way of a 32-bit integer, as the reference's x86
+// host conversion does it
+static inline uint8_t u8_of(float v) { return (uint8_t)((int)v & 0xff); }
+static inline void put4(uint8_t* o, size_t i, uint8_t r, uint8_t g, uint8_t b, uint8_t a) {
+  o[4 * i] = r; o[4 * i + 1] = g; o[4 * i + 2] = b; o[4 * i + 3] = a;
+}
+// renderTrackKernel / renderRGBTrackKernel / render_RGBD_TrackKernel
+// (rendering.cpp:395-487, 183-237) of an instance's trackresult_
+int orc_render_track(void* h, int id, int mode, uint8_t* out) {
+  Pipe* p = (Pipe*)h;
+  const std::vector<TrackRow>& t = p->objs[id]->trackresult;
+  const size_t n = (size_t)p->c.w * p->c.h;
+  for (size_t i = 0; i < n; i++) {
+    const TrackRow& a = t[i];
+    const TrackRow& b = t[n + i];
+    if (mode == 0) {
+      switch (a.result) {
+        case 1: put4(out, i, 128, 128, 128, 0); break;
+        case -1: put4(out, i, 0, 0, 0, 0); break;
+        case -2: put4(out, i, 255, 0, 0, 0); break;
+        case -3: put4(out, i, 0, 255, 0, 0); break;
+        case -4: put4(out, i, 0, 0, 255, 0); break;
+        case -5: put4(out, i, 255, 255, 0, 0); break;
+        case -6: put4(out, i, 255, 175, 175, 0); break;
+        default: put4(out, i, 255, 200, 128, 0); break;
+      }
+    } else if (mode == 1) {
+      switch (b.result) {
+        case 1: { const uint8_t e = u8_of(std::fabs((float)b.error * 255.0f)); put4(out, i, e, e, e, 255); break; }
+        case -1: put4(out, i, 0, 255, 0, 0); break;
+        case -2: put4(out, i, 0, 0, 0, 0); break;
+        case -3: put4(out, i, 255, 0, 0, 0); break;
+        case -4: put4(out, i, 0, 0, 255, 0); break;
+        case -5: put4(out, i, 255, 255, 0, 0); break;
+        default: put4(out, i, 255, 128, 128, 0); break;
+      }
+    } else {
+      const int ri = a.result, gi = b.result;
+      if (ri == 1 && gi == 1) { const uint8_t e = u8_of(std::fabs((float)b.error * 255.0f)); put4(out, i, e, e, e, 255); }
+      else if ((ri < -3 && ri > -6) || (gi < -3 && gi > -6)) put4(out, i, 255, 0, 0, 0);
+      else if (ri == -2 || gi == -2) put4(out, i, 255, 255, 0, 0);
+      else if (ri == -3 || gi == -3) put4(out, i, 255, 255, 0, 0);
+      else if (ri == -1 || gi == -1) put4(out, i, 0, 255, 0, 0);
+      else if (ri == -6 || gi == -6) put4(out, i, 255, 175, 175, 0);
+      else put4(out, i, 128, 128, 128, 0);
+    }
+  }
+  return 0;
+}
+// gs2rgb (commons.h:87-147)
+static void gs2rgb(double hh, uint8_t* o, size_t i) {
+  const double v = 0.75, m = 0.25, sv = 0.6667;
+  double r = 0, g = 0, b = 0;
+  hh *= 6.0;
+  const int sextant = (int)hh;
+  const double fract = hh - sextant;
+  const double vsf = v * sv * fract, mid1 = m + vsf, mid2 = v - vsf;
+  switch (sextant) {
+    case 0: r = v; g = mid1; b = m; break;
+    case 1: r = mid2; g = v; b = m; break;
+    case 2: r = m; g = v; b = mid1; break;
+    case 3: r = m; g = mid2; b = v; break;
+    case 4: r = mid1; g = m; b = v; break;
+    case 5: r = v; g = m; b = mid2; break;
+    default: r = g = b = 0; break;
+  }
+  put4(o, i, (uint8_t)(r * 255), (uint8_t)(g * 255), (uint8_t)(b * 255), 0);
+}
+// renderDepthKernel (rendering.cpp:368-394), nearPlane 0.1, farPlane 4
+int orc_render_depth(void* h, uint8_t* out) {
+  Pipe* p = (Pipe*)h;
+  const float nearp = 0.1f, farp = 4.0f, rangeScale = 1 / (farp - nearp);
+  for (size_t i = 0; i < p->depth.size(); i++) {
+    const float d = (float)p->depth[i];
+    if (d < nearp) put4(out, i, 255, 255, 255, 0);
+    else if (d > farp) put4(out, i, 0, 0, 0, 0);
+    else gs2rgb((double)((d - nearp) * rangeScale), out, i);
+  }
+  return 0;
+}
+// renderIntensityKernel (rendering.cpp:648-664)
+int orc_render_intensity(void* h, uint8_t* out) {
+  Pipe* p = (Pipe*)h;
+  for (size_t i = 0; i < p->grey.size(); i++) {
+    const uint8_t g = u8_of(255.f * (float)p->grey[i]);
+    put4(out, i, g, g, g, 255);
+  }
+  return 0;
+}
+// volume2MaskKernel (rendering.cpp:240-343): every object, labels only
+int orc_volume2mask(void* h, const float k[4], int* labels) {
+  Pipe* p = (Pipe*)h;
+  const size_t n = (size_t)p->c.w * p->c.h;
+  std::vector<V3> v(n), nn(n);
+  // the take rule and far plane are those of raycastObjectList with every
+  // object in view (its initial nearest hit of 1e10 instead of 1e3 changes
+  // nothing: both exceed every hit distance and the far plane); only the
+  // labels are kept
+  raycast_all(*p, k, true, v.data(), nn.data(), labels, nullptr, false);
+  return 0;
+}
+// Shading of renderVolume_many_Kernel (rendering.cpp:613-641)
+static void shade(const Pipe& p, V3 w_origin, V3 w_hit, V3 w_n, int w_label, int render_color, const uint8_t* pal,
+                  int npal, uint8_t* out, size_t pix) {
+  const R amb = (R)0.1f;
+  V3 col = v3(0, 0, 0);
+  for (const Object* o : p.objs) {
+    if (w_label != o->label) continue;
+    const V3 o_hit = affine(inverse(o->pose), w_hit);
+    const V3 diff = normalize(w_origin - w_hit);
+    const R dir = std::fmax(dot(normalize(w_n), diff), R(0));
+    auto cl = [](R f) { return std::fmax(R(0), std::fmin(f, R(1))); };
+    if (render_color) {
+      const V3 q = o_hit * ((R)o->res / o->size);
+      const R r = interp(o->vol, q, F_R), g = interp(o->vol, q, F_G), b = interp(o->vol, q, F_B);
+      col = v3(cl(dir * r + amb), cl(dir * g + amb), cl(dir * b + amb)) * R(255);
+    } else if (w_label == 0) {
+      col = v3(cl(dir + amb), cl(dir + amb), cl(dir + amb)) * R(255);
+    } else {
+      const uint8_t* c = (w_label >= 0 && w_label < npal) ? pal + 4 * w_label : nullptr;
+      const V3 rgb = c ? v3((R)c[0] / 255, (R)c[1] / 255, (R)c[2] / 255) : v3(0, 0, 0);
+      col = v3(cl(dir * rgb.x + amb), cl(dir * rgb.y + amb), cl(dir * rgb.z + amb)) * R(255);
+    }
+  }
+  put4(out, pix, u8_of((float)col.x), u8_of((float)col.y), u8_of((float)col.z), 0);
+}
+// Kfusion::renderVolume (kfusion.cpp:1284-1328) / renderVolume_many_Kernel
+// (rendering.cpp:489-646).  Returns 1 if it raycast, 0 if it shaded the last
+// raycast's maps.
+int orc_render_volume(void* h, const float k[4], const float view16[16], int w, int hh, float largestep,
+                      int render_color, const uint8_t* pal, int npal, uint8_t* out) {
+  Pipe* p = (Pipe*)h;
+  const M4 T = m4_from(view16);
+  bool same = p->ray_done;
+  for (int i = 0; i < 16 && same; i++) same = T.m[i / 4][i % 4] == p->ray_pose.m[i / 4][i % 4];
+  const bool doray = !same || w != p->c.w || hh != p->c.h;
+  const V3 w_origin = translation(T);
+  if (!doray) {
+    for (size_t i = 0; i < (size_t)w * hh; i++)
+      shade(*p, w_origin, p->vertex[i], p->normal[i], p->label[i], render_color, pal, npal, out, i);
+    return 0;
+  }
+  const M4 view = mul(T, camKinv(k[0], k[1], k[2], k[3]));
+  const R nearp = R(0.1), farp = R(8.0), mu = (R)p->c.mu;
+  if (p->ray_eps < 0 && !p->objs.empty()) {
+    int lg = 0; while ((1 << lg) < p->objs[0]->res) lg++;
+    p->ray_eps = std::exp2f(-(float)lg);
+  }
+  std::vector<M4> inv_pose(p->objs.size());
+  for (size_t i = 0; i < p->objs.size(); i++) inv_pose[i] = inverse(p->objs[i]->pose);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int y = 0; y < hh; y++)
+    for (int x = 0; x < w; x++) {
+      const size_t pix = x + (size_t)y * w;
+      R nearest = R(1.0e10);
+      double high_fg = 0;
+      size_t miss = 0;
+      V3 w_hit = v3(0, 0, 0), w_n = v3(0, 0, 0);
+      int w_label = 0;
+      for (size_t oi = 0; oi < p->objs.size(); oi++) {
+        const Object& o = *p->objs[oi];
+        const R step = o.size / (R)o.res;
+        const V3 oo = affine(inv_pose[oi], w_origin);
+        const V3 wd = rotate(view, v3(R(x), R(y), R(1)));
+        const V3 od = normalize(rotate(inv_pose[oi], wd));
+        const R nfar = fmin_r(nearest, farp);
+        const float of[3] = {(float)oo.x, (float)oo.y, (float)oo.z}, df[3] = {(float)od.x, (float)od.y, (float)od.z};
+        RayIter it(o.vol, of, df, (float)nearp, (float)nfar, p->ray_eps);
+        const R tmin = (R)it.first();
+        V3 hit = v3(0, 0, 0); R hw = 0;
+        if (tmin > 0) raycast_ray(o.vol, oo, od, tmin * o.size / (R)o.res, nfar, mu, step, (R)largestep, hit, hw);
+        const R hd = (o.label == 0) ? hw + R(0.05) : hw;
+        if (hd > nearest) { miss++; continue; }
+        const double fg = (double)interp(o.vol, hit * ((R)o.res / o.size), F_FG);
+        if (fg <= 0.5f) { miss++; continue; }
+        if (hd == nearest) {
+          if (fg > high_fg) high_fg = fg;
+          else { miss++; continue; }
+        }
+        if (hw > 0) {
+          nearest = hd;
+          const V3 g = grad(o.vol, hit * ((R)o.res / o.size));
+          if (length(g) > 0) { w_n = rotate(o.pose, g); w_hit = affine(o.pose, hit); w_label = o.label; }
+        } else {
+          miss++;
+        }
+      }
+      if (miss == p->objs.size()) { put4(out, pix, 0, 0, 0, 0); continue; }
+      shade(*p, w_origin, w_hit, w_n, w_label, render_color, pal, npal, out, pix);
+    }
+  return 1;
+}
+
 // Ground-truth camera poses (row-major 4x4 each); n = 0 turns the mode off.
 // The translation of poses[f] is moved by (initial position - poses[0]
 // translation), as kfusion.cpp:263-266 does, in R arithmetic.
@@ -803,6 +1002,8 @@ int orc_raycast(void* h, const float k[4], unsigned frame) {
   (void)frame;
   Pipe* p = (Pipe*)h;
   raycast_all(*p, k, true, p->vertex.data(), p->normal.data(), p->label.data(), &p->in_view);
+  p->ray_pose = p->camera_pose;
+  p->ray_done = true;
   return 1;
 }
 

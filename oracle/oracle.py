@@ -58,6 +58,11 @@ def _lib(fp64):
         'orc_motion_rows': (C.c_int, [P, i32p, f32p]),
         'orc_motion_state': (C.c_int, [P, i32p, i32p, i32p, C.c_int]),
         'orc_set_gt_poses': (None, [P, f32p, C.c_long]),
+        'orc_render_track': (C.c_int, [P, C.c_int, C.c_int, u8p]),
+        'orc_render_depth': (C.c_int, [P, u8p]),
+        'orc_render_intensity': (C.c_int, [P, u8p]),
+        'orc_volume2mask': (C.c_int, [P, f32p, i32p]),
+        'orc_render_volume': (C.c_int, [P, f32p, f32p, C.c_int, C.c_int, C.c_float, C.c_int, u8p, C.c_int, u8p]),
         'orc_motion_residuals': (C.c_int, [P, f32p, f32p, f32p, f32p, f32p, i32p]),
         'orc_get_maps_bf': (None, [P, f32p, f32p, i32p]),
         'orc_object_vertex_bef': (None, [P, C.c_int, f32p]),
@@ -202,6 +207,35 @@ class Oracle:
         px, inl, mv = np.zeros(cap, np.int64), np.zeros(cap, np.int64), np.zeros(cap, np.int32)
         n = self.L.orc_motion_result(self.h_, px, inl, mv, cap)
         return px[:n], inl[:n], mv[:n]
+
+    def render_track(self, oid, mode):
+        out = np.zeros((self.h, self.w, 4), np.uint8)
+        self.L.orc_render_track(self.h_, oid, mode, out.reshape(-1))
+        return out
+
+    def render_depth(self):
+        out = np.zeros((self.h, self.w, 4), np.uint8)
+        self.L.orc_render_depth(self.h_, out.reshape(-1))
+        return out
+
+    def render_intensity(self):
+        out = np.zeros((self.h, self.w, 4), np.uint8)
+        self.L.orc_render_intensity(self.h_, out.reshape(-1))
+        return out
+
+    def volume2mask(self, k):
+        out = np.zeros((self.h, self.w), np.int32)
+        self.L.orc_volume2mask(self.h_, f32(k), out.reshape(-1))
+        return out
+
+    def render_volume(self, k, view_pose, w=None, h=None, largestep=0.075, render_color=True, palette=None):
+        w, h = w or self.w, h or self.h
+        out = np.zeros((h, w, 4), np.uint8)
+        pal = np.zeros((1, 4), np.uint8) if palette is None else np.ascontiguousarray(palette, np.uint8)
+        r = self.L.orc_render_volume(self.h_, f32(k), f32(np.asarray(view_pose).reshape(16)), w, h, largestep,
+                                     int(render_color), pal.reshape(-1), 0 if palette is None else len(pal),
+                                     out.reshape(-1))
+        return out, bool(r)
 
     def set_gt_poses(self, poses):
         """Ground-truth camera poses (n x 4 x 4); an empty array turns the mode off."""

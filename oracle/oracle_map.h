@@ -186,8 +186,16 @@ struct Octree {
 
 // ---- trilinear interpolation (octree.hpp:479-501, interp_gather.hpp:107-237).
 // A missing block reads as tsdf 1, fg 0 in every gather case.
-enum Field { F_TSDF = 0, F_FG = 1 };
-static inline R sel(const Voxel& v, int f) { return f == F_TSDF ? v.x : (R)v.fg; }
+enum Field { F_TSDF = 0, F_FG = 1, F_R = 2, F_G = 3, F_B = 4 };
+static inline R sel(const Voxel& v, int f) {
+  switch (f) {
+    case F_TSDF: return v.x;
+    case F_FG: return (R)v.fg;
+    case F_R: return v.r;
+    case F_G: return v.g;
+    default: return v.b;
+  }
+}
 static inline R interp(const Octree& o, V3 pos, int f) {
   const int bx = (int)std::floor(pos.x), by = (int)std::floor(pos.y), bz = (int)std::floor(pos.z);
   const R fx = pos.x - std::floor(pos.x), fy = pos.y - std::floor(pos.y), fz = pos.z - std::floor(pos.z);

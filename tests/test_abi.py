@@ -13,7 +13,7 @@ HEADER = os.path.join(ROOT, 'include', 'midfusion.h')
 def declared():
     txt = open(HEADER).read()
     txt = re.sub(r'/\*.*?\*/', '', txt, flags=re.S)
-    return sorted(set(re.findall(r'\b(mf_[a-z_]+)\s*\(', txt)))
+    return sorted(set(re.findall(r'\b(mf_[a-z0-9_]+)\s*\(', txt)))
 
 
 def test_header_declares_entry_points():
