@@ -119,7 +119,8 @@ def main():
     ap.add_argument('--plain', action='store_true', help='one object per launch instead of batched launches')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline leg')
     ap.add_argument('--render', action='store_true',
-                    help='also time the display renderers and volume2mask (ms per call, image on the host)')
+                    help='also time the display renderers, volume2mask, the background mesh and the '
+                         'instance-mask generalisation (ms per call, results on the host)')
     ap.add_argument('--raw-stream', action='store_true',
                     help='also time the same frames read from a .raw sequence file through pinned staging '
                          '(disk + PCIe inclusive; never the headline value)')
@@ -265,7 +266,9 @@ def main():
                  'render_volume_raycast': lambda: g.render_volume(K, Pv, largestep=0.075),
                  'volume2mask': lambda: g.volume2mask(K),
                  'render_track_rgbd': lambda: g.render_track(0, 2),
-                 'render_depth': lambda: g.render_depth()}
+                 'render_depth': lambda: g.render_depth(),
+                 'object_mesh_bg': lambda: g.mesh(0),
+                 'instance_mask': lambda: g.set_instance_mask(1, (frames[0][2][1] > 0).astype(np.uint8) * 255)}
         render = {}
         for name, fn in calls.items():
             fn()

@@ -91,6 +91,17 @@ int mf_motion_residuals(mf_ctx* ctx, const float k[4], const float T_w_l[16], co
  * output, segmentation.cpp:10-47): fg value >= 0 inside, -1 outside.  NULL
  * clears it (the instance is then not integrated). */
 int mf_object_set_mask(mf_ctx* ctx, int id, const float* mask);
+/* instance_seg::generalize_label (segmentation.cpp:10-47) on the device: from
+ * a W*H instance mask (0 / non-zero, as Mask R-CNN gives it) the integration
+ * mask: the foreground value on the instance (1; for a mask rendered from
+ * the volumes, 0.1 if it covers under 3% of the image, else 0.5), 0 on the
+ * ring of its 5x5 elliptic dilation (and on the optional model mask, OR-ed
+ * into the dilation), -1 elsewhere. */
+int mf_object_set_instance_mask(mf_ctx* ctx, int id, const uint8_t* mask, int rendered, const uint8_t* model_mask);
+int mf_object_set_instance_mask_device(mf_ctx* ctx, int id, const uint8_t* d_mask, int rendered,
+                                       const uint8_t* d_model_mask);
+/* The instance's current integration mask (W*H floats). */
+int mf_object_get_mask(mf_ctx* ctx, int id, float* mask);
 int mf_object_set_mask_device(mf_ctx* ctx, int id, const float* d_mask);
 
 /* Kfusion::preprocessing(depth, rgb, inSize, filter) (kfusion.cpp:210-235):
@@ -288,6 +299,19 @@ int mf_render_volume(mf_ctx* ctx, const float k[4], const float view_pose[16], i
  * on) of instance id: read, or overwrite (e.g. to render rows computed
  * elsewhere). */
 int mf_get_track_errors(mf_ctx* ctx, int id, float* rgb_err);
+/* algorithms::marching_cube (lib/algorithms/meshing.hpp:157-208) of object
+ * id's volume, as Kfusion::dump_mesh calls it (tsdf field, inside = tsdf < 0;
+ * kfusion.cpp:1555-1588): triangles as 9 floats (object frame, metres), up
+ * to cap written; returns the number of triangles.  Cubes are meshed only
+ * where all 8 corners are observed foreground (weight != 0, fg > 0.5, more
+ * than 3 witnesses); triangles with a vertex at or below 0 or beyond the
+ * volume size are dropped.  The triangle table is this
+ * library's own: every cube yields the reference's edge vertices, but its
+ * polygons may be split into other triangles. */
+long mf_object_mesh(mf_ctx* ctx, int id, float* triangles, long cap);
+/* Kfusion::dump_mesh: "<prefix>_<id>.vtk" per object in the writeVtkMesh
+ * format (commons.h:945-1011). */
+int mf_dump_mesh(mf_ctx* ctx, const char* prefix);
 int mf_set_track_errors(mf_ctx* ctx, int id, const float* rgb_err);
 
 #ifdef __cplusplus

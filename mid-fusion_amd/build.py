@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 OUT = os.path.join(HERE, 'libmidfusion.so')
-SOURCES = ['mf_pre.hip', 'mf_track.hip', 'mf_map.hip', 'mf_raycast.hip', 'mf_api.hip', 'mf_io.hip', 'mf_render.hip']
+SOURCES = ['mf_pre.hip', 'mf_track.hip', 'mf_map.hip', 'mf_raycast.hip', 'mf_api.hip', 'mf_io.hip', 'mf_render.hip', 'mf_mesh.hip']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['-O3', '-std=c++17', '-fPIC', '--offload-arch=gfx950', '-ffp-contract=off', '-fno-fast-math',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-Wno-unused-result']

@@ -122,6 +122,9 @@ struct mf_ctx {
   size_t render_cap = 0;
   RayObj* rayobjs_lab = nullptr;      // raycast table writing no per-object maps
   RayObj ray_lab_host[MAX_INST];
+  unsigned long long* mesh_total = nullptr;   // marching-cubes triangle counter
+  uint8_t* mask_stage = nullptr;      // instance / model masks from the host
+  int* mask_count = nullptr;          // rendered-mask pixel count
   float* eval_maps = nullptr;         // mf_motion_residuals inputs (9 planes), allocated on first use
   Mat4* eval_T = nullptr;
   int8_t* eval_codes = nullptr;
@@ -608,6 +611,43 @@ int mf_object_set_mask(mf_ctx* c, int id, const float* mask) {
   o->has_mask = true;
   return 0;
 }
+// instance_seg::generalize_label on the device (segmentation.cpp:10-47)
+int mf_object_set_instance_mask_device(mf_ctx* c, int id, const uint8_t* d_mask, int rendered,
+                                       const uint8_t* d_model_mask) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  Object* o = c->objs[id];
+  if (!d_mask) { o->has_mask = false; return 0; }
+  if (!c->mask_count) {
+    c->mask_count = dalloc<int>(c, 1);
+    if (!c->mask_count) return fail(c, MF_E_OOM, "mask scratch");
+  }
+  launch_generalize(c->stream, d_mask, d_model_mask, c->W, c->H, rendered ? 1 : 0, c->mask_count, o->mask);
+  o->has_mask = true;
+  return 0;
+}
+int mf_object_set_instance_mask(mf_ctx* c, int id, const uint8_t* mask, int rendered, const uint8_t* model_mask) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  if (!mask) { c->objs[id]->has_mask = false; return 0; }
+  if (!c->mask_stage) {
+    c->mask_stage = dalloc<uint8_t>(c, 2 * c->N0);
+    if (!c->mask_stage) return fail(c, MF_E_OOM, "mask staging");
+  }
+  hipMemcpyAsync(c->mask_stage, mask, c->N0, hipMemcpyHostToDevice, c->stream);
+  if (model_mask) hipMemcpyAsync(c->mask_stage + c->N0, model_mask, c->N0, hipMemcpyHostToDevice, c->stream);
+  const int r = mf_object_set_instance_mask_device(c, id, c->mask_stage, rendered,
+                                                   model_mask ? c->mask_stage + c->N0 : nullptr);
+  // the staging buffer is reused by the next call: wait for this one
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  return r;
+}
+
+int mf_object_get_mask(mf_ctx* c, int id, float* mask) {
+  if (id < 0 || id >= (int)c->objs.size() || !mask) return fail(c, MF_E_ARG, "bad object id");
+  hipMemcpyAsync(mask, c->objs[id]->mask, c->N0 * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
 int mf_object_set_mask_device(mf_ctx* c, int id, const float* d_mask) {
   if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
   Object* o = c->objs[id];
@@ -1014,6 +1054,60 @@ int mf_raycast(mf_ctx* c, const float k[4], unsigned frame) {
                    c->md->in_view, nullptr);
     launch_copy_mat(s, c->ray_pose, c->mats + 0);   // raycastPose (kfusion.cpp:554)
   }) < 0 ? -1 : 1;
+}
+
+// ---- meshing (algorithms::marching_cube, meshing.hpp:157-208; Kfusion::
+// dump_mesh, kfusion.cpp:1555-1588)
+
+long mf_object_mesh(mf_ctx* c, int id, float* tri, long cap) {
+  if (id < 0 || id >= (int)c->objs.size() || cap < 0 || (cap && !tri)) return fail(c, MF_E_ARG, "bad mesh request");
+  hipStream_t s = c->stream;
+  const VolDev& v = c->objs[id]->v;
+  int cnt[4];
+  hipMemcpyAsync(cnt, v.counters, sizeof cnt, hipMemcpyDeviceToHost, s);
+  if (!c->mesh_total) c->mesh_total = dalloc<unsigned long long>(c, 1);
+  if (!c->mesh_total) return fail(c, MF_E_OOM, "mesh counter");
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  const int nblocks = (int)std::min<long>(cnt[0], v.cap);
+  unsigned long long total = 0;
+  hipMemsetAsync(c->mesh_total, 0, sizeof(unsigned long long), s);
+  launch_mesh(s, v, nblocks, nullptr, 0, c->mesh_total);
+  hipMemcpyAsync(&total, c->mesh_total, sizeof total, hipMemcpyDeviceToHost, s);
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  const long want = std::min<long>((long)total, cap);
+  if (want > 0) {
+    float* d = nullptr;
+    if (hipMalloc((void**)&d, (size_t)want * 9 * sizeof(float)) != hipSuccess) return fail(c, MF_E_OOM, "mesh buffer");
+    hipMemsetAsync(c->mesh_total, 0, sizeof(unsigned long long), s);
+    launch_mesh(s, v, nblocks, d, want, c->mesh_total);
+    hipMemcpyAsync(tri, d, (size_t)want * 9 * sizeof(float), hipMemcpyDeviceToHost, s);
+    const hipError_t e = hipStreamSynchronize(s);
+    hipFree(d);
+    if (e != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(e));
+  }
+  return (long)total;
+}
+
+int mf_dump_mesh(mf_ctx* c, const char* prefix) {
+  if (!prefix) return fail(c, MF_E_ARG, "no file prefix");
+  for (Object* o : c->objs) {
+    const long n = mf_object_mesh(c, o->id, nullptr, 0);
+    if (n < 0) return (int)n;
+    std::vector<float> t((size_t)std::max(n, 1L) * 9);
+    if (n > 0 && mf_object_mesh(c, o->id, t.data(), n) < 0) return -1;
+    // writeVtkMesh (commons.h:945-1011), object frame, metres
+    const std::string path = std::string(prefix) + "_" + std::to_string(o->id) + ".vtk";
+    FILE* f = fopen(path.c_str(), "w");
+    if (!f) return fail(c, MF_E_ARG, "cannot open mesh file");
+    fprintf(f, "# vtk DataFile Version 1.0\nvtk mesh generated from KFusion\nASCII\nDATASET POLYDATA\n");
+    fprintf(f, "POINTS %ld FLOAT\n", 3 * n);
+    for (long i = 0; i < 3 * n; i++) fprintf(f, "%g %g %g\n", t[3 * i], t[3 * i + 1], t[3 * i + 2]);
+    fprintf(f, "POLYGONS %ld %ld\n", n, 4 * n);
+    for (long i = 0; i < n; i++) fprintf(f, "3 %ld %ld %ld\n", 3 * i, 3 * i + 1, 3 * i + 2);
+    fprintf(f, "\n");
+    fclose(f);
+  }
+  return 0;
 }
 
 // ---- display rendering (rendering/rendering.cpp, kfusion.cpp:1284-1475)

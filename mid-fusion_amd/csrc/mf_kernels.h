@@ -99,6 +99,14 @@ void launch_motion_rows(hipStream_t s, const MotionLaunch& m);
 void launch_render_track(hipStream_t s, const int8_t* codes, const float* err, int n, int mode, uint32_t* out);
 void launch_render_depth(hipStream_t s, const float* depth, int n, float nearp, float farp, uint32_t* out);
 void launch_render_intensity(hipStream_t s, const float* grey, int n, uint32_t* out);
+// Marching cubes over the first `nblocks` pool slots of a volume: counts
+// triangles into *total and, with `out`, writes up to `cap` of them.
+void launch_mesh(hipStream_t s, const VolDev& v, int nblocks, float* out, long cap, unsigned long long* total);
+// generalize_label of a w*h 0/non-zero instance mask (global: optional model
+// mask) into an integration mask; `count` is device scratch for the
+// rendered-mask size.
+void launch_generalize(hipStream_t s, const uint8_t* mask, const uint8_t* global, int w, int h, int rendered,
+                       int* count, float* out);
 // *dst = table[*index] (ground-truth camera pose of the frame)
 void launch_pose_from_table(hipStream_t s, Mat4* dst, const Mat4* table, const int* index);
 void launch_set_int(hipStream_t s, int* dst, int v);

@@ -97,6 +97,48 @@ __global__ void k_render_intensity(const float* __restrict__ grey, int n, uint32
   out[i] = rgba(gs, gs, gs, 255);
 }
 
+// ---- instance_seg::generalize_label (segmentation/segmentation.cpp:10-47)
+// on 0/non-zero instance masks: the instance dilated by the 5x5 elliptic
+// element (cv::getStructuringElement(MORPH_ELLIPSE, 5x5): rows 0 and 4 hold
+// the centre column only; pixels outside the image do not take part), OR the
+// model mask if one is given; -1 outside that, 0 on it but off the instance
+// (the background layer), the foreground value on the instance.
+__global__ void k_mask_count(const uint8_t* __restrict__ mask, int n, int* __restrict__ count) {
+  int c = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c += mask[i] != 0;
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+__global__ void k_generalize(const uint8_t* __restrict__ mask, const uint8_t* __restrict__ global, int w, int h,
+                             int rendered, const int* __restrict__ count, float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w || y >= h) return;
+  float fg = 1.0f;
+  if (rendered)   // render_mask_size_ratio_ 0.03, large 0.5, small 0.1 (segmentation.h:110-113)
+    fg = ((float)*count < 0.03f * (float)w * (float)h) ? 0.1f : 0.5f;
+  bool dil = false;
+  for (int dy = -2; dy <= 2 && !dil; dy++) {
+    const int yy = y + dy;
+    if (yy < 0 || yy >= h) continue;
+    const int r = (dy == -2 || dy == 2) ? 0 : 2;
+    for (int dx = -r; dx <= r; dx++) {
+      const int xx = x + dx;
+      if (xx >= 0 && xx < w && mask[(size_t)yy * w + xx]) { dil = true; break; }
+    }
+  }
+  const size_t i = (size_t)y * w + x;
+  if (global && global[i]) dil = true;
+  out[i] = !dil ? -1.0f : (mask[i] ? fg : 0.0f);
+}
+void launch_generalize(hipStream_t s, const uint8_t* mask, const uint8_t* global, int w, int h, int rendered,
+                       int* count, float* out) {
+  if (rendered) {
+    launch_set_int(s, count, 0);
+    k_mask_count<<<256, 256, 0, s>>>(mask, w * h, count);
+  }
+  k_generalize<<<dim3((w + 255) / 256, h), 256, 0, s>>>(mask, global, w, h, rendered, count, out);
+}
+
 void launch_render_track(hipStream_t s, const int8_t* codes, const float* err, int n, int mode, uint32_t* out) {
   k_render_track<<<(n + 255) / 256, 256, 0, s>>>(codes, err, n, mode, out);
 }

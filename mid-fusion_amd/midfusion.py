@@ -68,6 +68,11 @@ SIGNATURES = {
     'mf_set_track_errors': (C.c_int, [P, C.c_int, f32p]),
     'mf_object_set_mask': (C.c_int, [P, C.c_int, f32p]),
     'mf_object_set_mask_device': (C.c_int, [P, C.c_int, P]),
+    'mf_object_set_instance_mask': (C.c_int, [P, C.c_int, u8p, C.c_int, P]),
+    'mf_object_set_instance_mask_device': (C.c_int, [P, C.c_int, P, C.c_int, P]),
+    'mf_object_get_mask': (C.c_int, [P, C.c_int, f32p]),
+    'mf_object_mesh': (C.c_long, [P, C.c_int, P, C.c_long]),
+    'mf_dump_mesh': (C.c_int, [P, C.c_char_p]),
     'mf_preprocess': (C.c_int, [P, u16p, u8p, C.c_int, C.c_int, C.c_int]),
     'mf_preprocess_device': (C.c_int, [P, P, P, C.c_int, C.c_int, C.c_int]),
     'mf_track': (C.c_int, [P, f32p, C.c_uint]),
@@ -294,6 +299,29 @@ class Kfusion:
 
     def set_trackresult(self, oid, codes):
         self._chk(self.L.mf_set_trackresult(self.ctx, oid, np.ascontiguousarray(codes, np.int32).reshape(-1)))
+
+    def set_instance_mask(self, oid, mask, rendered=False, model_mask=None):
+        """generalize_label on the device from a 0/non-zero instance mask."""
+        m = np.ascontiguousarray(mask, np.uint8).reshape(-1)
+        g = None if model_mask is None else np.ascontiguousarray(model_mask, np.uint8).reshape(-1)
+        self._chk(self.L.mf_object_set_instance_mask(self.ctx, oid, m, int(rendered),
+                                                     None if g is None else g.ctypes.data))
+
+    def mesh(self, oid):
+        """Marching-cubes triangles of object oid: [n, 3, 3] float32 (object frame)."""
+        n = self._chk(self.L.mf_object_mesh(self.ctx, oid, None, 0))
+        out = np.zeros((max(n, 1), 3, 3), np.float32)
+        if n:
+            self._chk(self.L.mf_object_mesh(self.ctx, oid, out.ctypes.data, n))
+        return out[:n]
+
+    def dump_mesh(self, prefix):
+        self._chk(self.L.mf_dump_mesh(self.ctx, prefix.encode()))
+
+    def get_mask(self, oid):
+        out = np.zeros(self.w * self.h, np.float32)
+        self._chk(self.L.mf_object_get_mask(self.ctx, oid, out))
+        return out.reshape(self.h, self.w)
 
     def set_mask(self, oid, mask):
         self._chk(self.L.mf_object_set_mask(self.ctx, oid, _f32(mask).reshape(-1)))

@@ -18,6 +18,8 @@
 
 using namespace orc;
 
+#include "../mid-fusion_amd/csrc/mf_mc_table.h"   // the library's marching-cubes table (generated)
+
 namespace {
 
 struct Cfg {
@@ -908,6 +910,68 @@ int orc_render_volume(void* h, const float k[4], const float view16[16], int w, 
   return 1;
 }
 
+// -------------------------------------------------------------- meshing
+// algorithms::marching_cube (lib/algorithms/meshing.hpp:157-208) with the
+// tsdf field and inside = tsdf < 0 (kfusion.cpp:1562-1577), over every
+// allocated block, with the library's triangle table (the reference's
+// edge_tables.h is not reused): same cubes, same edge vertices.
+static const int kMcCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1},
+                                    {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}};
+static const int kMcEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {4, 5}, {5, 6},
+                                   {6, 7}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+// meshing::compute_index (meshing.hpp:88-146)
+static int mc_index(const Voxel (&p)[8]) {
+  for (int k = 0; k < 8; k++) if (p[k].y == 0) return 0;
+  for (int k = 0; k < 8; k++) if (p[k].fg <= 0.5 || p[k].w <= 3) return 0;
+  int idx = 0;
+  for (int k = 0; k < 8; k++) if (p[k].x < 0) idx |= 1 << k;
+  return idx;
+}
+static long mesh_volume(const Octree& vol, float* out, long cap) {
+  const int size = vol.size;
+  const float vsz = (float)vol.dim / (float)size;   // volume.dim() / volume.size()
+  const float dim = (float)vol.dim;   // checkVertex bound in metres (the reference's int truncation is not kept)
+  long n = 0;
+  for (const Node* b : vol.blocks) {
+    const int xb = std::min(std::max(b->cx + 8, 0), size - 1), yb = std::min(std::max(b->cy + 8, 0), size - 1),
+              zb = std::min(std::max(b->cz + 8, 0), size - 1);
+    for (int x = b->cx; x < xb; x++)
+      for (int y = b->cy; y < yb; y++)
+        for (int z = b->cz; z < zb; z++) {
+          Voxel c[8];
+          for (int k = 0; k < 8; k++) c[k] = vol.get_fine(x + kMcCorner[k][0], y + kMcCorner[k][1], z + kMcCorner[k][2]);
+          const int idx = mc_index(c);
+          const int8_t* e = mf::kMcTri[idx];
+          for (int t = 0; t < 5 && e[3 * t] >= 0; t++) {
+            float v[3][3];
+            bool bad = false;
+            for (int q = 0; q < 3; q++) {
+              const int a = kMcEdge[e[3 * t + q]][0], bb = kMcEdge[e[3 * t + q]][1];
+              // compute_intersection (meshing.hpp:21-30), float
+              float s[3], d[3];
+              for (int ax = 0; ax < 3; ax++) {
+                const int o[3] = {x, y, z};
+                s[ax] = (float)(unsigned)(o[ax] + kMcCorner[a][ax]) * vsz;
+                d[ax] = (float)(unsigned)(o[ax] + kMcCorner[bb][ax]) * vsz;
+              }
+              const float v1 = (float)c[a].x, v2 = (float)c[bb].x, tt = -v1;
+              for (int ax = 0; ax < 3; ax++) v[q][ax] = s[ax] + (tt * (d[ax] - s[ax])) / (v2 - v1);
+              // checkVertex (meshing.hpp:148-150)
+              if (v[q][0] <= 0 || v[q][1] <= 0 || v[q][2] <= 0 || v[q][0] > dim || v[q][1] > dim || v[q][2] > dim)
+                bad = true;
+            }
+            if (bad) continue;
+            if (out && n < cap)
+              for (int q = 0; q < 3; q++) for (int ax = 0; ax < 3; ax++) out[9 * n + 3 * q + ax] = v[q][ax];
+            n++;
+          }
+        }
+  }
+  return n;
+}
+long orc_object_mesh(void* h, int id, float* out, long cap) { return mesh_volume(((Pipe*)h)->objs[id]->vol, out, cap); }
+long orc_octree_mesh(void* h, float* out, long cap) { return mesh_volume(*(Octree*)h, out, cap); }
+
 // Ground-truth camera poses (row-major 4x4 each); n = 0 turns the mode off.
 // The translation of poses[f] is moved by (initial position - poses[0]
 // translation), as kfusion.cpp:263-266 does, in R arithmetic.
@@ -1260,6 +1324,15 @@ int orc_octree_set(void* h, int x, int y, int z, float tsdf, float fg) {
   if (!b) return 0;
   Voxel& v = b->vox[(x - b->cx) + 8 * (y - b->cy) + 64 * (z - b->cz)];
   v.x = tsdf; v.fg = fg; v.y = 1;
+  return 1;
+}
+// All voxel fields the mesher reads: tsdf, weight, fg and its witness count.
+int orc_octree_set_full(void* h, int x, int y, int z, float tsdf, float weight, double fg, int fgw) {
+  Octree* o = (Octree*)h;
+  Node* b = o->fetch(x, y, z);
+  if (!b) return 0;
+  Voxel& v = b->vox[(x - b->cx) + 8 * (y - b->cy) + 64 * (z - b->cz)];
+  v.x = tsdf; v.y = weight; v.fg = fg; v.w = fgw;
   return 1;
 }
 float orc_octree_interp(void* h, float x, float y, float z, int field) { return (float)interp(*(Octree*)h, v3(x, y, z), field); }

@@ -58,6 +58,9 @@ def _lib(fp64):
         'orc_motion_rows': (C.c_int, [P, i32p, f32p]),
         'orc_motion_state': (C.c_int, [P, i32p, i32p, i32p, C.c_int]),
         'orc_set_gt_poses': (None, [P, f32p, C.c_long]),
+        'orc_object_mesh': (C.c_long, [P, C.c_int, P, C.c_long]),
+        'orc_octree_mesh': (C.c_long, [P, P, C.c_long]),
+        'orc_octree_set_full': (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_double, C.c_int]),
         'orc_render_track': (C.c_int, [P, C.c_int, C.c_int, u8p]),
         'orc_render_depth': (C.c_int, [P, u8p]),
         'orc_render_intensity': (C.c_int, [P, u8p]),
@@ -236,6 +239,13 @@ class Oracle:
                                      int(render_color), pal.reshape(-1), 0 if palette is None else len(pal),
                                      out.reshape(-1))
         return out, bool(r)
+
+    def mesh(self, oid):
+        n = self.L.orc_object_mesh(self.h_, oid, None, 0)
+        out = np.zeros((max(n, 1), 3, 3), np.float32)
+        if n:
+            self.L.orc_object_mesh(self.h_, oid, out.ctypes.data, n)
+        return out[:n]
 
     def set_gt_poses(self, poses):
         """Ground-truth camera poses (n x 4 x 4); an empty array turns the mode off."""
