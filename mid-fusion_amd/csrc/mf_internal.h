@@ -63,10 +63,9 @@ struct GNState {
   Mat4 B;          // camera: K * inverse(T_w_r); object: inverse(T_w_o1)
   Mat4 C;          // object: inverse(T_c1_o1)
   Mat4 T_wo1;      // object: pose at the previous frame
-  int iters_run;
   int object;      // 1: object instance (zero-inlier break, pose composed with T_w_c2)
   Mat4* out_pose;  // written after every update: T_w_l, or volume_pose = T_w_c2 * T_c2_o2
-  const Mat4* T_w_c2;
+  Mat4 Twc2;       // object: the camera pose T_w_c2 of this frame (fixed during the run)
   Mat4 T_final;    // T after the last solve (object: T_c2_o2)
   int off;         // 1: this run is skipped entirely (no rows, no solves)
   double sums[32]; // last reduction (FP64)

@@ -310,17 +310,15 @@ __device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst
 
 // ---- FP64 6x6 solve (commons.h:672-696 semantics: TooN GR_SVD backsub with
 // condition 1e6).  C is the symmetric JtWJ of the reduction, b its JtWe.
-// Wave 0 runs Gauss-Jordan on [C | I | b]; when every pivot is positive and
-// tr(C)*tr(C^-1) < 1e6 (so lambda_max/lambda_min < 1e6 and backsub would
-// truncate nothing), x = C^-1 b.  Otherwise lane 0 computes the pseudo-inverse
-// from a cyclic Jacobi eigendecomposition with the reference's truncation
-// rule.  All work arrays live in LDS, so the solve adds few registers to the
-// kernels that embed it.
+// Wave 0 runs Gauss-Jordan on [C | I | b], lane j holding column j in
+// registers and taking the pivot column from lane k with v_readlane; when
+// every pivot is positive and tr(C)*tr(C^-1) < 1e6 (so lambda_max/lambda_min
+// < 1e6 and backsub would truncate nothing), x = C^-1 b.  Otherwise lane 0
+// computes the pseudo-inverse from a cyclic Jacobi eigendecomposition (LDS
+// work arrays) with the reference's truncation rule.
 struct SolveLDS {
-  double M[6][13];           // [C | I | b] during Gauss-Jordan
   double A[6][6], V[6][6];   // Jacobi work arrays
   double x[6];
-  int ok;
 };
 __device__ __forceinline__ int tri(int i, int j) {   // packed upper-triangular index of the 21 JtWJ sums
   if (i > j) { const int t = i; i = j; j = t; }
@@ -330,43 +328,15 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
-// s27 = sums[1..27] (b then the packed JtWJ); called by the 64 lanes of one wave
-__device__ void gj_solve(SolveLDS& W, const double* s27, int lane) {
-  for (int e = lane; e < 78; e += 64) {
-    const int i = e / 13, j = e % 13;
-    W.M[i][j] = j < 6 ? s27[tri(i, j)] : (j < 12 ? (j - 6 == i ? 1.0 : 0.0) : s27[i]);
-  }
-  wave_sync();
-  bool ok = true;
-  for (int k = 0; k < 6; k++) {
-    const double pk = W.M[k][k];
-    ok = ok && (pk > 0.0);
-    double nv[2];
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int e = lane + 64 * u;
-      nv[u] = 0.0;
-      if (e < 78) {
-        const int i = e / 13, j = e % 13;
-        const double r = W.M[k][j] / pk;
-        nv[u] = (i == k) ? r : W.M[i][j] - W.M[i][k] * r;
-      }
-    }
-    wave_sync();
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int e = lane + 64 * u;
-      if (e < 78) W.M[e / 13][e % 13] = nv[u];
-    }
-    wave_sync();
-  }
-  if (lane == 0) {
-    double trC = 0, trI = 0;
-    for (int i = 0; i < 6; i++) { trC += s27[tri(i, i)]; trI += W.M[i][6 + i]; }
-    W.ok = ok && trI > 0.0 && trC * trI < 1e6;
-    for (int i = 0; i < 6; i++) W.x[i] = W.M[i][12];
-  }
-  wave_sync();
+// value of v in lane l (l uniform)
+__device__ __forceinline__ float rlf(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double rld(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 // Pseudo-inverse solve by cyclic Jacobi (one lane, LDS arrays).
 __device__ void jacobi_pinv(SolveLDS& W, const double* s27) {
@@ -400,6 +370,75 @@ __device__ void jacobi_pinv(SolveLDS& W, const double* s27) {
     for (int k = 0; k < 6; k++) W.x[k] += pr * W.V[k][i];
   }
 }
+// s27 = sums[1..27] (b then the packed JtWJ) in LDS; called by the 64 lanes of
+// one wave, x uniform on return.  Returns 1 (Gauss-Jordan, certified) or 2
+// (Jacobi pseudo-inverse).
+__device__ int solve6(SolveLDS& W, const double* s27, int lane, double (&x)[6]) {
+  const int j = lane < 13 ? lane : 12;
+  double m[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) m[i] = j < 6 ? s27[tri(i, j)] : (j < 12 ? (j - 6 == i ? 1.0 : 0.0) : s27[i]);
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    double ck[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) ck[i] = rld(m[i], k);
+    const double pk = ck[k];
+    ok = ok && (pk > 0.0);
+    const double r = m[k] / pk;
+#pragma unroll
+    for (int i = 0; i < 6; i++) m[i] = (i == k) ? r : m[i] - ck[i] * r;
+  }
+  double trC = 0, trI = 0;
+#pragma unroll
+  for (int i = 0; i < 6; i++) { trC += s27[tri(i, i)]; trI += rld(m[i], 6 + i); }
+  if (ok && trI > 0.0 && trC * trI < 1e6) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) x[i] = rld(m[i], 12);
+    return 1;
+  }
+  if (lane == 0) jacobi_pinv(W, s27);
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < 6; i++) x[i] = W.x[i];
+  return 2;
+}
+// inverse(Matrix4) of mf_math.h, column-parallel over lanes: lane l holds
+// column (l & 7) of [A | I] (rows a[0..3]); on return lanes with (l & 4) hold
+// the columns of the inverse.  Every element sees the operations of the
+// serial form in the same order, so the result is bit-identical to it.
+__device__ __forceinline__ void inverse_cols(float (&a)[4], int lane) {
+  const int cc = lane & 7;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    float ci[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) ci[r] = rlf(a[r], i);
+    int am = i;
+    float mx = fabsf(ci[i]);
+#pragma unroll
+    for (int ii = i + 1; ii < 4; ii++) { const float v = fabsf(ci[ii]); if (v > mx) { mx = v; am = ii; } }
+    float piv = ci[i];
+#pragma unroll
+    for (int r = i + 1; r < 4; r++)
+      if (r == am) { piv = ci[r]; ci[r] = ci[i]; const float t = a[i]; a[i] = a[r]; a[r] = t; }
+    const float inv = 1.0f / piv;
+    if (cc > i) a[i] = a[i] * inv;
+#pragma unroll
+    for (int u = i + 1; u < 4; u++)
+      if (cc > i) a[u] = a[u] - ci[u] * a[i];
+  }
+#pragma unroll
+  for (int i = 3; i > 0; i--) {
+    float f[4];
+#pragma unroll
+    for (int u = 0; u < i; u++) f[u] = rlf(a[u], i);
+#pragma unroll
+    for (int u = 0; u < i; u++)
+      if (cc >= 4) a[u] = a[u] - f[u] * a[i];
+  }
+}
 // SE3 exponential, x[0..2] translation, x[3..5] rotation (TooN SE3::exp as
 // used at tracking.cpp:1361): Rodrigues + left Jacobian in FP64.
 __device__ Mat4 se3_exp(const double x[6]) {
@@ -426,35 +465,35 @@ __device__ Mat4 se3_exp(const double x[6]) {
   return M;
 }
 
-// FP64 accumulation of one instance's workgroup partials, the 6x6 solve and
-// the GN control flow of trackLiveFrame / trackEachObject (tracking.cpp:
-// 300-334, 664-688) for the rows computed with X's pose at (X.pend_level,
-// X.pend_iter), by one 256-thread workgroup on its LDS copy X.  Partials
-// are summed in a fixed order with eight float4 loads in flight per thread
-// (thread t always reads sums 4*(t%8)..4*(t%8)+3), so every workgroup that
-// runs this on the same partials reaches bit-identical results.  Only the
-// `writer` workgroup publishes the per-iteration outputs (pose, trace).
-// FP64 sum of `nparts` workgroup partials (32 floats each) into sums (LDS) and,
-// when `out` is set, into out (device), by an NT-thread workgroup: eight
-// float4 loads in flight per thread, fixed order (deterministic); thread t
-// always reads sums 4*(t%8)..4*(t%8)+3.
-template <int NT>
-__device__ __forceinline__ void accumulate_sums(const float* __restrict__ part, int nparts, double* sums,
-                                                double* out) {
-  __shared__ double red[NT / 64][8][4];
-  const int tid = threadIdx.x;
+// FP64 sum of `nparts` workgroup partials (32 floats each) into sums (LDS)
+// and, when `out` is set, into out (device), by an NT-thread workgroup: U
+// float4 loads in flight per thread and round, fixed order (deterministic);
+// thread t always reads sums 4*(t%8)..4*(t%8)+3.  The caller issues round 0
+// (load_round with q0 = 0) ahead of its own state loads, so both latencies
+// overlap.
+template <int NT, int U>
+__device__ __forceinline__ void load_round(const float* __restrict__ part, int nparts, int q0, float4 (&v)[U]) {
   const float4* P = reinterpret_cast<const float4*>(part);
   const int nq = nparts * 8;
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int q = q0 + u * NT + (int)threadIdx.x;
+    v[u] = q < nq ? P[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+template <int NT, int U>
+__device__ __forceinline__ void accumulate_sums(const float* __restrict__ part, int nparts, float4 (&v)[U],
+                                                double* sums, double* out) {
+  __shared__ double red[NT / 64][8][4];
+  const int tid = threadIdx.x;
+  const int nq = nparts * 8;
   double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-  for (int q0 = 0; q0 < nq; q0 += 8 * NT) {
-    float4 v[8];
+  for (int q0 = 0;;) {
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int q = q0 + u * NT + tid;
-      v[u] = q < nq ? P[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-#pragma unroll
-    for (int u = 0; u < 8; u++) { a0 += (double)v[u].x; a1 += (double)v[u].y; a2 += (double)v[u].z; a3 += (double)v[u].w; }
+    for (int u = 0; u < U; u++) { a0 += (double)v[u].x; a1 += (double)v[u].y; a2 += (double)v[u].z; a3 += (double)v[u].w; }
+    q0 += U * NT;
+    if (q0 >= nq) break;
+    load_round<NT, U>(part, nparts, q0, v);
   }
   // lanes with equal (lane % 8) hold the same four sums
 #pragma unroll
@@ -474,66 +513,130 @@ __device__ __forceinline__ void accumulate_sums(const float* __restrict__ part, 
   __syncthreads();
 }
 
-template <int NT>
-__device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, int state, const float* __restrict__ part,
-                                         int nparts, bool writer, const TrackSync& sy) {
-  __shared__ double sums[32];
-  __shared__ SolveLDS W;
-  __shared__ int mode;
-  const int tid = threadIdx.x;
-  const uint64_t t0 = wall_clock64();
-  accumulate_sums<NT>(part, nparts, sums, writer ? S.sums : nullptr);
-  const int level = X.pend_level, iter = X.pend_iter;
-  // 3 = no inliers (objects), 2 = error grew (step back), 0 = solve
-  if (tid == 0) {
-    const float s0 = (float)sums[0], s28 = (float)sums[28];
-    const float err = s0 / s28;
-    const float prev_err = (iter == 0) ? INFINITY : X.prev_err;
-    mode = (S.object && s28 == 0) ? 3 : (err > prev_err ? 2 : 0);
-  }
-  __syncthreads();
+// Per-run constants of an instance that the solve needs, read with the
+// iteration state at kernel start.
+struct SolvePre {
+  Mat4 Twc2;
+  Mat4* out_pose;
+  int object;
+};
+__device__ __forceinline__ void load_pre(SolvePre& pre, const GNState& S) {
+  pre.Twc2 = S.Twc2; pre.out_pose = S.out_pose; pre.object = S.object;
+}
+
+// One GN step of trackLiveFrame / trackEachObject (tracking.cpp:300-334,
+// 664-688) for the rows computed with X's pose at (X.pend_level,
+// X.pend_iter), from the FP64 sums in LDS, by the 64 lanes of wave 0: lanes
+// 0-3 hold the columns of T and previous_pose, lanes 4-7 those of the
+// inverse; only the `writer` workgroup publishes the outputs (pose, trace).
+__device__ void gn_update(GNIter& X, GNState& S, const SolvePre& pre, SolveLDS& W, const double* sums, int state,
+                          bool writer, const TrackSync& sy, uint64_t t0) {
+  const int lane = threadIdx.x;
   const uint64_t t1 = wall_clock64();
-  if (mode == 0 && tid < 64) {
-    gj_solve(W, sums + 1, tid);
-    if (tid == 0 && !W.ok) jacobi_pinv(W, sums + 1);
-  }
-  __syncthreads();
-  if (tid == 0) {
-    GNIter Y = X;   // registers: the pose algebra below is serial
-    const float s0 = (float)sums[0], s28 = (float)sums[28];
-    const float err = s0 / s28;
-    int flag = mode;
-    if (mode == 3) { Y.done_level = level; }
-    else if (mode == 2) { Y.T = Y.prev; Y.invT = inverse(Y.T); Y.done_level = level; }
-    else {
-      Y.prev_err = err;
-      double x[6];
-      for (int q = 0; q < 6; q++) x[q] = W.x[q];
-      const Mat4 upd = se3_exp(x);
-      const double nx = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
-      if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) { Y.done_level = level; flag = 1; }
-      else { Y.prev = Y.T; Y.T = mul(upd, Y.prev); Y.invT = inverse(Y.T); }
+  const int level = X.pend_level, iter = X.pend_iter;
+  const float s0 = (float)sums[0], s28 = (float)sums[28];
+  const float err = s0 / s28;
+  const float prev_err = (iter == 0) ? INFINITY : X.prev_err;
+  // 3 = no inliers (objects), 2 = error grew (step back), 0 = solve
+  const int mode = (pre.object && s28 == 0) ? 3 : (err > prev_err ? 2 : 0);
+  const int c = lane & 3;
+  float t[4], pv[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) { t[r] = X.T.m[r][c]; pv[r] = X.prev.m[r][c]; }
+  int flag = mode, done = X.done_level, solver = 0;
+  float perr = X.prev_err;
+  bool newT = false;
+  if (mode == 3) {
+    done = level;
+  } else if (mode == 2) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) t[r] = pv[r];
+    newT = true;
+    done = level;
+  } else {
+    perr = err;
+    double x[6];
+    solver = solve6(W, sums + 1, lane, x);
+    const Mat4 upd = se3_exp(x);
+    const double nx = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+    if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) {
+      done = level;
+      flag = 1;
+    } else {
+      // previous_pose = T; T = upd * previous_pose (commons.h:657-662 operand order)
+#pragma unroll
+      for (int r = 0; r < 4; r++) pv[r] = t[r];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        float s = upd.m[r][0] * pv[0];
+        s = s + upd.m[r][1] * pv[1];
+        s = s + upd.m[r][2] * pv[2];
+        s = s + upd.m[r][3] * pv[3];
+        t[r] = s;
+      }
+      newT = true;
     }
-    Y.pend_level = -1;
-    X = Y;
-    if (writer) {
-      S.iters_run++;
-      S.T_final = Y.T;
-      // camera: T_w_l; object: volume_pose_ = T_w_c2 * T_c2_o2 (tracking.cpp:695)
-      *S.out_pose = S.object ? mul(*S.T_w_c2, Y.T) : Y.T;
-      if (sy.trace) {
-        const uint64_t t2 = wall_clock64();
-        const int ti = atomicAdd(sy.trace_n, 1);
-        if (ti < sy.trace_cap) {
-          TraceRec& r = sy.trace[ti];
-          r.inst = state; r.level = level; r.iter = iter; r.flag = flag; r.err = err; r.s28 = s28;
-          for (int q = 0; q < 16; q++) r.pose[q] = Y.T.m[q / 4][q % 4];
-          r.t_accum = (unsigned)(t1 - t0); r.t_solve = (unsigned)(t2 - t1);
-          r.solver = (mode == 0) ? (W.ok ? 1 : 2) : 0;
-        }
+  }
+  if (newT) {
+    float a[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) a[r] = (lane & 4) ? (r == c ? 1.f : 0.f) : t[r];
+    inverse_cols(a, lane);
+    if (lane >= 4 && lane < 8) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) X.invT.m[r][c] = a[r];
+    }
+  }
+  if (lane < 4) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) { X.T.m[r][c] = t[r]; X.prev.m[r][c] = pv[r]; }
+  }
+  if (lane == 0) { X.prev_err = perr; X.done_level = done; X.pend_level = -1; }
+  if (!writer) return;
+  if (lane < 4) {
+    // camera: T_w_l; object: volume_pose_ = T_w_c2 * T_c2_o2 (tracking.cpp:695)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      S.T_final.m[r][c] = t[r];
+      float o = t[r];
+      if (pre.object) {
+        o = pre.Twc2.m[r][0] * t[0];
+        o = o + pre.Twc2.m[r][1] * t[1];
+        o = o + pre.Twc2.m[r][2] * t[2];
+        o = o + pre.Twc2.m[r][3] * t[3];
+      }
+      pre.out_pose->m[r][c] = o;
+    }
+  }
+  if (sy.trace) {
+    const uint64_t t2 = wall_clock64();
+    int ti = 0;
+    if (lane == 0) ti = atomicAdd(sy.trace_n, 1);
+    ti = __shfl(ti, 0);
+    if (ti < sy.trace_cap) {
+      TraceRec& rec = sy.trace[ti];
+      if (lane < 4) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) rec.pose[4 * r + c] = t[r];
+      }
+      if (lane == 0) {
+        rec.inst = state; rec.level = level; rec.iter = iter; rec.flag = flag; rec.err = err; rec.s28 = s28;
+        rec.t_accum = (unsigned)(t1 - t0); rec.t_solve = (unsigned)(t2 - t1);
+        rec.solver = solver;
       }
     }
   }
+}
+
+template <int NT, int U>
+__device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, const SolvePre& pre, int state,
+                                         const float* __restrict__ part, int nparts, float4 (&v)[U], bool writer,
+                                         const TrackSync& sy) {
+  __shared__ double sums[32];
+  __shared__ SolveLDS W;
+  const uint64_t t0 = wall_clock64();
+  accumulate_sums<NT, U>(part, nparts, v, sums, writer ? S.sums : nullptr);
+  if (threadIdx.x < 64) gn_update(X, S, pre, W, sums, state, writer, sy, t0);
   __syncthreads();
 }
 
@@ -571,26 +674,33 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
 // Two-kernel form, solve: the iteration's FP64 accumulation, solve and GN
 // control, one workgroup per instance, after k_track (the kernel boundary
 // publishes the partials; a device-scope fence in every track workgroup costs
-// far more on gfx950).
+// far more on gfx950).  SOLVE_U float4 loads per thread cover the 600
+// partials of a 640x480 level in one round.
 static constexpr int SOLVE_NT = 512;    // threads of the solve workgroup (partials in flight)
+static constexpr int SOLVE_U = 10;
 __global__ __launch_bounds__(SOLVE_NT) void k_solve(int level, int iter, const TrackInst* __restrict__ insts,
                                               GNState* __restrict__ st, const float* __restrict__ partial,
                                               int nblk, int q, TrackSync sy) {
   __shared__ GNIter X;
+  __shared__ SolvePre pre;
   __shared__ int skip;
   const int tid = threadIdx.x;
   const int j = blockIdx.x;
+  const float* part = partial + (size_t)j * nblk * 32;
+  float4 v[SOLVE_U];
+  load_round<SOLVE_NT, SOLVE_U>(part, nblk, 0, v);
   const int state = insts[j].state;
   GNState& S = st[state];
   if (tid == 0) {
     X = S.it[q & 1];
+    load_pre(pre, S);
     skip = S.off || (iter > 0 && X.done_level == level);
     if (skip) S.it[(q + 1) & 1] = X;
     X.pend_level = level; X.pend_iter = iter;
   }
   __syncthreads();
   if (skip) return;
-  gn_solve<SOLVE_NT>(X, S, state, partial + (size_t)j * nblk * 32, nblk, true, sy);
+  gn_solve<SOLVE_NT, SOLVE_U>(X, S, pre, state, part, nblk, v, true, sy);
   if (tid == 0) S.it[(q + 1) & 1] = X;
 }
 
@@ -610,16 +720,19 @@ __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* 
   __shared__ float red[NW][32];
   __shared__ Mats Msh;
   __shared__ GNIter X;
+  __shared__ SolvePre pre;
   __shared__ int skip;
   const int tid = threadIdx.x;
   const int j = blockIdx.y;
+  const float* prev = part_prev + (size_t)j * nblk_prev * 32;
+  float4 v[8];
+  load_round<TB, 8>(prev, nblk_prev, 0, v);
   const int state = insts[j].state;
   GNState& S = st[state];
   const bool writer = blockIdx.x == 0;
-  if (tid == 0) X = S.it[q & 1];
+  if (tid == 0) { X = S.it[q & 1]; load_pre(pre, S); }
   __syncthreads();
-  if (X.pend_level >= 0)
-    gn_solve<TB>(X, S, state, part_prev + (size_t)j * nblk_prev * 32, nblk_prev, writer, sy);
+  if (X.pend_level >= 0) gn_solve<TB, 8>(X, S, pre, state, prev, nblk_prev, v, writer, sy);
   if (tid == 0) {
     skip = S.off || !rows || (p.iter > 0 && X.done_level == p.level);
     if (!skip) {
@@ -646,7 +759,9 @@ __global__ void k_eval_set(GNState* st, int state, Mat4 T) {
 __global__ __launch_bounds__(SOLVE_NT) void k_eval_sums(GNState* st, int state, const float* __restrict__ partial,
                                                         int nblk) {
   __shared__ double sums[32];
-  accumulate_sums<SOLVE_NT>(partial, nblk, sums, st[state].sums);
+  float4 v[SOLVE_U];
+  load_round<SOLVE_NT, SOLVE_U>(partial, nblk, 0, v);
+  accumulate_sums<SOLVE_NT, SOLVE_U>(partial, nblk, v, sums, st[state].sums);
 }
 
 __device__ void gn_iter_init(GNIter& X, const Mat4& T) {
@@ -659,8 +774,8 @@ __device__ void gn_init_camera(GNState& S, Mat4* T_w_l, const Mat4& Twr, float f
   S.A = Twr;
   S.B = mul(camK(fx, fy, cx, cy), inverse(Twr));
   S.C = ident(); S.T_wo1 = ident();
-  S.iters_run = 0; S.object = 0; S.off = 0;
-  S.out_pose = T_w_l; S.T_w_c2 = nullptr;
+  S.object = 0; S.off = 0;
+  S.out_pose = T_w_l; S.Twc2 = ident();
 }
 // Tracking frame start: T_w_r = camera pose (kfusion.cpp:243), its inverse for
 // the rendered reference, and the camera's GN state.
@@ -691,9 +806,9 @@ __global__ void k_gn_begin_objects(GNState* st, int first, int n, const int* ids
   S.T_final = T;
   S.B = inverse(To);
   S.C = inverse(S.A);
-  S.iters_run = 0; S.object = 1;
+  S.object = 1;
   S.off = (gate && *gate == 0) ? 1 : 0;
-  S.out_pose = pose; S.T_w_c2 = T_w_c2;
+  S.out_pose = pose; S.Twc2 = *T_w_c2;
   if (publish && !S.off) *pose = mul(*T_w_c2, T);
 }
 
