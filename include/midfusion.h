@@ -189,6 +189,13 @@ int mf_get_frame(mf_ctx* ctx, float* depth, float* grey, float* rgb);
  * frame; the robust function is the context's. */
 int mf_residual_eval(mf_ctx* ctx, int id, const float k[4], int level, const float T[16], double out[32]);
 
+/* The tracking solve (solve / makeJTJ, commons.h:672-696: TooN GR_SVD backsub
+ * with condition 1e6) on n caller-given systems of 27 doubles each {JtWe[6],
+ * upper triangle of JtWJ row-major[21]}, computed by the device code the GN
+ * iterations use: x[6n] the solutions, solver[n] 1 (certified Gauss-Jordan) or
+ * 2 (Jacobi pseudo-inverse).  Blocking; n <= 2^20. */
+int mf_solve6(mf_ctx* ctx, const double* s27, long n, double* x, int32_t* solver);
+
 /* Kernel timing: when enabled every launch is bracketed by HIP events on the
  * context stream.  out[i] = {total_ms, launches} for each kernel class in
  * mf_kernel_names order; returns the number of classes. */

@@ -1532,6 +1532,28 @@ int mf_residual_eval(mf_ctx* c, int id, const float k[4], int level, const float
   return 0;
 }
 
+int mf_solve6(mf_ctx* c, const double* s27, long n, double* x, int32_t* solver) {
+  if (n < 0 || (n > 0 && (!s27 || !x || !solver))) return fail(c, MF_E_ARG, "bad solve6 arguments");
+  if (n == 0) return 0;
+  if (n > (1 << 20)) return fail(c, MF_E_CAPACITY, "at most 2^20 systems per call");
+  hipStream_t s = c->stream;
+  double *d_s = nullptr, *d_x = nullptr;
+  int* d_v = nullptr;
+  if (hipMalloc(&d_s, n * 27 * sizeof(double)) != hipSuccess || hipMalloc(&d_x, n * 6 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&d_v, n * sizeof(int)) != hipSuccess) {
+    hipFree(d_s); hipFree(d_x); hipFree(d_v);
+    return fail(c, MF_E_OOM, "solve6 buffers");
+  }
+  hipMemcpyAsync(d_s, s27, n * 27 * sizeof(double), hipMemcpyHostToDevice, s);
+  launch_solve6(s, d_s, (int)n, d_x, d_v);
+  hipMemcpyAsync(x, d_x, n * 6 * sizeof(double), hipMemcpyDeviceToHost, s);
+  hipMemcpyAsync(solver, d_v, n * sizeof(int), hipMemcpyDeviceToHost, s);
+  const hipError_t e = hipStreamSynchronize(s);
+  hipFree(d_s); hipFree(d_x); hipFree(d_v);
+  if (e != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(e));
+  return 0;
+}
+
 // Replays the camera's row kernel of one pyramid level `reps` times back to
 // back (iteration 0 at the current camera state), bracketed by two events on
 // the context stream, and returns the average launch time and the

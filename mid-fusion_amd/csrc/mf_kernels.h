@@ -58,6 +58,9 @@ void launch_gn_fused(hipStream_t s, const TrackLaunch& p, const TrackInst* inst,
                      const TrackSync& sy);
 // Residual sums of one instance (`inst` points at its descriptor) at pose T:
 // GN state `state` gets T, then rows and the FP64 sums (into GNState::sums).
+// The tracking 6x6 solve on n systems (27 doubles each) into x[6n] and
+// solver[n] (1 certified Gauss-Jordan, 2 Jacobi pseudo-inverse).
+void launch_solve6(hipStream_t s, const double* s27, int n, double* x, int* solver);
 void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNState* st, int state, const Mat4& T,
                  float* partial, int nblk);
 // Motion gating state on the device (Kfusion::tracking, kfusion.cpp:286-471),

@@ -28,10 +28,16 @@ __device__ __forceinline__ f3 rf3(const TrackLaunch& p, int pl, size_t i) {
   const float* b = p.ref + (size_t)pl * p.lstride + i;
   return mk3(b[0], b[p.lstride], b[2 * (size_t)p.lstride]);
 }
+// Load from global memory.  Pointers read out of a descriptor are generic to
+// the compiler, which would otherwise issue flat loads (counted against LDS
+// waits as well).
+__device__ __forceinline__ float gld(const float* a) {
+  return *(const __attribute__((address_space(1))) float*)a;
+}
 __device__ __forceinline__ f3 mdl3(const TrackLaunch& p, const TrackInst& in, int pl, size_t i) {
   const size_t n = (size_t)p.W * p.H;
   const float* b = in.mvn + pl * n + i;
-  return mk3(b[0], b[n], b[2 * n]);
+  return mk3(gld(b), gld(b + n), gld(b + 2 * n));
 }
 
 // bilinear_interp<T> (utils/math_utils.h:149-166)
@@ -53,7 +59,7 @@ __device__ __forceinline__ f3 bilin3(const float* f0, size_t stride, int w, floa
   float* rr = &r.x;
   for (int k = 0; k < 3; k++) {
     const float* p = f0 + k * stride;
-    rr[k] = p[i0] * w0 * w1 + p[i0 + 1] * w1 * fx + p[i0 + w] * w0 * fy + p[i0 + w + 1] * fx * fy;
+    rr[k] = gld(p + i0) * w0 * w1 + gld(p + i0 + 1) * w1 * fx + gld(p + i0 + w) * w0 * fy + gld(p + i0 + w + 1) * fx * fy;
   }
   return r;
 }
@@ -88,29 +94,69 @@ __device__ __forceinline__ void robust(float e, int fn, double k, float& w, floa
 // Pose matrices a row reads, staged in LDS by the workgroup and re-read from
 // LDS (broadcast) inside the row loop: holding 60 uniform floats in scalar
 // registers spills them, and in vector registers costs occupancy.
-struct Mats { Mat4 T, invT, A, B, C; };
+// `mov` has bit l set when label l is a moving object (camera re-track).
+struct Mats { Mat4 T, invT, A, B, C; uint64_t mov; };
+
+// Marks loaded values as used at this point, so that loads issued above it
+// complete under one wait instead of being sunk, one by one, behind the
+// validity tests that read them: a row then costs one round trip for its
+// pixel's own inputs and one for its gathers.
+__device__ __forceinline__ void hold(float v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void hold(int v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void hold(const f3& v) { hold(v.x); hold(v.y); hold(v.z); }
+
+// Camera re-track outlier mask from the raycast labels of moving objects,
+// resized as cv::resize does for a 0/255 mask (tracking.cpp:1782-1788): the
+// label taps of pixel (x, y) at the launch's level (1 at level 0, else 2x2).
+struct LabelTaps { int l[4]; };
+__device__ __forceinline__ int label_taps(const TrackLaunch& p, int x, int y, LabelTaps& t) {
+  const int s = 1 << p.level;
+  const int o = (p.level <= 1) ? 0 : (s / 2 - 1);
+  const int n = (p.level == 0) ? 1 : 2;
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+      t.l[2 * j + i] = (i < n && j < n) ? p.label[(size_t)(x * s + o + i) + (size_t)(y * s + o + j) * p.W] : 0;
+  return n;
+}
+__device__ __forceinline__ bool taps_moving(const TrackLaunch& p, const LabelTaps& t, uint64_t mov) {
+  bool m = false;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int l = t.l[k];
+    if (l > 0 && l < p.n_labels && ((mov >> l) & 1)) m = true;
+  }
+  return m;
+}
 
 // ICP row (camera), track_ICP_Kernel mask overload (tracking.cpp:923-1020)
 template <class MT>
 __device__ __forceinline__ int icp_row(const TrackLaunch& p, const TrackInst& in, const MT& M, int x, int y,
-                                       size_t i, bool masked, float& e, float J[6]) {
-  if (masked) return -6;
-  const f3 nl = lv3(p, PL_N, i);
+                                       size_t i, float& e, float J[6]) {
+  const f3 nl = lv3(p, PL_N, i), vl = lv3(p, PL_V, i), P = lv3(p, PL_COV, i);
+  LabelTaps lt;
+  if (p.label) label_taps(p, x, y, lt);
+  hold(nl); hold(vl); hold(P);
+  if (p.label) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) hold(lt.l[k]);
+    if (taps_moving(p, lt, M.mov)) return -6;
+  }
   if (nl.x == INVALID) return -1;
-  const f3 vl = lv3(p, PL_V, i);
   const f3 pv = affine(M.T, vl);
   const f3 pp = affine(M.B, pv);
   const float ux = pp.x / pp.z, uy = pp.y / pp.z;
   if (ux < 1 || ux > (float)(p.W - 2) || uy < 1 || uy > (float)(p.H - 2) || isnan(ux) || isnan(uy)) return -2;
   const unsigned rx = (unsigned)ux, ry = (unsigned)uy;
   const size_t ri = rx + (size_t)ry * p.W;
-  const f3 rn = mdl3(p, in, 3, ri);
+  const f3 rn = mdl3(p, in, 3, ri), rv = mdl3(p, in, 0, ri);
+  hold(rn); hold(rv);
   if (rn.x == INVALID) return -3;
-  const f3 diff = mdl3(p, in, 0, ri) - pv;
+  const f3 diff = rv - pv;
   const f3 pn = rotate(M.T, nl);
   if (length(diff) > 0.1f) return -4;
   if (dot(pn, rn) < 0.8f) return -5;
-  const f3 P = lv3(p, PL_COV, i);
   const float sig = rn.x * rn.x * P.x + rn.y * rn.y * P.y + rn.z * rn.z * P.z;
   const float w = sqrtf((float)(1.0 / (double)sig));
   e = w * dot(rn, diff);
@@ -123,9 +169,9 @@ __device__ __forceinline__ int icp_row(const TrackLaunch& p, const TrackInst& in
 template <class MT>
 __device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst& in, const MT& M, int x, int y,
                                            size_t i, float& e, float J[6]) {
-  const f3 nl = lv3(p, PL_N, i);
+  const f3 nl = lv3(p, PL_N, i), c2v = lv3(p, PL_V, i), P = lv3(p, PL_COV, i);
+  hold(nl); hold(c2v); hold(P);
   if (nl.x == INVALID) return -1;
-  const f3 c2v = lv3(p, PL_V, i);
   if (c2v.x == 0 && c2v.y == 0 && c2v.z == 0) return -1;
   const f3 o2v = affine(M.invT, c2v);
   const f3 c1v = affine(M.A, o2v);
@@ -135,6 +181,7 @@ __device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst
   const f3 o2n = rotate(M.invT, nl);
   const f3 wrv = bilin3(in.mvn, (size_t)p.W * p.H, p.W, ux, uy);
   const f3 wrn = bilin3(in.mvn + (size_t)3 * p.W * p.H, (size_t)p.W * p.H, p.W, ux, uy);
+  hold(wrv); hold(wrn);
   if (wrn.x == INVALID) return -3;
   const f3 o1rv = affine(M.B, wrv);
   const f3 o1rn = rotate(M.B, wrn);
@@ -142,7 +189,6 @@ __device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst
   const float residual = dot(o1rn, diff);
   if (length(diff) > 0.1f) return -4;
   if (dot(o2n, o1rn) < 0.8f) return -5;
-  const f3 P = lv3(p, PL_COV, i);
   const float sig = o1rn.x * o1rn.x * P.x + o1rn.y * o1rn.y * P.y + o1rn.z * o1rn.z * P.z;
   const float w = sqrtf((float)(1.0 / (double)sig));
   e = w * residual;
@@ -160,9 +206,11 @@ __device__ __forceinline__ int rgb_row(const TrackLaunch& p, const TrackInst& in
                                        size_t i, float& e, float J[6]) {
   const float s = (float)(1 << p.level);
   const float fx = p.fx / s, fy = p.fy / s, cx = p.cx / s, cy = p.cy / s;
-  const float dr = in.rD[p.rd_off + i];
-  const f3 vr = dr > 0.0f ? dr * rotate(camKinv(fx, fy, cx, cy), mk3((float)x, (float)y, 1.0f)) : mk3(0.f, 0.f, 0.f);
+  const float dr = gld(in.rD + p.rd_off + i);
   const f3 vl = rf3(p, PL_V, i);
+  const float ir = p.ref[(size_t)PL_I * p.lstride + i];
+  hold(dr); hold(vl); hold(ir);
+  const f3 vr = dr > 0.0f ? dr * rotate(camKinv(fx, fy, cx, cy), mk3((float)x, (float)y, 1.0f)) : mk3(0.f, 0.f, 0.f);
   if (vr.z <= 0.f || vr.z == INVALID) return -1;
   if (length(vr - vl) > 0.1f) { if (vl.z > 0.f) return -3; }
   f3 lv, wv;
@@ -181,7 +229,7 @@ __device__ __forceinline__ int rgb_row(const TrackLaunch& p, const TrackInst& in
     if (ux < 1 || ux > (float)(p.w - 2) || uy < 1 || uy > (float)(p.h - 2) || isnan(ux) || isnan(uy)) return -2;
   }
   const float li = bilin(p.live + (size_t)PL_I * p.lstride, p.w, ux, uy);
-  const float res = p.ref[(size_t)PL_I * p.lstride + i] - li;
+  const float res = ir - li;
   const float g_x = bilin(p.live + (size_t)PL_GX * p.lstride, p.w, ux, uy),
               g_y = bilin(p.live + (size_t)PL_GY * p.lstride, p.w, ux, uy);
   const float sc = 1.f / lv.z;
@@ -223,29 +271,15 @@ __device__ __forceinline__ void accum(float* s, int res, float e, const float* J
   s[28] += 1.f;
 }
 
-// Camera re-track outlier mask from the raycast labels of moving objects,
-// resized as cv::resize does for a 0/255 mask (tracking.cpp:1782-1788).
-__device__ __forceinline__ bool dyn_masked(const TrackLaunch& p, int x, int y) {
-  if (!p.label) return false;
-  const int s = 1 << p.level;
-  const int o = (p.level <= 1) ? 0 : (s / 2 - 1);
-  const int n = (p.level == 0) ? 1 : 2;
-  bool m = false;
-  for (int j = 0; j < n; j++)
-    for (int i = 0; i < n; i++) {
-      const int l = p.label[(size_t)(x * s + o + i) + (size_t)(y * s + o + j) * p.W];
-      if (l > 0 && l < p.n_labels && p.moving[l]) m = true;
-    }
-  return m;
-}
 
 // Rows of one instance at one level over the workgroups [0, nactive): FP32
 // per-thread sums, the wave butterfly, and this workgroup's 32 partial sums
 // in `part` (32 floats).
 template <bool ICP, bool RGB, bool OBJ, class MT>
-__device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst& in, const MT& M, int state,
+__device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst& inst, const MT& M, int state,
                                            int wg, int nactive, float* __restrict__ part, float (*red)[32]) {
   const int tid = threadIdx.x;
+  const TrackInst in = inst;   // descriptor held in registers across the row loop
   float s[32];
 #pragma unroll
   for (int k = 0; k < 32; k++) s[k] = 0.f;
@@ -266,7 +300,7 @@ __device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst
     int res;
     if (icp) {
       if (OBJ) res = obj_icp_row(p, in, M, x, y, (size_t)i, e, J);
-      else res = icp_row(p, in, M, x, y, (size_t)i, dyn_masked(p, x, y), e, J);
+      else res = icp_row(p, in, M, x, y, (size_t)i, e, J);
     } else {
       res = rgb_row<OBJ>(p, in, M, x, y, (size_t)i, e, J);
     }
@@ -313,20 +347,12 @@ __device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst
 // Wave 0 runs Gauss-Jordan on [C | I | b], lane j holding column j in
 // registers and taking the pivot column from lane k with v_readlane; when
 // every pivot is positive and tr(C)*tr(C^-1) < 1e6 (so lambda_max/lambda_min
-// < 1e6 and backsub would truncate nothing), x = C^-1 b.  Otherwise lane 0
-// computes the pseudo-inverse from a cyclic Jacobi eigendecomposition (LDS
-// work arrays) with the reference's truncation rule.
-struct SolveLDS {
-  double A[6][6], V[6][6];   // Jacobi work arrays
-  double x[6];
-};
+// < 1e6 and backsub would truncate nothing), x = C^-1 b.  Otherwise the wave
+// computes the pseudo-inverse from a parallel Jacobi eigendecomposition with
+// the reference's truncation rule.
 __device__ __forceinline__ int tri(int i, int j) {   // packed upper-triangular index of the 21 JtWJ sums
   if (i > j) { const int t = i; i = j; j = t; }
   return 6 + 6 * i - (i * (i - 1)) / 2 + (j - i);
-}
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
 }
 // value of v in lane l (l uniform)
 __device__ __forceinline__ float rlf(float v, int l) {
@@ -338,42 +364,102 @@ __device__ __forceinline__ double rld(double v, int l) {
   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
-// Pseudo-inverse solve by cyclic Jacobi (one lane, LDS arrays).
-__device__ void jacobi_pinv(SolveLDS& W, const double* s27) {
-  for (int i = 0; i < 6; i++)
-    for (int j = 0; j < 6; j++) { W.A[i][j] = s27[tri(i, j)]; W.V[i][j] = (i == j) ? 1.0 : 0.0; }
-  for (int sweep = 0; sweep < 60; sweep++) {
-    double off = 0;
-    for (int pp = 0; pp < 6; pp++)
-      for (int q = pp + 1; q < 6; q++) off += W.A[pp][q] * W.A[pp][q];
-    if (off == 0) break;
-    for (int pp = 0; pp < 6; pp++)
-      for (int q = pp + 1; q < 6; q++) {
-        const double apq = W.A[pp][q];
-        if (apq == 0) continue;
-        const double theta = (W.A[q][q] - W.A[pp][pp]) / (2 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
-        const double c = 1 / sqrt(t * t + 1), sn = t * c;
-        for (int k = 0; k < 6; k++) { const double akp = W.A[k][pp], akq = W.A[k][q]; W.A[k][pp] = c * akp - sn * akq; W.A[k][q] = sn * akp + c * akq; }
-        for (int k = 0; k < 6; k++) { const double apk = W.A[pp][k], aqk = W.A[q][k]; W.A[pp][k] = c * apk - sn * aqk; W.A[q][k] = sn * apk + c * aqk; }
-        for (int k = 0; k < 6; k++) { const double vkp = W.V[k][pp], vkq = W.V[k][q]; W.V[k][pp] = c * vkp - sn * vkq; W.V[k][q] = sn * vkp + c * vkq; }
+// Pseudo-inverse x = sum_i [|l_i| * 1e6 > max |l|] (v_i . b / l_i) v_i of the
+// symmetric C by cyclic Jacobi, on the 64 lanes of one wave: lane j (< 6)
+// holds column j of A and of the eigenvector matrix V.  Each step applies
+// three disjoint rotations A <- J^T A J, V <- V J; a round-robin schedule
+// covers the 15 index pairs in 5 steps.  Disjoint rotations leave each
+// other's (p,p), (q,q), (p,q) entries alone, so each angle is the one the
+// serial order would compute at that point.  Column updates exchange a column
+// with the partner lane, row updates are lane-local.  Sweeps stop when the
+// off-diagonal mass is under 1e-30 of the diagonal's (eigenvalues at full
+// FP64 accuracy) or after 40 sweeps.  x is uniform on return.
+__device__ void jacobi_pinv(const double* s27, int lane, double (&x)[6]) {
+  constexpr int RR[5][6] = {{0, 1, 2, 3, 4, 5}, {0, 2, 1, 4, 3, 5}, {0, 3, 1, 5, 2, 4},
+                            {0, 4, 1, 3, 2, 5}, {0, 5, 1, 2, 3, 4}};
+  const int j = lane < 6 ? lane : 5;
+  double a[6], v[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) { a[i] = s27[tri(i, j)]; v[i] = (i == j) ? 1.0 : 0.0; }
+  for (int sweep = 0; sweep < 40; sweep++) {
+    double offl = 0, dl = 0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      if (i < j) offl += a[i] * a[i];
+      if (i == j) dl = a[i] * a[i];
+    }
+    double off = 0, dd = 0;
+#pragma unroll
+    for (int l = 0; l < 6; l++) { off += rld(offl, l); dd += rld(dl, l); }
+    if (off <= 1e-30 * dd) break;
+#pragma unroll
+    for (int st = 0; st < 5; st++) {
+      // this lane's pair: its index k, partner lane and role (p or q)
+      int kk = 0, partner = j;
+      bool isp = false;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const int pp = RR[st][2 * k], qq = RR[st][2 * k + 1];
+        if (j == pp) { kk = k; partner = qq; isp = true; }
+        if (j == qq) { kk = k; partner = pp; isp = false; }
       }
+      double app = 0, aqq = 0, apq = 0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const int pp = RR[st][2 * k], qq = RR[st][2 * k + 1];
+        const double e_pp = rld(a[pp], pp), e_qq = rld(a[qq], qq), e_pq = rld(a[pp], qq);
+        if (kk == k) { app = e_pp; aqq = e_qq; apq = e_pq; }
+      }
+      double c = 1.0, sn = 0.0;
+      if (apq != 0) {
+        const double theta = (aqq - app) / (2 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+        c = 1 / sqrt(t * t + 1);
+        sn = t * c;
+      }
+      // columns: A J and V J
+#pragma unroll
+      for (int i = 0; i < 6; i++) {
+        const double pa = __shfl(a[i], partner), pv = __shfl(v[i], partner);
+        a[i] = isp ? c * a[i] - sn * pa : sn * pa + c * a[i];
+        v[i] = isp ? c * v[i] - sn * pv : sn * pv + c * v[i];
+      }
+      // rows: J^T (A J), every pair's rotation on this lane's column
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const int pp = RR[st][2 * k], qq = RR[st][2 * k + 1];
+        const double ck = rld(c, pp), sk = rld(sn, pp);
+        const double ap = a[pp], aq = a[qq];
+        a[pp] = ck * ap - sk * aq;
+        a[qq] = sk * ap + ck * aq;
+      }
+    }
   }
+  double lam = a[0];
+#pragma unroll
+  for (int i = 1; i < 6; i++) if (i == j) lam = a[i];
   double mx = 0;
-  for (int i = 0; i < 6; i++) mx = fmax(mx, fabs(W.A[i][i]));
-  for (int i = 0; i < 6; i++) W.x[i] = 0;
-  for (int i = 0; i < 6; i++) {
-    if (fabs(W.A[i][i]) * 1e6 <= mx) continue;
-    double pr = 0;
-    for (int k = 0; k < 6; k++) pr += W.V[k][i] * s27[k];
-    pr /= W.A[i][i];
-    for (int k = 0; k < 6; k++) W.x[k] += pr * W.V[k][i];
+#pragma unroll
+  for (int l = 0; l < 6; l++) mx = fmax(mx, fabs(rld(lam, l)));
+  double pr = 0;
+  if (fabs(lam) * 1e6 > mx) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) pr += v[k] * s27[k];
+    pr /= lam;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const double y = pr * v[k];
+    double t = 0;
+#pragma unroll
+    for (int l = 0; l < 6; l++) t += rld(y, l);
+    x[k] = t;
   }
 }
 // s27 = sums[1..27] (b then the packed JtWJ) in LDS; called by the 64 lanes of
 // one wave, x uniform on return.  Returns 1 (Gauss-Jordan, certified) or 2
 // (Jacobi pseudo-inverse).
-__device__ int solve6(SolveLDS& W, const double* s27, int lane, double (&x)[6]) {
+__device__ int solve6(const double* s27, int lane, double (&x)[6]) {
   const int j = lane < 13 ? lane : 12;
   double m[6];
 #pragma unroll
@@ -398,10 +484,7 @@ __device__ int solve6(SolveLDS& W, const double* s27, int lane, double (&x)[6]) 
     for (int i = 0; i < 6; i++) x[i] = rld(m[i], 12);
     return 1;
   }
-  if (lane == 0) jacobi_pinv(W, s27);
-  wave_sync();
-#pragma unroll
-  for (int i = 0; i < 6; i++) x[i] = W.x[i];
+  jacobi_pinv(s27, lane, x);
   return 2;
 }
 // inverse(Matrix4) of mf_math.h, column-parallel over lanes: lane l holds
@@ -529,7 +612,7 @@ __device__ __forceinline__ void load_pre(SolvePre& pre, const GNState& S) {
 // X.pend_iter), from the FP64 sums in LDS, by the 64 lanes of wave 0: lanes
 // 0-3 hold the columns of T and previous_pose, lanes 4-7 those of the
 // inverse; only the `writer` workgroup publishes the outputs (pose, trace).
-__device__ void gn_update(GNIter& X, GNState& S, const SolvePre& pre, SolveLDS& W, const double* sums, int state,
+__device__ void gn_update(GNIter& X, GNState& S, const SolvePre& pre, const double* sums, int state,
                           bool writer, const TrackSync& sy, uint64_t t0) {
   const int lane = threadIdx.x;
   const uint64_t t1 = wall_clock64();
@@ -556,7 +639,7 @@ __device__ void gn_update(GNIter& X, GNState& S, const SolvePre& pre, SolveLDS& 
   } else {
     perr = err;
     double x[6];
-    solver = solve6(W, sums + 1, lane, x);
+    solver = solve6(sums + 1, lane, x);
     const Mat4 upd = se3_exp(x);
     const double nx = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
     if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) {
@@ -633,10 +716,9 @@ __device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, const SolvePre& 
                                          const float* __restrict__ part, int nparts, float4 (&v)[U], bool writer,
                                          const TrackSync& sy) {
   __shared__ double sums[32];
-  __shared__ SolveLDS W;
   const uint64_t t0 = wall_clock64();
   accumulate_sums<NT, U>(part, nparts, v, sums, writer ? S.sums : nullptr);
-  if (threadIdx.x < 64) gn_update(X, S, pre, W, sums, state, writer, sy, t0);
+  if (threadIdx.x < 64) gn_update(X, S, pre, sums, state, writer, sy, t0);
   __syncthreads();
 }
 
@@ -644,6 +726,13 @@ __device__ __forceinline__ void gn_solve(GNIter& X, GNState& S, const SolvePre& 
 // GNState::it[q & 1] and the solve step of iteration q writes it[(q+1) & 1],
 // so no workgroup ever reads a slot that another workgroup of the same
 // launch writes.
+
+// Moving flags of the labels as one mask (wave 0; tid < 64).
+__device__ __forceinline__ void stage_moving(const TrackLaunch& p, Mats& M, int tid) {
+  const bool mv = p.label && tid < p.n_labels && p.moving[tid] != 0;
+  const uint64_t b = __ballot(mv);
+  if (tid == 0) M.mov = b;
+}
 
 // Two-kernel form, rows: GN iteration q of every instance (grid.y =
 // instance), one partial of 32 FP32 sums per workgroup.
@@ -665,6 +754,7 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
     skip = S.off || (p.iter > 0 && X.done_level == p.level);
     if (!skip) { Msh.T = X.T; Msh.invT = X.invT; Msh.A = S.A; Msh.B = S.B; Msh.C = S.C; }
   }
+  if (tid < 64) stage_moving(p, Msh, tid);
   __syncthreads();
   if (skip) return;
   track_rows<ICP, RGB, OBJ>(p, insts[j], Msh, state, blockIdx.x, nblk,
@@ -741,6 +831,7 @@ __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* 
     }
     if (writer) S.it[(q + 1) & 1] = X;
   }
+  if (tid < 64) stage_moving(p, Msh, tid);
   __syncthreads();
   if (skip) return;
   track_rows<ICP, RGB, OBJ>(p, insts[j], Msh, state, blockIdx.x, nblk, part_cur + ((size_t)j * nblk + blockIdx.x) * 32,
@@ -1044,6 +1135,25 @@ void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNS
   launch_track_rows(s, p, inst, 1, st, partial, nblk, 0);
   k_eval_sums<<<1, SOLVE_NT, 0, s>>>(st, state, partial, nblk);
 }
+// The 6x6 solve on caller-given systems (27 doubles each: JtWe, then the
+// packed upper JtWJ), one wave per system; the tracking solve runs the same
+// solve6 on its FP64 sums.
+__global__ __launch_bounds__(64) void k_solve6(const double* __restrict__ s27, int n, double* __restrict__ x,
+                                               int* __restrict__ solver) {
+  const int b = blockIdx.x;
+  if (b >= n) return;
+  double xx[6];
+  const int sv = solve6(s27 + (size_t)b * 27, threadIdx.x, xx);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) x[(size_t)b * 6 + i] = xx[i];
+    solver[b] = sv;
+  }
+}
+void launch_solve6(hipStream_t s, const double* s27, int n, double* x, int* solver) {
+  if (n > 0) k_solve6<<<n, 64, 0, s>>>(s27, n, x, solver);
+}
+
 __global__ void k_pose_from_table(Mat4* dst, const Mat4* table, const int* index) { *dst = table[*index]; }
 __global__ void k_set_int(int* dst, int v) { *dst = v; }
 __global__ void k_copy_mat(Mat4* dst, const Mat4* src) { *dst = *src; }

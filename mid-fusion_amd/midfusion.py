@@ -92,6 +92,7 @@ SIGNATURES = {
     'mf_object_node_count': (C.c_long, [P, C.c_int]),
     'mf_get_trackresult': (C.c_int, [P, C.c_int, i32p]),
     'mf_get_trace': (C.c_long, [P, f64p, C.c_long]),
+    'mf_solve6': (C.c_int, [P, f64p, C.c_long, f64p, i32p]),
     'mf_clear_trace': (None, [P]),
     'mf_get_level': (C.c_long, [P, C.c_int, C.c_int, f32p]),
     'mf_debug_rows': (C.c_int, [P, C.c_int]),
@@ -466,6 +467,15 @@ class Kfusion:
 
     def device_bytes(self):
         return self.L.mf_device_bytes(self.ctx)
+
+    def solve6(self, s27):
+        """Device 6x6 solve of systems s27 [n, 27] (JtWe, packed upper JtWJ):
+        returns (x [n, 6], solver [n]; 1 Gauss-Jordan, 2 pseudo-inverse)."""
+        s27 = np.ascontiguousarray(s27, np.float64).reshape(-1, 27)
+        n = s27.shape[0]
+        x, sv = np.zeros((n, 6)), np.zeros(n, np.int32)
+        self._chk(self.L.mf_solve6(self.ctx, s27, n, x, sv))
+        return x, sv
 
     def residual_eval(self, oid, k, level, T):
         """32 FP64 sums of the tracking residual of the camera (oid < 0, T =

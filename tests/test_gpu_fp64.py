@@ -41,8 +41,18 @@ def big_box():
                            0.512, 128)
 
 
+def sphere():
+    """A uniform-albedo sphere: rotations about its centre change neither its
+    ideal depth nor its intensity."""
+    c = 2.56
+    T0 = np.eye(4)
+    T0[:3, 3] = np.array([c, c, c - 1.5]) + np.array([0.05, 0.05, 1.0]) - 0.256
+    return scene.MovingBox([0.1, 0.1, 0.1], T0, [0.003, -0.002, 0.002, 0.01, 0.012, -0.008], [0.7, 0.6, 0.5],
+                           0.512, 128, shape='ellipsoid')
+
+
 def run(n_frames, moving):
-    mov = [big_box()] if moving else []
+    mov = moving if isinstance(moving, list) else ([big_box()] if moving else [])
     sc = scene.Scene(seed=1, width=W, height=H, K=tuple(K320), moving=mov)
     P0 = sc.camera_pose(0).astype(np.float32)
     gpu = midfusion.Kfusion(W, H, P0, bg_size=5.12, bg_res=256, use_rgb=True)
@@ -119,10 +129,13 @@ def run(n_frames, moving):
         o32.track_objects(K320, f)
         o64.track_objects(K320, f)
         gpu.sync()
+        tg = gpu.trace()
         for i in range(1, 1 + len(mov)):
             Q64 = o64.pose(i)
             r['obj%d_gpu' % i] = pose_err(gpu.getPose(i), Q64)
             r['obj%d_o32' % i] = pose_err(o32.pose(i), Q64)
+            # column 24: 1 certified Gauss-Jordan, 2 Jacobi pseudo-inverse
+            r['obj%d_pinv' % i] = int(np.sum((tg[:, 0] == i) & (tg[:, 24] == 2)))
         for x in (gpu, o32, o64):
             x.integrate(K320, f) if x is not gpu else x.integration(K320, f)
             x.raycast(K320, f) if x is not gpu else x.raycasting(K320, f)
@@ -147,3 +160,18 @@ def test_camera_and_object_pose_vs_fp64_oracle():
         # from the same camera pose the object pose also holds 1e-5 of the
         # FP64 result (the FP32 oracle's own error is recorded beside it)
         assert r['obj1_gpu'][0] < 1e-5 and r['obj1_gpu'][1] < 1e-5, r
+
+
+def test_weakly_observable_object_vs_fp64_oracle():
+    """A uniform sphere's rotations about its centre are observable only
+    through the TSDF's discretisation and the depth noise, so its 6x6 systems
+    are ill-conditioned and both FP32 paths drift from the FP64 result along
+    them.  The object pose must stay as close to the FP64 oracle as the FP32
+    oracle itself is (twice its error, plus 1e-5).  Systems past the solve's
+    certificate (the pseudo-inverse branch) are checked directly in
+    test_gpu_solve."""
+    rep = run(4, moving=[sphere()])
+    for r in rep[1:]:
+        assert r['cam_gpu'][0] < 1e-5 and r['cam_gpu'][1] < 1e-5, r
+        eg, eo = r['obj1_gpu'], r['obj1_o32']
+        assert eg[0] <= 2 * eo[0] + 1e-5 and eg[1] <= 2 * eo[1] + 1e-5, r
