@@ -42,6 +42,7 @@ struct Object {
   float* rgb_err = nullptr;      // level-0 photometric row errors of that pass
   float* mvbf[3] = {nullptr, nullptr, nullptr};   // vertex map of the last raycast before fusion
   float* rD[MAX_LEVELS] = {};    // rendered reference depth pyramid
+  uint8_t* tiles = nullptr;      // model-map tile flags (TrackInst::tiles)
   ObjParams* prm = nullptr;
 };
 
@@ -237,6 +238,7 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   }
   o->codes = dalloc<int8_t>(c, 2 * c->N0);
   o->rgb_err = dalloc<float>(c, c->N0);
+  o->tiles = dalloc<uint8_t>(c, (size_t)((c->W + MAP_TILE - 1) / MAP_TILE) * ((c->H + MAP_TILE - 1) / MAP_TILE));
   {
     float* b = dalloc<float>(c, 3 * c->N0);
     for (int k = 0; k < 3; k++) o->mvbf[k] = b ? b + k * c->N0 : nullptr;
@@ -358,13 +360,13 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
 }
 
 void set_inst(mf_ctx* c, int slot, float* const mv[3], float* const mn[3], float* const rD[MAX_LEVELS],
-              int8_t* codes, float* rgb_err, int state) {
+              int8_t* codes, float* rgb_err, int state, uint8_t* tiles = nullptr) {
   TrackInst ti;
   memset(&ti, 0, sizeof ti);
   ti.mvn = mv[0];      // planes: mv[0..2], mn[0..2] back to back
   ti.rD = rD[0];
   (void)mn;
-  ti.codes = codes; ti.rgb_err = rgb_err; ti.state = state;
+  ti.codes = codes; ti.rgb_err = rgb_err; ti.state = state; ti.tiles = tiles;
   // descriptors only change with the object set: upload on change, ordered
   // after every pending launch that may still read the old table
   if (memcmp(&ti, &c->inst_host[slot], sizeof ti) != 0) {
@@ -938,7 +940,7 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
     for (size_t j = 0; j < mov.size(); j++) {
       Object* o = mov[j];
       ids[j] = 3 + o->id;
-      set_inst(c, 1 + (int)j, o->mv, o->mn, o->rD, o->codes, o->rgb_err, 1 + (int)j);
+      set_inst(c, 1 + (int)j, o->mv, o->mn, o->rD, o->codes, o->rgb_err, 1 + (int)j, o->tiles);
     }
     if (!mov.empty()) set_ids(c, ids);
     std::vector<uint64_t> key = base_key(c, 2, k);
@@ -950,6 +952,9 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
         // trackEachObject for every moving object when the last raycast shows
         // more than one instance (kfusion.cpp:292, 395-414)
         { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, 1, (int)mov.size(), c->obj_ids, c->mats, c->mats + 0, c->mats + 1, 1, &md->gate); }
+        { Timed t(c, KC_GNSMALL);
+          if (c->cfg.batch_objects) launch_model_tiles(s, c->inst + 1, (int)mov.size(), c->W, c->H);
+          else for (size_t j = 0; j < mov.size(); j++) launch_model_tiles(s, c->inst + 1 + j, 1, c->W, c->H); }
         if (c->cfg.use_rgb) {
           if (c->cfg.batch_objects) render_reference(c, 1, (int)mov.size());
           else for (size_t j = 0; j < mov.size(); j++) render_reference(c, 1 + (int)j, 1);

@@ -77,9 +77,12 @@ struct TrackInst {
   const float* rD;      // rendered reference depth pyramid, levels back to back
   int8_t* codes;        // level-0 result codes out [2*W*H] (ICP, RGB) or null
   float* rgb_err;       // level-0 photometric row errors out [W*H] or null
+  uint8_t* tiles;       // objects: per 8x8 pixel tile of the model maps, 0 when every
+                        // pixel is a raycast miss (vertex 0, normal (INVALID, 0, 0)); or null
   int state;            // index into GNState array
   int pad;
 };
+static constexpr int MAP_TILE = 8;   // model-map tile side of TrackInst::tiles
 
 // Pyramid buffers of one frame (live or reference) at one level: 13 planes
 // of w*h floats in one allocation, in the order of the PL_* indices.

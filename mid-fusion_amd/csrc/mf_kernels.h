@@ -4,6 +4,21 @@
 
 namespace mf {
 
+// Load through the global address space.  Pointers read out of a descriptor
+// are generic to the compiler, which would otherwise issue flat loads (which
+// count against LDS waits as well).
+template <class T>
+__device__ __forceinline__ T gload(const T* a) {
+  return *(const __attribute__((address_space(1))) T*)a;
+}
+// base[idx] for a wave-uniform base and a 32-bit index: addressed as scalar
+// base + 32-bit vector byte offset, half the vector registers of a 64-bit
+// per-lane address.
+template <class T>
+__device__ __forceinline__ T ldx(const T* base, uint32_t idx) {
+  return gload((const T*)((const char*)base + (size_t)(idx * (uint32_t)sizeof(T))));
+}
+
 // preprocessing (mf_pre.hip)
 void launch_convert(hipStream_t s, const uint16_t* din, const uint8_t* rgb, int iw, int ratio, int ow, int oh,
                     float* depth, float* grey, float* r, float* g, float* b);
@@ -58,6 +73,8 @@ void launch_gn_fused(hipStream_t s, const TrackLaunch& p, const TrackInst* inst,
                      const TrackSync& sy);
 // Residual sums of one instance (`inst` points at its descriptor) at pose T:
 // GN state `state` gets T, then rows and the FP64 sums (into GNState::sums).
+// TrackInst::tiles of the n instances at insts (every one with tiles set).
+void launch_model_tiles(hipStream_t s, const TrackInst* insts, int n, int W, int H);
 // The tracking 6x6 solve on n systems (27 doubles each) into x[6n] and
 // solver[n] (1 certified Gauss-Jordan, 2 Jacobi pseudo-inverse).
 void launch_solve6(hipStream_t s, const double* s27, int n, double* x, int* solver);

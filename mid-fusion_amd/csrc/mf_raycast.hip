@@ -16,7 +16,7 @@ namespace mf {
 
 __device__ __forceinline__ int32_t vslot(const VolDev& v, int x, int y, int z) {
   if (x < 0 || y < 0 || z < 0 || x >= v.res || y >= v.res || z >= v.res) return -1;
-  return v.slot[morton3((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3)];
+  return gload(v.slot + morton3((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3));
 }
 
 // One-entry block cache: consecutive voxel reads of a ray (march steps, the
@@ -33,25 +33,29 @@ __device__ __forceinline__ int32_t cslot(const VolDev& v, VCache& c, int x, int 
   const int key = ((x >> 3) << 20) | ((y >> 3) << 10) | (z >> 3);
   if (key != c.key) {
     c.key = key;
-    c.slot = v.slot[morton3((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3)];
+    c.slot = gload(v.slot + morton3((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3));
   }
   return c.slot;
 }
 __device__ __forceinline__ size_t vox(int32_t s, int x, int y, int z) {
   return (size_t)s * BLOCK_VOX + (x & 7) + 8 * (y & 7) + 64 * (z & 7);
 }
-// missing voxels read as voxel_traits<SDF>::empty(): tsdf 1, fg 0
-__device__ __forceinline__ float tsdf_at(const VolDev& v, VCache& c, int x, int y, int z) {
-  const int32_t s = cslot(v, c, x, y, z);
-  return s < 0 ? 1.f : v.tsdf[vox(s, x, y, z)];
+// Field value at voxel (x, y, z) of block slot s; missing voxels read as
+// voxel_traits<SDF>::empty(): tsdf 1, fg 0.
+template <bool FG>
+__device__ __forceinline__ float field_at(const VolDev& v, int32_t s, int x, int y, int z) {
+  if (s < 0) return FG ? 0.f : 1.f;
+  return FG ? (float)gload(v.fg + vox(s, x, y, z)) : gload(v.tsdf + vox(s, x, y, z));
 }
-__device__ __forceinline__ float fg_at(const VolDev& v, VCache& c, int x, int y, int z) {
-  const int32_t s = cslot(v, c, x, y, z);
-  return s < 0 ? 0.f : (float)v.fg[vox(s, x, y, z)];
+__device__ __forceinline__ float tsdf_at(const VolDev& v, VCache& c, int x, int y, int z) {
+  return field_at<false>(v, cslot(v, c, x, y, z), x, y, z);
 }
 
 // Octree::interp (octree.hpp:479-501): voxel values at integer corners, base
-// clamped at 0, fraction of the unclamped position.
+// clamped at 0, fraction of the unclamped position.  The 8 corner loads are
+// issued together: when the corners share one 8^3 block (no corner
+// coordinate crosses a multiple of 8) after one block-table probe, otherwise
+// after 8 independent probes.
 template <bool FG>
 __device__ __forceinline__ float interp(const VolDev& v, VCache& c, f3 pos) {
   const float flx = floorf(pos.x), fly = floorf(pos.y), flz = floorf(pos.z);
@@ -59,15 +63,27 @@ __device__ __forceinline__ float interp(const VolDev& v, VCache& c, f3 pos) {
   const float fx = pos.x - flx, fy = pos.y - fly, fz = pos.z - flz;
   const int lx = max(bx, 0), ly = max(by, 0), lz = max(bz, 0);
   float p[8];
+  if ((lx & 7) != 7 && (ly & 7) != 7 && (lz & 7) != 7) {
+    // res is a multiple of 8, so the +1 corners are inside exactly when the
+    // base corner is
+    const int32_t sl = cslot(v, c, lx, ly, lz);
 #pragma unroll
-  for (int i = 0; i < 8; i++)
-    p[i] = FG ? fg_at(v, c, lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1))
-              : tsdf_at(v, c, lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1));
+    for (int i = 0; i < 8; i++) p[i] = field_at<FG>(v, sl, lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1));
+  } else {
+    int32_t sl[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) sl[i] = vslot(v, lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1));
+#pragma unroll
+    for (int i = 0; i < 8; i++) p[i] = field_at<FG>(v, sl[i], lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1));
+  }
   return (((p[0] * (1 - fx) + p[1] * fx) * (1 - fy) + (p[2] * (1 - fx) + p[3] * fx) * fy) * (1 - fz) +
           ((p[4] * (1 - fx) + p[5] * fx) * (1 - fy) + (p[6] * (1 - fx) + p[7] * fx) * fy) * fz);
 }
 
-// Octree::grad(pos, select) (octree.hpp:593-680), tsdf field.
+// Octree::grad(pos, select) (octree.hpp:593-680), tsdf field: trilinearly
+// weighted central differences over the clamped 4^3 stencil.  (Issuing the
+// stencil's loads in batches per axis measured no faster: the raycast is
+// bound by its divergent traversal, and the batch cost registers.)
 __device__ f3 grad(const VolDev& v, VCache& cc, f3 pos) {
   const float fl[3] = {floorf(pos.x), floorf(pos.y), floorf(pos.z)};
   const int b[3] = {(int)fl[0], (int)fl[1], (int)fl[2]};
@@ -100,11 +116,16 @@ __device__ f3 grad(const VolDev& v, VCache& cc, f3 pos) {
   return mk3(g[0] * s * 0.5f, g[1] * s * 0.5f, g[2] * s * 0.5f);
 }
 
+// Byte offset of internal-node level lv >= 1 in VolDev::nodes: levels are
+// stored back to back from level 1, level l holding 8^l occupancy bytes, so
+// the offset is sum_{k=1}^{lv-1} 8^k = (8^lv - 8) / 7.
+__device__ __forceinline__ uint32_t node_offset(int lv) { return ((1u << (3 * lv)) - 8u) / 7u; }
+
 // Child occupancy of an octree node: level `lv` (1 = children of the root),
 // Morton prefix `pre` of the child.
 __device__ __forceinline__ bool child_exists(const VolDev& v, int lv, uint32_t pre) {
-  if (lv == v.leaf_level) return v.slot[pre] >= 0;
-  return v.nodes[v.node_off[lv] + pre] != 0;
+  if (lv == v.leaf_level) return gload(v.slot + pre) >= 0;
+  return gload(v.nodes + node_offset(lv) + pre) != 0;
 }
 
 // ray_iterator::next() first call (octree.hpp:923-1093): entry distance, in
@@ -288,7 +309,7 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
   for (int oi = 0; oi < nobj; oi++) {
     const RayObj& O = objs[oi];
     if (!all && !in_view[O.label]) continue;
-    const VolDev& v = O.v;
+    const VolDev v = O.v;
     const ObjParams& P = *O.prm;
     const float step = v.size / (float)v.res;
     const float largestep = step * 8;
@@ -325,7 +346,7 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
     // vertex and normal of the last take (the values a later take would
     // overwrite are never needed, so the gradient runs once per pixel)
     const RayObj& O = objs[win];
-    const VolDev& v = O.v;
+    const VolDev v = O.v;
     const ObjParams& P = *O.prm;
     out_l = O.label;
     out_v = affine(P.pose, win_hit);
@@ -389,7 +410,7 @@ __device__ uint32_t shade(const RayObj* __restrict__ objs, int nobj, f3 w_origin
   for (int oi = 0; oi < nobj; oi++) {
     const RayObj& O = objs[oi];
     if (w_label != O.label) continue;
-    const VolDev& v = O.v;
+    const VolDev v = O.v;
     const ObjParams& P = *O.prm;
     const f3 o_hit = affine(P.inv_pose, w_hit);
     const f3 diff = normalize(w_origin - w_hit);
@@ -447,7 +468,7 @@ __global__ __launch_bounds__(RT) void k_render_volume_ray(const RayObj* __restri
   unsigned dummy = 0;
   for (int oi = 0; oi < nobj; oi++) {
     const RayObj& O = objs[oi];
-    const VolDev& v = O.v;
+    const VolDev v = O.v;
     const ObjParams& P = *O.prm;
     const float step = v.size / (float)v.res;
     const f3 oo = affine(P.inv_pose, w_origin);

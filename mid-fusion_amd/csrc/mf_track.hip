@@ -21,23 +21,17 @@ static constexpr int NW = TB / 64;
 // Level planes (live / reference pyramid level, stride lstride) and the
 // instance's model vertex (planes 0-2) and normal (planes 3-5) maps.
 __device__ __forceinline__ f3 lv3(const TrackLaunch& p, int pl, size_t i) {
-  const float* b = p.live + (size_t)pl * p.lstride + i;
-  return mk3(b[0], b[p.lstride], b[2 * (size_t)p.lstride]);
+  const float* b = p.live + (size_t)pl * p.lstride;
+  return mk3(ldx(b, (uint32_t)i), ldx(b + p.lstride, (uint32_t)i), ldx(b + 2 * (size_t)p.lstride, (uint32_t)i));
 }
 __device__ __forceinline__ f3 rf3(const TrackLaunch& p, int pl, size_t i) {
-  const float* b = p.ref + (size_t)pl * p.lstride + i;
-  return mk3(b[0], b[p.lstride], b[2 * (size_t)p.lstride]);
-}
-// Load from global memory.  Pointers read out of a descriptor are generic to
-// the compiler, which would otherwise issue flat loads (counted against LDS
-// waits as well).
-__device__ __forceinline__ float gld(const float* a) {
-  return *(const __attribute__((address_space(1))) float*)a;
+  const float* b = p.ref + (size_t)pl * p.lstride;
+  return mk3(ldx(b, (uint32_t)i), ldx(b + p.lstride, (uint32_t)i), ldx(b + 2 * (size_t)p.lstride, (uint32_t)i));
 }
 __device__ __forceinline__ f3 mdl3(const TrackLaunch& p, const TrackInst& in, int pl, size_t i) {
   const size_t n = (size_t)p.W * p.H;
-  const float* b = in.mvn + pl * n + i;
-  return mk3(gld(b), gld(b + n), gld(b + 2 * n));
+  const float* b = in.mvn + pl * n;
+  return mk3(ldx(b, (uint32_t)i), ldx(b + n, (uint32_t)i), ldx(b + 2 * n, (uint32_t)i));
 }
 
 // bilinear_interp<T> (utils/math_utils.h:149-166)
@@ -45,23 +39,44 @@ __device__ __forceinline__ float bilin(const float* __restrict__ f, int w, float
   const float fpx = floorf(px), fpy = floorf(py);
   const int bx = (int)fpx, by = (int)fpy;
   const float fx = px - fpx, fy = py - fpy;
-  const size_t i0 = (size_t)bx + (size_t)by * w;
-  const float a = f[i0], b = f[i0 + 1], c = f[i0 + w], d = f[i0 + w + 1];
+  const uint32_t i0 = (uint32_t)bx + (uint32_t)by * (uint32_t)w;
+  const float a = ldx(f, i0), b = ldx(f, i0 + 1), c = ldx(f, i0 + w), d = ldx(f, i0 + w + 1);
   return a * (1.f - fx) * (1.f - fy) + b * (1.f - fy) * fx + c * (1.f - fx) * fy + d * fx * fy;
+}
+__device__ __forceinline__ float bmix(float a, float b, float c, float d, float w0, float w1, float fx, float fy) {
+  return a * w0 * w1 + b * w1 * fx + c * w0 * fy + d * fx * fy;
 }
 __device__ __forceinline__ f3 bilin3(const float* f0, size_t stride, int w, float px, float py) {
   const float fpx = floorf(px), fpy = floorf(py);
   const int bx = (int)fpx, by = (int)fpy;
   const float fx = px - fpx, fy = py - fpy;
-  const size_t i0 = (size_t)bx + (size_t)by * w;
+  const uint32_t i0 = (uint32_t)bx + (uint32_t)by * (uint32_t)w;
   const float w0 = 1.f - fx, w1 = 1.f - fy;
   f3 r;
   float* rr = &r.x;
   for (int k = 0; k < 3; k++) {
     const float* p = f0 + k * stride;
-    rr[k] = gld(p + i0) * w0 * w1 + gld(p + i0 + 1) * w1 * fx + gld(p + i0 + w) * w0 * fy + gld(p + i0 + w + 1) * fx * fy;
+    rr[k] = bmix(ldx(p, i0), ldx(p, i0 + 1), ldx(p, i0 + w), ldx(p, i0 + w + 1), w0, w1, fx, fy);
   }
   return r;
+}
+// bilin3 over taps that all hold the same value c per component (the same
+// arithmetic, so the same result bits as loading them)
+__device__ __forceinline__ f3 bilin3_const(f3 c, float px, float py) {
+  const float fpx = floorf(px), fpy = floorf(py);
+  const float fx = px - fpx, fy = py - fpy;
+  const float w0 = 1.f - fx, w1 = 1.f - fy;
+  return mk3(bmix(c.x, c.x, c.x, c.x, w0, w1, fx, fy), bmix(c.y, c.y, c.y, c.y, w0, w1, fx, fy),
+             bmix(c.z, c.z, c.z, c.z, w0, w1, fx, fy));
+}
+// True when the 2x2 bilinear taps at (px, py) (inside [1, W-2] x [1, H-2])
+// all lie in model-map tiles holding only raycast misses.
+__device__ __forceinline__ bool taps_all_miss(const uint8_t* tiles, int W, float px, float py) {
+  const int bx = (int)floorf(px), by = (int)floorf(py);
+  const int ntx = (W + MAP_TILE - 1) / MAP_TILE;
+  const int tx0 = bx / MAP_TILE, tx1 = (bx + 1) / MAP_TILE, ty0 = by / MAP_TILE, ty1 = (by + 1) / MAP_TILE;
+  return (ldx(tiles, (uint32_t)(ty0 * ntx + tx0)) | ldx(tiles, (uint32_t)(ty0 * ntx + tx1)) |
+          ldx(tiles, (uint32_t)(ty1 * ntx + tx0)) | ldx(tiles, (uint32_t)(ty1 * ntx + tx1))) == 0;
 }
 
 // calc_robust_weight / calc_robust_residual (tracking.cpp:1474-1532).  The
@@ -117,7 +132,7 @@ __device__ __forceinline__ int label_taps(const TrackLaunch& p, int x, int y, La
   for (int j = 0; j < 2; j++)
 #pragma unroll
     for (int i = 0; i < 2; i++)
-      t.l[2 * j + i] = (i < n && j < n) ? p.label[(size_t)(x * s + o + i) + (size_t)(y * s + o + j) * p.W] : 0;
+      t.l[2 * j + i] = (i < n && j < n) ? ldx(p.label, (uint32_t)(x * s + o + i) + (uint32_t)(y * s + o + j) * (uint32_t)p.W) : 0;
   return n;
 }
 __device__ __forceinline__ bool taps_moving(const TrackLaunch& p, const LabelTaps& t, uint64_t mov) {
@@ -179,9 +194,17 @@ __device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst
   const float ux = pj.x / pj.z, uy = pj.y / pj.z;
   if (ux < 1 || ux > (float)(p.W - 2) || uy < 1 || uy > (float)(p.H - 2) || isnan(ux) || isnan(uy)) return -2;
   const f3 o2n = rotate(M.invT, nl);
-  const f3 wrv = bilin3(in.mvn, (size_t)p.W * p.H, p.W, ux, uy);
-  const f3 wrn = bilin3(in.mvn + (size_t)3 * p.W * p.H, (size_t)p.W * p.H, p.W, ux, uy);
-  hold(wrv); hold(wrn);
+  f3 wrv, wrn;
+  if (in.tiles && taps_all_miss(in.tiles, p.W, ux, uy)) {
+    // every tap is a raycast miss of this object: vertex 0, normal
+    // (INVALID, 0, 0), so the interpolation needs no loads
+    wrv = bilin3_const(mk3(0.f, 0.f, 0.f), ux, uy);
+    wrn = bilin3_const(mk3((float)INVALID, 0.f, 0.f), ux, uy);
+  } else {
+    wrv = bilin3(in.mvn, (size_t)p.W * p.H, p.W, ux, uy);
+    wrn = bilin3(in.mvn + (size_t)3 * p.W * p.H, (size_t)p.W * p.H, p.W, ux, uy);
+    hold(wrv); hold(wrn);
+  }
   if (wrn.x == INVALID) return -3;
   const f3 o1rv = affine(M.B, wrv);
   const f3 o1rn = rotate(M.B, wrn);
@@ -206,7 +229,10 @@ __device__ __forceinline__ int rgb_row(const TrackLaunch& p, const TrackInst& in
                                        size_t i, float& e, float J[6]) {
   const float s = (float)(1 << p.level);
   const float fx = p.fx / s, fy = p.fy / s, cx = p.cx / s, cy = p.cy / s;
-  const float dr = gld(in.rD + p.rd_off + i);
+  const float dr = ldx(in.rD + p.rd_off, (uint32_t)i);
+  // an object's rendered depth is 0 outside its silhouette, where the row
+  // ends at the validity test below: test it before the other loads
+  if (OBJ && !(dr > 0.0f)) return -1;
   const f3 vl = rf3(p, PL_V, i);
   const float ir = p.ref[(size_t)PL_I * p.lstride + i];
   hold(dr); hold(vl); hold(ir);
@@ -1135,6 +1161,30 @@ void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNS
   launch_track_rows(s, p, inst, 1, st, partial, nblk, 0);
   k_eval_sums<<<1, SOLVE_NT, 0, s>>>(st, state, partial, nblk);
 }
+// TrackInst::tiles of instances [0, n): one wave per 8x8 tile of the model
+// maps, flag 1 when any pixel is not a raycast miss.
+__global__ __launch_bounds__(64) void k_model_tiles(const TrackInst* __restrict__ insts, int W, int H) {
+  const TrackInst& in = insts[blockIdx.y];
+  const int ntx = (W + MAP_TILE - 1) / MAP_TILE;
+  const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  const int x = tx * MAP_TILE + (threadIdx.x & 7), y = ty * MAP_TILE + (threadIdx.x >> 3);
+  bool hit = false;
+  if (x < W && y < H) {
+    const size_t n = (size_t)W * H, i = (size_t)x + (size_t)y * W;
+    const float* m = in.mvn;
+    const unsigned inv = __float_as_uint((float)INVALID);
+    hit = __float_as_uint(m[i]) != 0u || __float_as_uint(m[n + i]) != 0u || __float_as_uint(m[2 * n + i]) != 0u ||
+          __float_as_uint(m[3 * n + i]) != inv || __float_as_uint(m[4 * n + i]) != 0u ||
+          __float_as_uint(m[5 * n + i]) != 0u;
+  }
+  const uint64_t b = __ballot(hit);
+  if (threadIdx.x == 0) in.tiles[blockIdx.x] = b != 0 ? 1 : 0;
+}
+void launch_model_tiles(hipStream_t s, const TrackInst* insts, int n, int W, int H) {
+  const int nt = ((W + MAP_TILE - 1) / MAP_TILE) * ((H + MAP_TILE - 1) / MAP_TILE);
+  if (n > 0) k_model_tiles<<<dim3(nt, n), 64, 0, s>>>(insts, W, H);
+}
+
 // The 6x6 solve on caller-given systems (27 doubles each: JtWe, then the
 // packed upper JtWJ), one wave per system; the tracking solve runs the same
 // solve6 on its FP64 sums.
