@@ -129,6 +129,9 @@ def main():
                     help='1 per pyramid level: one fused launch per GN iteration (default: library default)')
     ap.add_argument('--track-rpt', type=str, default='', metavar='R0,R1,R2',
                     help='tracking rows per thread per pyramid level (default: library default)')
+    ap.add_argument('--obj-ppt', type=str, default='', metavar='P0,P1,P2',
+                    help='moving objects\' rows per pyramid level: pixels per thread of the shared pixel grid '
+                         '(1, 2) or 0 for one row grid per object (default: library default)')
     ap.add_argument('--copy-ref', type=int, default=0, metavar='MB',
                     help='only launch a plain copy kernel of MB megabytes (traffic-counter calibration)')
     args = ap.parse_args()
@@ -159,6 +162,8 @@ def main():
         g.set_tuning(g.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl, int(v))
     for lvl, v in enumerate(x for x in args.track_fused.split(',') if x):
         g.set_tuning(g.TUNE_TRACK_FUSED_L0 + lvl, int(v))
+    for lvl, v in enumerate(x for x in args.obj_ppt.split(',') if x):
+        g.set_tuning(g.TUNE_OBJ_PIXELS_PER_THREAD_L0 + lvl, int(v))
     for i, m in enumerate(movers):
         g.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
         g.set_moving(i + 1, True)

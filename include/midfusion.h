@@ -230,7 +230,11 @@ long mf_device_bytes(mf_ctx* ctx);
 /*   MF_TUNE_TRACK_FUSED_L0 + l: 1 runs each GN iteration at level l as one
  *   launch whose workgroups first finish the previous iteration themselves;
  *   0 (default) as a row launch and a one-workgroup solve launch. */
-enum { MF_TUNE_TRACK_ROWS_PER_THREAD_L0 = 0, MF_TUNE_TRACK_FUSED_L0 = 8 };
+/*   MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 + l: moving objects' rows at level l in
+ *   one grid over the pixels, each thread loading its pixels' frame inputs
+ *   once for every object, with 1 or 2 pixels per thread; 0 (default) gives
+ *   each object its own row grid in the same launch. */
+enum { MF_TUNE_TRACK_ROWS_PER_THREAD_L0 = 0, MF_TUNE_TRACK_FUSED_L0 = 8, MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 = 16 };
 int mf_set_tuning(mf_ctx* ctx, int knob, int value);
 int mf_get_tuning(mf_ctx* ctx, int knob, int* value);
 

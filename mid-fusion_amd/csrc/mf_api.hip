@@ -74,6 +74,10 @@ struct mf_ctx {
   int nblk0 = 0;
   int track_rpt[MAX_LEVELS] = {4, 1, 1, 1};   // tracking rows per thread, per pyramid level
   int track_fused[MAX_LEVELS] = {0, 0, 0, 0};   // 1: one fused launch per iteration at that level
+  // objects' rows in one grid over the pixels with this many pixels per
+  // thread (1, 2), or 0: one row grid per object (grid.y), per pyramid level.
+  // 0 everywhere measured fastest on MI355X at 1, 8 and 16 objects.
+  int track_obj_ppt[MAX_LEVELS] = {0, 0, 0, 0};
   uint8_t* moving = nullptr;     // [MAX_INST] label-indexed flags
   TraceRec* trace = nullptr;
   int* trace_n = nullptr;
@@ -336,8 +340,10 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
     p = base_launch(c, k, l, object);
     p.label = label_mask;
     if (mask_labels) p.moving = mask_labels;
-    const int nblk = track_blocks(p.w * p.h * kinds, c->track_rpt[l]);
     const bool fused = c->track_fused[l] != 0;
+    const bool objs_grid = object && c->cfg.batch_objects && !fused && c->track_obj_ppt[l] > 0;
+    const int nblk = objs_grid ? track_obj_blocks(p.w * p.h, c->track_obj_ppt[l])
+                               : track_blocks(p.w * p.h * kinds, c->track_rpt[l]);
     if (!fused) finish();
     for (int it = 0; it < c->cfg.iters[l]; ++it, ++q) {
       p.iter = it;
@@ -348,7 +354,11 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
         pending = true;
       } else {
         { Timed t2(c, object ? KC_TRACK_OBJ : KC_TRACK);
-          launch_track_rows(c->stream, p, c->inst + first_inst, n, c->gn, c->partial + (q & 1) * half, nblk, q); }
+          if (objs_grid)
+            launch_track_rows_objs(c->stream, p, c->inst + first_inst, n, c->gn, c->partial + (q & 1) * half, nblk,
+                                   c->track_obj_ppt[l], q);
+          else
+            launch_track_rows(c->stream, p, c->inst + first_inst, n, c->gn, c->partial + (q & 1) * half, nblk, q); }
         { Timed t2(c, KC_SOLVE);
           launch_track_solve(c->stream, p, c->inst + first_inst, n, c->gn, c->partial + (q & 1) * half, nblk, q, sy); }
       }
@@ -1477,6 +1487,11 @@ long mf_device_bytes(mf_ctx* c) { return c->bytes; }
 
 int mf_set_tuning(mf_ctx* c, int knob, int value) {
   if (!c) return MF_E_ARG;
+  if (knob >= MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 && knob < MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 + MAX_LEVELS) {
+    if (value < 0 || value > 2) return fail(c, MF_E_ARG, "object pixels per thread must be 0, 1 or 2");
+    c->track_obj_ppt[knob - MF_TUNE_OBJ_PIXELS_PER_THREAD_L0] = value;
+    return 0;
+  }
   if (knob >= MF_TUNE_TRACK_FUSED_L0 && knob < MF_TUNE_TRACK_FUSED_L0 + MAX_LEVELS) {
     c->track_fused[knob - MF_TUNE_TRACK_FUSED_L0] = value ? 1 : 0;
     return 0;
@@ -1490,6 +1505,10 @@ int mf_set_tuning(mf_ctx* c, int knob, int value) {
 }
 int mf_get_tuning(mf_ctx* c, int knob, int* value) {
   if (!c || !value) return MF_E_ARG;
+  if (knob >= MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 && knob < MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 + MAX_LEVELS) {
+    *value = c->track_obj_ppt[knob - MF_TUNE_OBJ_PIXELS_PER_THREAD_L0];
+    return 0;
+  }
   if (knob >= MF_TUNE_TRACK_FUSED_L0 && knob < MF_TUNE_TRACK_FUSED_L0 + MAX_LEVELS) {
     *value = c->track_fused[knob - MF_TUNE_TRACK_FUSED_L0];
     return 0;

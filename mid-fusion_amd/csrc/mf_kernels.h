@@ -63,6 +63,12 @@ int track_blocks(int nrows, int rows_per_thread);
 // (nblk per instance), then one solve workgroup per instance.
 void launch_track_rows(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
                        float* partial, int nblk, int q);
+// GN iteration q's rows of `ninst` objects in one grid over the pixels
+// (track_obj_blocks(w*h, pixels_per_thread) workgroups, each writing one
+// partial per object); pixels_per_thread is 1 or 2.
+int track_obj_blocks(int npix, int pixels_per_thread);
+void launch_track_rows_objs(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                            float* partial, int nblk, int pixels_per_thread, int q);
 void launch_track_solve(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
                         const float* partial, int nblk, int q, const TrackSync& sy);
 // GN iteration q, fused form: finishes iteration q-1 (part_prev, nblk_prev

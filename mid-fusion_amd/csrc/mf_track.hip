@@ -180,12 +180,11 @@ __device__ __forceinline__ int icp_row(const TrackLaunch& p, const TrackInst& in
   return 1;
 }
 
-// Object ICP row, track_Obj_ICP_Kernel (tracking.cpp:392-486)
+// Object ICP row, track_Obj_ICP_Kernel (tracking.cpp:392-486), from the live
+// pixel's normal, vertex and covariance.
 template <class MT>
-__device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst& in, const MT& M, int x, int y,
-                                           size_t i, float& e, float J[6]) {
-  const f3 nl = lv3(p, PL_N, i), c2v = lv3(p, PL_V, i), P = lv3(p, PL_COV, i);
-  hold(nl); hold(c2v); hold(P);
+__device__ __forceinline__ int obj_icp_core(const TrackLaunch& p, const TrackInst& in, const MT& M, const f3& nl,
+                                            const f3& c2v, const f3& P, float& e, float J[6]) {
   if (nl.x == INVALID) return -1;
   if (c2v.x == 0 && c2v.y == 0 && c2v.z == 0) return -1;
   const f3 o2v = affine(M.invT, c2v);
@@ -221,21 +220,22 @@ __device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst
   J[0] = j0.x; J[1] = j0.y; J[2] = j0.z; J[3] = j1.x; J[4] = j1.y; J[5] = j1.z;
   return 1;
 }
+template <class MT>
+__device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst& in, const MT& M, int x, int y,
+                                           size_t i, float& e, float J[6]) {
+  const f3 nl = lv3(p, PL_N, i), c2v = lv3(p, PL_V, i), P = lv3(p, PL_COV, i);
+  hold(nl); hold(c2v); hold(P);
+  return obj_icp_core(p, in, M, nl, c2v, P, e, J);
+}
 
 // Photometric rows: trackRGB (tracking.cpp:1023-1111) for the camera,
-// track_obj_RGB_kernel (tracking.cpp:491-578) for objects.
+// track_obj_RGB_kernel (tracking.cpp:491-578) for objects, from the reference
+// pixel's rendered depth dr, live vertex vl and intensity ir.
 template <bool OBJ, class MT>
-__device__ __forceinline__ int rgb_row(const TrackLaunch& p, const TrackInst& in, const MT& M, int x, int y,
-                                       size_t i, float& e, float J[6]) {
+__device__ __forceinline__ int rgb_core(const TrackLaunch& p, const MT& M, int x, int y, float dr, const f3& vl,
+                                        float ir, float& e, float J[6]) {
   const float s = (float)(1 << p.level);
   const float fx = p.fx / s, fy = p.fy / s, cx = p.cx / s, cy = p.cy / s;
-  const float dr = ldx(in.rD + p.rd_off, (uint32_t)i);
-  // an object's rendered depth is 0 outside its silhouette, where the row
-  // ends at the validity test below: test it before the other loads
-  if (OBJ && !(dr > 0.0f)) return -1;
-  const f3 vl = rf3(p, PL_V, i);
-  const float ir = p.ref[(size_t)PL_I * p.lstride + i];
-  hold(dr); hold(vl); hold(ir);
   const f3 vr = dr > 0.0f ? dr * rotate(camKinv(fx, fy, cx, cy), mk3((float)x, (float)y, 1.0f)) : mk3(0.f, 0.f, 0.f);
   if (vr.z <= 0.f || vr.z == INVALID) return -1;
   if (length(vr - vl) > 0.1f) { if (vl.z > 0.f) return -3; }
@@ -273,6 +273,18 @@ __device__ __forceinline__ int rgb_row(const TrackLaunch& p, const TrackInst& in
   e = ic * res;
   return 1;
 }
+template <bool OBJ, class MT>
+__device__ __forceinline__ int rgb_row(const TrackLaunch& p, const TrackInst& in, const MT& M, int x, int y,
+                                       size_t i, float& e, float J[6]) {
+  const float dr = ldx(in.rD + p.rd_off, (uint32_t)i);
+  // an object's rendered depth is 0 outside its silhouette, where the row
+  // ends at the validity test of the core: test it before the other loads
+  if (OBJ && !(dr > 0.0f)) return -1;
+  const f3 vl = rf3(p, PL_V, i);
+  const float ir = ldx(p.ref + (size_t)PL_I * p.lstride, (uint32_t)i);
+  hold(dr); hold(vl); hold(ir);
+  return rgb_core<OBJ>(p, M, x, y, dr, vl, ir, e, J);
+}
 
 __device__ __forceinline__ void accum(float* s, int res, float e, const float* J, int fn, double k) {
   if (res < 1) {
@@ -298,48 +310,13 @@ __device__ __forceinline__ void accum(float* s, int res, float e, const float* J
 }
 
 
-// Rows of one instance at one level over the workgroups [0, nactive): FP32
-// per-thread sums, the wave butterfly, and this workgroup's 32 partial sums
-// in `part` (32 floats).
-template <bool ICP, bool RGB, bool OBJ, class MT>
-__device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst& inst, const MT& M, int state,
-                                           int wg, int nactive, float* __restrict__ part, float (*red)[32]) {
+// The 32 sums of a workgroup into part (32 floats).  Wave butterfly: at each
+// step a lane keeps half of its sums and adds the partner lane's copy of that
+// half, so after 5 exchange steps (16+8+4+2+1 lane swaps) lane l holds sum
+// l>>1 over 32 lanes, and one more swap adds the other 32.  Fixed order, so
+// the partial is deterministic.  Called by every thread of the workgroup.
+__device__ __forceinline__ void wg_reduce(float (&s)[32], float* __restrict__ part, float (*red)[32]) {
   const int tid = threadIdx.x;
-  const TrackInst in = inst;   // descriptor held in registers across the row loop
-  float s[32];
-#pragma unroll
-  for (int k = 0; k < 32; k++) s[k] = 0.f;
-  const int n = p.w * p.h;
-  const int nrows = (ICP ? n : 0) + (RGB ? n : 0);
-  const size_t N0 = (size_t)p.W * p.H;
-  TrackRow* dbg = p.dbg ? p.dbg + ((size_t)state * p.n_levels + p.level) * 2 * N0 : nullptr;
-  // rows [0, n) are ICP rows, [n, 2n) photometric rows of the same pixels
-  // (the order of TrackData in the reference); n is a multiple of 64 at
-  // every level of a 640x480 or 320x240 pyramid, so waves do not mix kinds
-  for (int r = wg * TB + tid; r < nrows; r += nactive * TB) {
-    asm volatile("" ::: "memory");   // keep the LDS matrix reads inside the loop
-    const bool icp = ICP && (!RGB || r < n);
-    const int i = (ICP && !icp) ? r - n : r;
-    const int x = i % p.w, y = i / p.w;
-    const size_t ro = (size_t)x + (size_t)y * p.W;   // row stride W at every level (tracking.cpp:49-50)
-    float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
-    int res;
-    if (icp) {
-      if (OBJ) res = obj_icp_row(p, in, M, x, y, (size_t)i, e, J);
-      else res = icp_row(p, in, M, x, y, (size_t)i, e, J);
-    } else {
-      res = rgb_row<OBJ>(p, in, M, x, y, (size_t)i, e, J);
-    }
-    accum(s, res, e, J, p.robust, p.robust_k);
-    const size_t oo = icp ? ro : N0 + ro;
-    if (in.codes && p.level == 0) in.codes[oo] = (int8_t)res;
-    if (!icp && in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
-    if (dbg) { TrackRow& d = dbg[oo]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
-  }
-  // Wave butterfly: at each step a lane keeps half of its sums and adds the
-  // partner lane's copy of that half, so after 5 exchange steps (16+8+4+2+1
-  // lane swaps) lane l holds sum l>>1 over 32 lanes, and one more swap adds
-  // the other 32.  Fixed order, so the partial is deterministic.
   const int lane = tid & 63;
 #define MF_BFLY(HALF, OFF)                                                   \
   {                                                                          \
@@ -367,6 +344,49 @@ __device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst
   }
   __syncthreads();   // red is reused by the caller's next instance
 }
+
+// Rows of one instance at one level over the workgroups [0, nactive): FP32
+// per-thread sums, the wave butterfly, and this workgroup's 32 partial sums
+// in `part` (32 floats).
+template <bool ICP, bool RGB, bool OBJ, class MT>
+__device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst& inst, const MT& M, int state,
+                                           int wg, int nactive, float* __restrict__ part, float (*red)[32]) {
+  const int tid = threadIdx.x;
+  const TrackInst in = inst;   // descriptor held in registers across the row loop
+  float s[32];
+#pragma unroll
+  for (int k = 0; k < 32; k++) s[k] = 0.f;
+  const int n = p.w * p.h;
+  const int nrows = (ICP ? n : 0) + (RGB ? n : 0);
+  const size_t N0 = (size_t)p.W * p.H;
+  // the row dump holds GN states 0 and 1
+  TrackRow* dbg = (p.dbg && state < 2) ? p.dbg + ((size_t)state * p.n_levels + p.level) * 2 * N0 : nullptr;
+  // rows [0, n) are ICP rows, [n, 2n) photometric rows of the same pixels
+  // (the order of TrackData in the reference); n is a multiple of 64 at
+  // every level of a 640x480 or 320x240 pyramid, so waves do not mix kinds
+  for (int r = wg * TB + tid; r < nrows; r += nactive * TB) {
+    asm volatile("" ::: "memory");   // keep the LDS matrix reads inside the loop
+    const bool icp = ICP && (!RGB || r < n);
+    const int i = (ICP && !icp) ? r - n : r;
+    const int x = i % p.w, y = i / p.w;
+    const size_t ro = (size_t)x + (size_t)y * p.W;   // row stride W at every level (tracking.cpp:49-50)
+    float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
+    int res;
+    if (icp) {
+      if (OBJ) res = obj_icp_row(p, in, M, x, y, (size_t)i, e, J);
+      else res = icp_row(p, in, M, x, y, (size_t)i, e, J);
+    } else {
+      res = rgb_row<OBJ>(p, in, M, x, y, (size_t)i, e, J);
+    }
+    accum(s, res, e, J, p.robust, p.robust_k);
+    const size_t oo = icp ? ro : N0 + ro;
+    if (in.codes && p.level == 0) in.codes[oo] = (int8_t)res;
+    if (!icp && in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
+    if (dbg) { TrackRow& d = dbg[oo]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
+  }
+  wg_reduce(s, part, red);
+}
+
 
 // ---- FP64 6x6 solve (commons.h:672-696 semantics: TooN GR_SVD backsub with
 // condition 1e6).  C is the symmetric JtWJ of the reduction, b its JtWe.
@@ -787,6 +807,83 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
                             partial + ((size_t)j * nblk + blockIdx.x) * 32, red);
 }
 
+// Two-kernel form, rows of every moving object in one grid over the pixels
+// (trackEachObject's rows, tracking.cpp:392-578, for all objects at once):
+// each thread takes PPT pixels TB apart, loads their live vertex, normal and
+// covariance and their reference vertex and intensity once, then for each
+// object computes the object's ICP row of the live pixel and photometric row
+// of the reference pixel from those values, and writes the object's partial
+// of this workgroup at partial[(object * nblk + workgroup) * 32].  Only the
+// rendered depth and, near the object's silhouette, the model-map taps are
+// per-object loads.  Per-object pose matrices sit in dynamic LDS.
+template <bool ICP, bool RGB, int PPT>
+__global__ __launch_bounds__(TB) void k_track_objs(TrackLaunch p, const TrackInst* __restrict__ insts, int ninst,
+                                                   const GNState* __restrict__ st, float* __restrict__ partial,
+                                                   int nblk, int q) {
+  extern __shared__ __align__(16) unsigned char mf_dyn[];
+  Mats* Mo = reinterpret_cast<Mats*>(mf_dyn);
+  __shared__ float red[NW][32];
+  __shared__ int skip[MAX_INST];
+  const int tid = threadIdx.x;
+  for (int j = tid; j < ninst; j += TB) {
+    const GNState& S = st[insts[j].state];
+    const GNIter& X = S.it[q & 1];
+    // a level starts afresh (tracking.cpp:246); later iterations are skipped
+    // once this object's loop at this level has broken
+    const int sk = S.off || (p.iter > 0 && X.done_level == p.level);
+    skip[j] = sk;
+    if (!sk) { Mo[j].T = X.T; Mo[j].invT = X.invT; Mo[j].A = S.A; Mo[j].B = S.B; Mo[j].C = S.C; Mo[j].mov = 0; }
+  }
+  __syncthreads();
+  const int n = p.w * p.h;
+  const size_t N0 = (size_t)p.W * p.H;
+  int pix[PPT];
+  f3 nl[PPT], vl[PPT], cv[PPT], rv[PPT];
+  float ir[PPT];
+#pragma unroll
+  for (int k = 0; k < PPT; k++) {
+    const int i = blockIdx.x * TB * PPT + k * TB + tid;
+    pix[k] = i < n ? i : -1;
+    const size_t ii = i < n ? (size_t)i : 0;
+    if (ICP) { nl[k] = lv3(p, PL_N, ii); vl[k] = lv3(p, PL_V, ii); cv[k] = lv3(p, PL_COV, ii); }
+    if (RGB) { rv[k] = rf3(p, PL_V, ii); ir[k] = ldx(p.ref + (size_t)PL_I * p.lstride, (uint32_t)ii); }
+  }
+  for (int j = 0; j < ninst; j++) {
+    if (skip[j]) continue;   // uniform over the workgroup
+    const TrackInst in = insts[j];
+    const Mats& M = Mo[j];
+    TrackRow* dbg = (p.dbg && in.state < 2) ? p.dbg + ((size_t)in.state * p.n_levels + p.level) * 2 * N0 : nullptr;
+    float s[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++) s[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+      asm volatile("" ::: "memory");   // keep the LDS matrix reads inside the loop
+      const int i = pix[k];
+      if (i < 0) continue;
+      const int x = i % p.w, y = i / p.w;
+      const size_t ro = (size_t)x + (size_t)y * p.W;   // row stride W at every level (tracking.cpp:49-50)
+      if (ICP) {
+        float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
+        const int res = obj_icp_core(p, in, M, nl[k], vl[k], cv[k], e, J);
+        accum(s, res, e, J, p.robust, p.robust_k);
+        if (in.codes && p.level == 0) in.codes[ro] = (int8_t)res;
+        if (dbg) { TrackRow& d = dbg[ro]; d.result = res; d.error = e; for (int u = 0; u < 6; u++) d.J[u] = J[u]; }
+      }
+      if (RGB) {
+        float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
+        const float dr = ldx(in.rD + p.rd_off, (uint32_t)i);
+        const int res = (dr > 0.0f) ? rgb_core<true>(p, M, x, y, dr, rv[k], ir[k], e, J) : -1;
+        accum(s, res, e, J, p.robust, p.robust_k);
+        if (in.codes && p.level == 0) in.codes[N0 + ro] = (int8_t)res;
+        if (in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
+        if (dbg) { TrackRow& d = dbg[N0 + ro]; d.result = res; d.error = e; for (int u = 0; u < 6; u++) d.J[u] = J[u]; }
+      }
+    }
+    wg_reduce(s, partial + ((size_t)j * nblk + blockIdx.x) * 32, red);
+  }
+}
+
 // Two-kernel form, solve: the iteration's FP64 accumulation, solve and GN
 // control, one workgroup per instance, after k_track (the kernel boundary
 // publishes the partials; a device-scope fence in every track workgroup costs
@@ -1136,6 +1233,21 @@ void launch_track_rows(hipStream_t s, const TrackLaunch& p, const TrackInst* ins
     else if (p.use_icp) k_track<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
     else k_track<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
   }
+}
+int track_obj_blocks(int npix, int pixels_per_thread) {
+  const int ppt = pixels_per_thread < 1 ? 1 : (pixels_per_thread > 2 ? 2 : pixels_per_thread);
+  return track_blocks(npix, ppt);
+}
+void launch_track_rows_objs(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
+                            float* partial, int nblk, int pixels_per_thread, int q) {
+  const size_t lds = (size_t)ninst * sizeof(Mats);
+  dim3 grd(nblk), blk(TB);
+#define MF_OBJS(PPT)                                                                                                  \
+  if (p.use_icp && p.use_rgb) k_track_objs<true, true, PPT><<<grd, blk, lds, s>>>(p, inst, ninst, st, partial, nblk, q); \
+  else if (p.use_icp) k_track_objs<true, false, PPT><<<grd, blk, lds, s>>>(p, inst, ninst, st, partial, nblk, q);        \
+  else k_track_objs<false, true, PPT><<<grd, blk, lds, s>>>(p, inst, ninst, st, partial, nblk, q);
+  if (pixels_per_thread >= 2) { MF_OBJS(2) } else { MF_OBJS(1) }
+#undef MF_OBJS
 }
 void launch_track_solve(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
                         const float* partial, int nblk, int q, const TrackSync& sy) {

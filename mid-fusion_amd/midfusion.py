@@ -492,6 +492,7 @@ class Kfusion:
 
     TUNE_TRACK_ROWS_PER_THREAD_L0 = 0   # + pyramid level
     TUNE_TRACK_FUSED_L0 = 8             # + pyramid level
+    TUNE_OBJ_PIXELS_PER_THREAD_L0 = 16  # + pyramid level
 
     def set_tuning(self, knob, value):
         self._chk(self.L.mf_set_tuning(self.ctx, knob, int(value)))

@@ -158,7 +158,8 @@ def test_pipeline_icp_rgb_moving_object():
 
 def test_fused_iteration_form_matches_two_kernel_form():
     # Both forms sum the same workgroup partials in the same FP64 order, so
-    # the tracked poses must be bit-identical.
+    # the tracked poses must be bit-identical (objects on their per-object row
+    # grid, the partition the fused form uses).
     W, H = 320, 240
     mb_T0 = np.eye(4)
     mb_T0[:3, 3] = np.array([2.56, 2.56, 1.06]) + np.array([0.05, 0.05, 1.0]) - 0.256
@@ -173,6 +174,7 @@ def test_fused_iteration_form_matches_two_kernel_form():
         g.set_moving(1, True)
         for lvl in range(3):
             g.set_tuning(g.TUNE_TRACK_FUSED_L0 + lvl, fused)
+            g.set_tuning(g.TUNE_OBJ_PIXELS_PER_THREAD_L0 + lvl, 0)
         ctx.append(g)
     for f in range(4):
         d, rgb, lab = sc.render(f)
@@ -189,5 +191,50 @@ def test_fused_iteration_form_matches_two_kernel_form():
             Pa, Pb = a.getPose(oid), b.getPose(oid)
             print(f, oid, np.abs(Pa - Pb).max())
             assert np.array_equal(Pa, Pb), (f, oid)
+    for g in ctx:
+        g.close()
+
+
+def test_object_pixel_grid_rows_match_per_object_grid():
+    # Two moving objects tracked in one launch per GN iteration, on the shared
+    # pixel grid (1 and 2 pixels per thread) and on one row grid per object.
+    # The grids compute the same rows from the same state, so the first
+    # object launch of a frame (coarsest level, iteration 0) must dump
+    # bit-identical rows; later launches start from poses solved from the
+    # same rows summed in another FP32 order.  (Pose accuracy of the pixel
+    # grid against the FP64 oracle: test_gpu_fp64.)
+    W, H = 320, 240
+    movers = scene.batch_objects(2, 4002)
+    sc = scene.Scene(seed=4, width=W, height=H, K=tuple(K320), moving=movers)
+    P0 = sc.camera_pose(0).astype(np.float32)
+    ctx = []
+    for ppt in ((0, 0, 0), (2, 1, 1), (1, 1, 1)):
+        g = midfusion.Kfusion(W, H, P0, bg_size=5.12, bg_res=256, use_rgb=True, iters=(1, 1, 1))
+        for i, m in enumerate(movers):
+            g.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
+            g.set_moving(i + 1, True)
+        for lvl in range(3):
+            g.set_tuning(g.TUNE_OBJ_PIXELS_PER_THREAD_L0 + lvl, ppt[lvl])
+        g.debug_rows(True)
+        ctx.append(g)
+    for f in range(2):
+        d, rgb, lab = sc.render(f)
+        masks = scene.masks_for_labels(lab, len(movers))
+        for g in ctx:
+            for i, m in enumerate(masks):
+                g.set_mask(i, m)
+            g.preprocessing(d, rgb)
+            g.tracking(K320, f)
+            g.sync()
+        if f == 1:
+            r0 = ctx[0].rows(1, 2)
+            assert np.count_nonzero(r0[0] == 1) > 0
+            for g in ctx[1:]:
+                r = g.rows(1, 2)
+                for a, b in zip(r0, r):
+                    assert np.array_equal(a, b)
+        for g in ctx:
+            g.integration(K320, f)
+            g.raycasting(K320, f)
     for g in ctx:
         g.close()
