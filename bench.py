@@ -132,6 +132,9 @@ def main():
     ap.add_argument('--obj-ppt', type=str, default='', metavar='P0,P1,P2',
                     help='moving objects\' rows per pyramid level: pixels per thread of the shared pixel grid '
                          '(1, 2) or 0 for one row grid per object (default: library default)')
+    ap.add_argument('--autotune', action='store_true',
+                    help='choose the tracking rows per thread of every pyramid level with the library autotuner '
+                         '(mf_autotune) after the second warm-up frame, before timing')
     ap.add_argument('--copy-ref', type=int, default=0, metavar='MB',
                     help='only launch a plain copy kernel of MB megabytes (traffic-counter calibration)')
     args = ap.parse_args()
@@ -181,8 +184,18 @@ def main():
         g.integration(K, f)
         g.raycasting(K, f)
 
+    tuned = None
     for f in range(args.warmup):
         step(f)
+        if args.autotune and f == 1:
+            # both pyramids now hold real frames; the remaining warm-up frames
+            # capture the graphs of the chosen shapes
+            g.sync()
+            chosen, us = g.autotune(K)
+            tuned = {'rows_per_thread': [int(v) for v in chosen],
+                     'us_per_iteration': {str(c): [round(float(x), 2) for x in us[:, i]]
+                                          for i, c in enumerate((1, 2, 4, 8, 16))}}
+            print('autotune: %s' % json.dumps(tuned), file=sys.stderr, flush=True)
     g.sync()
     t = time.perf_counter()
     for f in range(args.warmup, n_frames):
@@ -338,6 +351,9 @@ def main():
         'work_per_frame': {k: v // 3 for k, v in st.items()},
         'camera_translation_error_vs_gt_m': round(pose_err, 5),
     }
+    out['tracking_rows_per_thread'] = rpt[:3]
+    if tuned:
+        out['autotune'] = tuned
     if raw_leg:
         out['host_input_rate'] = raw_leg
     if render:

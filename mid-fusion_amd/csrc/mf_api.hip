@@ -132,6 +132,7 @@ struct mf_ctx {
   int* mask_count = nullptr;          // rendered-mask pixel count
   float* eval_maps = nullptr;         // mf_motion_residuals inputs (9 planes), allocated on first use
   Mat4* eval_T = nullptr;
+  Mat4* tune_pose = nullptr;     // autotuner: pose output of its scratch GN state
   int8_t* eval_codes = nullptr;
   std::vector<int> ids_host;
   std::vector<Object*> objs;
@@ -421,7 +422,8 @@ std::vector<uint64_t> base_key(mf_ctx* c, int stage, const float k[4]) {
   std::vector<uint64_t> key = {(uint64_t)stage, (uint64_t)(uintptr_t)c->live[0].D, (uint64_t)(uintptr_t)c->ref[0].D,
                                fbits(k[0]), fbits(k[1]), fbits(k[2]), fbits(k[3]), (uint64_t)c->trace_on,
                                (uint64_t)(uintptr_t)c->dbg_rows};
-  for (int l = 0; l < MAX_LEVELS; l++) key.push_back((uint64_t)c->track_rpt[l] | ((uint64_t)c->track_fused[l] << 32));
+  for (int l = 0; l < MAX_LEVELS; l++)
+    key.push_back((uint64_t)c->track_rpt[l] | ((uint64_t)c->track_fused[l] << 32) | ((uint64_t)c->track_obj_ppt[l] << 40));
   return key;
 }
 
@@ -1605,6 +1607,61 @@ int mf_replay_track_rows(mf_ctx* c, const float k[4], int level, int reps, doubl
   // live v, n, cov + model v, n + mask (61 B) and the photometric inputs
   // (41 B) per pixel, one 128-B partial per workgroup
   if (alg_bytes) *alg_bytes = (double)p.w * p.h * ((c->cfg.use_icp ? 61 : 0) + (c->cfg.use_rgb ? 41 : 0)) + 128.0 * nblk;
+  return 0;
+}
+
+int mf_autotune(mf_ctx* c, const float k[4], int reps, int32_t* chosen, double* us) {
+  static const int cand[MF_TUNE_CANDIDATES] = {1, 2, 4, 8, 16};
+  if (!k || reps < 1 || reps > 10000) return fail(c, MF_E_ARG, "bad arguments");
+  if (!c->have_ref) return fail(c, MF_E_ARG, "no tracked frame yet");
+  hipStream_t s = c->stream;
+  if (!c->tune_pose && !(c->tune_pose = dalloc<Mat4>(c, 1))) return fail(c, MF_E_OOM, "device allocation failed");
+  // scratch GN state and descriptor (the residual evaluation's spare slot):
+  // a copy of the camera's state whose pose output goes to tune_pose, and
+  // the camera's inputs without code or error outputs
+  const int spare = MAX_INST - 1;
+  GNState* sg = c->gn + spare;
+  hipMemcpyAsync(sg, c->gn, sizeof(GNState), hipMemcpyDeviceToDevice, s);
+  Mat4* tp = c->tune_pose;
+  const int zero = 0;
+  hipMemcpyAsync(&sg->out_pose, &tp, sizeof tp, hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(&sg->off, &zero, sizeof zero, hipMemcpyHostToDevice, s);
+  hipStreamSynchronize(s);
+  Object* bg = c->objs[0];
+  set_inst(c, spare, c->vtx, c->nrm, bg->rD, nullptr, nullptr, spare);
+  TrackSync sy;
+  memset(&sy, 0, sizeof sy);
+  const int kinds = (c->cfg.use_icp ? 1 : 0) + (c->cfg.use_rgb ? 1 : 0);
+  hipEvent_t a = ev_get(c), b = ev_get(c);
+  int q = 0;
+  for (int l = 0; l < c->cfg.n_levels; l++) {
+    TrackLaunch p = base_launch(c, k, l, 0);
+    p.iter = 0;          // rows and solves of iteration 0 never skip
+    p.dbg = nullptr;
+    double best = -1;
+    int arg = c->track_rpt[l];
+    for (int ci = 0; ci < MF_TUNE_CANDIDATES; ci++) {
+      const int nblk = track_blocks(p.w * p.h * kinds, cand[ci]);
+      auto iter = [&]() {
+        launch_track_rows(s, p, c->inst + spare, 1, c->gn, c->partial, nblk, q);
+        launch_track_solve(s, p, c->inst + spare, 1, c->gn, c->partial, nblk, q, sy);
+        ++q;
+      };
+      iter();   // warm
+      hipEventRecord(a, s);
+      for (int r = 0; r < reps; r++) iter();
+      hipEventRecord(b, s);
+      if (hipEventSynchronize(b) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+      float ms = 0;
+      hipEventElapsedTime(&ms, a, b);
+      const double t = 1000.0 * ms / reps;
+      if (us) us[l * MF_TUNE_CANDIDATES + ci] = t;
+      if (best < 0 || t < best) { best = t; arg = cand[ci]; }
+    }
+    c->track_rpt[l] = arg;
+    if (chosen) chosen[l] = arg;
+  }
+  c->pool.push_back(a); c->pool.push_back(b);
   return 0;
 }
 

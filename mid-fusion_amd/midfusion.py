@@ -93,6 +93,7 @@ SIGNATURES = {
     'mf_get_trackresult': (C.c_int, [P, C.c_int, i32p]),
     'mf_get_trace': (C.c_long, [P, f64p, C.c_long]),
     'mf_solve6': (C.c_int, [P, f64p, C.c_long, f64p, i32p]),
+    'mf_autotune': (C.c_int, [P, f32p, C.c_int, i32p, f64p]),
     'mf_clear_trace': (None, [P]),
     'mf_get_level': (C.c_long, [P, C.c_int, C.c_int, f32p]),
     'mf_debug_rows': (C.c_int, [P, C.c_int]),
@@ -191,6 +192,7 @@ class Kfusion:
                  batch_objects=True, bg_capacity=0):
         self.L = load()
         self.w, self.h = width, height
+        self.n_levels = len(iters)
         cfg = MFConfig()
         cfg.width, cfg.height, cfg.n_levels = width, height, len(iters)
         for i, v in enumerate(iters):
@@ -467,6 +469,15 @@ class Kfusion:
 
     def device_bytes(self):
         return self.L.mf_device_bytes(self.ctx)
+
+    def autotune(self, k, reps=30):
+        """Launch-shape autotuner: times each pyramid level's rows-per-thread
+        candidates {1, 2, 4, 8, 16} on the last tracked frame and keeps the
+        fastest.  Returns (chosen per level, us per iteration [level, cand])."""
+        nl = self.n_levels
+        chosen, us = np.zeros(nl, np.int32), np.zeros(nl * 5)
+        self._chk(self.L.mf_autotune(self.ctx, _f32(k), int(reps), chosen, us))
+        return chosen, us.reshape(nl, 5)
 
     def solve6(self, s27):
         """Device 6x6 solve of systems s27 [n, 27] (JtWe, packed upper JtWJ):

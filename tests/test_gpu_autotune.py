@@ -1,0 +1,52 @@
+"""The launch-shape autotuner (mf_autotune) on a live context: it times every
+pyramid level's rows-per-thread candidates on a scratch copy of the camera's
+GN state, so the tracked poses, result codes and maps are untouched; it sets
+the chosen values as the context's tuning; and the next frame, tracked from
+the same state as an untuned context's, gives the same pose within 1e-5
+(rows-per-thread changes only the FP32 summation order).  Later frames are
+not compared: a last-iterate difference in the 7th digit can move a few
+threshold pixels of the result codes, which refine the integration mask,
+so the maps and then the poses of the two contexts drift apart by design."""
+import numpy as np
+import pytest
+
+from conftest import scene, midfusion, K320
+
+pytestmark = pytest.mark.gpu
+W, H = 320, 240
+
+
+def test_autotune_leaves_state_and_sets_tuning():
+    sc = scene.Scene(seed=3, width=W, height=H, K=tuple(K320))
+    P0 = sc.camera_pose(0).astype(np.float32)
+    ga = midfusion.Kfusion(W, H, P0, bg_size=5.12, bg_res=256)
+    gb = midfusion.Kfusion(W, H, P0, bg_size=5.12, bg_res=256)
+    for f in range(3):
+        d, rgb, lab = sc.render(f)
+        for g in (ga, gb):
+            g.set_mask(0, np.ones((H, W), np.float32))
+            g.preprocessing(d, rgb)
+            g.tracking(K320, f)
+            g.integration(K320, f)
+            g.raycasting(K320, f)
+            g.sync()
+        if f == 1:
+            pose, codes = ga.getPose(), ga.trackresult(0)
+            vtx, nrm, lb = ga.maps()
+            chosen, us = ga.autotune(K320, reps=5)
+            print(chosen, us)
+            assert np.array_equal(ga.getPose(), pose)
+            assert np.array_equal(ga.trackresult(0), codes)
+            v2, n2, l2 = ga.maps()
+            assert np.array_equal(v2, vtx) and np.array_equal(n2, nrm) and np.array_equal(l2, lb)
+            assert chosen.shape == (3,) and all(c in (1, 2, 4, 8, 16) for c in chosen)
+            assert np.all(us > 0)
+            for lvl in range(3):
+                assert ga.get_tuning(ga.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl) == chosen[lvl]
+                assert us[lvl, [1, 2, 4, 8, 16].index(chosen[lvl])] == us[lvl].min()
+        if f == 2:
+            A, B = ga.getPose(), gb.getPose()
+            assert np.linalg.norm(A[:3, 3] - B[:3, 3]) < 1e-5
+            assert np.abs(A[:3, :3] - B[:3, :3]).max() < 1e-5
+    ga.close()
+    gb.close()
