@@ -51,15 +51,39 @@ def sphere():
                            0.512, 128, shape='ellipsoid')
 
 
-def run(n_frames, moving):
+def tsdf_vs(gpu, orc, i, max_blocks=3000):
+    """tsdf and weight of instance i's allocated blocks (up to max_blocks,
+    evenly spaced over the sorted keys), GPU against an oracle: worst
+    differences and the fraction of voxels with |dtsdf| > 1e-4 or dw != 0."""
+    L = oracle.lib(False)
+    keys = orc.keys(i)
+    keys = keys[np.linspace(0, len(keys) - 1, min(len(keys), max_blocks)).astype(int)] if len(keys) else keys
+    xyz = np.zeros(3, np.uint32)
+    wt = ww = 0.0
+    bad = n = 0
+    for k in keys:
+        L.orc_unpack(int(k) & ~0x1FF, xyz)
+        bg, bo = gpu.block(i, *map(int, xyz)), orc.block(i, *map(int, xyz))
+        if bg is None or bo is None:
+            bad += 512
+            n += 512
+            continue
+        dt, dw = np.abs(bg[:, 0] - bo[:, 0]), np.abs(bg[:, 1] - bo[:, 1])
+        wt, ww = max(wt, float(dt.max())), max(ww, float(dw.max()))
+        bad += int(np.sum((dt > 1e-4) | (dw > 0)))
+        n += 512
+    return {'blocks': len(keys), 'tsdf_max': wt, 'w_max': ww, 'outside': bad / max(n, 1)}
+
+
+def run(n_frames, moving, W=W, H=H, K=K320, bg_res=256, invalid_frac=0.0, seed=1, tsdf=False):
     mov = moving if isinstance(moving, list) else ([big_box()] if moving else [])
-    sc = scene.Scene(seed=1, width=W, height=H, K=tuple(K320), moving=mov)
+    sc = scene.Scene(seed=seed, width=W, height=H, K=tuple(K), moving=mov, invalid_frac=invalid_frac)
     P0 = sc.camera_pose(0).astype(np.float32)
-    gpu = midfusion.Kfusion(W, H, P0, bg_size=5.12, bg_res=256, use_rgb=True)
+    gpu = midfusion.Kfusion(W, H, P0, bg_size=5.12, bg_res=bg_res, use_rgb=True)
     o32 = oracle.Oracle(W, H, P0, use_rgb=True)
     o64 = oracle.Oracle(W, H, P0, use_rgb=True, fp64=True)
     for o in (o32, o64):
-        o.add_object(5.12, 256, np.eye(4))
+        o.add_object(5.12, bg_res, np.eye(4))
     for j, m in enumerate(mov):
         T = m.pose(0).astype(np.float32)
         for x in (gpu, o32, o64):
@@ -90,9 +114,9 @@ def run(n_frames, moving):
                 gpu.setPose(i, T)
         for x in (gpu, o32, o64):
             x.trace_clear()
-        gpu.track_camera(K320, f)
-        o32.track_camera(K320, f)
-        o64.track_camera(K320, f)
+        gpu.track_camera(K, f)
+        o32.track_camera(K, f)
+        o64.track_camera(K, f)
         gpu.sync()
         r = {'frame': f}
         P64 = o64.pose()
@@ -125,9 +149,9 @@ def run(n_frames, moving):
         # objects start from the FP64 camera pose everywhere
         gpu.setPose(-1, P64.astype(np.float32))
         o32.set_pose(-1, P64.astype(np.float32))
-        gpu.track_objects(K320, f)
-        o32.track_objects(K320, f)
-        o64.track_objects(K320, f)
+        gpu.track_objects(K, f)
+        o32.track_objects(K, f)
+        o64.track_objects(K, f)
         gpu.sync()
         tg = gpu.trace()
         for i in range(1, 1 + len(mov)):
@@ -137,8 +161,14 @@ def run(n_frames, moving):
             # column 24: 1 certified Gauss-Jordan, 2 Jacobi pseudo-inverse
             r['obj%d_pinv' % i] = int(np.sum((tg[:, 0] == i) & (tg[:, 24] == 2)))
         for x in (gpu, o32, o64):
-            x.integrate(K320, f) if x is not gpu else x.integration(K320, f)
-            x.raycast(K320, f) if x is not gpu else x.raycasting(K320, f)
+            x.integrate(K, f) if x is not gpu else x.integration(K, f)
+        if tsdf:
+            gpu.sync()
+            r['tsdf_bg_vs_o64'] = tsdf_vs(gpu, o64, 0)
+            r['tsdf_bg_o32_vs_o64'] = tsdf_vs(o32, o64, 0)
+            r['tsdf_bg_vs_o32'] = tsdf_vs(gpu, o32, 0)
+        for x in (gpu, o32, o64):
+            x.raycast(K, f) if x is not gpu else x.raycasting(K, f)
         print(r)
         report.append(r)
     gpu.close()
@@ -199,3 +229,33 @@ def test_two_objects_batched_vs_fp64_oracle():
         for i in (1, 2):
             eg, eo = r['obj%d_gpu' % i], r['obj%d_o32' % i]
             assert eg[0] <= 2 * eo[0] + 1e-5 and eg[1] <= 2 * eo[1] + 1e-5, r
+
+
+def test_precision_scale_1280x720_16_objects():
+    """The precision + scale configuration at full size: 1280x720 (fx = fy =
+    1050), a 1024^3 background at 5 mm, 16 moving boxes and ellipsoids of
+    0.05-0.4 m, 2 % random invalid depth pixels, three frames.  Camera pose
+    within 1e-5 m / 1e-5 rad of the FP64 oracle.  Background TSDF and weights:
+    on the first frame (identical poses everywhere) bit-identical to the FP32
+    oracle's; against the FP64 oracle's, about 1 % of the voxels differ in
+    both FP32 paths, because a projection rounded in FP32 picks another pixel
+    at half-pixel boundaries (an invalid pixel or the far side of a depth
+    edge, so one side updates the voxel and the other does not), and the
+    GPU's share stays within 1.5x the FP32 oracle's.  Each object pose stays
+    within twice the FP32 oracle's own error against the FP64 oracle, plus
+    1e-5."""
+    K1050 = np.array([1050.0, 1050.0, 639.5, 359.5], np.float32)
+    rep = run(3, moving=scene.batch_objects(16, 5000), W=1280, H=720, K=K1050, bg_res=1024, invalid_frac=0.02,
+              seed=5000, tsdf=True)
+    for r in rep[1:]:
+        errs = [(r['obj%d_gpu' % i], r['obj%d_o32' % i]) for i in range(1, 17)]
+        print('objects (gpu, o32) vs o64:', errs)
+    assert rep[0]['tsdf_bg_vs_o32']['outside'] == 0.0, rep[0]['tsdf_bg_vs_o32']
+    for r in rep:
+        g, o = r['tsdf_bg_vs_o64'], r['tsdf_bg_o32_vs_o64']
+        assert g['outside'] <= 1.5 * o['outside'] + 1e-4, (g, o)
+    for r in rep[1:]:
+        assert r['cam_gpu'][0] < 1e-5 and r['cam_gpu'][1] < 1e-5, r
+        for i in range(1, 17):
+            eg, eo = r['obj%d_gpu' % i], r['obj%d_o32' % i]
+            assert eg[0] <= 2 * eo[0] + 1e-5 and eg[1] <= 2 * eo[1] + 1e-5, (i, eg, eo)

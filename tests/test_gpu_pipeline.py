@@ -98,6 +98,38 @@ def run_pair(W, H, K, n_frames, use_rgb, res=256, noise=True, moving=False, chec
     return report
 
 
+def voxel_check(stage, f, gpu, orc, r):
+    """After integration on identical inputs: every allocated block's voxels,
+    GPU against oracle.  tsdf, rgb within 1e-4; weights and counts exact;
+    fg (double) within 1e-6.  Records the worst differences and the
+    fraction of voxels outside those bounds."""
+    if stage != 'integrate':
+        return
+    L = oracle.lib(False)
+    xyz = np.zeros(3, np.uint32)
+    for i in range(len(gpu_instances(r))):
+        keys = orc.keys(i)
+        worst = np.zeros(8)
+        bad = n = 0
+        for k in keys:
+            L.orc_unpack(int(k) & ~0x1FF, xyz)
+            bg = gpu.block(i, int(xyz[0]), int(xyz[1]), int(xyz[2]))
+            bo = orc.block(i, int(xyz[0]), int(xyz[1]), int(xyz[2]))
+            assert bg is not None and bo is not None, (i, k)
+            d = np.abs(bg - bo)
+            worst = np.maximum(worst, d.max(axis=0))
+            tol = np.array([1e-4, 0, 1e-4, 1e-4, 1e-4, 0, 1e-6, 0])
+            bad += int(np.any(d > tol, axis=1).sum())
+            n += 512
+        r['vox%d' % i] = {'blocks': len(keys), 'tsdf_max': float(worst[0]), 'w_max': float(worst[1]),
+                          'rgb_max': float(worst[2:5].max()), 'fg_max': float(worst[6]),
+                          'count_max': float(max(worst[5], worst[7])), 'outside': bad / max(n, 1)}
+
+
+def gpu_instances(r):
+    return [k for k in r if k.startswith('rows') and k.endswith('_mismatch')]
+
+
 def test_stage_preprocess_and_pyramid():
     W, H = 320, 240
     sc = scene.Scene(seed=3, width=W, height=H, K=tuple(K320))
@@ -131,8 +163,9 @@ def test_stage_preprocess_and_pyramid():
 
 
 def test_pipeline_icp_only_cfg1_like():
-    rep = run_pair(320, 240, K320, 4, use_rgb=False)
+    rep = run_pair(320, 240, K320, 4, use_rgb=False, check=voxel_check)
     for r in rep:
+        assert r['vox0']['outside'] < 1e-4, r['vox0']
         assert r['rows0_mismatch'] < 5e-3, r
         assert r['pose_t'] < 1e-5 and r['pose_r'] < 1e-5, r
         assert r['keys0'][2] == 0, r
@@ -147,8 +180,10 @@ def test_pipeline_icp_rgb_moving_object():
     # FP32 oracle's error beside it): camera within 5e-5 here.  Object GN is
     # far worse conditioned than the camera's, so whole-pipeline object poses
     # are held to 1e-3 while test_gpu_rows pins every object row exactly.
-    rep = run_pair(320, 240, K320, 4, use_rgb=True, moving=True)
+    rep = run_pair(320, 240, K320, 4, use_rgb=True, moving=True, check=voxel_check)
     for r in rep:
+        for i in (0, 1):
+            assert r['vox%d' % i]['outside'] < 1e-4, r['vox%d' % i]
         assert r['rows0_mismatch'] < 5e-3 and r['rows1_mismatch'] < 5e-3, r
         assert r['pose_t'] < 5e-5 and r['pose_r'] < 5e-5, r
         assert r.get('obj1_t', 0) < 1e-3 and r.get('obj1_r', 0) < 1e-3, r
