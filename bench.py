@@ -31,12 +31,18 @@ def _load(name, path):
 HBM_PEAK_GBS = 8000.0
 W, H = 640, 480
 K = np.array([525.0, 525.0, 319.5, 239.5], np.float32)
+BG_RES = 512            # background volume resolution (5.12 m cube)
+HD = False              # --hd: the precision + scale workload
 
 
 def make_scene(scene, n_objects):
     """One object: the textured 0.2x0.15x0.1 m box in a 128^3 volume at 2 mm.
     K > 1 objects: seeded boxes and ellipsoids of 0.05-0.4 m, each in a
     64^3-256^3 volume at 2-5 mm, on constant twists."""
+    if HD:
+        # 1280x720, 16 seeded boxes/ellipsoids, 2 % random invalid depth pixels
+        movers = scene.batch_objects(n_objects, 5000)
+        return scene.Scene(seed=5000, width=W, height=H, K=tuple(K), moving=movers, invalid_frac=0.02), movers
     if n_objects == 1:
         rng = np.random.default_rng(2)
         T0 = np.eye(4)
@@ -135,9 +141,19 @@ def main():
     ap.add_argument('--autotune', action='store_true',
                     help='choose the tracking rows per thread of every pyramid level with the library autotuner '
                          '(mf_autotune) after the second warm-up frame, before timing')
+    ap.add_argument('--hd', action='store_true',
+                    help='precision + scale workload instead: 1280x720 (fx=fy=1050), background 1024^3 at 5 mm, '
+                         '16 moving objects (default --objects), 2%% invalid depth pixels')
     ap.add_argument('--copy-ref', type=int, default=0, metavar='MB',
                     help='only launch a plain copy kernel of MB megabytes (traffic-counter calibration)')
     args = ap.parse_args()
+    global W, H, K, BG_RES, HD
+    if args.hd:
+        HD = True
+        W, H, BG_RES = 1280, 720, 1024
+        K = np.array([1050.0, 1050.0, 639.5, 359.5], np.float32)
+        if args.objects == 1:
+            args.objects = 16
     if args.gpus != 1:
         print('bench.py measures one GPU: this build runs the dense path on a single MI355X '
               '(multi-GPU sharding by instance is a design note only); use --gpus 1', file=sys.stderr)
@@ -160,7 +176,7 @@ def main():
     print('rendered %d frames in %.1fs' % (len(frames), time.time() - t0), file=sys.stderr, flush=True)
 
     P0 = sc.camera_pose(0).astype(np.float32)
-    g = mf.Kfusion(W, H, P0, bg_size=5.12, bg_res=512, batch_objects=not args.plain)
+    g = mf.Kfusion(W, H, P0, bg_size=5.12, bg_res=BG_RES, batch_objects=not args.plain)
     for lvl, v in enumerate(x for x in args.track_rpt.split(',') if x):
         g.set_tuning(g.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl, int(v))
     for lvl, v in enumerate(x for x in args.track_fused.split(',') if x):
@@ -240,7 +256,9 @@ def main():
     achieved = rep_bytes / (rep_us * 1e-6) / 1e9
     roof = {'kernel': KERNEL_NAMES['track_rows'] + ' (level 0, %d workgroups)' % (-(-(2 * N0) // (TB * rpt[0]))),
             'bound': 'hbm', 'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS,
-            'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 5), 'traffic': traffic_per_dispatch('track_rows_l0'),
+            'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 5),
+            # the committed counter summary covers the 640x480 workload only
+            'traffic': None if HD else traffic_per_dispatch('track_rows_l0'),
             'avg_launch_us': round(rep_us, 3), 'launches': REPLAYS, 'alg_bytes_per_launch': int(rep_bytes)}
     roof_all = {}
     for k, (ms_k, n_k) in kern.items():
@@ -253,7 +271,7 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, 'oracle'))
         import oracle
         o = oracle.Oracle(W, H, P0)
-        o.add_object(5.12, 512, np.eye(4))
+        o.add_object(5.12, BG_RES, np.eye(4))
         for i, m in enumerate(movers):
             o.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
             o.set_moving(i + 1, True)
@@ -306,7 +324,7 @@ def main():
                     fh.write(struct.pack('<II', W, H) + np.ascontiguousarray(d, '<u2').tobytes())
                     fh.write(struct.pack('<II', W, H) + np.ascontiguousarray(rgb, np.uint8).tobytes())
             raw = mf.RawSequence(path)
-            g2 = mf.Kfusion(W, H, P0, bg_size=5.12, bg_res=512, batch_objects=not args.plain)
+            g2 = mf.Kfusion(W, H, P0, bg_size=5.12, bg_res=BG_RES, batch_objects=not args.plain)
             for i, m in enumerate(movers):
                 g2.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
                 g2.set_moving(i + 1, True)
@@ -332,11 +350,13 @@ def main():
             g2.close()
 
     out = {
-        'metric': 'frames/s (track+integrate+raycast, 640x480, cam+K objs)',
+        'metric': 'frames/s (track+integrate+raycast, %dx%d, cam+K objs)' % (W, H),
         'value': round(fps, 3), 'unit': 'frames/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': round(ms, 3), 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f32', 'data': 'synthetic',
-        'config': {'workload': ('640x480 RGB-D, bg 512^3 @1cm + 1 moving box 0.2x0.15x0.1 m in 128^3 @2mm'
+        'config': {'workload': ('1280x720 RGB-D (fx=fy=1050, 2%% invalid depth), bg 1024^3 @5mm + %d moving '
+                                'boxes/ellipsoids 0.05-0.4 m in 64^3-256^3 @2-5mm' % n_obj if HD else
+                                '640x480 RGB-D, bg 512^3 @1cm + 1 moving box 0.2x0.15x0.1 m in 128^3 @2mm'
                                 if n_obj == 1 else
                                 '640x480 RGB-D, bg 512^3 @1cm + %d moving boxes/ellipsoids 0.05-0.4 m in 64^3-256^3 '
                                 '@2-5mm' % n_obj) + ', ICP+RGB {10,5,4}, %s' % (
