@@ -273,6 +273,28 @@ __device__ float march(const VolDev& v, f3 origin, f3 dir, float nearp, float fa
   return 0.f;
 }
 
+// False only when no ray through the pixel tile [x0, x0 + n) x [y0, y0 + n)
+// can meet the object's volume cube [0, size]^3: every cube corner is in
+// front of the camera and the tile lies more than 2 pixels outside the
+// bounding box of their projections (the projection of a convex body in
+// front of the camera is the convex hull of its corners' projections).
+// Rays through pixel (x, y) follow view * (x, y, 1), so the corners are
+// projected with inverse(view) = K * inverse(T_w_c).  A ray that misses the
+// cube is clipped to nothing by march(), so skipping it changes no result.
+__device__ bool tile_meets_cube(const ObjParams& P, float size, int x0, int y0, int n) {
+  const Mat4 Vi = inverse(P.view);
+  float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+  for (int c = 0; c < 8; c++) {
+    const f3 q = affine(Vi, affine(P.pose, mk3((c & 1) ? size : 0.f, (c & 2) ? size : 0.f, (c & 4) ? size : 0.f)));
+    if (!(q.z > 1e-3f)) return true;   // corner behind or at the camera plane: no bound
+    const float u = q.x / q.z, v = q.y / q.z;
+    umin = fminf(umin, u); umax = fmaxf(umax, u); vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
+  }
+  const float m = 2.f;
+  return !((float)(x0 + n - 1) < umin - m || (float)x0 > umax + m || (float)(y0 + n - 1) < vmin - m ||
+           (float)y0 > vmax + m);
+}
+
 template <bool STATS>
 __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs, int nobj, int w, int h, float mu,
                                                  float nearp, float farp, float eps, float* __restrict__ vx,
@@ -285,7 +307,15 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
   __shared__ unsigned cnt[5];
   __shared__ uint32_t st_par[MAX_LG + 1][RT];
   __shared__ float st_tmax[MAX_LG + 1][RT];
+  __shared__ uint64_t tile_objs;   // bit oi: this tile's rays may meet object oi's volume
   if (threadIdx.x < 5) cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 64) {
+    const int oi = threadIdx.x;
+    const bool meets = oi >= nobj ||
+                       tile_meets_cube(*objs[oi].prm, objs[oi].v.size, blockIdx.x * RT_TILE, blockIdx.y * RT_TILE, RT_TILE);
+    const uint64_t b = __ballot(meets);
+    if (oi == 0) tile_objs = b;
+  }
   __syncthreads();
   // 16x16 pixel tile per workgroup, 8x8 per wave: neighbouring rays visit
   // the same blocks, so a wave's gathers share cache lines and its lanes
@@ -318,8 +348,12 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
     const f3 wd = rotate(P.view, mk3((float)x, (float)y, 1.f));
     const f3 od = normalize(rotate(P.inv_pose, wd));
     const float nfar = fmin_r(nearest, farp);
-    const float tmin = ray_first_block<STATS>(v, oo, od, nearp, nfar, eps, &st_par[0][threadIdx.x],
-                                              &st_tmax[0][threadIdx.x], nnodes);
+    // rays of a tile outside the object's projected volume: the miss outcome
+    // (no hit, hit point 0) without the traversal
+    const float tmin = ((tile_objs >> oi) & 1)
+                           ? ray_first_block<STATS>(v, oo, od, nearp, nfar, eps, &st_par[0][threadIdx.x],
+                                                    &st_tmax[0][threadIdx.x], nnodes)
+                           : -1.f;
     f3 hit = mk3(0.f, 0.f, 0.f);
     float hw = 0.f;
     if (STATS) npairs++;
