@@ -99,7 +99,8 @@ def solve_bytes(trace, rpt):
 
 
 # kernel class -> kernel name in rocprofv3 output (traffic counters)
-KERNEL_NAMES = {'track_rows': 'mf::k_track<true, true, false>', 'track_rows_l0': 'mf::k_track<true, true, false> level 0', 'track_rows_obj': 'mf::k_track<true, true, true>',
+KERNEL_NAMES = {'track_rows': 'mf::k_track<true, true, false, 3>', 'track_rows_l0': 'mf::k_track<true, true, false, 3> level 0',
+                'track_rows_obj': 'mf::k_track<true, true, true, 3>',
                 'track_solve': 'mf::k_solve', 'integrate': 'mf::k_integrate', 'raycast': 'mf::k_raycast',
                 'alloc_band': 'mf::k_alloc'}
 
@@ -210,7 +211,7 @@ def main():
             chosen, us = g.autotune(K)
             tuned = {'rows_per_thread': [int(v) for v in chosen],
                      'us_per_iteration': {str(c): [round(float(x), 2) for x in us[:, i]]
-                                          for i, c in enumerate((1, 2, 4, 8, 16))}}
+                                          for i, c in enumerate(g.TUNE_CANDIDATES)}}
             print('autotune: %s' % json.dumps(tuned), file=sys.stderr, flush=True)
     g.sync()
     t = time.perf_counter()

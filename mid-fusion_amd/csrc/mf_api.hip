@@ -1611,7 +1611,7 @@ int mf_replay_track_rows(mf_ctx* c, const float k[4], int level, int reps, doubl
 }
 
 int mf_autotune(mf_ctx* c, const float k[4], int reps, int32_t* chosen, double* us) {
-  static const int cand[MF_TUNE_CANDIDATES] = {1, 2, 4, 8, 16};
+  static const int cand[MF_TUNE_CANDIDATES] = {1, 2, 3, 4, 8, 16};
   if (!k || reps < 1 || reps > 10000) return fail(c, MF_E_ARG, "bad arguments");
   if (!c->have_ref) return fail(c, MF_E_ARG, "no tracked frame yet");
   hipStream_t s = c->stream;

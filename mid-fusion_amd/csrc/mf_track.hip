@@ -81,6 +81,13 @@ __device__ __forceinline__ bool taps_all_miss(const uint8_t* tiles, int W, float
 
 // calc_robust_weight / calc_robust_residual (tracking.cpp:1474-1532).  The
 // reference's sq() takes float: (e/k) in double, squared in float.
+// Cauchy, the reference's default (tracking.h:342-345):
+__device__ __forceinline__ void robust_cauchy(float e, double k, float& w, float& loss) {
+  const float r = (float)((double)e / k);
+  const float q = r * r;
+  w = 1.0f / (1.0f + q);
+  loss = (float)(0.5 * k * k * log(1.0 + (double)q));
+}
 __device__ __forceinline__ void robust(float e, int fn, double k, float& w, float& loss) {
   switch (fn) {
     case 0: w = 1.f; loss = (float)(0.5 * e * e); break;
@@ -97,12 +104,7 @@ __device__ __forceinline__ void robust(float e, int fn, double k, float& w, floa
         loss = (float)(k * k / 6.0 * (1.0 - pow(1.0 - (double)(r * r), 3)));
       } else { w = 0.f; loss = (float)(k * k / 6.0); }
       break;
-    default: {
-      const float r = (float)((double)e / k);
-      const float q = r * r;
-      w = 1.0f / (1.0f + q);
-      loss = (float)(0.5 * k * k * log(1.0 + (double)q));
-    }
+    default: robust_cauchy(e, k, w, loss);
   }
 }
 
@@ -286,6 +288,12 @@ __device__ __forceinline__ int rgb_row(const TrackLaunch& p, const TrackInst& in
   return rgb_core<OBJ>(p, M, x, y, dr, vl, ir, e, J);
 }
 
+// RB = ROBUST_CAUCHY compiles the Cauchy weight alone; any other value the
+// run-time choice of fn.  The switch over all four functions (FP64 log and
+// pow among them) sets the row kernels' register count: 144 VGPRs for the
+// camera rows against 106 with Cauchy alone.
+static constexpr int ROBUST_CAUCHY = 3, ROBUST_ANY = -1;
+template <int RB>
 __device__ __forceinline__ void accum(float* s, int res, float e, const float* J, int fn, double k) {
   if (res < 1) {
     s[29] += res == -4 ? 1.f : 0.f;
@@ -294,7 +302,8 @@ __device__ __forceinline__ void accum(float* s, int res, float e, const float* J
     return;
   }
   float w, loss;
-  robust(e, fn, k, w, loss);
+  if (RB == ROBUST_CAUCHY) robust_cauchy(e, k, w, loss);
+  else robust(e, fn, k, w, loss);
   s[0] += loss;
   const float we = w * e;
 #pragma unroll
@@ -348,7 +357,7 @@ __device__ __forceinline__ void wg_reduce(float (&s)[32], float* __restrict__ pa
 // Rows of one instance at one level over the workgroups [0, nactive): FP32
 // per-thread sums, the wave butterfly, and this workgroup's 32 partial sums
 // in `part` (32 floats).
-template <bool ICP, bool RGB, bool OBJ, class MT>
+template <bool ICP, bool RGB, bool OBJ, int RB, class MT>
 __device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst& inst, const MT& M, int state,
                                            int wg, int nactive, float* __restrict__ part, float (*red)[32]) {
   const int tid = threadIdx.x;
@@ -378,7 +387,7 @@ __device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst
     } else {
       res = rgb_row<OBJ>(p, in, M, x, y, (size_t)i, e, J);
     }
-    accum(s, res, e, J, p.robust, p.robust_k);
+    accum<RB>(s, res, e, J, p.robust, p.robust_k);
     const size_t oo = icp ? ro : N0 + ro;
     if (in.codes && p.level == 0) in.codes[oo] = (int8_t)res;
     if (!icp && in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
@@ -782,7 +791,7 @@ __device__ __forceinline__ void stage_moving(const TrackLaunch& p, Mats& M, int 
 
 // Two-kernel form, rows: GN iteration q of every instance (grid.y =
 // instance), one partial of 32 FP32 sums per workgroup.
-template <bool ICP, bool RGB, bool OBJ>
+template <bool ICP, bool RGB, bool OBJ, int RB>
 __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __restrict__ insts,
                                               const GNState* __restrict__ st, float* __restrict__ partial,
                                               int nblk, int q) {
@@ -803,7 +812,7 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
   if (tid < 64) stage_moving(p, Msh, tid);
   __syncthreads();
   if (skip) return;
-  track_rows<ICP, RGB, OBJ>(p, insts[j], Msh, state, blockIdx.x, nblk,
+  track_rows<ICP, RGB, OBJ, RB>(p, insts[j], Msh, state, blockIdx.x, nblk,
                             partial + ((size_t)j * nblk + blockIdx.x) * 32, red);
 }
 
@@ -816,7 +825,7 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
 // of this workgroup at partial[(object * nblk + workgroup) * 32].  Only the
 // rendered depth and, near the object's silhouette, the model-map taps are
 // per-object loads.  Per-object pose matrices sit in dynamic LDS.
-template <bool ICP, bool RGB, int PPT>
+template <bool ICP, bool RGB, int PPT, int RB>
 __global__ __launch_bounds__(TB) void k_track_objs(TrackLaunch p, const TrackInst* __restrict__ insts, int ninst,
                                                    const GNState* __restrict__ st, float* __restrict__ partial,
                                                    int nblk, int q) {
@@ -866,7 +875,7 @@ __global__ __launch_bounds__(TB) void k_track_objs(TrackLaunch p, const TrackIns
       if (ICP) {
         float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
         const int res = obj_icp_core(p, in, M, nl[k], vl[k], cv[k], e, J);
-        accum(s, res, e, J, p.robust, p.robust_k);
+        accum<RB>(s, res, e, J, p.robust, p.robust_k);
         if (in.codes && p.level == 0) in.codes[ro] = (int8_t)res;
         if (dbg) { TrackRow& d = dbg[ro]; d.result = res; d.error = e; for (int u = 0; u < 6; u++) d.J[u] = J[u]; }
       }
@@ -874,7 +883,7 @@ __global__ __launch_bounds__(TB) void k_track_objs(TrackLaunch p, const TrackIns
         float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
         const float dr = ldx(in.rD + p.rd_off, (uint32_t)i);
         const int res = (dr > 0.0f) ? rgb_core<true>(p, M, x, y, dr, rv[k], ir[k], e, J) : -1;
-        accum(s, res, e, J, p.robust, p.robust_k);
+        accum<RB>(s, res, e, J, p.robust, p.robust_k);
         if (in.codes && p.level == 0) in.codes[N0 + ro] = (int8_t)res;
         if (in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
         if (dbg) { TrackRow& d = dbg[N0 + ro]; d.result = res; d.error = e; for (int u = 0; u < 6; u++) d.J[u] = J[u]; }
@@ -925,7 +934,7 @@ __global__ __launch_bounds__(SOLVE_NT) void k_solve(int level, int iter, const T
 // rows == 0 the launch only finishes iteration q-1.  Each workgroup reads all
 // of the previous partials, so this form pays nblk^2 partial reads: it suits
 // levels with few workgroups.
-template <bool ICP, bool RGB, bool OBJ>
+template <bool ICP, bool RGB, bool OBJ, int RB>
 __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* __restrict__ insts,
                                                 GNState* __restrict__ st, const float* __restrict__ part_prev,
                                                 int nblk_prev, float* __restrict__ part_cur, int nblk, int q,
@@ -957,7 +966,7 @@ __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* 
   if (tid < 64) stage_moving(p, Msh, tid);
   __syncthreads();
   if (skip) return;
-  track_rows<ICP, RGB, OBJ>(p, insts[j], Msh, state, blockIdx.x, nblk, part_cur + ((size_t)j * nblk + blockIdx.x) * 32,
+  track_rows<ICP, RGB, OBJ, RB>(p, insts[j], Msh, state, blockIdx.x, nblk, part_cur + ((size_t)j * nblk + blockIdx.x) * 32,
                             red);
 }
 
@@ -1224,15 +1233,18 @@ int track_blocks(int nrows, int rows_per_thread) {
 void launch_track_rows(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
                        float* partial, int nblk, int q) {
   dim3 grd(nblk, ninst), blk(TB);
-  if (p.object) {
-    if (p.use_icp && p.use_rgb) k_track<true, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
-    else if (p.use_icp) k_track<true, false, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
-    else k_track<false, true, true><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
-  } else {
-    if (p.use_icp && p.use_rgb) k_track<true, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
-    else if (p.use_icp) k_track<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
-    else k_track<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
+#define MF_TRACK(RB)                                                                                   \
+  if (p.object) {                                                                                      \
+    if (p.use_icp && p.use_rgb) k_track<true, true, true, RB><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q); \
+    else if (p.use_icp) k_track<true, false, true, RB><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);        \
+    else k_track<false, true, true, RB><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);                       \
+  } else {                                                                                             \
+    if (p.use_icp && p.use_rgb) k_track<true, true, false, RB><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q); \
+    else if (p.use_icp) k_track<true, false, false, RB><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);        \
+    else k_track<false, true, false, RB><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);                       \
   }
+  if (p.robust == ROBUST_CAUCHY) { MF_TRACK(ROBUST_CAUCHY) } else { MF_TRACK(ROBUST_ANY) }
+#undef MF_TRACK
 }
 int track_obj_blocks(int npix, int pixels_per_thread) {
   const int ppt = pixels_per_thread < 1 ? 1 : (pixels_per_thread > 2 ? 2 : pixels_per_thread);
@@ -1242,11 +1254,16 @@ void launch_track_rows_objs(hipStream_t s, const TrackLaunch& p, const TrackInst
                             float* partial, int nblk, int pixels_per_thread, int q) {
   const size_t lds = (size_t)ninst * sizeof(Mats);
   dim3 grd(nblk), blk(TB);
-#define MF_OBJS(PPT)                                                                                                  \
-  if (p.use_icp && p.use_rgb) k_track_objs<true, true, PPT><<<grd, blk, lds, s>>>(p, inst, ninst, st, partial, nblk, q); \
-  else if (p.use_icp) k_track_objs<true, false, PPT><<<grd, blk, lds, s>>>(p, inst, ninst, st, partial, nblk, q);        \
-  else k_track_objs<false, true, PPT><<<grd, blk, lds, s>>>(p, inst, ninst, st, partial, nblk, q);
-  if (pixels_per_thread >= 2) { MF_OBJS(2) } else { MF_OBJS(1) }
+#define MF_OBJS(PPT, RB)                                                                                                  \
+  if (p.use_icp && p.use_rgb) k_track_objs<true, true, PPT, RB><<<grd, blk, lds, s>>>(p, inst, ninst, st, partial, nblk, q); \
+  else if (p.use_icp) k_track_objs<true, false, PPT, RB><<<grd, blk, lds, s>>>(p, inst, ninst, st, partial, nblk, q);        \
+  else k_track_objs<false, true, PPT, RB><<<grd, blk, lds, s>>>(p, inst, ninst, st, partial, nblk, q);
+  const bool cauchy = p.robust == ROBUST_CAUCHY;
+  if (pixels_per_thread >= 2) {
+    if (cauchy) { MF_OBJS(2, ROBUST_CAUCHY) } else { MF_OBJS(2, ROBUST_ANY) }
+  } else {
+    if (cauchy) { MF_OBJS(1, ROBUST_CAUCHY) } else { MF_OBJS(1, ROBUST_ANY) }
+  }
 #undef MF_OBJS
 }
 void launch_track_solve(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
@@ -1257,15 +1274,18 @@ void launch_gn_fused(hipStream_t s, const TrackLaunch& p, const TrackInst* inst,
                      const float* part_prev, int nblk_prev, float* part_cur, int nblk, int q, int rows,
                      const TrackSync& sy) {
   dim3 grd(rows ? nblk : 1, ninst), blk(TB);
-  if (p.object) {
-    if (p.use_icp && p.use_rgb) k_gn_iter<true, true, true><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
-    else if (p.use_icp) k_gn_iter<true, false, true><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
-    else k_gn_iter<false, true, true><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
-  } else {
-    if (p.use_icp && p.use_rgb) k_gn_iter<true, true, false><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
-    else if (p.use_icp) k_gn_iter<true, false, false><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
-    else k_gn_iter<false, true, false><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy);
+#define MF_FUSED(RB) \
+  if (p.object) { \
+    if (p.use_icp && p.use_rgb) k_gn_iter<true, true, true, RB><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy); \
+    else if (p.use_icp) k_gn_iter<true, false, true, RB><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy); \
+    else k_gn_iter<false, true, true, RB><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy); \
+  } else { \
+    if (p.use_icp && p.use_rgb) k_gn_iter<true, true, false, RB><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy); \
+    else if (p.use_icp) k_gn_iter<true, false, false, RB><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy); \
+    else k_gn_iter<false, true, false, RB><<<grd, blk, 0, s>>>(p, inst, st, part_prev, nblk_prev, part_cur, nblk, q, rows, sy); \
   }
+  if (p.robust == ROBUST_CAUCHY) { MF_FUSED(ROBUST_CAUCHY) } else { MF_FUSED(ROBUST_ANY) }
+#undef MF_FUSED
 }
 void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNState* st, int state, const Mat4& T,
                  float* partial, int nblk) {

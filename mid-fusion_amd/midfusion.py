@@ -472,12 +472,12 @@ class Kfusion:
 
     def autotune(self, k, reps=30):
         """Launch-shape autotuner: times each pyramid level's rows-per-thread
-        candidates {1, 2, 4, 8, 16} on the last tracked frame and keeps the
+        candidates TUNE_CANDIDATES on the last tracked frame and keeps the
         fastest.  Returns (chosen per level, us per iteration [level, cand])."""
-        nl = self.n_levels
-        chosen, us = np.zeros(nl, np.int32), np.zeros(nl * 5)
+        nl, nc = self.n_levels, len(self.TUNE_CANDIDATES)
+        chosen, us = np.zeros(nl, np.int32), np.zeros(nl * nc)
         self._chk(self.L.mf_autotune(self.ctx, _f32(k), int(reps), chosen, us))
-        return chosen, us.reshape(nl, 5)
+        return chosen, us.reshape(nl, nc)
 
     def solve6(self, s27):
         """Device 6x6 solve of systems s27 [n, 27] (JtWe, packed upper JtWJ):
@@ -504,6 +504,7 @@ class Kfusion:
     TUNE_TRACK_ROWS_PER_THREAD_L0 = 0   # + pyramid level
     TUNE_TRACK_FUSED_L0 = 8             # + pyramid level
     TUNE_OBJ_PIXELS_PER_THREAD_L0 = 16  # + pyramid level
+    TUNE_CANDIDATES = (1, 2, 3, 4, 8, 16)   # rows per thread tried by autotune()
 
     def set_tuning(self, knob, value):
         self._chk(self.L.mf_set_tuning(self.ctx, knob, int(value)))

@@ -39,11 +39,11 @@ def test_autotune_leaves_state_and_sets_tuning():
             assert np.array_equal(ga.trackresult(0), codes)
             v2, n2, l2 = ga.maps()
             assert np.array_equal(v2, vtx) and np.array_equal(n2, nrm) and np.array_equal(l2, lb)
-            assert chosen.shape == (3,) and all(c in (1, 2, 4, 8, 16) for c in chosen)
+            assert chosen.shape == (3,) and all(c in ga.TUNE_CANDIDATES for c in chosen)
             assert np.all(us > 0)
             for lvl in range(3):
                 assert ga.get_tuning(ga.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl) == chosen[lvl]
-                assert us[lvl, [1, 2, 4, 8, 16].index(chosen[lvl])] == us[lvl].min()
+                assert us[lvl, list(ga.TUNE_CANDIDATES).index(chosen[lvl])] == us[lvl].min()
         if f == 2:
             A, B = ga.getPose(), gb.getPose()
             assert np.linalg.norm(A[:3, 3] - B[:3, 3]) < 1e-5
