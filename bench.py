@@ -131,7 +131,9 @@ def main():
     ap.add_argument('--raw-stream', action='store_true',
                     help='also time the same frames read from a .raw sequence file through pinned staging '
                          '(disk + PCIe inclusive; never the headline value)')
-    ap.add_argument('--cpu-frames', type=int, default=3)
+    ap.add_argument('--cpu-frames', type=int, default=0,
+                    help='frames of the CPU baseline sample (default: 120, about 10 s of oracle work on 16 threads; '
+                         '12 with --hd)')
     ap.add_argument('--track-fused', type=str, default='', metavar='F0,F1,F2',
                     help='1 per pyramid level: one fused launch per GN iteration (default: library default)')
     ap.add_argument('--track-rpt', type=str, default='', metavar='R0,R1,R2',
@@ -173,7 +175,10 @@ def main():
     sc, movers = make_scene(scene, n_obj)
     n_frames = args.warmup + args.steps
     t0 = time.time()
-    frames = render_frames(sc, n_obj, n_frames + 3)
+    if args.cpu_frames <= 0:
+        args.cpu_frames = 12 if HD else 120
+    n_render = max(n_frames + 3, 0 if args.no_cpu else args.cpu_frames)
+    frames = render_frames(sc, n_obj, n_render)
     print('rendered %d frames in %.1fs' % (len(frames), time.time() - t0), file=sys.stderr, flush=True)
 
     P0 = sc.camera_pose(0).astype(np.float32)
@@ -291,8 +296,8 @@ def main():
             times.append(time.perf_counter() - t1)
         cpu_fps = 1.0 / float(np.mean(times[1:]))   # frame 0 only seeds the tracker
         cpu = {'value': round(cpu_fps, 4), 'unit': 'frames/s', 'cores': nthreads, 'kind': 'port',
-               'sample': 'oracle (OpenMP C++ restatement, -O2) on frames 1..%d of the same workload, frame 0 '
-                         'as warm-up' % (nc - 1)}
+               'sample': 'oracle (OpenMP C++ restatement, -O2) on frames 1..%d of the same workload (%.1f s of CPU '
+                         'work), frame 0 as warm-up' % (nc - 1, float(np.sum(times[1:])))}
 
     render = None
     if args.render:
