@@ -293,14 +293,12 @@ __device__ __forceinline__ int rgb_row(const TrackLaunch& p, const TrackInst& in
 // pow among them) sets the row kernels' register count: 144 VGPRs for the
 // camera rows against 106 with Cauchy alone.
 static constexpr int ROBUST_CAUCHY = 3, ROBUST_ANY = -1;
-// The four row counts (sums 28-31: inliers, result -4, result -5, result
-// > -4) are kept as 8-bit fields of `cnt` in the row loop (rows per thread <=
-// 64) and become floats in unpack_counts, before the reduction: the same
-// exact integer values, three registers fewer across the loop.
 template <int RB>
-__device__ __forceinline__ void accum(float* s, uint32_t& cnt, int res, float e, const float* J, int fn, double k) {
+__device__ __forceinline__ void accum(float* s, int res, float e, const float* J, int fn, double k) {
   if (res < 1) {
-    cnt += (res == -4 ? (1u << 8) : 0u) + (res == -5 ? (1u << 16) : 0u) + (res > -4 ? (1u << 24) : 0u);
+    s[29] += res == -4 ? 1.f : 0.f;
+    s[30] += res == -5 ? 1.f : 0.f;
+    s[31] += res > -4 ? 1.f : 0.f;
     return;
   }
   float w, loss;
@@ -317,13 +315,7 @@ __device__ __forceinline__ void accum(float* s, uint32_t& cnt, int res, float e,
 #pragma unroll
     for (int b = a; b < 6; b++) s[c++] += wa * J[b];
   }
-  cnt += 1u;
-}
-__device__ __forceinline__ void unpack_counts(float* s, uint32_t cnt) {
-  s[28] = (float)(cnt & 0xffu);
-  s[29] = (float)((cnt >> 8) & 0xffu);
-  s[30] = (float)((cnt >> 16) & 0xffu);
-  s[31] = (float)(cnt >> 24);
+  s[28] += 1.f;
 }
 
 
@@ -372,8 +364,7 @@ __device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst
   const TrackInst in = inst;   // descriptor held in registers across the row loop
   float s[32];
 #pragma unroll
-  for (int k = 0; k < 28; k++) s[k] = 0.f;
-  uint32_t cnt = 0;
+  for (int k = 0; k < 32; k++) s[k] = 0.f;
   const int n = p.w * p.h;
   const int nrows = (ICP ? n : 0) + (RGB ? n : 0);
   const size_t N0 = (size_t)p.W * p.H;
@@ -396,13 +387,12 @@ __device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst
     } else {
       res = rgb_row<OBJ>(p, in, M, x, y, (size_t)i, e, J);
     }
-    accum<RB>(s, cnt, res, e, J, p.robust, p.robust_k);
+    accum<RB>(s, res, e, J, p.robust, p.robust_k);
     const size_t oo = icp ? ro : N0 + ro;
     if (in.codes && p.level == 0) in.codes[oo] = (int8_t)res;
     if (!icp && in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
     if (dbg) { TrackRow& d = dbg[oo]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
   }
-  unpack_counts(s, cnt);
   wg_reduce(s, part, red);
 }
 
@@ -874,8 +864,7 @@ __global__ __launch_bounds__(TB) void k_track_objs(TrackLaunch p, const TrackIns
     TrackRow* dbg = (p.dbg && in.state < 2) ? p.dbg + ((size_t)in.state * p.n_levels + p.level) * 2 * N0 : nullptr;
     float s[32];
 #pragma unroll
-    for (int k = 0; k < 28; k++) s[k] = 0.f;
-    uint32_t cnt = 0;
+    for (int k = 0; k < 32; k++) s[k] = 0.f;
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
       asm volatile("" ::: "memory");   // keep the LDS matrix reads inside the loop
@@ -886,7 +875,7 @@ __global__ __launch_bounds__(TB) void k_track_objs(TrackLaunch p, const TrackIns
       if (ICP) {
         float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
         const int res = obj_icp_core(p, in, M, nl[k], vl[k], cv[k], e, J);
-        accum<RB>(s, cnt, res, e, J, p.robust, p.robust_k);
+        accum<RB>(s, res, e, J, p.robust, p.robust_k);
         if (in.codes && p.level == 0) in.codes[ro] = (int8_t)res;
         if (dbg) { TrackRow& d = dbg[ro]; d.result = res; d.error = e; for (int u = 0; u < 6; u++) d.J[u] = J[u]; }
       }
@@ -894,13 +883,12 @@ __global__ __launch_bounds__(TB) void k_track_objs(TrackLaunch p, const TrackIns
         float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
         const float dr = ldx(in.rD + p.rd_off, (uint32_t)i);
         const int res = (dr > 0.0f) ? rgb_core<true>(p, M, x, y, dr, rv[k], ir[k], e, J) : -1;
-        accum<RB>(s, cnt, res, e, J, p.robust, p.robust_k);
+        accum<RB>(s, res, e, J, p.robust, p.robust_k);
         if (in.codes && p.level == 0) in.codes[N0 + ro] = (int8_t)res;
         if (in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
         if (dbg) { TrackRow& d = dbg[N0 + ro]; d.result = res; d.error = e; for (int u = 0; u < 6; u++) d.J[u] = J[u]; }
       }
     }
-    unpack_counts(s, cnt);
     wg_reduce(s, partial + ((size_t)j * nblk + blockIdx.x) * 32, red);
   }
 }
