@@ -240,8 +240,9 @@ int mf_get_tuning(mf_ctx* ctx, int knob, int* value);
 /* Launch-shape autotuner (no counterpart in the reference): for every
  * pyramid level, times bursts of `reps` GN iterations of the camera (row
  * launch + solve launch) on the last tracked frame for each of the
- * MF_TUNE_CANDIDATES rows-per-thread values {1, 2, 3, 4, 8, 16}, and sets
- * MF_TUNE_TRACK_ROWS_PER_THREAD_L0 + level to the fastest.  The iterations
+ * MF_TUNE_CANDIDATES rows-per-thread values {1, 2, 3, 4, 8, 16} (best of
+ * three bursts each), and sets MF_TUNE_TRACK_ROWS_PER_THREAD_L0 + level to the
+ * fastest, keeping the current value unless another is more than 1 % faster.  The iterations
  * run on a scratch copy of the camera's GN state: poses, result codes and
  * maps are left as they were.  us[level * MF_TUNE_CANDIDATES + i] (or null)
  * gets the measured microseconds per iteration, chosen[level] (or null) the

@@ -43,7 +43,9 @@ def test_autotune_leaves_state_and_sets_tuning():
             assert np.all(us > 0)
             for lvl in range(3):
                 assert ga.get_tuning(ga.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl) == chosen[lvl]
-                assert us[lvl, list(ga.TUNE_CANDIDATES).index(chosen[lvl])] == us[lvl].min()
+                # the fastest, or the previous setting within 1 % of it
+                t = us[lvl, list(ga.TUNE_CANDIDATES).index(chosen[lvl])]
+                assert t == us[lvl].min() or t <= us[lvl].min() / 0.99
         if f == 2:
             A, B = ga.getPose(), gb.getPose()
             assert np.linalg.norm(A[:3, 3] - B[:3, 3]) < 1e-5
