@@ -138,6 +138,8 @@ def main():
                     help='1 per pyramid level: one fused launch per GN iteration (default: library default)')
     ap.add_argument('--track-rpt', type=str, default='', metavar='R0,R1,R2',
                     help='tracking rows per thread per pyramid level (default: library default)')
+    ap.add_argument('--obj-rpt', type=str, default='', metavar='R0,R1,R2',
+                    help='moving objects\' rows per thread per pyramid level (0: the camera\'s; default: library default)')
     ap.add_argument('--obj-ppt', type=str, default='', metavar='P0,P1,P2',
                     help='moving objects\' rows per pyramid level: pixels per thread of the shared pixel grid '
                          '(1, 2) or 0 for one row grid per object (default: library default)')
@@ -187,6 +189,8 @@ def main():
         g.set_tuning(g.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl, int(v))
     for lvl, v in enumerate(x for x in args.track_fused.split(',') if x):
         g.set_tuning(g.TUNE_TRACK_FUSED_L0 + lvl, int(v))
+    for lvl, v in enumerate(x for x in args.obj_rpt.split(',') if x):
+        g.set_tuning(g.TUNE_OBJ_ROWS_PER_THREAD_L0 + lvl, int(v))
     for lvl, v in enumerate(x for x in args.obj_ppt.split(',') if x):
         g.set_tuning(g.TUNE_OBJ_PIXELS_PER_THREAD_L0 + lvl, int(v))
     for i, m in enumerate(movers):
@@ -244,9 +248,10 @@ def main():
     # algorithmic bytes per kernel class over the 3 frames
     N0 = W * H
     rpt = [g.get_tuning(g.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl) for lvl in range(4)]
+    orpt = [g.get_tuning(g.TUNE_OBJ_ROWS_PER_THREAD_L0 + lvl) or rpt[lvl] for lvl in range(4)]
     alg = {
         'track_rows': track_bytes(tr, rpt, False),
-        'track_rows_obj': track_bytes(tr, rpt, True),
+        'track_rows_obj': track_bytes(tr, orpt, True),
         'track_solve': solve_bytes(tr, rpt),
         'integrate': 20 * N0 * (1 + n_obj) + 72 * st['voxel_updates'] + 16 * st['blocks_integrated'],
         'raycast': 12 * st['march_steps'] + 32 * st['trilinear_evals'] + 128 * st['hits'] + 28 * N0 * 3,

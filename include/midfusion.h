@@ -234,7 +234,10 @@ long mf_device_bytes(mf_ctx* ctx);
  *   one grid over the pixels, each thread loading its pixels' frame inputs
  *   once for every object, with 1 or 2 pixels per thread; 0 (default) gives
  *   each object its own row grid in the same launch. */
-enum { MF_TUNE_TRACK_ROWS_PER_THREAD_L0 = 0, MF_TUNE_TRACK_FUSED_L0 = 8, MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 = 16 };
+/*   MF_TUNE_OBJ_ROWS_PER_THREAD_L0 + l: rows per thread of the moving
+ *   objects' row grids at level l, 1..64, or 0 (default) for the camera's. */
+enum { MF_TUNE_TRACK_ROWS_PER_THREAD_L0 = 0, MF_TUNE_TRACK_FUSED_L0 = 8, MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 = 16,
+       MF_TUNE_OBJ_ROWS_PER_THREAD_L0 = 24 };
 int mf_set_tuning(mf_ctx* ctx, int knob, int value);
 int mf_get_tuning(mf_ctx* ctx, int knob, int* value);
 /* Launch-shape autotuner (no counterpart in the reference): for every

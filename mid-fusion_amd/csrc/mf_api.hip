@@ -78,6 +78,8 @@ struct mf_ctx {
   // thread (1, 2), or 0: one row grid per object (grid.y), per pyramid level.
   // 0 everywhere measured fastest on MI355X at 1, 8 and 16 objects.
   int track_obj_ppt[MAX_LEVELS] = {0, 0, 0, 0};
+  // rows per thread of the objects' row grids per pyramid level (0: the camera's)
+  int track_obj_rpt[MAX_LEVELS] = {0, 0, 0, 0};
   uint8_t* moving = nullptr;     // [MAX_INST] label-indexed flags
   TraceRec* trace = nullptr;
   int* trace_n = nullptr;
@@ -344,7 +346,8 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
     const bool fused = c->track_fused[l] != 0;
     const bool objs_grid = object && c->cfg.batch_objects && !fused && c->track_obj_ppt[l] > 0;
     const int nblk = objs_grid ? track_obj_blocks(p.w * p.h, c->track_obj_ppt[l])
-                               : track_blocks(p.w * p.h * kinds, c->track_rpt[l]);
+                               : track_blocks(p.w * p.h * kinds,
+                                              (object && c->track_obj_rpt[l] > 0) ? c->track_obj_rpt[l] : c->track_rpt[l]);
     if (!fused) finish();
     for (int it = 0; it < c->cfg.iters[l]; ++it, ++q) {
       p.iter = it;
@@ -423,7 +426,8 @@ std::vector<uint64_t> base_key(mf_ctx* c, int stage, const float k[4]) {
                                fbits(k[0]), fbits(k[1]), fbits(k[2]), fbits(k[3]), (uint64_t)c->trace_on,
                                (uint64_t)(uintptr_t)c->dbg_rows};
   for (int l = 0; l < MAX_LEVELS; l++)
-    key.push_back((uint64_t)c->track_rpt[l] | ((uint64_t)c->track_fused[l] << 32) | ((uint64_t)c->track_obj_ppt[l] << 40));
+    key.push_back((uint64_t)c->track_rpt[l] | ((uint64_t)c->track_fused[l] << 32) | ((uint64_t)c->track_obj_ppt[l] << 40) |
+                  ((uint64_t)c->track_obj_rpt[l] << 48));
   return key;
 }
 
@@ -1489,6 +1493,11 @@ long mf_device_bytes(mf_ctx* c) { return c->bytes; }
 
 int mf_set_tuning(mf_ctx* c, int knob, int value) {
   if (!c) return MF_E_ARG;
+  if (knob >= MF_TUNE_OBJ_ROWS_PER_THREAD_L0 && knob < MF_TUNE_OBJ_ROWS_PER_THREAD_L0 + MAX_LEVELS) {
+    if (value < 0 || value > 64) return fail(c, MF_E_ARG, "object rows per thread must be in [0, 64]");
+    c->track_obj_rpt[knob - MF_TUNE_OBJ_ROWS_PER_THREAD_L0] = value;
+    return 0;
+  }
   if (knob >= MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 && knob < MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 + MAX_LEVELS) {
     if (value < 0 || value > 2) return fail(c, MF_E_ARG, "object pixels per thread must be 0, 1 or 2");
     c->track_obj_ppt[knob - MF_TUNE_OBJ_PIXELS_PER_THREAD_L0] = value;
@@ -1507,6 +1516,10 @@ int mf_set_tuning(mf_ctx* c, int knob, int value) {
 }
 int mf_get_tuning(mf_ctx* c, int knob, int* value) {
   if (!c || !value) return MF_E_ARG;
+  if (knob >= MF_TUNE_OBJ_ROWS_PER_THREAD_L0 && knob < MF_TUNE_OBJ_ROWS_PER_THREAD_L0 + MAX_LEVELS) {
+    *value = c->track_obj_rpt[knob - MF_TUNE_OBJ_ROWS_PER_THREAD_L0];
+    return 0;
+  }
   if (knob >= MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 && knob < MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 + MAX_LEVELS) {
     *value = c->track_obj_ppt[knob - MF_TUNE_OBJ_PIXELS_PER_THREAD_L0];
     return 0;
