@@ -148,6 +148,13 @@ long mf_object_keys(mf_ctx* ctx, int id, uint64_t* out, long cap);
 /* Voxels of the block holding voxel (x,y,z): 512 x {tsdf, w, r, g, b, c, fg,
  * fgw} as double, index x + 8y + 64z.  Returns 1, or 0 if unallocated. */
 int mf_object_block(mf_ctx* ctx, int id, int x, int y, int z, double* out);
+/* Bulk form of mf_object_block: voxels of the n blocks with the given keys
+ * (mf_object_keys encoding), out[n][512][8] as double {tsdf, w, r, g, b, c,
+ * fg, fgw}, index x + 8y + 64z; a key without an allocated block reads as
+ * voxel_traits<SDF>::initValue() (tsdf 1, the rest 0).  Returns how many keys
+ * were allocated.  (Octree contents for checks against the reference's
+ * VoxelBlock data, node.hpp:83-157.) */
+long mf_object_voxels(mf_ctx* ctx, int id, const uint64_t* keys, long n, double* out);
 /* Number of internal octree nodes implied by the allocated blocks. */
 long mf_object_node_count(mf_ctx* ctx, int id);
 

@@ -1331,6 +1331,40 @@ int mf_object_block(mf_ctx* c, int id, int x, int y, int z, double* out) {
   return 1;
 }
 
+long mf_object_voxels(mf_ctx* c, int id, const uint64_t* keys, long n, double* out) {
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  if (n < 0 || (n > 0 && (!keys || !out))) return fail(c, MF_E_ARG, "bad voxel export arguments");
+  const VolDev& v = c->objs[id]->v;
+  hipStream_t s = c->stream;
+  const long chunk = 4096;   // keys per round: 128 MB of doubles on the device
+  const long cn = std::min(n, chunk);
+  uint64_t* d_keys = nullptr;
+  double* d_out = nullptr;
+  int* d_found = nullptr;
+  if (cn > 0 && (hipMalloc(&d_keys, cn * sizeof(uint64_t)) != hipSuccess ||
+                 hipMalloc(&d_out, cn * BLOCK_VOX * 8 * sizeof(double)) != hipSuccess ||
+                 hipMalloc(&d_found, cn * sizeof(int)) != hipSuccess)) {
+    hipFree(d_keys); hipFree(d_out); hipFree(d_found);
+    return fail(c, MF_E_OOM, "voxel export buffers");
+  }
+  long nfound = 0;
+  std::vector<int> f(cn);
+  for (long k0 = 0; k0 < n; k0 += chunk) {
+    const long k = std::min(chunk, n - k0);
+    hipMemcpyAsync(d_keys, keys + k0, k * sizeof(uint64_t), hipMemcpyHostToDevice, s);
+    launch_export_voxels(s, v, d_keys, (int)k, d_out, d_found);
+    hipMemcpyAsync(out + k0 * BLOCK_VOX * 8, d_out, k * BLOCK_VOX * 8 * sizeof(double), hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(f.data(), d_found, k * sizeof(int), hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      hipFree(d_keys); hipFree(d_out); hipFree(d_found);
+      return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+    }
+    for (long i = 0; i < k; i++) nfound += f[i];
+  }
+  hipFree(d_keys); hipFree(d_out); hipFree(d_found);
+  return nfound;
+}
+
 long mf_object_node_count(mf_ctx* c, int id) {
   if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
   const VolDev& v = c->objs[id]->v;

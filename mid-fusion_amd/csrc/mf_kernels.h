@@ -19,6 +19,10 @@ __device__ __forceinline__ T ldx(const T* base, uint32_t idx) {
   return gload((const T*)((const char*)base + (size_t)(idx * (uint32_t)sizeof(T))));
 }
 
+// Voxels of the leaf blocks with the given keys into out[n][512][8] (double
+// {tsdf, w, r, g, b, c, fg, fgw}) and found[n] (mf_map.hip).
+void launch_export_voxels(hipStream_t st, const VolDev& v, const uint64_t* keys, int n, double* out, int* found);
+
 // preprocessing (mf_pre.hip)
 void launch_convert(hipStream_t s, const uint16_t* din, const uint8_t* rgb, int iw, int ratio, int ow, int oh,
                     float* depth, float* grey, float* r, float* g, float* b);

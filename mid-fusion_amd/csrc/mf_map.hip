@@ -145,6 +145,33 @@ __global__ void k_alloc_commit(const MapObj* __restrict__ objs, int n, MapWork w
   }
 }
 
+// Voxels of the blocks with the given keys (keyops::encode of a leaf block:
+// the block's Morton index shifted left by 9, level in the low bits), one
+// 512-thread workgroup per key: out[k][voxel x + 8y + 64z][8] as double
+// {tsdf, w, r, g, b, c, fg, fgw}.  A key without an allocated block gives
+// voxel_traits<SDF>::initValue() (tsdf 1, everything else 0) and found 0.
+__global__ __launch_bounds__(512) void k_export_voxels(VolDev v, const uint64_t* __restrict__ keys, int n,
+                                                       double* __restrict__ out, int* __restrict__ found) {
+  const uint64_t nb3 = (uint64_t)v.nb * v.nb * v.nb;
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    const uint64_t m = keys[k] >> 9;
+    const int32_t s = m < nb3 ? v.slot[m] : -1;
+    const int i = threadIdx.x;
+    double* d = out + ((size_t)k * BLOCK_VOX + i) * 8;
+    if (s < 0) {
+      d[0] = 1.0; d[1] = 0.0; d[2] = 0.0; d[3] = 0.0; d[4] = 0.0; d[5] = 0.0; d[6] = 0.0; d[7] = 0.0;
+    } else {
+      const size_t vi = (size_t)s * BLOCK_VOX + i;
+      d[0] = v.tsdf[vi]; d[1] = v.wt[vi]; d[2] = v.r[vi]; d[3] = v.g[vi]; d[4] = v.b[vi];
+      d[5] = v.c[vi]; d[6] = v.fg[vi]; d[7] = v.fgw[vi];
+    }
+    if (i == 0) found[k] = s >= 0 ? 1 : 0;
+  }
+}
+void launch_export_voxels(hipStream_t st, const VolDev& v, const uint64_t* keys, int n, double* out, int* found) {
+  if (n > 0) k_export_voxels<<<n < 4096 ? n : 4096, BLOCK_VOX, 0, st>>>(v, keys, n, out, found);
+}
+
 // Voxel initialisation of this frame's new blocks (voxel_traits<SDF>::initValue,
 // continuous/volume_traits.hpp:38), one 512-thread workgroup per block.
 __global__ __launch_bounds__(512) void k_init_new(const MapObj* __restrict__ objs, MapWork wl) {

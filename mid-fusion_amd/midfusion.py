@@ -93,6 +93,7 @@ SIGNATURES = {
     'mf_get_trackresult': (C.c_int, [P, C.c_int, i32p]),
     'mf_get_trace': (C.c_long, [P, f64p, C.c_long]),
     'mf_solve6': (C.c_int, [P, f64p, C.c_long, f64p, i32p]),
+    'mf_object_voxels': (C.c_long, [P, C.c_int, np.ctypeslib.ndpointer(np.uint64, flags='C_CONTIGUOUS'), C.c_long, f64p]),
     'mf_autotune': (C.c_int, [P, f32p, C.c_int, i32p, f64p]),
     'mf_clear_trace': (None, [P]),
     'mf_get_level': (C.c_long, [P, C.c_int, C.c_int, f32p]),
@@ -407,6 +408,16 @@ class Kfusion:
         out = np.zeros(512 * 8)
         ok = self._chk(self.L.mf_object_block(self.ctx, oid, x, y, z, out))
         return out.reshape(512, 8) if ok else None
+
+    def voxels(self, oid, keys):
+        """Voxels of the blocks with `keys` (mf_object_keys encoding):
+        (array [n, 512, 8] of {tsdf, w, r, g, b, c, fg, fgw}, allocated count)."""
+        keys = np.ascontiguousarray(keys, np.uint64)
+        out = np.zeros(len(keys) * 512 * 8)
+        n = self.L.mf_object_voxels(self.ctx, oid, keys, len(keys), out)
+        if n < 0:
+            self._chk(n)
+        return out.reshape(len(keys), 512, 8), int(n)
 
     def node_count(self, oid):
         return self._chk(self.L.mf_object_node_count(self.ctx, oid))

@@ -61,9 +61,11 @@ def tsdf_vs(gpu, orc, i, max_blocks=3000):
     xyz = np.zeros(3, np.uint32)
     wt = ww = 0.0
     bad = n = 0
-    for k in keys:
+    bulk = gpu.voxels(i, keys)[0] if hasattr(gpu, 'voxels') else None
+    for kk, k in enumerate(keys):
         L.orc_unpack(int(k) & ~0x1FF, xyz)
-        bg, bo = gpu.block(i, *map(int, xyz)), orc.block(i, *map(int, xyz))
+        bg = bulk[kk] if bulk is not None else gpu.block(i, *map(int, xyz))
+        bo = orc.block(i, *map(int, xyz))
         if bg is None or bo is None:
             bad += 512
             n += 512

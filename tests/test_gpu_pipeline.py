@@ -111,11 +111,15 @@ def voxel_check(stage, f, gpu, orc, r):
         keys = orc.keys(i)
         worst = np.zeros(8)
         bad = n = 0
-        for k in keys:
+        vg, found = gpu.voxels(i, keys)      # bulk export (mf_object_voxels)
+        assert found == len(keys), (i, found, len(keys))
+        for kk, k in enumerate(keys):
             L.orc_unpack(int(k) & ~0x1FF, xyz)
-            bg = gpu.block(i, int(xyz[0]), int(xyz[1]), int(xyz[2]))
+            bg = vg[kk]
+            if kk < 3:   # the bulk export equals the per-block one
+                assert np.array_equal(bg, gpu.block(i, int(xyz[0]), int(xyz[1]), int(xyz[2])))
             bo = orc.block(i, int(xyz[0]), int(xyz[1]), int(xyz[2]))
-            assert bg is not None and bo is not None, (i, k)
+            assert bo is not None, (i, k)
             d = np.abs(bg - bo)
             worst = np.maximum(worst, d.max(axis=0))
             tol = np.array([1e-4, 0, 1e-4, 1e-4, 1e-4, 0, 1e-6, 0])
