@@ -58,7 +58,8 @@ void mf_destroy(mf_ctx* ctx);
 const char* mf_last_error(mf_ctx* ctx);
 
 /* Object::Object (object/object.cpp:19-46); res a power of two in [16, 8192].  Returns the instance id
- * (= instance_label_), capacity = max voxel blocks (0: default). */
+ * (= instance_label_), capacity = max voxel blocks (0: min(blocks in the volume, 262144); at most
+ * 2^21, else MF_E_ARG; the same limit holds for mf_create's bg_capacity). */
 int mf_object_create(mf_ctx* ctx, float size_m, int res, const float T_w_o[16], long capacity);
 /* Object::set_static_state (object.cpp:317-320): moving objects are tracked
  * by trackEachObject and masked out of the camera re-track. */

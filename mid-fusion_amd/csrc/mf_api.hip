@@ -223,6 +223,11 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   const long nb3 = (long)v.nb * v.nb * v.nb;
   if (cap <= 0) cap = std::min(nb3, (long)262144);
   cap = std::min(cap, nb3);
+  // the raycast addresses TSDF voxels by a 32-bit byte offset from the pool
+  if (cap > MAX_BLOCKS) {
+    delete o;
+    return fail(c, MF_E_ARG, "capacity above 2^21 voxel blocks per volume");
+  }
   v.cap = cap;
   v.slot = dalloc<int32_t>(c, nb3);
   if (v.slot) hipMemsetAsync(v.slot, 0xff, nb3 * sizeof(int32_t), c->stream);

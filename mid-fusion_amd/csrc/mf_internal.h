@@ -18,6 +18,8 @@ namespace mf {
 static constexpr int MAX_LEVELS = 4;
 static constexpr int MAX_INST = 64;     // camera + objects per context
 static constexpr int BLOCK_VOX = 512;   // 8^3 voxels per block (octree_defines.h:38)
+// blocks per volume: TSDF byte offsets cap * 512 * 4 stay below 2^32
+static constexpr long MAX_BLOCKS = 1L << 21;
 
 // Device view of one object volume (passed by value to kernels).
 struct VolDev {

@@ -104,3 +104,12 @@ def test_solve6_degenerate(ctx):
         assert np.array_equal(xg, x), (xg, x)
     assert np.array_equal(x_gpu[1], [0.25, 0, 3, 2, 0, 2])
     assert np.array_equal(x_gpu[2], [1, 0, 1, 1, 1, 1])
+
+
+def test_volume_capacity_limit(ctx):
+    """Voxel pools hold at most 2^21 blocks per volume (the raycast's 32-bit
+    TSDF byte offsets); a larger request fails with MF_E_ARG and leaves the
+    context usable."""
+    with pytest.raises(midfusion.MFError, match='2\\^21'):
+        ctx.add_object(2.0, 2048, np.eye(4, dtype=np.float32), capacity=(1 << 21) + 1)
+    assert ctx.add_object(0.5, 64, np.eye(4, dtype=np.float32), capacity=64) >= 1
