@@ -122,6 +122,17 @@ int mf_track(mf_ctx* ctx, const float k[4], unsigned frame);
  * tracked is pending. */
 int mf_track_camera(mf_ctx* ctx, const float k[4], unsigned frame);
 int mf_track_objects(mf_ctx* ctx, const float k[4], unsigned frame);
+/* Tracking::trackEachObject(ObjectPointer&, k, T_w_c2, T_w_c1, I, D)
+ * (tracking.h:63-68, tracking.cpp:580-708) for object id > 0 alone, between
+ * mf_track_camera and mf_track_objects: GN from the object's pose at the
+ * frame's start (T_c1_o1 = T_w_c1^-1 T_w_o1, T_c2_o2 = T_w_c2^-1 T_w_o1) at
+ * the current camera pose T_w_c2, updating volume_pose_ (mf_get_pose), the
+ * virtual camera pose and the object's level-0 trackresult_ codes.  It runs
+ * whatever the moving flags and the motion gate say.  mf_track_objects then
+ * keeps this result (it does not track the object again) and completes the
+ * frame.  MF_E_ARG if no camera phase is pending or the object was already
+ * tracked in this frame. */
+int mf_track_object(mf_ctx* ctx, int id, const float k[4]);
 /* Kfusion::integration(k, rate, mu, frame) (kfusion.cpp:573-677). */
 int mf_integrate(mf_ctx* ctx, const float k[4], unsigned frame);
 /* Kfusion::raycasting(k, mu, frame) (kfusion.cpp:539-569). */

@@ -44,6 +44,7 @@ struct Object {
   float* rD[MAX_LEVELS] = {};    // rendered reference depth pyramid
   uint8_t* tiles = nullptr;      // model-map tile flags (TrackInst::tiles)
   ObjParams* prm = nullptr;
+  bool tracked = false;          // tracked by mf_track_object in the pending object phase
 };
 
 }  // namespace
@@ -212,7 +213,18 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   if (res < 16 || res > 8192 || (res & (res - 1)) || size <= 0)
     return fail(c, MF_E_ARG, "object resolution must be a power of two in [16, 8192]");
   if ((int)c->objs.size() >= MAX_INST - 3) return fail(c, MF_E_ARG, "too many objects");
+  // a failed call frees what it allocated and leaves the context as it was
+  const size_t mark = c->allocs.size();
+  const long bytes_mark = c->bytes;
   Object* o = new Object();
+  auto rollback = [&](int code, const char* msg) {
+    hipStreamSynchronize(c->stream);
+    for (size_t i = mark; i < c->allocs.size(); i++) hipFree(c->allocs[i]);
+    c->allocs.resize(mark);
+    c->bytes = bytes_mark;
+    delete o;
+    return fail(c, code, msg);
+  };
   o->id = (int)c->objs.size();
   o->size = size; o->res = res;
   VolDev& v = o->v;
@@ -224,10 +236,7 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   if (cap <= 0) cap = std::min(nb3, (long)262144);
   cap = std::min(cap, nb3);
   // the raycast addresses TSDF voxels by a 32-bit byte offset from the pool
-  if (cap > MAX_BLOCKS) {
-    delete o;
-    return fail(c, MF_E_ARG, "capacity above 2^21 voxel blocks per volume");
-  }
+  if (cap > MAX_BLOCKS) return rollback(MF_E_ARG, "capacity above 2^21 voxel blocks per volume");
   v.cap = cap;
   v.slot = dalloc<int32_t>(c, nb3);
   if (v.slot) hipMemsetAsync(v.slot, 0xff, nb3 * sizeof(int32_t), c->stream);
@@ -261,10 +270,9 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   }
   o->prm = dalloc<ObjParams>(c, 1);
   if (!v.slot || !v.req || !v.nodes || !v.tsdf || !v.fg || !v.c || !v.fgw || !o->mask ||
-      !o->codes || !o->rgb_err || !o->prm || !o->mv[2] || !o->mn[2] || !o->mvbf[2]) {
-    delete o;
-    return fail(c, MF_E_OOM, "device allocation failed");
-  }
+      !o->codes || !o->rgb_err || !o->prm || !o->mv[2] || !o->mn[2] || !o->mvbf[2] || !o->tiles || !v.blk_morton ||
+      !v.blk_active || !v.wt || !v.r || !v.g || !v.b || !v.counters || !o->rD[0])
+    return rollback(MF_E_OOM, "device allocation failed");
   // INVALID normals in the model maps until the first raycast
   std::vector<float> inv(c->N0, (float)INVALID);
   hipMemcpyAsync(o->mn[0], inv.data(), c->N0 * sizeof(float), hipMemcpyHostToDevice, c->stream);
@@ -278,18 +286,13 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   hipStreamSynchronize(c->stream);
   // mapping work lists cover every object's pool (grown with each object;
   // the previous lists stay allocated until the context is destroyed)
-  if (!c->work.pool_pre) {
-    c->work.pool_pre = dalloc<long>(c, MAX_INST + 1);
-    c->work.new_count = dalloc<int>(c, 1);
-    c->work.act_count = dalloc<int>(c, 1);
-  }
-  c->work_cap += cap;
-  c->work.new_list = dalloc<int2>(c, c->work_cap);
-  c->work.act_list = dalloc<int2>(c, c->work_cap);
-  if (!c->work.pool_pre || !c->work.new_count || !c->work.act_count || !c->work.new_list || !c->work.act_list) {
-    delete o;
-    return fail(c, MF_E_OOM, "device allocation failed");
-  }
+  const long work_cap = c->work_cap + cap;
+  int2* new_list = dalloc<int2>(c, work_cap);
+  int2* act_list = dalloc<int2>(c, work_cap);
+  if (!new_list || !act_list) return rollback(MF_E_OOM, "device allocation failed");
+  c->work_cap = work_cap;
+  c->work.new_list = new_list;
+  c->work.act_list = act_list;
   c->objs.push_back(o);
   return o->id;
 }
@@ -531,6 +534,10 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->gauss = dalloc<float>(c, 5);
   c->stats = dalloc<unsigned long long>(c, (size_t)N_STATS * STAT_SLOTS);
   c->obj_ids = dalloc<int>(c, MAX_INST);
+  c->work.pool_pre = dalloc<long>(c, MAX_INST + 1);
+  c->work.new_count = dalloc<int>(c, 1);
+  c->work.act_count = dalloc<int>(c, 1);
+  ok = ok && c->work.pool_pre && c->work.new_count && c->work.act_count;
   ok = ok && c->vbf[2] && c->label_bf && c->motion_codes && c->md && c->vcam && c->rayobjs_bf && c->rayobjs_lab &&
        c->ray_pose && c->view_dev && c->palette_dev;
   ok = ok && c->stats && c->obj_ids && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
@@ -849,6 +856,7 @@ MotionLaunch motion_launch(mf_ctx* c, const float k[4], const Mat4* T_w_l, const
 
 int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
   c->pending_objects = false;
+  for (Object* o : c->objs) o->tracked = false;
   if (frame % c->cfg.track_rate != 0) return 0;
   hipStream_t s = c->stream;
   const int L = c->cfg.n_levels;
@@ -945,6 +953,34 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
   return 1;
 }
 
+// Tracking::trackEachObject (tracking.cpp:580-708) of one object: GN state
+// and descriptor slot OBJ_SLOT, ids entry OBJ_SLOT of obj_ids.
+static constexpr int OBJ_SLOT = MAX_INST - 2;
+
+int mf_track_object(mf_ctx* c, int id, const float k[4]) {
+  if (!c->pending_objects) return fail(c, MF_E_ARG, "call between mf_track_camera and mf_track_objects");
+  if (id <= 0 || id >= (int)c->objs.size() || !k) return fail(c, MF_E_ARG, "bad object id");
+  Object* o = c->objs[id];
+  if (o->tracked) return fail(c, MF_E_ARG, "object already tracked in this frame");
+  hipStream_t s = c->stream;
+  set_inst(c, OBJ_SLOT, o->mv, o->mn, o->rD, o->codes, o->rgb_err, OBJ_SLOT, o->tiles);
+  const int oid = 3 + id;
+  hipMemcpyAsync(c->obj_ids + OBJ_SLOT, &oid, sizeof oid, hipMemcpyHostToDevice, s);
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  std::vector<uint64_t> key = base_key(c, 6, k);
+  key.push_back((uint64_t)id);
+  const int r = run_graph(c, key, [&]() {
+    { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, OBJ_SLOT, 1, c->obj_ids + OBJ_SLOT, c->mats, c->mats + 0, c->mats + 1, 1, nullptr); }
+    { Timed t(c, KC_GNSMALL); launch_model_tiles(s, c->inst + OBJ_SLOT, 1, c->W, c->H); }
+    if (c->cfg.use_rgb) render_reference(c, OBJ_SLOT, 1);
+    gn_run(c, k, 1, OBJ_SLOT, 1, nullptr);
+    { Timed t(c, KC_GNSMALL); launch_virtual_cams(s, c->gn, OBJ_SLOT, 1, c->obj_ids + OBJ_SLOT, c->vcam); }
+  });
+  if (r < 0) return r;
+  o->tracked = true;
+  return 0;
+}
+
 int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
   (void)frame;
   if (!c->pending_objects) return 0;
@@ -952,37 +988,43 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
   hipStream_t s = c->stream;
   const int L = c->cfg.n_levels;
   const int nobj = (int)c->objs.size();
-  std::vector<Object*> mov, stat;
+  std::vector<Object*> mov, stat, gnmov;
   for (int i = 1; i < nobj; i++) (c->objs[i]->moving ? mov : stat).push_back(c->objs[i]);
+  // objects already tracked by mf_track_object keep that result
+  for (Object* o : mov) if (!o->tracked) gnmov.push_back(o);
   if (nobj >= 2) {
     MotionDev* md = c->md;
     Object* bg = c->objs[0];
-    std::vector<int> ids(mov.size());
-    for (size_t j = 0; j < mov.size(); j++) {
-      Object* o = mov[j];
+    std::vector<int> ids(gnmov.size());
+    for (size_t j = 0; j < gnmov.size(); j++) {
+      Object* o = gnmov[j];
       ids[j] = 3 + o->id;
       set_inst(c, 1 + (int)j, o->mv, o->mn, o->rD, o->codes, o->rgb_err, 1 + (int)j, o->tiles);
     }
-    if (!mov.empty()) set_ids(c, ids);
+    if (!gnmov.empty()) set_ids(c, ids);
     std::vector<uint64_t> key = base_key(c, 2, k);
     key.push_back((uint64_t)c->cfg.batch_objects);
     key.push_back((uint64_t)nobj);
     for (int id : ids) key.push_back((uint64_t)id);
+    for (Object* o : mov) key.push_back((uint64_t)o->id | (1ull << 32));
+    const int ng = (int)gnmov.size();
     const int r = run_graph(c, key, [&]() {
       if (!mov.empty()) {
         // trackEachObject for every moving object when the last raycast shows
         // more than one instance (kfusion.cpp:292, 395-414)
-        { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, 1, (int)mov.size(), c->obj_ids, c->mats, c->mats + 0, c->mats + 1, 1, &md->gate); }
-        { Timed t(c, KC_GNSMALL);
-          if (c->cfg.batch_objects) launch_model_tiles(s, c->inst + 1, (int)mov.size(), c->W, c->H);
-          else for (size_t j = 0; j < mov.size(); j++) launch_model_tiles(s, c->inst + 1 + j, 1, c->W, c->H); }
-        if (c->cfg.use_rgb) {
-          if (c->cfg.batch_objects) render_reference(c, 1, (int)mov.size());
-          else for (size_t j = 0; j < mov.size(); j++) render_reference(c, 1 + (int)j, 1);
+        if (ng > 0) {
+          { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, 1, ng, c->obj_ids, c->mats, c->mats + 0, c->mats + 1, 1, &md->gate); }
+          { Timed t(c, KC_GNSMALL);
+            if (c->cfg.batch_objects) launch_model_tiles(s, c->inst + 1, ng, c->W, c->H);
+            else for (int j = 0; j < ng; j++) launch_model_tiles(s, c->inst + 1 + j, 1, c->W, c->H); }
+          if (c->cfg.use_rgb) {
+            if (c->cfg.batch_objects) render_reference(c, 1, ng);
+            else for (int j = 0; j < ng; j++) render_reference(c, 1 + j, 1);
+          }
+          if (c->cfg.batch_objects) gn_run(c, k, 1, 1, ng, nullptr);
+          else for (int j = 0; j < ng; j++) gn_run(c, k, 1, 1 + j, 1, nullptr);
+          { Timed t(c, KC_GNSMALL); launch_virtual_cams(s, c->gn, 1, ng, c->obj_ids, c->vcam); }
         }
-        if (c->cfg.batch_objects) gn_run(c, k, 1, 1, (int)mov.size(), nullptr);
-        else for (size_t j = 0; j < mov.size(); j++) gn_run(c, k, 1, 1 + (int)j, 1, nullptr);
-        { Timed t(c, KC_GNSMALL); launch_virtual_cams(s, c->gn, 1, (int)mov.size(), c->obj_ids, c->vcam); }
         // objects moving this frame: the camera's rows become motion residuals
         // against the maps before fusion, and a second raycast before fusion
         // renders the objects at their new poses (kfusion.cpp:416-441)
@@ -1012,6 +1054,7 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
     });
     if (r < 0) return r;
   }
+  for (Object* o : c->objs) o->tracked = false;
   for (int l = 0; l < L; l++) std::swap(c->live[l], c->ref[l]);   // setRefImgFromCurr
   return 1;
 }

@@ -697,7 +697,9 @@ __device__ void gn_update(GNIter& X, GNState& S, const SolvePre& pre, const doub
     solver = solve6(sums + 1, lane, x);
     const Mat4 upd = se3_exp(x);
     const double nx = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
-    if (nx < 1e-5 && (double)sqrtf(s0 / s28) < 2e-3) {
+    // icp_threshold is a float constant (tracking.h:387) compared with the
+    // double norm(x) (tracking.cpp:1365): its value is (double)1e-5f
+    if (nx < (double)1e-5f && (double)sqrtf(s0 / s28) < 2e-3) {
       done = level;
       flag = 1;
     } else {

@@ -78,6 +78,7 @@ SIGNATURES = {
     'mf_track': (C.c_int, [P, f32p, C.c_uint]),
     'mf_track_camera': (C.c_int, [P, f32p, C.c_uint]),
     'mf_track_objects': (C.c_int, [P, f32p, C.c_uint]),
+    'mf_track_object': (C.c_int, [P, C.c_int, f32p]),
     'mf_integrate': (C.c_int, [P, f32p, C.c_uint]),
     'mf_raycast': (C.c_int, [P, f32p, C.c_uint]),
     'mf_sync': (C.c_int, [P]),
@@ -360,6 +361,10 @@ class Kfusion:
 
     def track_objects(self, k, frame):
         return self._chk(self.L.mf_track_objects(self.ctx, _f32(k), frame))
+
+    def track_object(self, oid, k):
+        """trackEachObject of one object (between track_camera and track_objects)."""
+        return self._chk(self.L.mf_track_object(self.ctx, oid, _f32(k)))
 
     def integration(self, k, frame):
         return self._chk(self.L.mf_integrate(self.ctx, _f32(k), frame))

@@ -295,7 +295,9 @@ static inline bool solve_pose(M4& upd, const double sums[32], double xout[6]) {
   if (xout) for (int i = 0; i < 6; i++) xout[i] = x[i];
   double nx = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
   const R s0 = (R)sums[0], s28 = (R)sums[28];
-  return (nx < 1e-5) && ((double)std::sqrt(s0 / s28) < 2e-3);
+  // icp_threshold is a float constant (tracking.h:387) compared with the
+  // double norm(x) (tracking.cpp:1365): its value is (double)1e-5f
+  return (nx < (double)1e-5f) && ((double)std::sqrt(s0 / s28) < 2e-3);
 }
 
 }  // namespace orc
