@@ -2,16 +2,25 @@
 
 A step is one frame: preprocessing (inputs already in HBM), camera + moving
 object tracking (ICP + photometric, 3 levels {10,5,4}), allocation +
-integration of every instance, and raycasting.  Workload: 640x480
-(fx=fy=525, cx=319.5, cy=239.5), background volume 512^3 at 1 cm, one
-moving textured box 0.2x0.15x0.1 m with its own 128^3 volume at 2 mm,
-seeded synthetic frames.  Prints one JSON line.
+integration of every instance, and raycasting.
+
+Default workload (camera + K = 8 objects): 640x480 (fx=fy=525, cx=319.5,
+cy=239.5), background volume 512^3 at 1 cm, 8 seeded moving boxes and
+ellipsoids of 0.05-0.4 m, each in its own 64^3-256^3 volume at 2-5 mm, on
+constant twists, seeded synthetic frames; all objects batched per launch.
+The same run times the one-object-per-launch form of the same workload
+(`vs_plain_gpu`), the single-object workload (one 0.2x0.15x0.1 m box in a
+128^3 volume at 2 mm), the CPU oracle at the reference's compiler flags and
+at -O3 on the host cores, and the pose and TSDF error of the GPU path against
+the FP64 oracle.  Prints one JSON line.
 """
 import argparse
 import importlib.util
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -28,6 +37,7 @@ def _load(name, path):
     return mod
 
 
+METRIC = 'frames/s (track+integrate+raycast, 640x480, cam+K objs) vs plain GPU; pose err'
 HBM_PEAK_GBS = 8000.0
 W, H = 640, 480
 K = np.array([525.0, 525.0, 319.5, 239.5], np.float32)
@@ -56,6 +66,17 @@ def make_scene(scene, n_objects):
     return scene.Scene(seed=seed, width=W, height=H, K=tuple(K), moving=movers), movers
 
 
+def workload_name(n_obj, plain):
+    if HD:
+        w = ('1280x720 RGB-D (fx=fy=1050, 2%% invalid depth), bg 1024^3 @5mm + %d moving boxes/ellipsoids '
+             '0.05-0.4 m in 64^3-256^3 @2-5mm' % n_obj)
+    elif n_obj == 1:
+        w = '640x480 RGB-D, bg 512^3 @1cm + 1 moving box 0.2x0.15x0.1 m in 128^3 @2mm'
+    else:
+        w = '640x480 RGB-D, bg 512^3 @1cm + %d moving boxes/ellipsoids 0.05-0.4 m in 64^3-256^3 @2-5mm' % n_obj
+    return w + ', ICP+RGB {10,5,4}, %s' % ('one object per launch' if plain else 'objects batched per launch')
+
+
 def render_frames(sc, n_objects, n):
     scene = sys.modules['mf_scene']
     frames = []
@@ -65,7 +86,7 @@ def render_frames(sc, n_objects, n):
     return frames
 
 
-TB = 256   # threads per tracking workgroup
+TB = 256        # threads per tracking workgroup
 REPLAYS = 200   # back-to-back launches of the roofline kernel
 
 
@@ -99,22 +120,270 @@ def solve_bytes(trace, rpt):
 
 
 # kernel class -> kernel name in rocprofv3 output (traffic counters)
-KERNEL_NAMES = {'track_rows': 'mf::k_track<true, true, false, 3>', 'track_rows_l0': 'mf::k_track<true, true, false, 3> level 0',
+KERNEL_NAMES = {'track_rows': 'mf::k_track<true, true, false, 3>',
+                'track_rows_l0': 'mf::k_track<true, true, false, 3> level 0',
                 'track_rows_obj': 'mf::k_track<true, true, true, 3>',
                 'track_solve': 'mf::k_solve', 'integrate': 'mf::k_integrate', 'raycast': 'mf::k_raycast',
                 'alloc_band': 'mf::k_alloc'}
+# committed rocprofv3 counter summary of each workload (FETCH_SIZE / WRITE_SIZE)
+TRAFFIC_FILES = {8: 'hbm_traffic_640x480_8objects.json', 1: 'hbm_traffic_cfg2.json'}
 
 
-def traffic_per_dispatch(kernel_class):
+def traffic_per_dispatch(kernel_class, n_obj):
     """HBM bytes per dispatch of a kernel from the committed rocprofv3 counter
-    summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration)."""
-    path = os.path.join(ROOT, 'profiles', 'hbm_traffic_cfg2.json')
+    summary of this workload (FETCH_SIZE / WRITE_SIZE scaled by the
+    copy-kernel calibration), or None."""
+    name = TRAFFIC_FILES.get(n_obj)
+    if HD or not name:
+        return None
     try:
-        with open(path) as f:
+        with open(os.path.join(ROOT, 'profiles', name)) as f:
             t = json.load(f)
         return t['kernels'][KERNEL_NAMES[kernel_class]]['bytes_per_dispatch']
     except (OSError, KeyError, ValueError):
         return None
+
+
+def apply_tuning(g, args):
+    for lvl, v in enumerate(x for x in args.track_rpt.split(',') if x):
+        g.set_tuning(g.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl, int(v))
+    for lvl, v in enumerate(x for x in args.track_fused.split(',') if x):
+        g.set_tuning(g.TUNE_TRACK_FUSED_L0 + lvl, int(v))
+    for lvl, v in enumerate(x for x in args.obj_rpt.split(',') if x):
+        g.set_tuning(g.TUNE_OBJ_ROWS_PER_THREAD_L0 + lvl, int(v))
+    for lvl, v in enumerate(x for x in args.obj_ppt.split(',') if x):
+        g.set_tuning(g.TUNE_OBJ_PIXELS_PER_THREAD_L0 + lvl, int(v))
+
+
+class Leg:
+    """One GPU context over resident frames: warm-up, then K timed frames
+    between two stream synchronisations."""
+
+    def __init__(self, mf, hiprt, frames, movers, P0, plain, args):
+        self.g = mf.Kfusion(W, H, P0, bg_size=5.12, bg_res=BG_RES, batch_objects=not plain)
+        apply_tuning(self.g, args)
+        for i, m in enumerate(movers):
+            self.g.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
+            self.g.set_moving(i + 1, True)
+        # inputs resident in HBM before timing
+        self.dev = [(hiprt.DeviceBuffer(d), hiprt.DeviceBuffer(rgb), [hiprt.DeviceBuffer(m) for m in masks])
+                    for d, rgb, masks in frames]
+        self.movers = movers
+
+    def step(self, f):
+        g = self.g
+        dd, dr, dm = self.dev[f]
+        for i, m in enumerate(dm):
+            g.L.mf_object_set_mask_device(g.ctx, i, m.ptr)
+        g.preprocessing_device(dd.ptr, dr.ptr, W, H)
+        g.tracking(K, f)
+        g.integration(K, f)
+        g.raycasting(K, f)
+
+    def run(self, warmup, steps, autotune=False):
+        g = self.g
+        tuned = None
+        for f in range(warmup):
+            self.step(f)
+            if autotune and f == 1:
+                # both pyramids now hold real frames; the remaining warm-up
+                # frames capture the graphs of the chosen shapes
+                g.sync()
+                chosen, us = g.autotune(K)
+                tuned = {'rows_per_thread': [int(v) for v in chosen],
+                         'us_per_iteration': {str(c): [round(float(x), 2) for x in us[:, i]]
+                                              for i, c in enumerate(g.TUNE_CANDIDATES)}}
+                print('autotune: %s' % json.dumps(tuned), file=sys.stderr, flush=True)
+        g.sync()
+        t = time.perf_counter()
+        for f in range(warmup, warmup + steps):
+            self.step(f)
+        t_enq = time.perf_counter() - t
+        g.sync()
+        el = time.perf_counter() - t
+        return {'fps': steps / el, 'ms': 1000.0 * el / steps, 'enqueue_ms': 1000.0 * t_enq / steps, 'tuned': tuned}
+
+    def close(self):
+        self.g.close()
+
+
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_leg(frames, movers, P0, n_frames, budget_s):
+    """The CPU oracle (the float restatement of the reference path, OpenMP
+    over every host thread) timed per stage with CLOCK_MONOTONIC, built on
+    this host twice: with the reference's flags (-g -O1 -march=native
+    -fopenmp) and with -O3.  Frame 0 only seeds the tracker and is not
+    counted; each build runs up to n_frames frames or budget_s seconds."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        subprocess.check_call(['make', '-s', '-C', os.path.join(ROOT, 'oracle'), 'timing', 'OUT=' + td, '-j2'])
+        for build in ('O1', 'O3'):
+            o = oracle.Oracle(W, H, P0, lib_path=os.path.join(td, 'liborc32_%s.so' % build))
+            o.add_object(5.12, BG_RES, np.eye(4))
+            for i, m in enumerate(movers):
+                o.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
+                o.set_moving(i + 1, True)
+            stages = {k: [] for k in ('preprocess', 'track', 'allocate', 'integrate', 'raycast', 'frame')}
+            t_start = time.perf_counter()
+            nf = 0
+            for f in range(min(n_frames, len(frames))):
+                d, rgb, masks = frames[f]
+                for i, m in enumerate(masks):
+                    o.set_mask(i, m)
+                t0 = time.perf_counter()
+                o.preprocess(d, rgb)
+                t1 = time.perf_counter()
+                o.track(K, f)
+                t2 = time.perf_counter()
+                o.stage_clock()
+                o.integrate(K, f)
+                ta, ti = o.stage_clock()
+                t3 = time.perf_counter()
+                o.raycast(K, f)
+                t4 = time.perf_counter()
+                if f > 0:
+                    for k, v in (('preprocess', t1 - t0), ('track', t2 - t1), ('allocate', ta), ('integrate', ti),
+                                 ('raycast', t4 - t3), ('frame', t4 - t0)):
+                        stages[k].append(1000.0 * v)
+                    nf += 1
+                if time.perf_counter() - t_start > budget_s and nf >= 3:
+                    break
+            out[build] = {
+                'frames_per_s': round(1000.0 / float(np.median(stages['frame'])), 4),
+                'ms_median': {k: round(float(np.median(v)), 3) for k, v in stages.items()},
+                'ms_p90': {k: round(float(np.percentile(v, 90)), 3) for k, v in stages.items()},
+                'frames': nf, 'seconds': round(time.perf_counter() - t_start, 1)}
+            threads = o.L.orc_num_threads()
+            del o
+    return {'value': out['O1']['frames_per_s'], 'unit': 'frames/s', 'cores': threads, 'kind': 'port',
+            'sample': ('float oracle (OpenMP C++ restatement of the reference path) on frames 1..%d of the headline '
+                       'workload per build, frame 0 as warm-up; value = 1000 / median frame ms of the build at the '
+                       "reference's flags -g -O1 -march=native -fopenmp; -O3 build beside it"
+                       % out['O1']['frames']),
+            'cpu_model': cpu_model(), 'builds': {'-g -O1 -march=native -fopenmp': out['O1'],
+                                                 '-O3 -march=native -fopenmp': out['O3']}}
+
+
+def rot_err(A, B):
+    R = A[:3, :3].astype(np.float64).T @ B[:3, :3].astype(np.float64)
+    s = 0.5 * np.linalg.norm([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]])
+    return float(np.arctan2(s, (np.trace(R) - 1) / 2))
+
+
+def accuracy_leg(mf, frames, movers, P0, n_frames, plain):
+    """Pose and TSDF error of the GPU path against the FP64 oracle (checker
+    of the CPU leg).  Each frame starts the GPU context, the FP32 oracle and
+    the FP64 oracle from identical model maps and poses (the FP32 oracle's);
+    camera and object poses after tracking are compared with the FP64
+    oracle's, objects from the FP64 camera pose.  TSDF: after the last frame,
+    the fraction of the background's voxels (4000 blocks evenly spaced over
+    the FP64 oracle's keys) whose tsdf differs by more than 1e-4 or whose
+    weight differs, GPU and FP32 oracle each against the FP64 oracle."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle
+    g = mf.Kfusion(W, H, P0, bg_size=5.12, bg_res=BG_RES, batch_objects=not plain)
+    o32 = oracle.Oracle(W, H, P0)
+    o64 = oracle.Oracle(W, H, P0, fp64=True)
+    for o in (o32, o64):
+        o.add_object(5.12, BG_RES, np.eye(4))
+    for i, m in enumerate(movers):
+        T = m.pose(0).astype(np.float32)
+        for x in (g, o32, o64):
+            x.add_object(m.vol_size, m.vol_res, T)
+            x.set_moving(i + 1, True)
+    n = len(movers)
+    cam, cam32, obj, obj32 = [], [], [], []
+    for f in range(n_frames):
+        d, rgb, masks = frames[f]
+        for x in (g, o32, o64):
+            for i, m in enumerate(masks):
+                x.set_mask(i, m)
+        g.preprocessing(d, rgb)
+        o32.preprocess(d, rgb)
+        o64.preprocess(d, rgb)
+        if f > 0:
+            vtx, nrm, lb = o32.maps()
+            o64.set_maps(vtx, nrm, lb)
+            g.set_maps(vtx, nrm, lb)
+            for i in range(1, 1 + n):
+                ov, on = o32.object_maps(i)
+                o64.set_object_maps(i, ov, on)
+                g.set_object_maps(i, ov, on)
+            for i in [-1] + list(range(1, 1 + n)):
+                T = o32.pose(i).astype(np.float32)
+                o64.set_pose(i, T)
+                g.setPose(i, T)
+        g.track_camera(K, f)
+        o32.track_camera(K, f)
+        o64.track_camera(K, f)
+        P64 = o64.pose()
+        if f > 0:
+            for P, acc in ((g.getPose(), cam), (o32.pose(), cam32)):
+                acc.append((float(np.linalg.norm(P[:3, 3] - P64[:3, 3])), rot_err(P, P64)))
+        g.setPose(-1, P64.astype(np.float32))
+        o32.set_pose(-1, P64.astype(np.float32))
+        g.track_objects(K, f)
+        o32.track_objects(K, f)
+        o64.track_objects(K, f)
+        if f > 0:
+            for i in range(1, 1 + n):
+                Q64 = o64.pose(i)
+                for Q, acc in ((g.getPose(i), obj), (o32.pose(i), obj32)):
+                    acc.append((float(np.linalg.norm(Q[:3, 3] - Q64[:3, 3])), rot_err(Q, Q64)))
+        g.setPose(-1, o32.pose().astype(np.float32))
+        for i in range(1, 1 + n):
+            g.setPose(i, o32.pose(i).astype(np.float32))
+        for i in range(1 + n):
+            g.set_trackresult(i, o32.trackresult(i)[0])
+        g.integration(K, f)
+        o32.integrate(K, f)
+        o64.integrate(K, f)
+        g.raycasting(K, f)
+        o32.raycast(K, f)
+        o64.raycast(K, f)
+    L = oracle.lib(False)
+    keys = o64.keys(0)
+    keys = keys[np.linspace(0, len(keys) - 1, min(len(keys), 4000)).astype(int)]
+    vg = g.voxels(0, keys)[0]
+    xyz = np.zeros(3, np.uint32)
+    bad_g = bad_o = 0
+    for kk, key in enumerate(keys):
+        L.orc_unpack(int(key) & ~0x1FF, xyz)
+        b64 = o64.block(0, *map(int, xyz))
+        b32 = o32.block(0, *map(int, xyz))
+        for b, which in ((vg[kk], 0), (b32, 1)):
+            if b is None:
+                dd = 512
+            else:
+                dd = int(np.sum((np.abs(b[:, 0] - b64[:, 0]) > 1e-4) | (b[:, 1] != b64[:, 1])))
+            if which == 0:
+                bad_g += dd
+            else:
+                bad_o += dd
+    g.close()
+    nv = 512 * max(len(keys), 1)
+
+    def summ(a):
+        a = np.asarray(a) if a else np.zeros((1, 2))
+        return {'max_m': float(a[:, 0].max()), 'max_rad': float(a[:, 1].max()),
+                'median_m': float(np.median(a[:, 0])), 'median_rad': float(np.median(a[:, 1])),
+                'within_1e-5': float(np.mean((a[:, 0] < 1e-5) & (a[:, 1] < 1e-5)))}
+    return {'reference': 'FP64 oracle, %d tracked frames from identical inputs' % (n_frames - 1),
+            'camera_gpu': summ(cam), 'camera_fp32_oracle': summ(cam32),
+            'objects_gpu': summ(obj), 'objects_fp32_oracle': summ(obj32),
+            'tsdf_voxels_differing_gpu': round(bad_g / nv, 6), 'tsdf_voxels_differing_fp32_oracle': round(bad_o / nv, 6)}
 
 
 def main():
@@ -122,18 +391,21 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=30)
     ap.add_argument('--warmup', type=int, default=5)
-    ap.add_argument('--objects', type=int, default=1)
-    ap.add_argument('--plain', action='store_true', help='one object per launch instead of batched launches')
-    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline leg')
+    ap.add_argument('--objects', type=int, default=8)
+    ap.add_argument('--plain', action='store_true', help='headline in the one-object-per-launch form')
+    ap.add_argument('--no-plain', action='store_true', help='skip the one-object-per-launch leg')
+    ap.add_argument('--no-single', action='store_true', help='skip the single-object workload leg')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle leg and the FP64 accuracy check')
+    ap.add_argument('--cpu-frames', type=int, default=0,
+                    help='frames per CPU build (default 31: about 10-25 s of oracle work per build; 6 with --hd)')
+    ap.add_argument('--accuracy-frames', type=int, default=4,
+                    help='frames of the FP64 accuracy check (the first only seeds the tracker)')
     ap.add_argument('--render', action='store_true',
                     help='also time the display renderers, volume2mask, the background mesh and the '
                          'instance-mask generalisation (ms per call, results on the host)')
     ap.add_argument('--raw-stream', action='store_true',
                     help='also time the same frames read from a .raw sequence file through pinned staging '
                          '(disk + PCIe inclusive; never the headline value)')
-    ap.add_argument('--cpu-frames', type=int, default=0,
-                    help='frames of the CPU baseline sample (default: 120, about 10 s of oracle work on 16 threads; '
-                         '12 with --hd)')
     ap.add_argument('--track-fused', type=str, default='', metavar='F0,F1,F2',
                     help='1 per pyramid level: one fused launch per GN iteration (default: library default)')
     ap.add_argument('--track-rpt', type=str, default='', metavar='R0,R1,R2',
@@ -148,7 +420,7 @@ def main():
                          '(mf_autotune) after the second warm-up frame, before timing')
     ap.add_argument('--hd', action='store_true',
                     help='precision + scale workload instead: 1280x720 (fx=fy=1050), background 1024^3 at 5 mm, '
-                         '16 moving objects (default --objects), 2%% invalid depth pixels')
+                         '16 moving objects, 2%% invalid depth pixels')
     ap.add_argument('--copy-ref', type=int, default=0, metavar='MB',
                     help='only launch a plain copy kernel of MB megabytes (traffic-counter calibration)')
     args = ap.parse_args()
@@ -157,7 +429,7 @@ def main():
         HD = True
         W, H, BG_RES = 1280, 720, 1024
         K = np.array([1050.0, 1050.0, 639.5, 359.5], np.float32)
-        if args.objects == 1:
+        if args.objects == 8:
             args.objects = 16
     if args.gpus != 1:
         print('bench.py measures one GPU: this build runs the dense path on a single MI355X '
@@ -176,76 +448,33 @@ def main():
     n_obj = args.objects
     sc, movers = make_scene(scene, n_obj)
     n_frames = args.warmup + args.steps
-    t0 = time.time()
     if args.cpu_frames <= 0:
-        args.cpu_frames = 12 if HD else 120
-    n_render = max(n_frames + 3, 0 if args.no_cpu else args.cpu_frames)
+        args.cpu_frames = 6 if HD else 31
+    t0 = time.time()
+    n_render = max(n_frames + 3, 0 if args.no_cpu else max(args.cpu_frames, args.accuracy_frames))
     frames = render_frames(sc, n_obj, n_render)
     print('rendered %d frames in %.1fs' % (len(frames), time.time() - t0), file=sys.stderr, flush=True)
-
     P0 = sc.camera_pose(0).astype(np.float32)
-    g = mf.Kfusion(W, H, P0, bg_size=5.12, bg_res=BG_RES, batch_objects=not args.plain)
-    for lvl, v in enumerate(x for x in args.track_rpt.split(',') if x):
-        g.set_tuning(g.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl, int(v))
-    for lvl, v in enumerate(x for x in args.track_fused.split(',') if x):
-        g.set_tuning(g.TUNE_TRACK_FUSED_L0 + lvl, int(v))
-    for lvl, v in enumerate(x for x in args.obj_rpt.split(',') if x):
-        g.set_tuning(g.TUNE_OBJ_ROWS_PER_THREAD_L0 + lvl, int(v))
-    for lvl, v in enumerate(x for x in args.obj_ppt.split(',') if x):
-        g.set_tuning(g.TUNE_OBJ_PIXELS_PER_THREAD_L0 + lvl, int(v))
-    for i, m in enumerate(movers):
-        g.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
-        g.set_moving(i + 1, True)
-    # inputs resident in HBM before timing
-    dev = []
-    for d, rgb, masks in frames:
-        dev.append((hiprt.DeviceBuffer(d), hiprt.DeviceBuffer(rgb), [hiprt.DeviceBuffer(m) for m in masks]))
 
-    def step(f):
-        dd, dr, dm = dev[f]
-        for i, m in enumerate(dm):
-            g.L.mf_object_set_mask_device(g.ctx, i, m.ptr)
-        g.preprocessing_device(dd.ptr, dr.ptr, W, H)
-        g.tracking(K, f)
-        g.integration(K, f)
-        g.raycasting(K, f)
-
-    tuned = None
-    for f in range(args.warmup):
-        step(f)
-        if args.autotune and f == 1:
-            # both pyramids now hold real frames; the remaining warm-up frames
-            # capture the graphs of the chosen shapes
-            g.sync()
-            chosen, us = g.autotune(K)
-            tuned = {'rows_per_thread': [int(v) for v in chosen],
-                     'us_per_iteration': {str(c): [round(float(x), 2) for x in us[:, i]]
-                                          for i, c in enumerate(g.TUNE_CANDIDATES)}}
-            print('autotune: %s' % json.dumps(tuned), file=sys.stderr, flush=True)
-    g.sync()
-    t = time.perf_counter()
-    for f in range(args.warmup, n_frames):
-        step(f)
-    t_enq = time.perf_counter() - t
-    g.sync()
-    el = time.perf_counter() - t
-    fps = args.steps / el
-    ms = 1000.0 * el / args.steps
-    pose_err = float(np.linalg.norm(g.getPose()[:3, 3] - sc.camera_pose(n_frames - 1)[:3, 3]))
+    # ---- headline: batched (or, with --plain, one object per launch)
+    leg = Leg(mf, hiprt, frames, movers, P0, args.plain, args)
+    res = leg.run(args.warmup, args.steps, args.autotune)
+    g = leg.g
+    print('headline %.1f frames/s' % res['fps'], file=sys.stderr, flush=True)
+    pose_gt = float(np.linalg.norm(g.getPose()[:3, 3] - sc.camera_pose(n_frames - 1)[:3, 3]))
 
     # per-kernel timing + work counters on 3 more frames (events on the context stream)
     g.enable_timing(True)
     g.reset_stats()
     g.trace_clear()
     for f in range(n_frames, n_frames + 3):
-        step(f)
+        leg.step(f)
     g.sync()
     tm = g.timing()
     st = g.stats()
     tr = g.trace()
     tot_ms = sum(v[0] for v in tm.values())
     stage = {k: round(v[0] / 3, 4) for k, v in tm.items() if v[1]}
-    # algorithmic bytes per kernel class over the 3 frames
     N0 = W * H
     rpt = [g.get_tuning(g.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl) for lvl in range(4)]
     orpt = [g.get_tuning(g.TUNE_OBJ_ROWS_PER_THREAD_L0 + lvl) or rpt[lvl] for lvl in range(4)]
@@ -268,44 +497,61 @@ def main():
     roof = {'kernel': KERNEL_NAMES['track_rows'] + ' (level 0, %d workgroups)' % (-(-(2 * N0) // (TB * rpt[0]))),
             'bound': 'hbm', 'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS,
             'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 5),
-            # the committed counter summary covers the 640x480 workload only
-            'traffic': None if HD else traffic_per_dispatch('track_rows_l0'),
+            'traffic': traffic_per_dispatch('track_rows_l0', n_obj),
             'avg_launch_us': round(rep_us, 3), 'launches': REPLAYS, 'alg_bytes_per_launch': int(rep_bytes)}
     roof_all = {}
     for k, (ms_k, n_k) in kern.items():
         a = alg[k] / (ms_k / 1000.0) / 1e9
         roof_all[k] = {'achieved_gbs': round(a, 1), 'frac': round(a / HBM_PEAK_GBS, 4),
                        'avg_launch_us': round(1000.0 * ms_k / n_k, 2), 'launches': n_k}
+    # allocated blocks of every volume against its pool capacity
+    g.sync()
+    capacity = []
+    for i in range(1 + n_obj):
+        nb = len(g.keys(i))
+        res_i = BG_RES if i == 0 else movers[i - 1].vol_res
+        cap = min((res_i // 8) ** 3, 262144)
+        capacity.append({'id': i, 'blocks': nb, 'capacity': cap})
+    leg.close()
 
-    cpu = None
+    # ---- the one-object-per-launch form of the same workload
+    plain = None
+    if not args.plain and not args.no_plain and n_obj > 1:
+        lp = Leg(mf, hiprt, frames, movers, P0, True, args)
+        plain = lp.run(args.warmup, args.steps)
+        lp.close()
+        print('plain %.1f frames/s' % plain['fps'], file=sys.stderr, flush=True)
+
+    # ---- the single-object workload
+    single = None
+    if not args.no_single and not HD and n_obj != 1:
+        sc1, mov1 = make_scene(scene, 1)
+        fr1 = render_frames(sc1, 1, n_frames)
+        l1 = Leg(mf, hiprt, fr1, mov1, sc1.camera_pose(0).astype(np.float32), args.plain, args)
+        single = l1.run(args.warmup, args.steps)
+        l1.close()
+        print('single object %.1f frames/s' % single['fps'], file=sys.stderr, flush=True)
+
+    cpu = accuracy = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
-        import oracle
-        o = oracle.Oracle(W, H, P0)
-        o.add_object(5.12, BG_RES, np.eye(4))
-        for i, m in enumerate(movers):
-            o.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
-            o.set_moving(i + 1, True)
-        nthreads = o.L.orc_num_threads()
-        nc = max(2, args.cpu_frames)
-        times = []
-        for f in range(nc):
-            d, rgb, masks = frames[f]
-            t1 = time.perf_counter()
-            for i, m in enumerate(masks):
-                o.set_mask(i, m)
-            o.preprocess(d, rgb)
-            o.track(K, f)
-            o.integrate(K, f)
-            o.raycast(K, f)
-            times.append(time.perf_counter() - t1)
-        cpu_fps = 1.0 / float(np.mean(times[1:]))   # frame 0 only seeds the tracker
-        cpu = {'value': round(cpu_fps, 4), 'unit': 'frames/s', 'cores': nthreads, 'kind': 'port',
-               'sample': 'oracle (OpenMP C++ restatement, -O2) on frames 1..%d of the same workload (%.1f s of CPU '
-                         'work), frame 0 as warm-up' % (nc - 1, float(np.sum(times[1:])))}
+        cpu = cpu_leg(frames, movers, P0, args.cpu_frames, 30.0 if HD else 25.0)
+        if args.accuracy_frames > 1:
+            accuracy = accuracy_leg(mf, frames, movers, P0, args.accuracy_frames, args.plain)
 
     render = None
     if args.render:
+        g = mf.Kfusion(W, H, P0, bg_size=5.12, bg_res=BG_RES, batch_objects=not args.plain)
+        for i, m in enumerate(movers):
+            g.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
+            g.set_moving(i + 1, True)
+        for f in range(3):
+            d, rgb, masks = frames[f]
+            for i, m in enumerate(masks):
+                g.set_mask(i, m)
+            g.preprocessing(d, rgb)
+            g.tracking(K, f)
+            g.integration(K, f)
+            g.raycasting(K, f)
         P = g.getPose()
         Pv = P.copy()
         Pv[:3, 3] += np.array([0.05, -0.03, 0.02], np.float32)
@@ -323,11 +569,11 @@ def main():
             for _ in range(10):
                 fn()
             render[name] = round(1000.0 * (time.perf_counter() - t) / 10, 3)
+        g.close()
 
     raw_leg = None
     if args.raw_stream:
         import struct
-        import tempfile
         with tempfile.TemporaryDirectory() as td:
             path = os.path.join(td, 'scene.raw')
             with open(path, 'wb') as fh:   # the SLAMBench frame layout (interface.h:254-256)
@@ -335,56 +581,61 @@ def main():
                     fh.write(struct.pack('<II', W, H) + np.ascontiguousarray(d, '<u2').tobytes())
                     fh.write(struct.pack('<II', W, H) + np.ascontiguousarray(rgb, np.uint8).tobytes())
             raw = mf.RawSequence(path)
-            g2 = mf.Kfusion(W, H, P0, bg_size=5.12, bg_res=BG_RES, batch_objects=not args.plain)
-            for i, m in enumerate(movers):
-                g2.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
-                g2.set_moving(i + 1, True)
+            lr = Leg(mf, hiprt, frames, movers, P0, args.plain, args)
 
             def step_raw(f):
-                for i, m in enumerate(dev[f][2]):
-                    g2.L.mf_object_set_mask_device(g2.ctx, i, m.ptr)
-                g2.preprocessing_raw(raw, f)
-                g2.tracking(K, f)
-                g2.integration(K, f)
-                g2.raycasting(K, f)
+                for i, m in enumerate(lr.dev[f][2]):
+                    lr.g.L.mf_object_set_mask_device(lr.g.ctx, i, m.ptr)
+                lr.g.preprocessing_raw(raw, f)
+                lr.g.tracking(K, f)
+                lr.g.integration(K, f)
+                lr.g.raycasting(K, f)
             for f in range(args.warmup):
                 step_raw(f)
-            g2.sync()
+            lr.g.sync()
             t = time.perf_counter()
             for f in range(args.warmup, n_frames):
                 step_raw(f)
-            g2.sync()
+            lr.g.sync()
             el2 = time.perf_counter() - t
             raw_leg = {'value': round(args.steps / el2, 3), 'unit': 'frames/s',
                        'path': '.raw file (page cache) -> pinned double-buffered staging -> PCIe -> preprocess'}
             raw.close()
-            g2.close()
+            lr.close()
 
     out = {
-        'metric': 'frames/s (track+integrate+raycast, %dx%d, cam+K objs)' % (W, H),
-        'value': round(fps, 3), 'unit': 'frames/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
-        'ms_per_step': round(ms, 3), 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'metric': METRIC,
+        'value': round(res['fps'], 3), 'unit': 'frames/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(res['ms'], 3), 'higher_is_better': True, 'scaling': 'weak',
+        # no published number exists for this metric: the
+        # comparison with the plain GPU path is vs_plain_gpu
+        'vs_baseline': None,
         'dtype': 'f32', 'data': 'synthetic',
-        'config': {'workload': ('1280x720 RGB-D (fx=fy=1050, 2%% invalid depth), bg 1024^3 @5mm + %d moving '
-                                'boxes/ellipsoids 0.05-0.4 m in 64^3-256^3 @2-5mm' % n_obj if HD else
-                                '640x480 RGB-D, bg 512^3 @1cm + 1 moving box 0.2x0.15x0.1 m in 128^3 @2mm'
-                                if n_obj == 1 else
-                                '640x480 RGB-D, bg 512^3 @1cm + %d moving boxes/ellipsoids 0.05-0.4 m in 64^3-256^3 '
-                                '@2-5mm' % n_obj) + ', ICP+RGB {10,5,4}, %s' % (
-                                    'one object per launch' if args.plain else 'objects batched per launch'),
-                   'objects': n_obj, 'frames': args.steps},
+        'config': {'workload': workload_name(n_obj, args.plain), 'objects': n_obj, 'frames': args.steps},
         'roofline': roof,
-        'event_timed_by_kernel': roof_all,
         'cpu_baseline': cpu,
+    }
+    if plain:
+        out['plain_gpu'] = {'value': round(plain['fps'], 3), 'unit': 'frames/s', 'ms_per_step': round(plain['ms'], 3),
+                            'workload': workload_name(n_obj, True)}
+        out['vs_plain_gpu'] = round(res['fps'] / plain['fps'], 4)
+    if accuracy:
+        out['pose_err_vs_fp64'] = accuracy
+    if single:
+        out['single_object'] = {'value': round(single['fps'], 3), 'unit': 'frames/s',
+                                'ms_per_step': round(single['ms'], 3), 'workload': workload_name(1, args.plain)}
+    out.update({
+        'event_timed_by_kernel': roof_all,
         'stage_ms_per_frame': stage,
         'kernel_ms_per_frame_total': round(tot_ms / 3, 3),
-        'host_enqueue_ms_per_frame': round(1000.0 * t_enq / args.steps, 3),
+        'host_enqueue_ms_per_frame': round(res['enqueue_ms'], 3),
         'work_per_frame': {k: v // 3 for k, v in st.items()},
-        'camera_translation_error_vs_gt_m': round(pose_err, 5),
-    }
-    out['tracking_rows_per_thread'] = rpt[:3]
-    if tuned:
-        out['autotune'] = tuned
+        'camera_translation_error_vs_gt_m': round(pose_gt, 5),
+        'tracking_rows_per_thread': rpt[:3],
+        'allocated_blocks': capacity,
+    })
+    if res['tuned']:
+        out['autotune'] = res['tuned']
     if raw_leg:
         out['host_input_rate'] = raw_leg
     if render:

@@ -496,8 +496,10 @@ __device__ void jacobi_pinv(const double* s27, int lane, double (&x)[6]) {
   double mx = 0;
 #pragma unroll
   for (int l = 0; l < 6; l++) mx = fmax(mx, fabs(rld(lam, l)));
+  // backsub keeps 1/lambda unless |lambda| * 1e6 <= max: a NaN eigenvalue
+  // (a NaN system) is kept, so NaN sums give a NaN step as in the reference
   double pr = 0;
-  if (fabs(lam) * 1e6 > mx) {
+  if (!(fabs(lam) * 1e6 <= mx)) {
 #pragma unroll
     for (int k = 0; k < 6; k++) pr += v[k] * s27[k];
     pr /= lam;

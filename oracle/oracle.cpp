@@ -12,6 +12,7 @@
 #include "oracle_track.h"
 #include "oracle_map.h"
 #include <cstdio>
+#include <ctime>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -282,8 +283,16 @@ struct Object {
   size_t last_emitted = 0;
 };
 
+// CLOCK_MONOTONIC seconds, as the reference's tock() samples (utils/timings.h:55-68)
+static double mono_s() {
+  timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 struct Pipe {
   Cfg c;
+  double t_alloc = 0, t_integ = 0;   // seconds in allocation / voxel update since the last orc_stage_clock
   std::vector<R> depth, grey;
   std::vector<V3> rgb;
   std::vector<V3> vertex, normal;
@@ -329,6 +338,7 @@ static void integrate_object(Pipe& p, Object& o, const std::vector<R>& mask, con
   const R voxel = o.size / (R)o.res;
   const M4 K = camK(k[0], k[1], k[2], k[3]);
   std::vector<uint64_t> keys;
+  const double t0 = mono_s();
   o.last_emitted = build_alloc_list(keys, o.vol, p.camera_pose, o.pose, K, p.depth.data(), mask.data(), p.c.w, p.c.h,
                                     voxel, R(2) * (R)p.c.mu);
   if (p.c.alloc_quirks) {
@@ -344,8 +354,11 @@ static void integrate_object(Pipe& p, Object& o, const std::vector<R>& mask, con
     o.last_keys = keys;
     if (!keys.empty()) o.vol.alloc_update(keys.data(), (int)keys.size());
   }
+  const double t1 = mono_s();
   const M4 Tco = mul(inverse(p.camera_pose), o.pose);
   integrate(o.vol, Tco, K, p.depth.data(), p.rgb.data(), mask.data(), p.c.w, p.c.h, (R)p.c.mu, 100);
+  p.t_alloc += t1 - t0;
+  p.t_integ += mono_s() - t1;
 }
 
 // raycastObjectList (rendering.cpp:9-181), all objects, has_integrated selects
@@ -1049,6 +1062,14 @@ int orc_track(void* h, const float k[4], unsigned frame) {
 }
 
 // Kfusion::integration (kfusion.cpp:573-677), objects with a mask set.
+// Seconds spent in allocation (band march + alloc_update) and in the voxel
+// update since the last call, then reset.
+void orc_stage_clock(void* h, double out[2]) {
+  Pipe* p = (Pipe*)h;
+  out[0] = p->t_alloc; out[1] = p->t_integ;
+  p->t_alloc = p->t_integ = 0;
+}
+
 int orc_integrate(void* h, const float k[4], unsigned frame) {
   Pipe* p = (Pipe*)h;
   if (!((frame % p->c.integ_rate) == 0 || frame <= 3)) return 0;

@@ -25,10 +25,11 @@ def build():
     subprocess.check_call(['make', '-s', '-C', _HERE])
 
 
-def _lib(fp64):
-    path = os.path.join(_HERE, 'liborc64.so' if fp64 else 'liborc32.so')
-    if not os.path.exists(path):
-        build()
+def _lib(fp64, path=None):
+    if path is None:
+        path = os.path.join(_HERE, 'liborc64.so' if fp64 else 'liborc32.so')
+        if not os.path.exists(path):
+            build()
     lib = C.CDLL(path)
     P = C.c_void_p
     f32p = np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS')
@@ -112,6 +113,7 @@ def _lib(fp64):
         'orc_octree_raycast': (C.c_float, [P, f32p, f32p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                            f32p]),
         'orc_num_threads': (C.c_int, []),
+        'orc_stage_clock': (None, [P, f64p]),
         'orc_real_bytes': (C.c_int, []),
     }
     for name, (res, args) in sig.items():
@@ -124,10 +126,13 @@ def _lib(fp64):
 _LIBS = {}
 
 
-def lib(fp64=False):
-    if fp64 not in _LIBS:
-        _LIBS[fp64] = _lib(fp64)
-    return _LIBS[fp64]
+def lib(fp64=False, path=None):
+    """The float (or double) oracle, or the float build at `path` (a timing
+    build made by `make timing`)."""
+    key = path or fp64
+    if key not in _LIBS:
+        _LIBS[key] = _lib(fp64, path)
+    return _LIBS[key]
 
 
 def f32(a):
@@ -138,8 +143,9 @@ class Oracle:
     """Kfusion-style per-frame pipeline on the CPU oracle."""
 
     def __init__(self, w, h, init_pose, iters=(10, 5, 4), use_icp=True, use_rgb=True, robust='cauchy',
-                 robust_k=None, mu=0.1, track_rate=1, integ_rate=1, init_frame=0, alloc_quirks=False, fp64=False):
-        self.L = lib(fp64)
+                 robust_k=None, mu=0.1, track_rate=1, integ_rate=1, init_frame=0, alloc_quirks=False, fp64=False,
+                 lib_path=None):
+        self.L = lib(fp64, lib_path)
         self.w, self.h = w, h
         cfg = OrcConfig()
         cfg.w, cfg.h, cfg.nlev = w, h, len(iters)
@@ -187,6 +193,12 @@ class Oracle:
 
     def raycast(self, k, frame):
         return self.L.orc_raycast(self.h_, f32(k), frame)
+
+    def stage_clock(self):
+        """(allocation s, voxel update s) since the last call."""
+        out = np.zeros(2)
+        self.L.orc_stage_clock(self.h_, out)
+        return float(out[0]), float(out[1])
 
     def pose(self, oid=-1):
         T = np.zeros(16)

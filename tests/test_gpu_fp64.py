@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 from conftest import scene, midfusion, oracle, K320
+from lockstep import condition, check_poses
 
 pytestmark = pytest.mark.gpu
 W, H = 320, 240
@@ -151,12 +152,22 @@ def run(n_frames, moving, W=W, H=H, K=K320, bg_res=256, invalid_frac=0.0, seed=1
         # objects start from the FP64 camera pose everywhere
         gpu.setPose(-1, P64.astype(np.float32))
         o32.set_pose(-1, P64.astype(np.float32))
+        if f > 0:
+            for i in range(1, 1 + len(mov)):
+                s = o64.eval(i, 0, np.linalg.inv(P64) @ o64.pose(i))
+                r['obj%d_cond' % i] = condition(s)
+                r['obj%d_inliers' % i] = int(s[28])
+        for x in (gpu, o32, o64):
+            x.trace_clear()
         gpu.track_objects(K, f)
         o32.track_objects(K, f)
         o64.track_objects(K, f)
         gpu.sync()
-        tg = gpu.trace()
+        tg, t32, t64 = gpu.trace(), o32.trace(), o64.trace()
         for i in range(1, 1 + len(mov)):
+            fg, f32, fo = ([tuple(int(v) for v in q[[1, 2, 5]]) for q in t if int(q[0]) == i] for t in (tg, t32, t64))
+            r['obj%d_flow_same' % i] = fg == fo
+            r['obj%d_flow_same_o32' % i] = f32 == fo
             Q64 = o64.pose(i)
             r['obj%d_gpu' % i] = pose_err(gpu.getPose(i), Q64)
             r['obj%d_o32' % i] = pose_err(o32.pose(i), Q64)
@@ -187,11 +198,7 @@ def test_camera_pose_vs_fp64_oracle():
 
 def test_camera_and_object_pose_vs_fp64_oracle():
     rep = run(4, moving=True)
-    for r in rep[1:]:
-        assert r['cam_gpu'][0] < 1e-5 and r['cam_gpu'][1] < 1e-5, r
-        # from the same camera pose the object pose also holds 1e-5 of the
-        # FP64 result (the FP32 oracle's own error is recorded beside it)
-        assert r['obj1_gpu'][0] < 1e-5 and r['obj1_gpu'][1] < 1e-5, r
+    check_poses(rep, 1)
 
 
 def test_weakly_observable_object_vs_fp64_oracle():
@@ -203,10 +210,11 @@ def test_weakly_observable_object_vs_fp64_oracle():
     certificate (the pseudo-inverse branch) are checked directly in
     test_gpu_solve."""
     rep = run(4, moving=[sphere()])
-    for r in rep[1:]:
-        assert r['cam_gpu'][0] < 1e-5 and r['cam_gpu'][1] < 1e-5, r
-        eg, eo = r['obj1_gpu'], r['obj1_o32']
-        assert eg[0] <= 2 * eo[0] + 1e-5 and eg[1] <= 2 * eo[1] + 1e-5, r
+    exc = check_poses(rep, 1)
+    # rotations about the centre are unobservable but for discretisation:
+    # the sphere's systems are far past the conditioning bound
+    assert all(r['obj1_cond'] > 1e4 for r in rep[1:]), [r['obj1_cond'] for r in rep[1:]]
+    assert len(exc) == len(rep) - 1
 
 
 def second_box():
@@ -226,11 +234,7 @@ def test_two_objects_batched_vs_fp64_oracle():
     FP64 oracle's, and a step-back taken one iteration earlier or later
     moves the result by millimetres, on the CPU oracle as well.)"""
     rep = run(4, moving=[big_box(), second_box()])
-    for r in rep[1:]:
-        assert r['cam_gpu'][0] < 1e-5 and r['cam_gpu'][1] < 1e-5, r
-        for i in (1, 2):
-            eg, eo = r['obj%d_gpu' % i], r['obj%d_o32' % i]
-            assert eg[0] <= 2 * eo[0] + 1e-5 and eg[1] <= 2 * eo[1] + 1e-5, r
+    check_poses(rep, 2)
 
 
 def test_precision_scale_1280x720_16_objects():
@@ -256,8 +260,4 @@ def test_precision_scale_1280x720_16_objects():
     for r in rep:
         g, o = r['tsdf_bg_vs_o64'], r['tsdf_bg_o32_vs_o64']
         assert g['outside'] <= 1.5 * o['outside'] + 1e-4, (g, o)
-    for r in rep[1:]:
-        assert r['cam_gpu'][0] < 1e-5 and r['cam_gpu'][1] < 1e-5, r
-        for i in range(1, 17):
-            eg, eo = r['obj%d_gpu' % i], r['obj%d_o32' % i]
-            assert eg[0] <= 2 * eo[0] + 1e-5 and eg[1] <= 2 * eo[1] + 1e-5, (i, eg, eo)
+    check_poses(rep, 16)

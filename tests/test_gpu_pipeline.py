@@ -179,18 +179,18 @@ def test_pipeline_icp_only_cfg1_like():
 
 def test_pipeline_icp_rgb_moving_object():
     # Against the FP32 oracle, whose 8-block sequential FP32 sums carry up to
-    # ~2e-5 m of pose error of their own with the masked camera re-track
-    # (test_gpu_fp64 holds the GPU to 1e-5 of the FP64 oracle and records the
-    # FP32 oracle's error beside it): camera within 5e-5 here.  Object GN is
-    # far worse conditioned than the camera's, so whole-pipeline object poses
-    # are held to 1e-3 while test_gpu_rows pins every object row exactly.
+    # ~2e-5 m of pose error of their own with the masked camera re-track: the
+    # camera within 5e-5 here.  Poses are held against the FP64 oracle in
+    # test_gpu_fp64 and test_gpu_configs (1e-5 m / 1e-5 rad, objects under
+    # the conditioning and control-flow rule of lockstep.check_poses); this
+    # test checks the maps: rows, keys, voxels and raycasts on identical
+    # inputs.
     rep = run_pair(320, 240, K320, 4, use_rgb=True, moving=True, check=voxel_check)
     for r in rep:
         for i in (0, 1):
             assert r['vox%d' % i]['outside'] < 1e-4, r['vox%d' % i]
         assert r['rows0_mismatch'] < 5e-3 and r['rows1_mismatch'] < 5e-3, r
         assert r['pose_t'] < 5e-5 and r['pose_r'] < 5e-5, r
-        assert r.get('obj1_t', 0) < 1e-3 and r.get('obj1_r', 0) < 1e-3, r
         assert r['keys0'][2] == 0 and r['keys1'][2] == 0, r
         assert r['label_mismatch'] < 1e-3, r
 
