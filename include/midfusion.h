@@ -167,6 +167,15 @@ int mf_object_block(mf_ctx* ctx, int id, int x, int y, int z, double* out);
  * were allocated.  (Octree contents for checks against the reference's
  * VoxelBlock data, node.hpp:83-157.) */
 long mf_object_voxels(mf_ctx* ctx, int id, const uint64_t* keys, long n, double* out);
+/* mf_object_voxels for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc);
+ * no counterpart in the reference.  d_keys (n keys) and d_out (n * 512 * 8
+ * doubles, same layout) are device pointers; n <= 65536, else MF_E_ARG.  It
+ * calls no hipMalloc or hipFree: the per-key found flags and their count live
+ * in a fixed 256-KiB device array of the library, one per GPU, held until the
+ * process ends and shared by every context on that GPU, so calls that
+ * overlap on one GPU race.  Blocking; returns how many keys were allocated. */
+long mf_object_voxels_device(mf_ctx* ctx, int id, const uint64_t* d_keys, long n, double* d_out);
 /* Number of internal octree nodes implied by the allocated blocks. */
 long mf_object_node_count(mf_ctx* ctx, int id);
 

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <algorithm>
 #include <map>
+#include <mutex>
 
 using namespace mf;
 
@@ -139,6 +140,7 @@ struct mf_ctx {
   int8_t* eval_codes = nullptr;
   std::vector<int> ids_host;
   std::vector<Object*> objs;
+  int* scratch = nullptr;        // this device's no-allocation scratch (address of `scratch` on the device)
   float ray_eps = -1.f;
   bool have_ref = false;
   bool pending_objects = false;   // camera phase done, object phase due
@@ -482,6 +484,57 @@ void upload_planes(mf_ctx* c, float* const dst[3], const float* src_aos, size_t 
 
 }  // namespace
 
+// ---- Scratch of the no-allocation voxel export (mf_object_voxels_device):
+// word 0 counts the keys found, words [64, 64 + n) hold the per-key found
+// flags.  A file-scope device array: every GPU the library runs on holds its
+// own copy until the process ends, shared by every context and call on that
+// GPU, so calls that overlap on one GPU race.  Host code never passes the
+// array's name to a kernel: it takes the device address from
+// hipGetSymbolAddress once per device (scratch_setup, from mf_create).
+static constexpr long NOALLOC_MAX_KEYS = 65536;
+static __device__ __align__(256) int scratch[64 + NOALLOC_MAX_KEYS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+int* g_scratch[MAX_DEVICES] = {};
+
+// Address of this device's `scratch`, looked up once per device with that
+// device current.
+int scratch_setup(mf_ctx* c) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES)
+    return fail(c, MF_E_HIP, "no current HIP device for the no-allocation scratch");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(c, MF_E_HIP, "hipGetSymbolAddress of the no-allocation scratch failed");
+    g_scratch[dev] = static_cast<int*>(p);
+  }
+  c->scratch = g_scratch[dev];
+  return 0;
+}
+
+__global__ void k_fill_i32(int* p, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+// *total += number of non-zero flags (one workgroup)
+__global__ __launch_bounds__(1024) void k_count_flags(const int* __restrict__ f, int n, int* total) {
+  __shared__ int part[1024 / 64];
+  int t = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) t += f[i] != 0;
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) a += part[w];
+    *total += a;
+  }
+}
+}  // namespace
+
 extern "C" {
 
 mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size_m, int bg_res, long bg_capacity) {
@@ -496,6 +549,8 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   if (c->cfg.integ_rate <= 0) c->cfg.integ_rate = 1;
   c->W = cfg->width; c->H = cfg->height; c->N0 = (size_t)c->W * c->H;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return nullptr; }
+  // a failed lookup leaves c->scratch null: mf_object_voxels_device then fails
+  scratch_setup(c);
   bool ok = true;
   c->depth = dalloc<float>(c, c->N0); c->grey = dalloc<float>(c, c->N0);
   for (int k = 0; k < 3; k++) c->rgb[k] = dalloc<float>(c, c->N0);
@@ -1411,6 +1466,30 @@ long mf_object_voxels(mf_ctx* c, int id, const uint64_t* keys, long n, double* o
   }
   hipFree(d_keys); hipFree(d_out); hipFree(d_found);
   return nfound;
+}
+
+// mf_object_voxels on device buffers with no allocation: the found flags and
+// their count live in this device's `scratch`.
+long mf_object_voxels_device(mf_ctx* c, int id, const uint64_t* d_keys, long n, double* d_out) {
+  if (!c) return MF_E_ARG;
+  if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  if (n < 0 || n > NOALLOC_MAX_KEYS)
+    return fail(c, MF_E_ARG, "mf_object_voxels_device takes at most 65536 keys per call");
+  if (n > 0 && (!d_keys || !d_out)) return fail(c, MF_E_ARG, "bad voxel export arguments");
+  if (!c->scratch) return fail(c, MF_E_HIP, "no-allocation scratch unavailable on this device");
+  hipStream_t s = c->stream;
+  int* count = c->scratch;
+  int* found = c->scratch + 64;
+  k_fill_i32<<<1, 64, 0, s>>>(count, 1, 0);
+  if (n > 0) {
+    launch_export_voxels(s, c->objs[id]->v, d_keys, (int)n, d_out, found);
+    k_count_flags<<<1, 1024, 0, s>>>(found, (int)n, count);
+  }
+  int total = 0;
+  hipMemcpyAsync(&total, count, sizeof total, hipMemcpyDeviceToHost, s);
+  const hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(e));
+  return total;
 }
 
 long mf_object_node_count(mf_ctx* c, int id) {

@@ -44,6 +44,15 @@ class DeviceBuffer:
         if hip().hipMemcpy(self.ptr, arr.ctypes.data_as(C.c_void_p), self.nbytes, 1) != 0:   # HostToDevice
             raise RuntimeError('hipMemcpy failed')
 
+    def download(self, dtype, shape):
+        """The buffer's first prod(shape) elements as a numpy array."""
+        out = np.empty(shape, dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError('download larger than the buffer')
+        if hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes, 2) != 0:   # DeviceToHost
+            raise RuntimeError('hipMemcpy failed')
+        return out
+
     def __del__(self):
         if getattr(self, 'ptr', None) and self.ptr.value:
             hip().hipFree(self.ptr)

@@ -96,6 +96,7 @@ SIGNATURES = {
     'mf_solve6': (C.c_int, [P, f64p, C.c_long, f64p, i32p]),
     'mf_object_voxels': (C.c_long, [P, C.c_int, np.ctypeslib.ndpointer(np.uint64, flags='C_CONTIGUOUS'), C.c_long, f64p]),
     'mf_autotune': (C.c_int, [P, f32p, C.c_int, i32p, f64p]),
+    'mf_object_voxels_device': (C.c_long, [P, C.c_int, P, C.c_long, P]),
     'mf_clear_trace': (None, [P]),
     'mf_get_level': (C.c_long, [P, C.c_int, C.c_int, f32p]),
     'mf_debug_rows': (C.c_int, [P, C.c_int]),
@@ -423,6 +424,10 @@ class Kfusion:
         if n < 0:
             self._chk(n)
         return out.reshape(len(keys), 512, 8), int(n)
+
+    def voxels_device(self, oid, d_keys, n, d_out):
+        """mf_object_voxels_device on device pointers; returns the allocated count."""
+        return self._chk(self.L.mf_object_voxels_device(self.ctx, oid, d_keys, n, d_out))
 
     def node_count(self, oid):
         return self._chk(self.L.mf_object_node_count(self.ctx, oid))
