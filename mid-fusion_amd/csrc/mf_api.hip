@@ -316,6 +316,12 @@ TrackLaunch base_launch(mf_ctx* c, const float k[4], int level, int object) {
   return p;
 }
 
+// Camera ICP + photometric rows run by pixel (both rows of a pixel on one
+// thread) when the level's rows per thread is even.
+void set_shape(TrackLaunch& p, int rows_per_thread) {
+  p.paired = (!p.object && p.use_icp && p.use_rgb && rows_per_thread % 2 == 0) ? 1 : 0;
+}
+
 // Rendered reference depth pyramids (tracking.cpp:218-238): vertex2depth at
 // T_w_r (inverse in mats[2]) then robust half-sampling, for the tracking
 // instances [first, first + n) of the descriptor table c->inst.
@@ -355,9 +361,9 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
     if (mask_labels) p.moving = mask_labels;
     const bool fused = c->track_fused[l] != 0;
     const bool objs_grid = object && c->cfg.batch_objects && !fused && c->track_obj_ppt[l] > 0;
-    const int nblk = objs_grid ? track_obj_blocks(p.w * p.h, c->track_obj_ppt[l])
-                               : track_blocks(p.w * p.h * kinds,
-                                              (object && c->track_obj_rpt[l] > 0) ? c->track_obj_rpt[l] : c->track_rpt[l]);
+    const int rpt = (object && c->track_obj_rpt[l] > 0) ? c->track_obj_rpt[l] : c->track_rpt[l];
+    set_shape(p, rpt);
+    const int nblk = objs_grid ? track_obj_blocks(p.w * p.h, c->track_obj_ppt[l]) : track_blocks(p.w * p.h * kinds, rpt);
     if (!fused) finish();
     for (int it = 0; it < c->cfg.iters[l]; ++it, ++q) {
       p.iter = it;
@@ -1026,7 +1032,7 @@ int mf_track_object(mf_ctx* c, int id, const float k[4]) {
   key.push_back((uint64_t)id);
   const int r = run_graph(c, key, [&]() {
     { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, OBJ_SLOT, 1, c->obj_ids + OBJ_SLOT, c->mats, c->mats + 0, c->mats + 1, 1, nullptr); }
-    { Timed t(c, KC_GNSMALL); launch_model_tiles(s, c->inst + OBJ_SLOT, 1, c->W, c->H); }
+    { Timed t(c, KC_GNSMALL); launch_model_tiles(s, c->inst + OBJ_SLOT, c->gn, 1, c->W, c->H); }
     if (c->cfg.use_rgb) render_reference(c, OBJ_SLOT, 1);
     gn_run(c, k, 1, OBJ_SLOT, 1, nullptr);
     { Timed t(c, KC_GNSMALL); launch_virtual_cams(s, c->gn, OBJ_SLOT, 1, c->obj_ids + OBJ_SLOT, c->vcam); }
@@ -1070,8 +1076,8 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
         if (ng > 0) {
           { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, 1, ng, c->obj_ids, c->mats, c->mats + 0, c->mats + 1, 1, &md->gate); }
           { Timed t(c, KC_GNSMALL);
-            if (c->cfg.batch_objects) launch_model_tiles(s, c->inst + 1, ng, c->W, c->H);
-            else for (int j = 0; j < ng; j++) launch_model_tiles(s, c->inst + 1 + j, 1, c->W, c->H); }
+            if (c->cfg.batch_objects) launch_model_tiles(s, c->inst + 1, c->gn, ng, c->W, c->H);
+            else for (int j = 0; j < ng; j++) launch_model_tiles(s, c->inst + 1 + j, c->gn, 1, c->W, c->H); }
           if (c->cfg.use_rgb) {
             if (c->cfg.batch_objects) render_reference(c, 1, ng);
             else for (int j = 0; j < ng; j++) render_reference(c, 1 + j, 1);
@@ -1724,6 +1730,7 @@ int mf_residual_eval(mf_ctx* c, int id, const float k[4], int level, const float
   }
   TrackLaunch p = base_launch(c, k, level, object);
   p.iter = 0;
+  set_shape(p, c->track_rpt[level]);
   const int kinds = (c->cfg.use_icp ? 1 : 0) + (c->cfg.use_rgb ? 1 : 0);
   const int nblk = track_blocks(p.w * p.h * kinds, c->track_rpt[level]);
   launch_eval(s, p, c->inst + slot, c->gn, state, m4(T16), c->partial, nblk);
@@ -1764,6 +1771,7 @@ int mf_replay_track_rows(mf_ctx* c, const float k[4], int level, int reps, doubl
   if (!c->have_ref) return fail(c, MF_E_ARG, "no tracked frame yet");
   TrackLaunch p = base_launch(c, k, level, 0);
   p.iter = 0;
+  set_shape(p, c->track_rpt[level]);
   // the reference pyramid is the last tracked frame, the live one the frame
   // before it: after a frame's hand-over both hold valid planes
   const int kinds = (c->cfg.use_icp ? 1 : 0) + (c->cfg.use_rgb ? 1 : 0);
@@ -1819,6 +1827,7 @@ int mf_autotune(mf_ctx* c, const float k[4], int reps, int32_t* chosen, double* 
     double tc[MF_TUNE_CANDIDATES];
     for (int ci = 0; ci < MF_TUNE_CANDIDATES; ci++) {
       const int nblk = track_blocks(p.w * p.h * kinds, cand[ci]);
+      set_shape(p, cand[ci]);
       auto iter = [&]() {
         launch_track_rows(s, p, c->inst + spare, 1, c->gn, c->partial, nblk, q);
         launch_track_solve(s, p, c->inst + spare, 1, c->gn, c->partial, nblk, q, sy);

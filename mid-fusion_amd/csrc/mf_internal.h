@@ -71,6 +71,8 @@ struct GNState {
   Mat4 T_final;    // T after the last solve (object: T_c2_o2)
   int off;         // 1: this run is skipped entirely (no rows, no solves)
   double sums[32]; // last reduction (FP64)
+  int tb[4];       // object: bounding box {x0, y0, x1, y1} of its model-map tiles holding a surface
+                   // (TrackInst::tiles), empty when x0 > x1
 };
 
 // Per-instance inputs of one tracking pass.

@@ -55,6 +55,8 @@ struct TrackLaunch {
   int n_labels;
   TrackRow* dbg;                       // optional row dump: [state][level][2*W*H], or null
   int n_levels;
+  int paired;                          // camera ICP + photometric rows: each thread takes whole pixels
+                                       // (both rows), two at a time (rows per thread even)
 };
 struct TrackSync {             // accumulation + solve of one launch
   TraceRec* trace;              // optional per-iteration trace
@@ -83,8 +85,10 @@ void launch_gn_fused(hipStream_t s, const TrackLaunch& p, const TrackInst* inst,
                      const TrackSync& sy);
 // Residual sums of one instance (`inst` points at its descriptor) at pose T:
 // GN state `state` gets T, then rows and the FP64 sums (into GNState::sums).
-// TrackInst::tiles of the n instances at insts (every one with tiles set).
-void launch_model_tiles(hipStream_t s, const TrackInst* insts, int n, int W, int H);
+// TrackInst::tiles of the n instances at insts (every one with tiles set),
+// and the box of tiles with a surface in each instance's GN state (reset by
+// launch_gn_begin_objects).
+void launch_model_tiles(hipStream_t s, const TrackInst* insts, GNState* st, int n, int W, int H);
 // The tracking 6x6 solve on n systems (27 doubles each) into x[6n] and
 // solver[n] (1 certified Gauss-Jordan, 2 Jacobi pseudo-inverse).
 void launch_solve6(hipStream_t s, const double* s27, int n, double* x, int* solver);

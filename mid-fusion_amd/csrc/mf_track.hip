@@ -79,6 +79,14 @@ __device__ __forceinline__ bool taps_all_miss(const uint8_t* tiles, int W, float
           ldx(tiles, (uint32_t)(ty1 * ntx + tx0)) | ldx(tiles, (uint32_t)(ty1 * ntx + tx1))) == 0;
 }
 
+// True when the 2x2 bilinear taps at (px, py) all lie in tiles outside the
+// box tb = {x0, y0, x1, y1} (in tiles) that holds every tile with a surface.
+__device__ __forceinline__ bool taps_outside_box(const int (&tb)[4], float px, float py) {
+  const int bx = (int)floorf(px), by = (int)floorf(py);
+  const int tx0 = bx / MAP_TILE, tx1 = (bx + 1) / MAP_TILE, ty0 = by / MAP_TILE, ty1 = (by + 1) / MAP_TILE;
+  return tx1 < tb[0] || tx0 > tb[2] || ty1 < tb[1] || ty0 > tb[3];
+}
+
 // calc_robust_weight / calc_robust_residual (tracking.cpp:1474-1532).  The
 // reference's sq() takes float: (e/k) in double, squared in float.
 // Cauchy, the reference's default (tracking.h:342-345):
@@ -112,7 +120,7 @@ __device__ __forceinline__ void robust(float e, int fn, double k, float& w, floa
 // LDS (broadcast) inside the row loop: holding 60 uniform floats in scalar
 // registers spills them, and in vector registers costs occupancy.
 // `mov` has bit l set when label l is a moving object (camera re-track).
-struct Mats { Mat4 T, invT, A, B, C; uint64_t mov; };
+struct Mats { Mat4 T, invT, A, B, C; uint64_t mov; int tb[4]; };
 
 // Marks loaded values as used at this point, so that loads issued above it
 // complete under one wait instead of being sunk, one by one, behind the
@@ -222,12 +230,54 @@ __device__ __forceinline__ int obj_icp_core(const TrackLaunch& p, const TrackIns
   J[0] = j0.x; J[1] = j0.y; J[2] = j0.z; J[3] = j1.x; J[4] = j1.y; J[5] = j1.z;
   return 1;
 }
+// The same row with its loads in the order the tests need them: the live
+// vertex and the normal's x first, the model-map taps only for projections
+// into the box of tiles with a surface (rows outside it interpolate the miss
+// constants with the same arithmetic, so their results are unchanged), and
+// the rest of the normal and the covariance only for rows that reach the
+// normal test.  Most of an object's rows cover pixels far from it and end at
+// the projection or at the taps.
 template <class MT>
 __device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst& in, const MT& M, int x, int y,
                                            size_t i, float& e, float J[6]) {
-  const f3 nl = lv3(p, PL_N, i), c2v = lv3(p, PL_V, i), P = lv3(p, PL_COV, i);
-  hold(nl); hold(c2v); hold(P);
-  return obj_icp_core(p, in, M, nl, c2v, P, e, J);
+  const float* nb = p.live + (size_t)PL_N * p.lstride;
+  const f3 c2v = lv3(p, PL_V, i);
+  const float nlx = ldx(nb, (uint32_t)i);
+  hold(c2v); hold(nlx);
+  if (nlx == INVALID) return -1;
+  if (c2v.x == 0 && c2v.y == 0 && c2v.z == 0) return -1;
+  const f3 o2v = affine(M.invT, c2v);
+  const f3 c1v = affine(M.A, o2v);
+  const f3 pj = rotate(camK(p.fx, p.fy, p.cx, p.cy), c1v);
+  const float ux = pj.x / pj.z, uy = pj.y / pj.z;
+  if (ux < 1 || ux > (float)(p.W - 2) || uy < 1 || uy > (float)(p.H - 2) || isnan(ux) || isnan(uy)) return -2;
+  f3 wrv, wrn;
+  if (in.tiles && (taps_outside_box(M.tb, ux, uy) || taps_all_miss(in.tiles, p.W, ux, uy))) {
+    wrv = bilin3_const(mk3(0.f, 0.f, 0.f), ux, uy);
+    wrn = bilin3_const(mk3((float)INVALID, 0.f, 0.f), ux, uy);
+  } else {
+    wrv = bilin3(in.mvn, (size_t)p.W * p.H, p.W, ux, uy);
+    wrn = bilin3(in.mvn + (size_t)3 * p.W * p.H, (size_t)p.W * p.H, p.W, ux, uy);
+    hold(wrv); hold(wrn);
+  }
+  if (wrn.x == INVALID) return -3;
+  const f3 o1rv = affine(M.B, wrv);
+  const f3 o1rn = rotate(M.B, wrn);
+  const f3 diff = o2v - o1rv;
+  const float residual = dot(o1rn, diff);
+  if (length(diff) > 0.1f) return -4;
+  const f3 nl = mk3(nlx, ldx(nb + p.lstride, (uint32_t)i), ldx(nb + 2 * (size_t)p.lstride, (uint32_t)i));
+  const f3 P = lv3(p, PL_COV, i);
+  const f3 o2n = rotate(M.invT, nl);
+  if (dot(o2n, o1rn) < 0.8f) return -5;
+  const float sig = o1rn.x * o1rn.x * P.x + o1rn.y * o1rn.y * P.y + o1rn.z * o1rn.z * P.z;
+  const float w = sqrtf((float)(1.0 / (double)sig));
+  e = w * residual;
+  const f3 jt = rotate(M.T, o1rn);
+  const f3 j0 = ((-1.0f) * w) * ((-1.0f) * jt);
+  const f3 j1 = w * cross(c2v, jt);
+  J[0] = j0.x; J[1] = j0.y; J[2] = j0.z; J[3] = j1.x; J[4] = j1.y; J[5] = j1.z;
+  return 1;
 }
 
 // Photometric rows: trackRGB (tracking.cpp:1023-1111) for the camera,
@@ -396,6 +446,175 @@ __device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst
   wg_reduce(s, part, red);
 }
 
+
+// Camera rows by pixel: each thread takes whole pixels, U at a time, and
+// computes both rows of each (track_ICP_Kernel's and trackRGB's, the same
+// arithmetic as icp_row and rgb_core<false>), issuing the own inputs of all
+// 2U rows as one batch and then all their gathers as one batch, so the 2U
+// rows wait on two memory round trips instead of 2U pairs (U = 1 keeps the
+// kernel at 143 VGPRs, 3 waves per SIMD; U = 2 needs 172, 2 waves, and a
+// level-0 launch of 2400 waves then runs in two rounds).  Pixels i = wg *
+// TB + tid + k * nactive * TB; ICP row then photometric row of each pixel
+// into the thread's FP32 sums.
+template <int RB, int U, class MT>
+__device__ __forceinline__ void camera_pixels(const TrackLaunch& p, const TrackInst& inst, const MT& M, int state,
+                                              int wg, int nactive, float* __restrict__ part, float (*red)[32]) {
+  const int tid = threadIdx.x;
+  const TrackInst in = inst;
+  float s[32];
+#pragma unroll
+  for (int k = 0; k < 32; k++) s[k] = 0.f;
+  const int n = p.w * p.h;
+  const size_t N0 = (size_t)p.W * p.H;
+  TrackRow* dbg = (p.dbg && state < 2) ? p.dbg + ((size_t)state * p.n_levels + p.level) * 2 * N0 : nullptr;
+  const int stride = nactive * TB;
+  const float lsc = (float)(1 << p.level);
+  const float fxl = p.fx / lsc, fyl = p.fy / lsc, cxl = p.cx / lsc, cyl = p.cy / lsc;
+  const float* Ip = p.live + (size_t)PL_I * p.lstride;
+  const float* Gx = p.live + (size_t)PL_GX * p.lstride;
+  const float* Gy = p.live + (size_t)PL_GY * p.lstride;
+  for (int base = wg * TB + tid; base < n; base += U * stride) {
+    asm volatile("" ::: "memory");   // keep the LDS matrix reads inside the loop
+    int idx[U];
+    f3 nl[U], vl[U], P[U], rvl[U];
+    float dr[U], ir[U];
+    LabelTaps lt[U];
+    // own inputs of both rows of every pixel
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      idx[u] = base + u * stride;
+      const uint32_t ii = idx[u] < n ? (uint32_t)idx[u] : 0u;
+      nl[u] = lv3(p, PL_N, ii); vl[u] = lv3(p, PL_V, ii); P[u] = lv3(p, PL_COV, ii);
+      if (p.label) label_taps(p, (int)ii % p.w, (int)ii / p.w, lt[u]);
+      dr[u] = ldx(in.rD + p.rd_off, ii); rvl[u] = rf3(p, PL_V, ii); ir[u] = ldx(p.ref + (size_t)PL_I * p.lstride, ii);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      hold(nl[u]); hold(vl[u]); hold(P[u]); hold(dr[u]); hold(rvl[u]); hold(ir[u]);
+      if (p.label) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) hold(lt[u].l[k]);
+      }
+    }
+    // projections (codes 0: pending)
+    int ci[U], cr[U];
+    f3 pv[U], wv[U], lvv[U];
+    uint32_t ri[U], i0[U];
+    float tfx[U], tfy[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int x = idx[u] % p.w, y = idx[u] / p.w;
+      ci[u] = 0; cr[u] = 0; ri[u] = 0; i0[u] = 0; tfx[u] = 0.f; tfy[u] = 0.f;
+      pv[u] = mk3(0.f, 0.f, 0.f); wv[u] = pv[u]; lvv[u] = pv[u];
+      // track_ICP_Kernel (tracking.cpp:923-1020), up to the reference lookup
+      if (p.label && taps_moving(p, lt[u], M.mov)) ci[u] = -6;
+      else if (nl[u].x == INVALID) ci[u] = -1;
+      else {
+        pv[u] = affine(M.T, vl[u]);
+        const f3 pp = affine(M.B, pv[u]);
+        const float ux = pp.x / pp.z, uy = pp.y / pp.z;
+        if (ux < 1 || ux > (float)(p.W - 2) || uy < 1 || uy > (float)(p.H - 2) || isnan(ux) || isnan(uy)) ci[u] = -2;
+        else ri[u] = (unsigned)ux + (unsigned)uy * (unsigned)p.W;
+      }
+      // trackRGB (tracking.cpp:1023-1111), up to the bilinear taps
+      const f3 vr = dr[u] > 0.0f ? dr[u] * rotate(camKinv(fxl, fyl, cxl, cyl), mk3((float)x, (float)y, 1.0f))
+                                 : mk3(0.f, 0.f, 0.f);
+      if (vr.z <= 0.f || vr.z == INVALID) cr[u] = -1;
+      else if (length(vr - rvl[u]) > 0.1f && rvl[u].z > 0.f) cr[u] = -3;
+      else {
+        wv[u] = affine(M.A, vr);
+        lvv[u] = affine(M.invT, wv[u]);
+        const f3 pj = rotate(camK(fxl, fyl, cxl, cyl), lvv[u]);
+        const float ux = pj.x / pj.z, uy = pj.y / pj.z;
+        if (ux < 1 || ux > (float)(p.w - 2) || uy < 1 || uy > (float)(p.h - 2) || isnan(ux) || isnan(uy)) cr[u] = -2;
+        else {
+          const float fpx = floorf(ux), fpy = floorf(uy);
+          i0[u] = (uint32_t)(int)fpx + (uint32_t)(int)fpy * (uint32_t)p.w;
+          tfx[u] = ux - fpx; tfy[u] = uy - fpy;
+        }
+      }
+      if (idx[u] >= n) { ci[u] = 1; cr[u] = 1; }   // no pixel: nothing to load or sum
+    }
+    // gathers of every pending row
+    f3 rn[U], rv[U];
+    float ta[U][12];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t r0 = ci[u] == 0 ? ri[u] : 0u;
+      rn[u] = mdl3(p, in, 3, r0); rv[u] = mdl3(p, in, 0, r0);
+      const uint32_t j0 = cr[u] == 0 ? i0[u] : 0u;
+      const uint32_t w = (uint32_t)p.w;
+      ta[u][0] = ldx(Ip, j0); ta[u][1] = ldx(Ip, j0 + 1); ta[u][2] = ldx(Ip, j0 + w); ta[u][3] = ldx(Ip, j0 + w + 1);
+      ta[u][4] = ldx(Gx, j0); ta[u][5] = ldx(Gx, j0 + 1); ta[u][6] = ldx(Gx, j0 + w); ta[u][7] = ldx(Gx, j0 + w + 1);
+      ta[u][8] = ldx(Gy, j0); ta[u][9] = ldx(Gy, j0 + 1); ta[u][10] = ldx(Gy, j0 + w); ta[u][11] = ldx(Gy, j0 + w + 1);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      hold(rn[u]); hold(rv[u]);
+#pragma unroll
+      for (int k = 0; k < 12; k++) hold(ta[u][k]);
+    }
+    // finish both rows of every pixel
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (idx[u] >= n) continue;
+      const int x = idx[u] % p.w, y = idx[u] / p.w;
+      const size_t ro = (size_t)x + (size_t)y * p.W;   // row stride W at every level (tracking.cpp:49-50)
+      {
+        float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
+        int res = ci[u];
+        if (res == 0) {
+          if (rn[u].x == INVALID) res = -3;
+          else {
+            const f3 diff = rv[u] - pv[u];
+            const f3 pn = rotate(M.T, nl[u]);
+            if (length(diff) > 0.1f) res = -4;
+            else if (dot(pn, rn[u]) < 0.8f) res = -5;
+            else {
+              const f3 Pu = P[u], rnu = rn[u];
+              const float sig = rnu.x * rnu.x * Pu.x + rnu.y * rnu.y * Pu.y + rnu.z * rnu.z * Pu.z;
+              const float w = sqrtf((float)(1.0 / (double)sig));
+              e = w * dot(rnu, diff);
+              const f3 j0 = w * rnu, j1 = w * cross(pv[u], rnu);
+              J[0] = j0.x; J[1] = j0.y; J[2] = j0.z; J[3] = j1.x; J[4] = j1.y; J[5] = j1.z;
+              res = 1;
+            }
+          }
+        }
+        accum<RB>(s, res, e, J, p.robust, p.robust_k);
+        if (in.codes && p.level == 0) in.codes[ro] = (int8_t)res;
+        if (dbg) { TrackRow& d = dbg[ro]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
+      }
+      {
+        float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
+        int res = cr[u];
+        if (res == 0) {
+          const float fx = tfx[u], fy = tfy[u];
+          const float* t = ta[u];
+          const float li = t[0] * (1.f - fx) * (1.f - fy) + t[1] * (1.f - fy) * fx + t[2] * (1.f - fx) * fy + t[3] * fx * fy;
+          const float g_x = t[4] * (1.f - fx) * (1.f - fy) + t[5] * (1.f - fy) * fx + t[6] * (1.f - fx) * fy + t[7] * fx * fy;
+          const float g_y = t[8] * (1.f - fx) * (1.f - fy) + t[9] * (1.f - fy) * fx + t[10] * (1.f - fx) * fy + t[11] * fx * fy;
+          const float rres = ir[u] - li;
+          const f3 lv = lvv[u];
+          const float sc = 1.f / lv.z;
+          f3 jt = sc * mk3(g_x * fxl, g_y * fyl, -(g_x * lv.x * fxl + g_y * lv.y * fyl) / lv.z);
+          jt = (-1.f) * rotate(M.T, jt);
+          const f3 jr = cross(wv[u], jt);
+          const float ic = p.inv_cov_rgb;
+          const f3 j0 = ic * jt, j1 = ic * jr;
+          J[0] = j0.x; J[1] = j0.y; J[2] = j0.z; J[3] = j1.x; J[4] = j1.y; J[5] = j1.z;
+          e = ic * rres;
+          res = 1;
+        }
+        accum<RB>(s, res, e, J, p.robust, p.robust_k);
+        if (in.codes && p.level == 0) in.codes[N0 + ro] = (int8_t)res;
+        if (in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
+        if (dbg) { TrackRow& d = dbg[N0 + ro]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
+      }
+    }
+  }
+  wg_reduce(s, part, red);
+}
 
 // ---- FP64 6x6 solve (commons.h:672-696 semantics: TooN GR_SVD backsub with
 // condition 1e6).  C is the symmetric JtWJ of the reduction, b its JtWe.
@@ -811,13 +1030,40 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
     // a level starts afresh (tracking.cpp:246); later iterations are skipped
     // once this level's loop has broken (uniform for the instance)
     skip = S.off || (p.iter > 0 && X.done_level == p.level);
-    if (!skip) { Msh.T = X.T; Msh.invT = X.invT; Msh.A = S.A; Msh.B = S.B; Msh.C = S.C; }
+    if (!skip) {
+      Msh.T = X.T; Msh.invT = X.invT; Msh.A = S.A; Msh.B = S.B; Msh.C = S.C;
+      if (OBJ) { Msh.tb[0] = S.tb[0]; Msh.tb[1] = S.tb[1]; Msh.tb[2] = S.tb[2]; Msh.tb[3] = S.tb[3]; }
+    }
   }
   if (tid < 64) stage_moving(p, Msh, tid);
   __syncthreads();
   if (skip) return;
   track_rows<ICP, RGB, OBJ, RB>(p, insts[j], Msh, state, blockIdx.x, nblk,
                             partial + ((size_t)j * nblk + blockIdx.x) * 32, red);
+}
+
+// Two-kernel form, camera rows by pixel (camera_pixels, one pixel per loop
+// iteration): the paired variant of k_track<true, true, false, RB>.
+template <int RB>
+__global__ __launch_bounds__(TB) void k_track_px(TrackLaunch p, const TrackInst* __restrict__ insts,
+                                                 const GNState* __restrict__ st, float* __restrict__ partial,
+                                                 int nblk, int q) {
+  __shared__ float red[NW][32];
+  __shared__ Mats Msh;
+  __shared__ int skip;
+  const int tid = threadIdx.x;
+  const int j = blockIdx.y;
+  const int state = insts[j].state;
+  const GNState& S = st[state];
+  if (tid == 0) {
+    const GNIter& X = S.it[q & 1];
+    skip = S.off || (p.iter > 0 && X.done_level == p.level);
+    if (!skip) { Msh.T = X.T; Msh.invT = X.invT; Msh.A = S.A; Msh.B = S.B; Msh.C = S.C; }
+  }
+  if (tid < 64) stage_moving(p, Msh, tid);
+  __syncthreads();
+  if (skip) return;
+  camera_pixels<RB, 1>(p, insts[j], Msh, state, blockIdx.x, nblk, partial + ((size_t)j * nblk + blockIdx.x) * 32, red);
 }
 
 // Two-kernel form, rows of every moving object in one grid over the pixels
@@ -963,6 +1209,7 @@ __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* 
     skip = S.off || !rows || (p.iter > 0 && X.done_level == p.level);
     if (!skip) {
       Msh.T = X.T; Msh.invT = X.invT; Msh.A = S.A; Msh.B = S.B; Msh.C = S.C;
+      if (OBJ) { Msh.tb[0] = S.tb[0]; Msh.tb[1] = S.tb[1]; Msh.tb[2] = S.tb[2]; Msh.tb[3] = S.tb[3]; }
       X.pend_level = p.level; X.pend_iter = p.iter;
     }
     if (writer) S.it[(q + 1) & 1] = X;
@@ -970,8 +1217,11 @@ __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* 
   if (tid < 64) stage_moving(p, Msh, tid);
   __syncthreads();
   if (skip) return;
-  track_rows<ICP, RGB, OBJ, RB>(p, insts[j], Msh, state, blockIdx.x, nblk, part_cur + ((size_t)j * nblk + blockIdx.x) * 32,
-                            red);
+  float* part = part_cur + ((size_t)j * nblk + blockIdx.x) * 32;
+  if constexpr (ICP && RGB && !OBJ) {
+    if (p.paired) { camera_pixels<RB, 1>(p, insts[j], Msh, state, blockIdx.x, nblk, part, red); return; }
+  }
+  track_rows<ICP, RGB, OBJ, RB>(p, insts[j], Msh, state, blockIdx.x, nblk, part, red);
 }
 
 // Residual evaluation at a given pose (forward/backward of the tracking
@@ -1035,6 +1285,7 @@ __global__ void k_gn_begin_objects(GNState* st, int first, int n, const int* ids
   S.C = inverse(S.A);
   S.object = 1;
   S.off = (gate && *gate == 0) ? 1 : 0;
+  S.tb[0] = S.tb[1] = 0x7fffffff; S.tb[2] = S.tb[3] = -1;   // no tile yet (k_model_tiles extends it)
   S.out_pose = pose; S.Twc2 = *T_w_c2;
   if (publish && !S.off) *pose = mul(*T_w_c2, T);
 }
@@ -1237,6 +1488,11 @@ int track_blocks(int nrows, int rows_per_thread) {
 void launch_track_rows(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
                        float* partial, int nblk, int q) {
   dim3 grd(nblk, ninst), blk(TB);
+  if (p.paired && !p.object && p.use_icp && p.use_rgb) {
+    if (p.robust == ROBUST_CAUCHY) k_track_px<ROBUST_CAUCHY><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
+    else k_track_px<ROBUST_ANY><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q);
+    return;
+  }
 #define MF_TRACK(RB)                                                                                   \
   if (p.object) {                                                                                      \
     if (p.use_icp && p.use_rgb) k_track<true, true, true, RB><<<grd, blk, 0, s>>>(p, inst, st, partial, nblk, q); \
@@ -1299,7 +1555,8 @@ void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNS
 }
 // TrackInst::tiles of instances [0, n): one wave per 8x8 tile of the model
 // maps, flag 1 when any pixel is not a raycast miss.
-__global__ __launch_bounds__(64) void k_model_tiles(const TrackInst* __restrict__ insts, int W, int H) {
+__global__ __launch_bounds__(64) void k_model_tiles(const TrackInst* __restrict__ insts, GNState* __restrict__ st,
+                                                    int W, int H) {
   const TrackInst& in = insts[blockIdx.y];
   const int ntx = (W + MAP_TILE - 1) / MAP_TILE;
   const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
@@ -1314,11 +1571,17 @@ __global__ __launch_bounds__(64) void k_model_tiles(const TrackInst* __restrict_
           __float_as_uint(m[5 * n + i]) != 0u;
   }
   const uint64_t b = __ballot(hit);
-  if (threadIdx.x == 0) in.tiles[blockIdx.x] = b != 0 ? 1 : 0;
+  if (threadIdx.x == 0) {
+    in.tiles[blockIdx.x] = b != 0 ? 1 : 0;
+    if (b != 0) {
+      int* tb = st[in.state].tb;
+      atomicMin(&tb[0], tx); atomicMin(&tb[1], ty); atomicMax(&tb[2], tx); atomicMax(&tb[3], ty);
+    }
+  }
 }
-void launch_model_tiles(hipStream_t s, const TrackInst* insts, int n, int W, int H) {
+void launch_model_tiles(hipStream_t s, const TrackInst* insts, GNState* st, int n, int W, int H) {
   const int nt = ((W + MAP_TILE - 1) / MAP_TILE) * ((H + MAP_TILE - 1) / MAP_TILE);
-  if (n > 0) k_model_tiles<<<dim3(nt, n), 64, 0, s>>>(insts, W, H);
+  if (n > 0) k_model_tiles<<<dim3(nt, n), 64, 0, s>>>(insts, st, W, H);
 }
 
 // The 6x6 solve on caller-given systems (27 doubles each: JtWe, then the

@@ -175,3 +175,18 @@ def test_parse_gt(tmp_path):
         assert np.allclose(P[i][:3, 3], [float('%.4f' % v) for v in t], atol=1e-6)
     with pytest.raises(midfusion.MFError):
         midfusion.parse_gt(str(tmp_path / 'missing.txt'))
+
+
+def test_tum_fixture_ground_truth_convention():
+    """The real-data fixture's poses follow the TUM line order tx ty tz qx qy
+    qz qw ("tum pose"): rotation from the quaternion (x, y, z, w), as scipy
+    reads it, and the two frames 33 ms apart about 1.2 cm apart."""
+    import os
+    from scipy.spatial.transform import Rotation
+    from conftest import ROOT
+    d = np.load(os.path.join(ROOT, 'tests', 'golden', 'tum_fr2_pair.npz'))
+    R = Rotation.from_quat([-0.7091, -0.2017, 0.2146, 0.6407]).as_matrix()
+    assert np.allclose(d['gt_r'][:3, :3], R, atol=1e-4)
+    assert np.allclose(d['gt_r'][:3, 3], [0.0068, -2.7651, 1.6077])
+    assert 0.005 < np.linalg.norm(d['gt_l'][:3, 3] - d['gt_r'][:3, 3]) < 0.02
+    assert d['depth_r'].dtype == np.uint16 and d['depth_r'].shape == (480, 640) and d['rgb_l'].shape == (480, 640, 3)

@@ -86,6 +86,39 @@ def pair(use_icp, use_rgb, obj, use_ramp, seed, W, H, K):
     return g, o, oid, T.astype(np.float32).astype(np.float64)
 
 
+def pair_fp32(o64, seed, obj):
+    """An FP32 oracle in the FP64 oracle's state of `pair` (maps and poses
+    copied from it, frame 1 preprocessed and its camera phase run)."""
+    W, H = o64.w, o64.h
+    K640 = np.array([525.0, 525.0, 319.5, 239.5], np.float32)
+    mov = [big_box()] if obj else []
+    sc = scene.Scene(seed=seed, width=W, height=H, K=tuple(K640), moving=mov)
+    kw = dict(use_icp=True, use_rgb=True, robust='huber', robust_k=0.1)
+    o = oracle.Oracle(W, H, sc.camera_pose(0).astype(np.float32), **kw)
+    o.add_object(5.12, 256, np.eye(4))
+    for m in mov:
+        o.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
+        o.set_moving(1, True)
+    for f in range(2):
+        d, rgb, lab = sc.render(f)
+        for i, m in enumerate(scene.masks_for_labels(lab, len(mov))):
+            o.set_mask(i, m)
+        if f == 1:
+            # the model maps of the FP64 oracle; the poses at the start of
+            # frame 1 are the initial ones (frame 0 only seeds the tracker)
+            o.set_maps(*o64.maps())
+            if obj:
+                o.set_object_maps(1, *o64.object_maps(1))
+        o.preprocess(d, rgb)
+        o.track_camera(K640, f)
+        if f == 1:
+            break
+        o.track_objects(K640, f)
+        o.integrate(K640, f)
+        o.raycast(K640, f)
+    return o
+
+
 def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-12))
 
@@ -124,23 +157,35 @@ def test_gpu_gradient_matches_fp64_central_differences(use_icp, use_rgb, obj, ra
 
 
 def test_sums_640x480_seed_sweep():
-    """The differentiation configuration's geometry: 640x480 pairs from seeds
-    1000-1019, camera and object reference poses perturbed by random twists
+    """The differentiation configuration: 640x480 pairs from its 100 seeds
+    1000-1099, camera and object reference poses perturbed by random twists
     of 1e-3 (rad and m), Huber delta = 0.1.  At every pyramid level the GPU's
-    loss, JtWe and JtWJ stay within 2e-3 of the FP64 oracle's and the inlier
-    counts within 0.2 %; the worst relative errors over the sweep are printed."""
+    loss, JtWe and JtWJ against the FP64 oracle's, beside the FP32 oracle's
+    (the reference's arithmetic) on the same inputs: inlier counts within
+    0.2 %; relative errors within 5e-3 everywhere, and over the sweep the
+    GPU's median error no larger than the FP32 oracle's.  A row whose
+    distance or normal gate flips between FP32 and FP64 moves the sums of a
+    small object by a few 1e-3; the worst cases are printed."""
     K640 = np.array([525.0, 525.0, 319.5, 239.5], np.float32)
-    worst = {'loss': 0.0, 'JtWe': 0.0, 'JtWJ': 0.0}
-    for seed in range(1000, 1020):
+    errs = []
+    for seed in range(1000, 1100):
         for obj in (False, True):
             g, o, oid, T = pair(True, True, obj, False, seed, 640, 480, K640)
+            o32 = pair_fp32(o, seed, obj)
             for level in (2, 1, 0):
-                sg, so = g.residual_eval(oid, K640, level, T), o.eval(oid, level, T)
-                r = {'loss': rel(sg[0], so[0]), 'JtWe': rel(sg[1:7], so[1:7]), 'JtWJ': rel(sg[7:28], so[7:28])}
-                for k in worst:
-                    worst[k] = max(worst[k], r[k])
+                sg, so, s32 = g.residual_eval(oid, K640, level, T), o.eval(oid, level, T), o32.eval(oid, level, T)
+                r = [rel(sg[0], so[0]), rel(sg[1:7], so[1:7]), rel(sg[7:28], so[7:28]),
+                     rel(s32[0], so[0]), rel(s32[1:7], so[1:7]), rel(s32[7:28], so[7:28])]
+                errs.append(r)
                 assert so[28] > 100, (seed, obj, level)
                 assert abs(sg[28] - so[28]) <= max(3, 2e-3 * so[28]), (seed, obj, level, sg[28], so[28])
-                assert r['loss'] < 2e-3 and r['JtWe'] < 2e-3 and r['JtWJ'] < 2e-3, (seed, obj, level, r)
+                if max(r[:3]) > 2e-3:
+                    print('seed', seed, 'object' if obj else 'camera', 'level', level, 'inliers', sg[28], so[28], s32[28],
+                          'gpu (loss, JtWe, JtWJ)', r[:3], 'fp32 oracle', r[3:])
             g.close()
-    print('worst relative error over 20 seeds x (camera, object) x 3 levels:', worst)
+        print('seed', seed, 'worst so far: gpu', np.array(errs)[:, :3].max(0), flush=True)
+    e = np.array(errs)
+    print('max relative error over 100 seeds x (camera, object) x 3 levels: gpu', e[:, :3].max(0),
+          'fp32 oracle', e[:, 3:].max(0), '; median: gpu', np.median(e[:, :3], 0), 'fp32 oracle', np.median(e[:, 3:], 0))
+    assert e[:, :3].max() < 5e-3, e[:, :3].max(0)
+    assert np.all(np.median(e[:, :3], 0) <= np.median(e[:, 3:], 0)), (np.median(e[:, :3], 0), np.median(e[:, 3:], 0))

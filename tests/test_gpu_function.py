@@ -1,0 +1,157 @@
+"""The C++ Function interface (include/midfusion_function.hpp) inside a full
+Gauss-Newton loop to convergence, through its test driver
+(tests/cpp/function_gn, built by __graft_entry__.build()).
+
+Differentiation configuration geometry: 640x480 pairs from seeds 1000-1009,
+camera and one textured box, Huber weight delta = 0.1; Gauss-Newton starts
+from the true frame-1 poses perturbed by random twists of 1e-3 (m and rad)
+and runs until its step is below 1e-8 (at most 100 iterations per level),
+undoing steps that raise the loss by more than 1e-3.
+The loop is the reference's Gauss-Newton control (tracking.cpp:242-335):
+{4, 5, 10} iterations at levels 2, 1, 0, a step that raises the loss undone
+and the level ended, a step below 1e-5 ending the level.  The same loop on
+the FP64 oracle's reduction sums (its own forward and backward) is the
+reference.  Most weighted residuals lie in Huber's linear region at
+delta = 0.1 and the correspondences are re-found at every step, so the loss
+is flat and piecewise along some directions (run without the iteration cap,
+both loops wander for a hundred iterations on some seeds): the solution is
+held by its exact loss.  For every seed and instance the GPU loop lowers the
+FP64 level-0 loss from the start pose, and ends within 1 % of the FP64
+loop's loss (median within 0.1 %); pose differences are printed.
+"""
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from conftest import ROOT, scene, oracle
+from lockstep import jtj, condition, pose_err
+
+pytestmark = pytest.mark.gpu
+DRIVER = os.path.join(ROOT, 'tests', 'cpp', 'function_gn')
+K640 = np.array([525.0, 525.0, 319.5, 239.5], np.float32)
+W, H = 640, 480
+DELTA = 0.1
+
+
+def twist(x):
+    M = np.zeros((4, 4))
+    M[:3, 3] = x[:3]
+    w = x[3:]
+    M[:3, :3] = [[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]]
+    return expm(M)
+
+
+def box():
+    c = 2.56
+    T0 = np.eye(4)
+    T0[:3, 3] = np.array([c, c, c - 1.5]) + np.array([0.05, 0.05, 1.0]) - 0.256
+    return scene.MovingBox([0.15, 0.12, 0.1], T0, [0.003, -0.002, 0.002, 0.01, 0.012, -0.008], [0.8, 0.3, 0.2],
+                           0.512, 128)
+
+
+# Gauss-Newton options of the runs: the reference's control flow
+MAX_ITERS, LEVEL_ITERS, STEP_TOL, LOSS_RTOL = 10, (10, 5, 4), 1e-5, 0.0
+
+
+def gauss_newton_fp64(o, oid, T):
+    """The loop of midfusion::gauss_newton on the FP64 oracle's sums."""
+    T = np.array(T, np.float64)
+    iters = []
+    for level in (2, 1, 0):
+        prev, prev_loss = T, np.inf
+        it = 0
+        for it in range(1, LEVEL_ITERS[level] + 1):
+            s = o.eval(oid, level, T)
+            if s[28] <= 0:
+                break
+            if s[0] > prev_loss * (1 + LOSS_RTOL):
+                T = prev
+                break
+            x = np.linalg.solve(jtj(s), s[1:7])
+            prev, prev_loss = T, s[0]
+            T = twist(x) @ T
+            if np.linalg.norm(x) < STEP_TOL:
+                break
+        iters.append(it)
+    return T, iters[::-1]
+
+
+def run_seed(seed, td):
+    mov = [box()]
+    sc = scene.Scene(seed=seed, width=W, height=H, K=tuple(K640), moving=mov)
+    P0 = sc.camera_pose(0).astype(np.float32)
+    frames = [sc.render(f) for f in range(2)]
+    with open(os.path.join(td, 'frames.raw'), 'wb') as fh:   # interface.h:254-256 layout
+        for d, rgb, _ in frames:
+            fh.write(struct.pack('<II', W, H) + np.ascontiguousarray(d, '<u2').tobytes())
+            fh.write(struct.pack('<II', W, H) + np.ascontiguousarray(rgb, np.uint8).tobytes())
+    masks = [scene.masks_for_labels(lab, 1) for _, _, lab in frames]
+    np.asarray(masks, np.float32).tofile(os.path.join(td, 'masks.bin'))
+    rng = np.random.default_rng(seed + 7)
+    T_cam = (twist(np.r_[rng.normal(0, 1e-3, 6)]) @ sc.camera_pose(1)).astype(np.float32)
+    T_obj = (twist(np.r_[rng.normal(0, 1e-3, 6)]) @ np.linalg.inv(sc.camera_pose(1)) @ mov[0].pose(1)).astype(np.float32)
+    T_wo = mov[0].pose(0).astype(np.float32)
+    with open(os.path.join(td, 'params.txt'), 'w') as fh:
+        vals = [W, H, *K640, 5.12, 256, DELTA, 1, *P0.reshape(-1), mov[0].vol_size, mov[0].vol_res, *T_wo.reshape(-1),
+                *T_cam.reshape(-1), *T_obj.reshape(-1), MAX_ITERS, *LEVEL_ITERS, STEP_TOL, LOSS_RTOL]
+        fh.write(' '.join(repr(float(v)) if not isinstance(v, int) else str(v) for v in vals) + '\n')
+    out = subprocess.run([DRIVER, td], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    got = {}
+    for line in out.stdout.split('\n'):
+        if line.strip():
+            v = line.split()
+            got[int(v[0])] = (np.array([float(x) for x in v[8:24]]).reshape(4, 4), [int(x) for x in v[1:7]])
+    # the FP64 oracle on the same frames, in the same state
+    o = oracle.Oracle(W, H, P0, fp64=True, robust='huber', robust_k=DELTA)
+    o.add_object(5.12, 256, np.eye(4))
+    o.add_object(mov[0].vol_size, mov[0].vol_res, T_wo)
+    o.set_moving(1, True)
+    for f, (d, rgb, _) in enumerate(frames):
+        for i, m in enumerate(masks[f]):
+            o.set_mask(i, m)
+        o.preprocess(d, rgb)
+        if f == 0:
+            o.track(K640, 0)
+            o.integrate(K640, 0)
+            o.raycast(K640, 0)
+        else:
+            o.track_camera(K640, 1)
+    ref_cam, it_cam = gauss_newton_fp64(o, -1, T_cam)
+    ref_obj, it_obj = gauss_newton_fp64(o, 1, T_obj)
+    # FP64 loss at level 0 of the FP64 optimum and of the GPU's solution
+    loss = {}
+    for oid, ref, start in ((-1, ref_cam, T_cam), (1, ref_obj, T_obj)):
+        l_ref = o.eval(oid, 0, ref)[0]
+        l_gpu = o.eval(oid, 0, got[oid][0])[0]
+        l_start = o.eval(oid, 0, start.astype(np.float64))[0]
+        loss[oid] = ((l_gpu - l_ref) / l_ref, l_gpu < l_start, l_ref < l_start)
+    kappa = {oid: condition(o.eval(oid, 0, ref)) for oid, ref in ((-1, ref_cam), (1, ref_obj))}
+    print('fp64 iterations: camera', it_cam, 'object', it_obj)
+    return got, ref_cam, ref_obj, kappa, loss
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason='tests/cpp/function_gn not built (__graft_entry__.build())')
+def test_function_gauss_newton_converges_to_fp64_solution():
+    rows = []
+    for seed in range(1000, 1010):
+        with tempfile.TemporaryDirectory() as td:
+            got, ref_cam, ref_obj, kappa, loss = run_seed(seed, td)
+        ec = pose_err(got[-1][0], ref_cam)
+        eo = pose_err(got[1][0], ref_obj)
+        print('seed', seed, 'camera', ec, 'dloss', loss[-1], 'cond', round(kappa[-1]), 'iterations/stops', got[-1][1],
+              '| object', eo, 'dloss', loss[1], 'cond', round(kappa[1]), got[1][1])
+        rows.append((seed, ec, eo, loss[-1], loss[1]))
+    d = []
+    for seed, ec, eo, lc, lo in rows:
+        for dl, gpu_down, ref_down in (lc, lo):
+            assert gpu_down and ref_down, (seed, lc, lo)
+            assert abs(dl) < 1e-2, (seed, lc, lo)
+            d.append(abs(dl))
+    print('median |relative loss difference|', np.median(d), 'max', np.max(d))
+    assert np.median(d) < 1e-3
