@@ -120,13 +120,13 @@ def solve_bytes(trace, rpt):
 
 
 # kernel class -> kernel name in rocprofv3 output (traffic counters)
-KERNEL_NAMES = {'track_rows': 'mf::k_track<true, true, false, 3>',
-                'track_rows_l0': 'mf::k_track<true, true, false, 3> level 0',
+KERNEL_NAMES = {'track_rows': 'mf::k_track_px<3>',
+                'track_rows_l0': 'mf::k_track_px<3> level 0',
                 'track_rows_obj': 'mf::k_track<true, true, true, 3>',
                 'track_solve': 'mf::k_solve', 'integrate': 'mf::k_integrate', 'raycast': 'mf::k_raycast',
                 'alloc_band': 'mf::k_alloc'}
 # committed rocprofv3 counter summary of each workload (FETCH_SIZE / WRITE_SIZE)
-TRAFFIC_FILES = {8: 'hbm_traffic_640x480_8objects.json', 1: 'hbm_traffic_cfg2.json'}
+TRAFFIC_FILES = {8: 'hbm_traffic_640x480_8objects.json'}
 
 
 def traffic_per_dispatch(kernel_class, n_obj):
