@@ -1012,6 +1012,18 @@ __device__ __forceinline__ void stage_moving(const TrackLaunch& p, Mats& M, int 
   if (tid == 0) M.mov = b;
 }
 
+// Chunk of rows taken by workgroup b of n, XCD-aware.  Workgroups are
+// dispatched to the 8 XCDs round-robin by linear id and each XCD has its own
+// L2; with n a multiple of 8 the linear id of (b, instance) is b modulo 8, and
+// the workgroups of one XCD take consecutive chunks, so the image rows that
+// neighbouring chunks share (bilinear taps, projected gathers) are fetched
+// into one L2.  The partial of a chunk is written at the chunk's index, so
+// the sums and their order do not depend on the mapping.
+__device__ __forceinline__ int xcd_chunk(int b, int n) {
+  if (n & 7) return b;
+  return (b & 7) * (n >> 3) + (b >> 3);
+}
+
 // Two-kernel form, rows: GN iteration q of every instance (grid.y =
 // instance), one partial of 32 FP32 sums per workgroup.
 template <bool ICP, bool RGB, bool OBJ, int RB>
@@ -1038,8 +1050,10 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
   if (tid < 64) stage_moving(p, Msh, tid);
   __syncthreads();
   if (skip) return;
-  track_rows<ICP, RGB, OBJ, RB>(p, insts[j], Msh, state, blockIdx.x, nblk,
-                            partial + ((size_t)j * nblk + blockIdx.x) * 32, red);
+  // object rows cluster where the object is: they keep the round-robin
+  // spread over the XCDs, which balances them better than L2 locality pays
+  const int c = OBJ ? (int)blockIdx.x : xcd_chunk(blockIdx.x, nblk);
+  track_rows<ICP, RGB, OBJ, RB>(p, insts[j], Msh, state, c, nblk, partial + ((size_t)j * nblk + c) * 32, red);
 }
 
 // Two-kernel form, camera rows by pixel (camera_pixels, one pixel per loop
@@ -1063,7 +1077,8 @@ __global__ __launch_bounds__(TB) void k_track_px(TrackLaunch p, const TrackInst*
   if (tid < 64) stage_moving(p, Msh, tid);
   __syncthreads();
   if (skip) return;
-  camera_pixels<RB, 1>(p, insts[j], Msh, state, blockIdx.x, nblk, partial + ((size_t)j * nblk + blockIdx.x) * 32, red);
+  const int c = xcd_chunk(blockIdx.x, nblk);
+  camera_pixels<RB, 1>(p, insts[j], Msh, state, c, nblk, partial + ((size_t)j * nblk + c) * 32, red);
 }
 
 // Two-kernel form, rows of every moving object in one grid over the pixels

@@ -205,10 +205,12 @@ def test_weakly_observable_object_vs_fp64_oracle():
     """A uniform sphere's rotations about its centre are observable only
     through the TSDF's discretisation and the depth noise, so its 6x6 systems
     are ill-conditioned and both FP32 paths drift from the FP64 result along
-    them.  The object pose must stay as close to the FP64 oracle as the FP32
-    oracle itself is (twice its error, plus 1e-5).  Systems past the solve's
-    certificate (the pseudo-inverse branch) are checked directly in
-    test_gpu_solve."""
+    them.  Its condition number is far past the bound of the per-object
+    contract (lockstep.check_poses), so every frame is an exception there,
+    held in aggregate: the GPU follows the FP64 control flow as often as the
+    FP32 oracle does, with a median error no larger than the FP32 oracle's.
+    Systems past the solve's certificate (the pseudo-inverse branch) are
+    checked directly in test_gpu_solve."""
     rep = run(4, moving=[sphere()])
     exc = check_poses(rep, 1)
     # rotations about the centre are unobservable but for discretisation:
@@ -226,9 +228,10 @@ def second_box():
 
 
 def test_two_objects_batched_vs_fp64_oracle():
-    """Two textured boxes tracked in one launch per GN iteration: each object
-    pose stays as close to the FP64 oracle as the FP32 oracle itself is
-    (twice its error, plus 1e-5).  (A near-degenerate
+    """Two textured boxes tracked in one launch per GN iteration, each object
+    under the per-object contract of lockstep.check_poses (1e-5 of the FP64
+    oracle when well conditioned with the FP64 control flow, else held in
+    aggregate against the FP32 oracle).  (A near-degenerate
     object, such as a small ellipsoid of ~1000 pixels, is no test of this:
     there the FP32 paths' errors at the finer levels differ by ~1% from the
     FP64 oracle's, and a step-back taken one iteration earlier or later
@@ -247,9 +250,8 @@ def test_precision_scale_1280x720_16_objects():
     both FP32 paths, because a projection rounded in FP32 picks another pixel
     at half-pixel boundaries (an invalid pixel or the far side of a depth
     edge, so one side updates the voxel and the other does not), and the
-    GPU's share stays within 1.5x the FP32 oracle's.  Each object pose stays
-    within twice the FP32 oracle's own error against the FP64 oracle, plus
-    1e-5."""
+    GPU's share stays within 1.5x the FP32 oracle's.  Object poses under the
+    per-object contract of lockstep.check_poses."""
     K1050 = np.array([1050.0, 1050.0, 639.5, 359.5], np.float32)
     rep = run(3, moving=scene.batch_objects(16, 5000), W=1280, H=720, K=K1050, bg_res=1024, invalid_frac=0.02,
               seed=5000, tsdf=True)

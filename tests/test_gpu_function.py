@@ -4,9 +4,7 @@ Gauss-Newton loop to convergence, through its test driver
 
 Differentiation configuration geometry: 640x480 pairs from seeds 1000-1009,
 camera and one textured box, Huber weight delta = 0.1; Gauss-Newton starts
-from the true frame-1 poses perturbed by random twists of 1e-3 (m and rad)
-and runs until its step is below 1e-8 (at most 100 iterations per level),
-undoing steps that raise the loss by more than 1e-3.
+from the true frame-1 poses perturbed by random twists of 1e-3 (m and rad).
 The loop is the reference's Gauss-Newton control (tracking.cpp:242-335):
 {4, 5, 10} iterations at levels 2, 1, 0, a step that raises the loss undone
 and the level ended, a step below 1e-5 ending the level.  The same loop on
