@@ -77,12 +77,35 @@ def workload_name(n_obj, plain):
     return w + ', ICP+RGB {10,5,4}, %s' % ('one object per launch' if plain else 'objects batched per launch')
 
 
+_RENDER_SCENE = None
+
+
+def _render_one(f):
+    return _RENDER_SCENE.render(f)
+
+
 def render_frames(sc, n_objects, n):
+    """Frames 0..n-1 of the seeded scene (each frame depends only on its
+    index), rendered by a pool of forked host processes when there are many;
+    called before anything touches the GPU."""
+    global _RENDER_SCENE
     scene = sys.modules['mf_scene']
     frames = []
-    for f in range(n):
-        d, rgb, lab = sc.render(f)
-        frames.append((d, rgb, scene.masks_for_labels(lab, n_objects)))
+    workers = min(16, os.cpu_count() or 1, n)
+    t0 = time.time()
+    if workers > 1 and n >= 16:
+        import multiprocessing as mp
+        _RENDER_SCENE = sc
+        with mp.get_context('fork').Pool(workers) as pool:
+            for f, (d, rgb, lab) in enumerate(pool.imap(_render_one, range(n), chunksize=1)):
+                frames.append((d, rgb, scene.masks_for_labels(lab, n_objects)))
+                if (f + 1) % 25 == 0:
+                    print('rendered %d / %d frames (%.0f s)' % (f + 1, n, time.time() - t0), file=sys.stderr, flush=True)
+        _RENDER_SCENE = None
+    else:
+        for f in range(n):
+            d, rgb, lab = sc.render(f)
+            frames.append((d, rgb, scene.masks_for_labels(lab, n_objects)))
     return frames
 
 
@@ -168,13 +191,15 @@ class Leg:
         # inputs resident in HBM before timing
         self.dev = [(hiprt.DeviceBuffer(d), hiprt.DeviceBuffer(rgb), [hiprt.DeviceBuffer(m) for m in masks])
                     for d, rgb, masks in frames]
+        self.mask_ids = np.arange(len(self.dev[0][2]) if self.dev else 0, dtype=np.int32)
+        self.mask_ptrs = [np.array([m.ptr.value for m in dm], np.uint64) for _, _, dm in self.dev]
         self.movers = movers
 
     def step(self, f):
         g = self.g
         dd, dr, dm = self.dev[f]
-        for i, m in enumerate(dm):
-            g.L.mf_object_set_mask_device(g.ctx, i, m.ptr)
+        # every instance's integration mask of the frame, one copy launch
+        g.L.mf_objects_set_masks_device(g.ctx, len(dm), self.mask_ids, self.mask_ptrs[f])
         g.preprocessing_device(dd.ptr, dr.ptr, W, H)
         g.tracking(K, f)
         g.integration(K, f)

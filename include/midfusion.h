@@ -104,6 +104,11 @@ int mf_object_set_instance_mask_device(mf_ctx* ctx, int id, const uint8_t* d_mas
 /* The instance's current integration mask (W*H floats). */
 int mf_object_get_mask(mf_ctx* ctx, int id, float* mask);
 int mf_object_set_mask_device(mf_ctx* ctx, int id, const float* d_mask);
+/* mf_object_set_mask_device for n objects at once (ids[i] gets d_masks[i];
+ * a null mask clears that object's mask), in one copy launch: the per-frame
+ * masks of all instances (Object::mask_ of every object, set per frame from
+ * the segmentation, segmentation.cpp:10-47).  Ids may not repeat. */
+int mf_objects_set_masks_device(mf_ctx* ctx, int n, const int32_t* ids, const float* const* d_masks);
 
 /* Kfusion::preprocessing(depth, rgb, inSize, filter) (kfusion.cpp:210-235):
  * mm->metres, grey + rgb/255, optional bilateral filter of the depth. */
@@ -242,10 +247,11 @@ int mf_reset_stats(mf_ctx* ctx);
  * hardware traffic counters). */
 int mf_copy_reference(mf_ctx* ctx, size_t bytes, int reps);
 /* Replays the camera's tracking-row kernel of pyramid level `level` `reps`
- * times back to back at the current camera state and returns its average
- * launch time (HIP events on the context stream) and algorithmic bytes per
- * launch (no counterpart in the reference: roofline measurement).  Call it
- * between frames; it overwrites the camera's track codes. */
+ * times back to back on a scratch copy of the camera's GN state (run even
+ * when the last frame's re-track left the camera's state off) and returns
+ * its average launch time (HIP events on the context stream) and algorithmic
+ * bytes per launch (no counterpart in the reference: roofline measurement).
+ * Call it between frames; it overwrites the camera's track codes. */
 int mf_replay_track_rows(mf_ctx* ctx, const float k[4], int level, int reps, double* avg_us, double* alg_bytes);
 /* Device bytes held by the context. */
 long mf_device_bytes(mf_ctx* ctx);

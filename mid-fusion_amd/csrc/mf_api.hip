@@ -44,6 +44,7 @@ struct Object {
   float* mvbf[3] = {nullptr, nullptr, nullptr};   // vertex map of the last raycast before fusion
   float* rD[MAX_LEVELS] = {};    // rendered reference depth pyramid
   uint8_t* tiles = nullptr;      // model-map tile flags (TrackInst::tiles)
+  int* tbox = nullptr;           // their box (TrackInst::tbox), filled by the main raycast
   ObjParams* prm = nullptr;
   bool tracked = false;          // tracked by mf_track_object in the pending object phase
 };
@@ -262,6 +263,7 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   o->codes = dalloc<int8_t>(c, 2 * c->N0);
   o->rgb_err = dalloc<float>(c, c->N0);
   o->tiles = dalloc<uint8_t>(c, (size_t)((c->W + MAP_TILE - 1) / MAP_TILE) * ((c->H + MAP_TILE - 1) / MAP_TILE));
+  o->tbox = dalloc<int>(c, 4);
   {
     float* b = dalloc<float>(c, 3 * c->N0);
     for (int k = 0; k < 3; k++) o->mvbf[k] = b ? b + k * c->N0 : nullptr;
@@ -272,12 +274,15 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   }
   o->prm = dalloc<ObjParams>(c, 1);
   if (!v.slot || !v.req || !v.nodes || !v.tsdf || !v.fg || !v.c || !v.fgw || !o->mask ||
-      !o->codes || !o->rgb_err || !o->prm || !o->mv[2] || !o->mn[2] || !o->mvbf[2] || !o->tiles || !v.blk_morton ||
+      !o->codes || !o->rgb_err || !o->prm || !o->mv[2] || !o->mn[2] || !o->mvbf[2] || !o->tiles || !o->tbox || !v.blk_morton ||
       !v.blk_active || !v.wt || !v.r || !v.g || !v.b || !v.counters || !o->rD[0])
     return rollback(MF_E_OOM, "device allocation failed");
   // INVALID normals in the model maps until the first raycast
   std::vector<float> inv(c->N0, (float)INVALID);
   hipMemcpyAsync(o->mn[0], inv.data(), c->N0 * sizeof(float), hipMemcpyHostToDevice, c->stream);
+  // every tile flag 0 (zeroed) and an empty tile box until then
+  static const int empty_box[4] = {0x7fffffff, 0x7fffffff, -1, -1};
+  hipMemcpyAsync(o->tbox, empty_box, sizeof empty_box, hipMemcpyHostToDevice, c->stream);
   const Mat4 M = m4(T);
   hipMemcpyAsync(c->mats + 3 + o->id, &M, sizeof(Mat4), hipMemcpyHostToDevice, c->stream);
   // virtual_camera_pose_ starts at the current camera pose
@@ -390,13 +395,13 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
 }
 
 void set_inst(mf_ctx* c, int slot, float* const mv[3], float* const mn[3], float* const rD[MAX_LEVELS],
-              int8_t* codes, float* rgb_err, int state, uint8_t* tiles = nullptr) {
+              int8_t* codes, float* rgb_err, int state, uint8_t* tiles = nullptr, const int* tbox = nullptr) {
   TrackInst ti;
   memset(&ti, 0, sizeof ti);
   ti.mvn = mv[0];      // planes: mv[0..2], mn[0..2] back to back
   ti.rD = rD[0];
   (void)mn;
-  ti.codes = codes; ti.rgb_err = rgb_err; ti.state = state; ti.tiles = tiles;
+  ti.codes = codes; ti.rgb_err = rgb_err; ti.state = state; ti.tiles = tiles; ti.tbox = tbox;
   // descriptors only change with the object set: upload on change, ordered
   // after every pending launch that may still read the old table
   if (memcmp(&ti, &c->inst_host[slot], sizeof ti) != 0) {
@@ -457,7 +462,7 @@ void set_mapobjs(mf_ctx* c, int& n_int, int& n_all, long& sum_nb3, long& sum_cap
   for (Object* o : c->objs) {
     MapObj m;
     memset(&m, 0, sizeof m);
-    m.v = o->v; m.prm = o->prm; m.mask = o->mask; m.id = o->id;
+    m.v = o->v; m.prm = o->prm; m.mask = o->mask; m.id = o->id; m.tbox = o->tbox;
     // refine_mask_use_motion with the instance's trackresult_ (static objects
     // hold copies of the camera's rows, kfusion.cpp:399-402, 450-454)
     m.codes = o->codes;
@@ -748,6 +753,24 @@ int mf_object_set_mask_device(mf_ctx* c, int id, const float* d_mask) {
   return 0;
 }
 
+int mf_objects_set_masks_device(mf_ctx* c, int n, const int32_t* ids, const float* const* d_masks) {
+  if (n < 0 || n > MAX_INST || (n > 0 && (!ids || !d_masks))) return fail(c, MF_E_ARG, "bad mask list");
+  for (int i = 0; i < n; i++)
+    if (ids[i] < 0 || ids[i] >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
+  MaskCopies mc;
+  int m = 0;
+  for (int i = 0; i < n; i++) {
+    Object* o = c->objs[ids[i]];
+    o->has_mask = d_masks[i] != nullptr;
+    if (!d_masks[i]) continue;
+    mc.src[m] = d_masks[i];
+    mc.dst[m] = o->mask;
+    ++m;
+  }
+  launch_copy_masks(c->stream, mc, m, c->N0);
+  return 0;
+}
+
 int mf_preprocess_device(mf_ctx* c, const uint16_t* d_depth, const uint8_t* d_rgb, int iw, int ih, int filter) {
   const int W = c->W, H = c->H;
   if (iw < W || ih < H || iw % W || ih % H || iw / W != ih / H) return fail(c, MF_E_ARG, "Invalid ratio.");
@@ -868,12 +891,16 @@ void upload_ray_tables(mf_ctx* c) {
     r.v = o->v; r.prm = o->prm; r.label = o->id;
     r.mvx = o->mv[0]; r.mvy = o->mv[1]; r.mvz = o->mv[2];
     r.mnx = o->mn[0]; r.mny = o->mn[1]; r.mnz = o->mn[2];
+    // moving objects track against these maps: the raycast flags their tiles
+    r.tiles = i > 0 ? o->tiles : nullptr; r.tbox = i > 0 ? o->tbox : nullptr;
     RayObj& lb = c->ray_lab_host[i];
     lb = r;
     lb.mvx = lb.mvy = lb.mvz = lb.mnx = lb.mny = lb.mnz = nullptr;
+    lb.tiles = nullptr; lb.tbox = nullptr;
     RayObj& b = c->ray_bf_host[i];
     b = r;
     b.mnx = b.mny = b.mnz = nullptr;
+    b.tiles = nullptr; b.tbox = nullptr;
     if (i == 0) { b.mvx = b.mvy = b.mvz = nullptr; }
     else { b.mvx = o->mvbf[0]; b.mvy = o->mvbf[1]; b.mvz = o->mvbf[2]; }
   }
@@ -1024,7 +1051,7 @@ int mf_track_object(mf_ctx* c, int id, const float k[4]) {
   Object* o = c->objs[id];
   if (o->tracked) return fail(c, MF_E_ARG, "object already tracked in this frame");
   hipStream_t s = c->stream;
-  set_inst(c, OBJ_SLOT, o->mv, o->mn, o->rD, o->codes, o->rgb_err, OBJ_SLOT, o->tiles);
+  set_inst(c, OBJ_SLOT, o->mv, o->mn, o->rD, o->codes, o->rgb_err, OBJ_SLOT, o->tiles, o->tbox);
   const int oid = 3 + id;
   hipMemcpyAsync(c->obj_ids + OBJ_SLOT, &oid, sizeof oid, hipMemcpyHostToDevice, s);
   if (hipStreamSynchronize(s) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
@@ -1032,7 +1059,6 @@ int mf_track_object(mf_ctx* c, int id, const float k[4]) {
   key.push_back((uint64_t)id);
   const int r = run_graph(c, key, [&]() {
     { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, OBJ_SLOT, 1, c->obj_ids + OBJ_SLOT, c->mats, c->mats + 0, c->mats + 1, 1, nullptr); }
-    { Timed t(c, KC_GNSMALL); launch_model_tiles(s, c->inst + OBJ_SLOT, c->gn, 1, c->W, c->H); }
     if (c->cfg.use_rgb) render_reference(c, OBJ_SLOT, 1);
     gn_run(c, k, 1, OBJ_SLOT, 1, nullptr);
     { Timed t(c, KC_GNSMALL); launch_virtual_cams(s, c->gn, OBJ_SLOT, 1, c->obj_ids + OBJ_SLOT, c->vcam); }
@@ -1060,7 +1086,7 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
     for (size_t j = 0; j < gnmov.size(); j++) {
       Object* o = gnmov[j];
       ids[j] = 3 + o->id;
-      set_inst(c, 1 + (int)j, o->mv, o->mn, o->rD, o->codes, o->rgb_err, 1 + (int)j, o->tiles);
+      set_inst(c, 1 + (int)j, o->mv, o->mn, o->rD, o->codes, o->rgb_err, 1 + (int)j, o->tiles, o->tbox);
     }
     if (!gnmov.empty()) set_ids(c, ids);
     std::vector<uint64_t> key = base_key(c, 2, k);
@@ -1075,9 +1101,6 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
         // more than one instance (kfusion.cpp:292, 395-414)
         if (ng > 0) {
           { Timed t(c, KC_GNSMALL); launch_gn_begin_objects(s, c->gn, 1, ng, c->obj_ids, c->mats, c->mats + 0, c->mats + 1, 1, &md->gate); }
-          { Timed t(c, KC_GNSMALL);
-            if (c->cfg.batch_objects) launch_model_tiles(s, c->inst + 1, c->gn, ng, c->W, c->H);
-            else for (int j = 0; j < ng; j++) launch_model_tiles(s, c->inst + 1 + j, c->gn, 1, c->W, c->H); }
           if (c->cfg.use_rgb) {
             if (c->cfg.batch_objects) render_reference(c, 1, ng);
             else for (int j = 0; j < ng; j++) render_reference(c, 1 + j, 1);
@@ -1153,7 +1176,7 @@ int mf_integrate(mf_ctx* c, const float k[4], unsigned frame) {
         wl.table_pre[j + 1] = wl.table_pre[j] + (long)v.nb * v.nb * v.nb;
         cap += v.cap;
       }
-      { Timed tt(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats, t, per, k[0], k[1], k[2], k[3], 1, wl); }
+      { Timed tt(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats, t, per, k[0], k[1], k[2], k[3], OBJ_ALLOC_RESET, wl); }
       { Timed tt(c, KC_ALLOC); launch_alloc(s, t, per, c->depth, refine, c->cfg.use_icp, c->cfg.use_rgb, c->W, c->H, band, stats); }
       { Timed tt(c, KC_COMMIT); launch_alloc_commit(s, t, per, wl); }
       { Timed tt(c, KC_INTEGRATE);
@@ -1176,7 +1199,8 @@ int mf_raycast(mf_ctx* c, const float k[4], unsigned frame) {
   long sum_nb3, sum_cap;
   set_mapobjs(c, n_int, n_all, sum_nb3, sum_cap);
   return run_graph(c, key, [&]() {
-    { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats, c->mapobjs + MAX_INST, n_all, k[0], k[1], k[2], k[3], 0, c->work); }
+    // the objects' model-map tile boxes restart empty: this raycast fills them
+    { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats, c->mapobjs + MAX_INST, n_all, k[0], k[1], k[2], k[3], OBJ_TBOX_RESET, c->work); }
     Timed t(c, KC_RAYCAST);
     // objects in view only (kfusion.cpp:557-558)
     launch_raycast(s, c->rayobjs, n, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, c->vtx[0], c->vtx[1],
@@ -1390,8 +1414,14 @@ int mf_get_object_maps(mf_ctx* c, int id, float* vertex, float* normal) {
 }
 int mf_set_object_maps(mf_ctx* c, int id, const float* vertex, const float* normal) {
   if (id < 0 || id >= (int)c->objs.size()) return fail(c, MF_E_ARG, "bad object id");
-  if (vertex) upload_planes(c, c->objs[id]->mv, vertex, c->N0);
-  if (normal) upload_planes(c, c->objs[id]->mn, normal, c->N0);
+  Object* o = c->objs[id];
+  if (vertex) upload_planes(c, o->mv, vertex, c->N0);
+  if (normal) upload_planes(c, o->mn, normal, c->N0);
+  // their tile flags and box, which the main raycast computes for its maps
+  static const int empty_box[4] = {0x7fffffff, 0x7fffffff, -1, -1};
+  hipMemcpyAsync(o->tbox, empty_box, sizeof empty_box, hipMemcpyHostToDevice, c->stream);
+  launch_map_tiles(c->stream, o->mv[0], o->tiles, o->tbox, c->W, c->H);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
   return 0;
 }
 
@@ -1766,9 +1796,34 @@ int mf_solve6(mf_ctx* c, const double* s27, long n, double* x, int32_t* solver) 
 // the context stream, and returns the average launch time and the
 // launch's algorithmic bytes.  Overwrites the camera's level-0 track codes and
 // the row partials; meant for measurement between frames.
+// Scratch GN state and descriptor in the spare slot (MAX_INST - 1, shared
+// with the residual evaluation): a copy of the camera's state, switched on
+// (the last frame's re-track may have left the camera's state off), whose
+// pose output goes to tune_pose; the descriptor takes the camera's inputs,
+// and its code and error outputs when `outputs` is set.
+static int scratch_camera(mf_ctx* c, bool outputs) {
+  hipStream_t s = c->stream;
+  if (!c->tune_pose && !(c->tune_pose = dalloc<Mat4>(c, 1))) return fail(c, MF_E_OOM, "device allocation failed");
+  const int spare = MAX_INST - 1;
+  GNState* sg = c->gn + spare;
+  hipMemcpyAsync(sg, c->gn, sizeof(GNState), hipMemcpyDeviceToDevice, s);
+  Mat4* tp = c->tune_pose;
+  const int zero = 0;
+  hipMemcpyAsync(&sg->out_pose, &tp, sizeof tp, hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(&sg->off, &zero, sizeof zero, hipMemcpyHostToDevice, s);
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
+  Object* bg = c->objs[0];
+  set_inst(c, spare, c->vtx, c->nrm, bg->rD, outputs ? bg->codes : nullptr, outputs ? bg->rgb_err : nullptr, spare);
+  return spare;
+}
+
 int mf_replay_track_rows(mf_ctx* c, const float k[4], int level, int reps, double* avg_us, double* alg_bytes) {
   if (level < 0 || level >= c->cfg.n_levels || reps < 1) return fail(c, MF_E_ARG, "bad level or repetition count");
   if (!c->have_ref) return fail(c, MF_E_ARG, "no tracked frame yet");
+  // the camera's launch on a scratch copy of its state, with the camera's
+  // code and error outputs (the same stores as in a frame)
+  const int spare = scratch_camera(c, true);
+  if (spare < 0) return spare;
   TrackLaunch p = base_launch(c, k, level, 0);
   p.iter = 0;
   set_shape(p, c->track_rpt[level]);
@@ -1777,9 +1832,9 @@ int mf_replay_track_rows(mf_ctx* c, const float k[4], int level, int reps, doubl
   const int kinds = (c->cfg.use_icp ? 1 : 0) + (c->cfg.use_rgb ? 1 : 0);
   const int nblk = track_blocks(p.w * p.h * kinds, c->track_rpt[level]);
   hipEvent_t a = ev_get(c), b = ev_get(c);
-  launch_track_rows(c->stream, p, c->inst, 1, c->gn, c->partial, nblk, 0);   // warm
+  launch_track_rows(c->stream, p, c->inst + spare, 1, c->gn, c->partial, nblk, 0);   // warm
   hipEventRecord(a, c->stream);
-  for (int r = 0; r < reps; r++) launch_track_rows(c->stream, p, c->inst, 1, c->gn, c->partial, nblk, 0);
+  for (int r = 0; r < reps; r++) launch_track_rows(c->stream, p, c->inst + spare, 1, c->gn, c->partial, nblk, 0);
   hipEventRecord(b, c->stream);
   if (hipEventSynchronize(b) != hipSuccess) return fail(c, MF_E_HIP, hipGetErrorString(hipGetLastError()));
   float ms = 0;
@@ -1797,20 +1852,8 @@ int mf_autotune(mf_ctx* c, const float k[4], int reps, int32_t* chosen, double* 
   if (!k || reps < 1 || reps > 10000) return fail(c, MF_E_ARG, "bad arguments");
   if (!c->have_ref) return fail(c, MF_E_ARG, "no tracked frame yet");
   hipStream_t s = c->stream;
-  if (!c->tune_pose && !(c->tune_pose = dalloc<Mat4>(c, 1))) return fail(c, MF_E_OOM, "device allocation failed");
-  // scratch GN state and descriptor (the residual evaluation's spare slot):
-  // a copy of the camera's state whose pose output goes to tune_pose, and
-  // the camera's inputs without code or error outputs
-  const int spare = MAX_INST - 1;
-  GNState* sg = c->gn + spare;
-  hipMemcpyAsync(sg, c->gn, sizeof(GNState), hipMemcpyDeviceToDevice, s);
-  Mat4* tp = c->tune_pose;
-  const int zero = 0;
-  hipMemcpyAsync(&sg->out_pose, &tp, sizeof tp, hipMemcpyHostToDevice, s);
-  hipMemcpyAsync(&sg->off, &zero, sizeof zero, hipMemcpyHostToDevice, s);
-  hipStreamSynchronize(s);
-  Object* bg = c->objs[0];
-  set_inst(c, spare, c->vtx, c->nrm, bg->rD, nullptr, nullptr, spare);
+  const int spare = scratch_camera(c, false);
+  if (spare < 0) return spare;
   TrackSync sy;
   memset(&sy, 0, sizeof sy);
   const int kinds = (c->cfg.use_icp ? 1 : 0) + (c->cfg.use_rgb ? 1 : 0);

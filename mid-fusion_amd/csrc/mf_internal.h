@@ -71,8 +71,6 @@ struct GNState {
   Mat4 T_final;    // T after the last solve (object: T_c2_o2)
   int off;         // 1: this run is skipped entirely (no rows, no solves)
   double sums[32]; // last reduction (FP64)
-  int tb[4];       // object: bounding box {x0, y0, x1, y1} of its model-map tiles holding a surface
-                   // (TrackInst::tiles), empty when x0 > x1
 };
 
 // Per-instance inputs of one tracking pass.
@@ -83,6 +81,7 @@ struct TrackInst {
   float* rgb_err;       // level-0 photometric row errors out [W*H] or null
   uint8_t* tiles;       // objects: per 8x8 pixel tile of the model maps, 0 when every
                         // pixel is a raycast miss (vertex 0, normal (INVALID, 0, 0)); or null
+  const int* tbox;      // with tiles: {x0, y0, x1, y1}, the box (in tiles) of the tiles flagged 1
   int state;            // index into GNState array
   int pad;
 };

@@ -83,15 +83,14 @@ void launch_track_solve(hipStream_t s, const TrackLaunch& p, const TrackInst* in
 void launch_gn_fused(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, int ninst, GNState* st,
                      const float* part_prev, int nblk_prev, float* part_cur, int nblk, int q, int rows,
                      const TrackSync& sy);
-// Residual sums of one instance (`inst` points at its descriptor) at pose T:
-// GN state `state` gets T, then rows and the FP64 sums (into GNState::sums).
-// TrackInst::tiles of the n instances at insts (every one with tiles set),
-// and the box of tiles with a surface in each instance's GN state (reset by
-// launch_gn_begin_objects).
-void launch_model_tiles(hipStream_t s, const TrackInst* insts, GNState* st, int n, int W, int H);
+// Tile flags and tile box (TrackInst::tiles, tbox) of caller-written model
+// maps (6 planes of W*H at mvn); tbox must hold the empty box.
+void launch_map_tiles(hipStream_t s, const float* mvn, uint8_t* tiles, int* tbox, int W, int H);
 // The tracking 6x6 solve on n systems (27 doubles each) into x[6n] and
 // solver[n] (1 certified Gauss-Jordan, 2 Jacobi pseudo-inverse).
 void launch_solve6(hipStream_t s, const double* s27, int n, double* x, int* solver);
+// Residual sums of one instance (`inst` points at its descriptor) at pose T:
+// GN state `state` gets T, then rows and the FP64 sums (into GNState::sums).
 void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNState* st, int state, const Mat4& T,
                  float* partial, int nblk);
 // Motion gating state on the device (Kfusion::tracking, kfusion.cpp:286-471),
@@ -162,6 +161,12 @@ void launch_motion_after(hipStream_t s, MotionDev* md, const int32_t* label_bf, 
 // [first, first + n), unless the run was skipped.
 void launch_virtual_cams(hipStream_t s, const GNState* st, int first, int n, const int* ids, Mat4* vcam);
 // dst <- src (2*n codes) and edst <- esrc (ne photometric errors) when *gate != 0
+// Copies of up to MAX_INST float images of n elements in one launch.
+struct MaskCopies {
+  const float* src[MAX_INST];
+  float* dst[MAX_INST];
+};
+void launch_copy_masks(hipStream_t s, const MaskCopies& mc, int m, size_t n);
 void launch_copy_codes(hipStream_t s, const int8_t* src, int8_t* dst, size_t n2, const float* esrc, float* edst,
                        size_t ne, const int* gate);
 // Frame start for the camera: T_w_r = T_w_c, inverse(T_w_r), GN state 0.
@@ -182,6 +187,7 @@ struct MapObj {
   ObjParams* prm;
   const float* mask;      // generalised integration mask
   const int8_t* codes;    // the instance's last level-0 track codes (refine_mask_use_motion)
+  int* tbox;              // the object's model-map tile box (TrackInst::tbox)
   int id;                 // object id: pose in mats[3 + id]
   int pad;
 };
@@ -194,8 +200,12 @@ struct MapWork {
   int* act_count;                 // active blocks this frame
   int2* act_list;                 // (object, slot) of each active block
 };
+// flags: OBJ_ALLOC_RESET starts an allocation pass (new-block count and the
+// volumes' frame-start counters), OBJ_TBOX_RESET empties the objects' model-map
+// tile boxes ahead of the raycast that fills them.
+enum { OBJ_ALLOC_RESET = 1, OBJ_TBOX_RESET = 2 };
 void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx,
-                       float fy, float cx, float cy, int alloc_reset, const MapWork& wl);
+                       float fy, float cx, float cy, int flags, const MapWork& wl);
 // Work counters accumulated on the device (algorithmic-byte accounting):
 // [0] voxel updates, [1] blocks integrated, [2] march steps, [3] trilinear
 // evaluations, [4] raycast hits, [5] allocation band samples, [6] pixels
@@ -221,6 +231,8 @@ struct RayObj {
   const ObjParams* prm;
   int label;
   float* mvx; float* mvy; float* mvz; float* mnx; float* mny; float* mnz;   // per-object output maps
+  uint8_t* tiles;         // set with the maps: their 8x8 tile flags and tile box (TrackInst::tiles,
+  int* tbox;              // tbox), the box reset beforehand by launch_obj_params(..., OBJ_TBOX_RESET)
 };
 // raycastObjectList over `objs`: objects whose label is not flagged in
 // `in_view` are skipped unless no object is flagged (null: every object); the

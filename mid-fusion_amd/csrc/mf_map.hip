@@ -42,13 +42,17 @@ __device__ void obj_params(const Mat4& Twc, const Mat4& Two, float fx, float fy,
 // Matrices of every object of a stage (one thread each), and the start of
 // this frame's allocation bookkeeping when `alloc_reset` is set.
 __global__ void k_obj_params(const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx, float fy,
-                             float cx, float cy, int alloc_reset, int* new_count) {
+                             float cx, float cy, int flags, int* new_count) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (alloc_reset && j == 0) *new_count = 0;
+  if ((flags & OBJ_ALLOC_RESET) && j == 0) *new_count = 0;
   if (j >= n) return;
   const MapObj& o = objs[j];
   obj_params(*T_w_c, mats[3 + o.id], fx, fy, cx, cy, o.prm);
-  if (alloc_reset) o.v.counters[3] = o.v.counters[0];
+  if (flags & OBJ_ALLOC_RESET) o.v.counters[3] = o.v.counters[0];
+  if ((flags & OBJ_TBOX_RESET) && o.tbox) {
+    o.tbox[0] = o.tbox[1] = 0x7fffffff;
+    o.tbox[2] = o.tbox[3] = -1;
+  }
 }
 
 // Band ray-march of buildVolumeAllocationList: every sample in a missing
@@ -311,8 +315,8 @@ __global__ __launch_bounds__(512) void k_integrate(const MapObj* __restrict__ ob
 // Every launcher covers the `n` objects of `objs` (device table) in one
 // launch; a one-object-per-launch caller passes n = 1 and a table offset.
 void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx,
-                       float fy, float cx, float cy, int alloc_reset, const MapWork& wl) {
-  k_obj_params<<<(n + 63) / 64, 64, 0, s>>>(T_w_c, mats, objs, n, fx, fy, cx, cy, alloc_reset, wl.new_count);
+                       float fy, float cx, float cy, int flags, const MapWork& wl) {
+  k_obj_params<<<(n + 63) / 64, 64, 0, s>>>(T_w_c, mats, objs, n, fx, fy, cx, cy, flags, wl.new_count);
 }
 void launch_alloc(hipStream_t s, const MapObj* objs, int n, const float* depth, int refine, int use_icp,
                   int use_rgb, int w, int h, float band, unsigned long long* stats) {

@@ -139,6 +139,7 @@ __device__ __forceinline__ bool child_exists(const VolDev& v, int lv, uint32_t p
 // its t_max, for the leaf_level + 1 scales a push or pop can reach.
 static constexpr int RT = 256;          // raycast threads per workgroup
 static constexpr int RT_TILE = 16;      // pixel tile side per workgroup (RT = RT_TILE^2)
+static_assert(RT_TILE == 2 * MAP_TILE, "each raycast wave covers one model-map tile");
 static constexpr int MAX_LG = 10;       // leaf level <= 10: res <= 8192
 template <bool STATS>
 __device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, float farp, float eps,
@@ -327,6 +328,7 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
   const int x = blockIdx.x * RT_TILE + (wv & 1) * 8 + (lane & 7);
   const int y = blockIdx.y * RT_TILE + (wv >> 1) * 8 + (lane >> 3);
   unsigned nsteps = 0, ninterp = 0, nhits = 0, npairs = 0, nnodes = 0;
+  int surf = -1;   // object whose maps get a surface pixel here (not the miss constants)
   if (x < w && y < h) {
   const size_t pix = (size_t)x + (size_t)y * w;
   float nearest = 1.0e3f;
@@ -401,7 +403,31 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
     const bool mine = (out_l == O.label);
     if (O.mvx) { O.mvx[pix] = mine ? out_v.x : 0.f; O.mvy[pix] = mine ? out_v.y : 0.f; O.mvz[pix] = mine ? out_v.z : 0.f; }
     if (O.mnx) { O.mnx[pix] = mine ? out_n.x : (float)INVALID; O.mny[pix] = mine ? out_n.y : 0.f; O.mnz[pix] = mine ? out_n.z : 0.f; }
+    if (mine && O.tiles &&
+        (__float_as_uint(out_v.x) | __float_as_uint(out_v.y) | __float_as_uint(out_v.z) | __float_as_uint(out_n.y) |
+         __float_as_uint(out_n.z) | (__float_as_uint(out_n.x) ^ __float_as_uint((float)INVALID))) != 0u)
+      surf = oi;
   }
+  }
+  // Tile flags of the objects' model maps (TrackInst::tiles; each wave
+  // covers one 8x8 tile): 1 when a pixel of the tile got anything but the
+  // miss constants, and the box of those tiles, which the object rows use to
+  // skip model-map loads.
+  {
+    const int ntx = (w + MAP_TILE - 1) / MAP_TILE, nty = (h + MAP_TILE - 1) / MAP_TILE;
+    const int ttx = (int)blockIdx.x * 2 + (wv & 1), tty = (int)blockIdx.y * 2 + (wv >> 1);
+    for (int oi = 0; oi < nobj; oi++) {
+      const RayObj& O = objs[oi];
+      if (!O.tiles) continue;
+      const uint64_t b = __ballot(surf == oi);
+      if (lane == 0 && ttx < ntx && tty < nty) {
+        O.tiles[tty * ntx + ttx] = b != 0 ? 1 : 0;
+        if (b != 0) {
+          atomicMin(&O.tbox[0], ttx); atomicMin(&O.tbox[1], tty);
+          atomicMax(&O.tbox[2], ttx); atomicMax(&O.tbox[3], tty);
+        }
+      }
+    }
   }
   if (!STATS) return;
   atomicAdd(&cnt[0], nsteps); atomicAdd(&cnt[1], ninterp); atomicAdd(&cnt[2], nhits); atomicAdd(&cnt[3], npairs);

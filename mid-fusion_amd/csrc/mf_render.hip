@@ -139,6 +139,32 @@ void launch_generalize(hipStream_t s, const uint8_t* mask, const uint8_t* global
   k_generalize<<<dim3((w + 255) / 256, h), 256, 0, s>>>(mask, global, w, h, rendered, count, out);
 }
 
+// Integration masks of several objects in one launch (grid.y = object):
+// float4 copies when every pointer is 16-byte aligned, else floats.
+__global__ void k_copy_masks(MaskCopies mc, size_t n, int vec) {
+  const float* __restrict__ src = mc.src[blockIdx.y];
+  float* __restrict__ dst = mc.dst[blockIdx.y];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (vec) {
+    const float4* __restrict__ s4 = reinterpret_cast<const float4*>(src);
+    float4* __restrict__ d4 = reinterpret_cast<float4*>(dst);
+    for (size_t i = t; i < n / 4; i += stride) d4[i] = s4[i];
+    for (size_t i = (n / 4) * 4 + t; i < n; i += stride) dst[i] = src[i];
+  } else {
+    for (size_t i = t; i < n; i += stride) dst[i] = src[i];
+  }
+}
+void launch_copy_masks(hipStream_t s, const MaskCopies& mc, int m, size_t n) {
+  if (m <= 0) return;
+  int vec = 1;
+  for (int i = 0; i < m; i++)
+    if (((uintptr_t)mc.src[i] | (uintptr_t)mc.dst[i]) & 15) vec = 0;
+  const size_t units = vec ? (n + 3) / 4 : n;
+  const unsigned gx = (unsigned)std::min<size_t>(1024, (units + 255) / 256);
+  k_copy_masks<<<dim3(gx, m), 256, 0, s>>>(mc, n, vec);
+}
+
 void launch_render_track(hipStream_t s, const int8_t* codes, const float* err, int n, int mode, uint32_t* out) {
   k_render_track<<<(n + 255) / 256, 256, 0, s>>>(codes, err, n, mode, out);
 }

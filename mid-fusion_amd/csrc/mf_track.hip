@@ -1044,7 +1044,10 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
     skip = S.off || (p.iter > 0 && X.done_level == p.level);
     if (!skip) {
       Msh.T = X.T; Msh.invT = X.invT; Msh.A = S.A; Msh.B = S.B; Msh.C = S.C;
-      if (OBJ) { Msh.tb[0] = S.tb[0]; Msh.tb[1] = S.tb[1]; Msh.tb[2] = S.tb[2]; Msh.tb[3] = S.tb[3]; }
+      if (OBJ && insts[j].tbox) {
+        const int* tb = insts[j].tbox;
+        Msh.tb[0] = tb[0]; Msh.tb[1] = tb[1]; Msh.tb[2] = tb[2]; Msh.tb[3] = tb[3];
+      }
     }
   }
   if (tid < 64) stage_moving(p, Msh, tid);
@@ -1224,7 +1227,10 @@ __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* 
     skip = S.off || !rows || (p.iter > 0 && X.done_level == p.level);
     if (!skip) {
       Msh.T = X.T; Msh.invT = X.invT; Msh.A = S.A; Msh.B = S.B; Msh.C = S.C;
-      if (OBJ) { Msh.tb[0] = S.tb[0]; Msh.tb[1] = S.tb[1]; Msh.tb[2] = S.tb[2]; Msh.tb[3] = S.tb[3]; }
+      if (OBJ && insts[j].tbox) {
+        const int* tb = insts[j].tbox;
+        Msh.tb[0] = tb[0]; Msh.tb[1] = tb[1]; Msh.tb[2] = tb[2]; Msh.tb[3] = tb[3];
+      }
       X.pend_level = p.level; X.pend_iter = p.iter;
     }
     if (writer) S.it[(q + 1) & 1] = X;
@@ -1300,7 +1306,6 @@ __global__ void k_gn_begin_objects(GNState* st, int first, int n, const int* ids
   S.C = inverse(S.A);
   S.object = 1;
   S.off = (gate && *gate == 0) ? 1 : 0;
-  S.tb[0] = S.tb[1] = 0x7fffffff; S.tb[2] = S.tb[3] = -1;   // no tile yet (k_model_tiles extends it)
   S.out_pose = pose; S.Twc2 = *T_w_c2;
   if (publish && !S.off) *pose = mul(*T_w_c2, T);
 }
@@ -1568,35 +1573,32 @@ void launch_eval(hipStream_t s, const TrackLaunch& p, const TrackInst* inst, GNS
   launch_track_rows(s, p, inst, 1, st, partial, nblk, 0);
   k_eval_sums<<<1, SOLVE_NT, 0, s>>>(st, state, partial, nblk);
 }
-// TrackInst::tiles of instances [0, n): one wave per 8x8 tile of the model
-// maps, flag 1 when any pixel is not a raycast miss.
-__global__ __launch_bounds__(64) void k_model_tiles(const TrackInst* __restrict__ insts, GNState* __restrict__ st,
-                                                    int W, int H) {
-  const TrackInst& in = insts[blockIdx.y];
+// Tile flags and tile box of model maps written by the caller (the main
+// raycast computes them for the maps it writes): one wave per 8x8 tile of
+// the 6 planes at mvn, flag 1 when any pixel is not a raycast miss (vertex 0,
+// normal (INVALID, 0, 0)); tbox must hold the empty box beforehand.
+__global__ __launch_bounds__(64) void k_map_tiles(const float* __restrict__ mvn, uint8_t* __restrict__ tiles,
+                                                  int* __restrict__ tbox, int W, int H) {
   const int ntx = (W + MAP_TILE - 1) / MAP_TILE;
   const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
   const int x = tx * MAP_TILE + (threadIdx.x & 7), y = ty * MAP_TILE + (threadIdx.x >> 3);
   bool hit = false;
   if (x < W && y < H) {
     const size_t n = (size_t)W * H, i = (size_t)x + (size_t)y * W;
-    const float* m = in.mvn;
     const unsigned inv = __float_as_uint((float)INVALID);
-    hit = __float_as_uint(m[i]) != 0u || __float_as_uint(m[n + i]) != 0u || __float_as_uint(m[2 * n + i]) != 0u ||
-          __float_as_uint(m[3 * n + i]) != inv || __float_as_uint(m[4 * n + i]) != 0u ||
-          __float_as_uint(m[5 * n + i]) != 0u;
+    hit = (__float_as_uint(mvn[i]) | __float_as_uint(mvn[n + i]) | __float_as_uint(mvn[2 * n + i]) |
+           (__float_as_uint(mvn[3 * n + i]) ^ inv) | __float_as_uint(mvn[4 * n + i]) |
+           __float_as_uint(mvn[5 * n + i])) != 0u;
   }
   const uint64_t b = __ballot(hit);
   if (threadIdx.x == 0) {
-    in.tiles[blockIdx.x] = b != 0 ? 1 : 0;
-    if (b != 0) {
-      int* tb = st[in.state].tb;
-      atomicMin(&tb[0], tx); atomicMin(&tb[1], ty); atomicMax(&tb[2], tx); atomicMax(&tb[3], ty);
-    }
+    tiles[blockIdx.x] = b != 0 ? 1 : 0;
+    if (b != 0) { atomicMin(&tbox[0], tx); atomicMin(&tbox[1], ty); atomicMax(&tbox[2], tx); atomicMax(&tbox[3], ty); }
   }
 }
-void launch_model_tiles(hipStream_t s, const TrackInst* insts, GNState* st, int n, int W, int H) {
+void launch_map_tiles(hipStream_t s, const float* mvn, uint8_t* tiles, int* tbox, int W, int H) {
   const int nt = ((W + MAP_TILE - 1) / MAP_TILE) * ((H + MAP_TILE - 1) / MAP_TILE);
-  if (n > 0) k_model_tiles<<<dim3(nt, n), 64, 0, s>>>(insts, st, W, H);
+  k_map_tiles<<<nt, 64, 0, s>>>(mvn, tiles, tbox, W, H);
 }
 
 // The 6x6 solve on caller-given systems (27 doubles each: JtWe, then the

@@ -68,6 +68,8 @@ SIGNATURES = {
     'mf_set_track_errors': (C.c_int, [P, C.c_int, f32p]),
     'mf_object_set_mask': (C.c_int, [P, C.c_int, f32p]),
     'mf_object_set_mask_device': (C.c_int, [P, C.c_int, P]),
+    'mf_objects_set_masks_device': (C.c_int, [P, C.c_int, np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'),
+                                              np.ctypeslib.ndpointer(np.uint64, flags='C_CONTIGUOUS')]),
     'mf_object_set_instance_mask': (C.c_int, [P, C.c_int, u8p, C.c_int, P]),
     'mf_object_set_instance_mask_device': (C.c_int, [P, C.c_int, P, C.c_int, P]),
     'mf_object_get_mask': (C.c_int, [P, C.c_int, f32p]),
@@ -331,6 +333,13 @@ class Kfusion:
 
     def set_mask(self, oid, mask):
         self._chk(self.L.mf_object_set_mask(self.ctx, oid, _f32(mask).reshape(-1)))
+
+    def set_masks_device(self, ids, ptrs):
+        """Integration masks of several objects from device pointers (ints;
+        0 clears a mask), one copy launch."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        ptrs = np.ascontiguousarray([int(p or 0) for p in ptrs], np.uint64)
+        self._chk(self.L.mf_objects_set_masks_device(self.ctx, len(ids), ids, ptrs))
 
     # ---- per-frame stages (kernels.h:242-250)
     def preprocessing(self, depth, rgb, filt=False):

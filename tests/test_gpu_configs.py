@@ -112,3 +112,35 @@ def test_track_object_arguments():
         g.raycasting(K, f)
     g.sync()
     g.close()
+
+
+def test_batched_device_masks_match_per_object_calls():
+    """mf_objects_set_masks_device copies every listed object's mask in one
+    launch: equal to mf_object_set_mask_device per object, float4 and scalar
+    paths (a pointer 4 bytes off alignment), a null entry clears that
+    object's mask, bad ids are MF_E_ARG."""
+    import os
+    from conftest import load_module, PKG
+    hiprt = load_module('mf_hiprt', os.path.join(PKG, 'hiprt.py'))
+    w, h = 160, 120
+    rng = np.random.default_rng(7)
+    masks = [rng.choice([-1.0, 0.0, 0.5, 1.0], size=(h, w)).astype(np.float32) for _ in range(4)]
+    bufs = [hiprt.DeviceBuffer(np.concatenate([[0.0], m.reshape(-1)]).astype(np.float32)) for m in masks]
+    g = midfusion.Kfusion(w, h, np.eye(4, dtype=np.float32), bg_res=64)
+    for _ in range(3):
+        g.add_object(0.256, 64, np.eye(4, dtype=np.float32))
+    # element 0 of each buffer is padding: the mask starts 4 bytes in, off
+    # 16-byte alignment, so the launch takes its scalar path
+    g.set_masks_device([0, 1, 2, 3], [b.ptr.value + 4 for b in bufs])
+    g.sync()
+    for i in range(4):
+        assert np.array_equal(g.get_mask(i).reshape(h, w), masks[i]), i
+    al = [hiprt.DeviceBuffer(m) for m in masks]
+    g.set_masks_device([3, 2, 1, 0], [b.ptr.value for b in al])   # aligned: float4 copies, permuted
+    g.sync()
+    for i in range(4):
+        assert np.array_equal(g.get_mask(i).reshape(h, w), masks[3 - i]), i
+    g.set_masks_device([2], [0])                                  # clears object 2's mask
+    with pytest.raises(midfusion.MFError):
+        g.set_masks_device([4], [al[0].ptr.value])
+    g.close()
