@@ -121,14 +121,22 @@ __device__ f3 grad(const VolDev& v, VCache& cc, f3 pos) {
 
 // Byte offset of internal-node level lv >= 1 in VolDev::nodes: levels are
 // stored back to back from level 1, level l holding 8^l occupancy bytes, so
-// the offset is sum_{k=1}^{lv-1} 8^k = (8^lv - 8) / 7.
-__device__ __forceinline__ uint32_t node_offset(int lv) { return ((1u << (3 * lv)) - 8u) / 7u; }
+// the offset is sum_{k=1}^{lv-1} 8^k = 8 + 64 + ..., the octal digits 1 at
+// positions 1..lv-1: the bit pattern 0x49249248 below bit 3*lv.
+__device__ __forceinline__ uint32_t node_offset(int lv) { return 0x49249248u & ((1u << (3 * lv)) - 1u); }
 
 // Child occupancy of an octree node: level `lv` (1 = children of the root),
-// Morton prefix `pre` of the child.
+// Morton prefix `pre` of the child.  One byte load on either kind of level,
+// so that a wave whose lanes test leaf and internal children in the same
+// step waits for one memory round trip, not two: at the leaf level the most
+// significant byte of the block-table entry (its sign: -1 = missing), above
+// it the occupancy byte.
 __device__ __forceinline__ bool child_exists(const VolDev& v, int lv, uint32_t pre) {
-  if (lv == v.leaf_level) return gload(v.slot + pre) >= 0;
-  return gload(v.nodes + node_offset(lv) + pre) != 0;
+  const bool leaf = lv == v.leaf_level;
+  const uint8_t* a = leaf ? reinterpret_cast<const uint8_t*>(v.slot) + 4 * (size_t)pre + 3
+                          : v.nodes + node_offset(lv) + pre;
+  const uint8_t b = gload(a);
+  return leaf ? (b & 0x80u) == 0 : b != 0;
 }
 
 // ray_iterator::next() first call (octree.hpp:923-1093): entry distance, in
