@@ -130,6 +130,8 @@ struct mf_ctx {
   uint32_t* render_out = nullptr;
   int32_t* label_scratch = nullptr;
   size_t render_cap = 0;
+  MotionObj* motion_tab = nullptr;    // moving objects' motion-residual inputs and outputs (one launch)
+  std::vector<MotionObj> motion_host;
   RayObj* rayobjs_lab = nullptr;      // raycast table writing no per-object maps
   RayObj ray_lab_host[MAX_INST];
   unsigned long long* mesh_total = nullptr;   // marching-cubes triangle counter
@@ -584,6 +586,7 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->vcam = dalloc<Mat4>(c, MAX_INST);
   c->rayobjs_bf = dalloc<RayObj>(c, MAX_INST);
   c->rayobjs_lab = dalloc<RayObj>(c, MAX_INST);
+  c->motion_tab = dalloc<MotionObj>(c, MAX_INST);
   c->ray_pose = dalloc<Mat4>(c, 1);
   c->view_dev = dalloc<Mat4>(c, 1);
   c->palette_dev = dalloc<uint32_t>(c, 256);
@@ -605,7 +608,7 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->work.act_count = dalloc<int>(c, 1);
   ok = ok && c->work.pool_pre && c->work.new_count && c->work.act_count;
   ok = ok && c->vbf[2] && c->label_bf && c->motion_codes && c->md && c->vcam && c->rayobjs_bf && c->rayobjs_lab &&
-       c->ray_pose && c->view_dev && c->palette_dev;
+       c->motion_tab && c->ray_pose && c->view_dev && c->palette_dev;
   ok = ok && c->stats && c->obj_ids && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&
        c->inst && c->partial && c->moving && c->trace && c->trace_n && c->rayobjs && c->mapobjs && c->gauss;
   if (!ok) { mf_destroy(c); return nullptr; }
@@ -1089,6 +1092,17 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
       set_inst(c, 1 + (int)j, o->mv, o->mn, o->rD, o->codes, o->rgb_err, 1 + (int)j, o->tiles, o->tbox);
     }
     if (!gnmov.empty()) set_ids(c, ids);
+    // the moving objects' motion-residual table (changes with the moving set)
+    std::vector<MotionObj> mtab;
+    for (Object* o : mov) mtab.push_back({o->mv[0], o->mvbf[0], c->vcam + o->id, o->codes, o->rgb_err});
+    if (mtab.size() != c->motion_host.size() ||
+        (!mtab.empty() && memcmp(mtab.data(), c->motion_host.data(), mtab.size() * sizeof(MotionObj)) != 0)) {
+      c->motion_host = mtab;
+      if (!mtab.empty()) {
+        hipMemcpyAsync(c->motion_tab, c->motion_host.data(), mtab.size() * sizeof(MotionObj), hipMemcpyHostToDevice, s);
+        hipStreamSynchronize(s);
+      }
+    }
     std::vector<uint64_t> key = base_key(c, 2, k);
     key.push_back((uint64_t)c->cfg.batch_objects);
     key.push_back((uint64_t)nobj);
@@ -1119,13 +1133,10 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
         raycast_before_fusion(c, k, md->in_view, &md->move_any);
         { Timed t(c, KC_MOTION); launch_motion_after(s, md, c->label_bf, (int)c->N0, nobj, c->cfg.use_rgb); }
         // each moving object's rows: motion residuals at its virtual camera
-        // pose (kfusion.cpp:445-467)
-        for (Object* o : mov) {
-          Timed t(c, KC_MOTION);
-          MotionLaunch m = motion_launch(c, k, c->vcam + o->id, o->mv[0], o->mvbf[0], o->codes, &md->cond2);
-          m.rgb_err = o->rgb_err;
-          launch_motion_rows(s, m);
-        }
+        // pose (kfusion.cpp:445-467), one launch for all of them
+        { Timed t(c, KC_MOTION);
+          launch_motion_rows_n(s, motion_launch(c, k, nullptr, nullptr, nullptr, nullptr, &md->cond2), c->motion_tab,
+                               (int)mov.size()); }
       } else {
         Timed t(c, KC_MOTION);
         launch_motion_after(s, md, nullptr, (int)c->N0, nobj, c->cfg.use_rgb);

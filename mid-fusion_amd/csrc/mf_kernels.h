@@ -128,6 +128,12 @@ struct MotionLaunch {
   int* counts;              //   per-label {pixels, inliers} counts [2*MAX_INST]
 };
 void launch_motion_rows(hipStream_t s, const MotionLaunch& m);
+// n instances' rows in one launch: `base` (no per-label counts) with each
+// instance's maps, pose and outputs from a device table; outputs disjoint.
+struct MotionObj {
+  const float* mvn; const float* vbf; const Mat4* T_w_l; int8_t* codes; float* rgb_err;
+};
+void launch_motion_rows_n(hipStream_t s, const MotionLaunch& base, const MotionObj* objs, int n);
 // Display images (uchar4 as one 32-bit word per pixel).
 void launch_render_track(hipStream_t s, const int8_t* codes, const float* err, int n, int mode, uint32_t* out);
 void launch_render_depth(hipStream_t s, const float* depth, int n, float nearp, float farp, uint32_t* out);

@@ -1405,7 +1405,7 @@ __global__ void k_copy_codes(const int8_t* __restrict__ src, int8_t* __restrict_
   if (t < n - 4 * n4) dst[4 * n4 + t] = src[4 * n4 + t];
   for (size_t i = t; i < ne; i += st) edst[i] = esrc[i];
 }
-__global__ __launch_bounds__(256) void k_motion_rows(MotionLaunch m) {
+__device__ __forceinline__ void motion_rows(const MotionLaunch& m) {
   if (m.gate && *m.gate == 0) return;
   __shared__ int hist[2 * MAX_INST];
   const bool count = m.label_bf != nullptr;
@@ -1497,6 +1497,16 @@ __global__ __launch_bounds__(256) void k_motion_rows(MotionLaunch m) {
     for (int j = threadIdx.x; j < 2 * MAX_INST; j += blockDim.x)
       if (hist[j]) atomicAdd(&m.counts[j], hist[j]);
   }
+}
+__global__ __launch_bounds__(256) void k_motion_rows(MotionLaunch m) { motion_rows(m); }
+// Several instances' rows in one launch (grid.z = instance): `base` with the
+// instance's own maps, pose and outputs.  Their outputs are disjoint, so this
+// equals one launch per instance.
+__global__ __launch_bounds__(256) void k_motion_rows_n(MotionLaunch base, const MotionObj* __restrict__ objs) {
+  MotionLaunch m = base;
+  const MotionObj o = objs[blockIdx.z];
+  m.mvn = o.mvn; m.vbf = o.vbf; m.T_w_l = o.T_w_l; m.codes = o.codes; m.rgb_err = o.rgb_err;
+  motion_rows(m);
 }
 
 // ------------------------------------------------------------ launchers
@@ -1630,6 +1640,9 @@ void launch_pose_from_table(hipStream_t s, Mat4* dst, const Mat4* table, const i
 }
 void launch_motion_rows(hipStream_t s, const MotionLaunch& m) {
   k_motion_rows<<<dim3((m.W + 255) / 256, m.H), 256, 0, s>>>(m);
+}
+void launch_motion_rows_n(hipStream_t s, const MotionLaunch& base, const MotionObj* objs, int n) {
+  if (n > 0) k_motion_rows_n<<<dim3((base.W + 255) / 256, base.H, n), 256, 0, s>>>(base, objs);
 }
 static unsigned presence_blocks(int n) { return (unsigned)std::min(256, (n + 1023) / 1024); }
 void launch_motion_gate(hipStream_t s, MotionDev* md, const int32_t* label, int n, int nobj) {
