@@ -320,6 +320,10 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
   __shared__ uint32_t st_par[MAX_LG + 1][RT];
   __shared__ float st_tmax[MAX_LG + 1][RT];
   __shared__ uint64_t tile_objs;   // bit oi: this tile's rays may meet object oi's volume
+  // fg of each volume at the hit point 0: every miss (hit 0) reads it
+  // (rendering.cpp:91-92), so it is looked up once per workgroup instead of
+  // once per ray and object
+  __shared__ float fg_miss[MAX_INST];
   if (threadIdx.x < 5) cnt[threadIdx.x] = 0;
   if (threadIdx.x < 64) {
     const int oi = threadIdx.x;
@@ -327,6 +331,10 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
                        tile_meets_cube(*objs[oi].prm, objs[oi].v.size, blockIdx.x * RT_TILE, blockIdx.y * RT_TILE, RT_TILE);
     const uint64_t b = __ballot(meets);
     if (oi == 0) tile_objs = b;
+  } else if (threadIdx.x < 64 + nobj) {
+    const int oi = threadIdx.x - 64;
+    VCache c;
+    fg_miss[oi] = interp<true>(objs[oi].v, c, mk3(0.f, 0.f, 0.f));
   }
   __syncthreads();
   // 16x16 pixel tile per workgroup, 8x8 per wave: neighbouring rays visit
@@ -353,29 +361,31 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
     const RayObj& O = objs[oi];
     if (!all && !in_view[O.label]) continue;
     const VolDev v = O.v;
-    const ObjParams& P = *O.prm;
-    const float step = v.size / (float)v.res;
-    const float largestep = step * 8;
-    const f3 wo = translation(P.view);
-    const f3 oo = affine(P.inv_pose, wo);
-    const f3 wd = rotate(P.view, mk3((float)x, (float)y, 1.f));
-    const f3 od = normalize(rotate(P.inv_pose, wd));
-    const float nfar = fmin_r(nearest, farp);
-    // rays of a tile outside the object's projected volume: the miss outcome
-    // (no hit, hit point 0) without the traversal
-    const float tmin = ((tile_objs >> oi) & 1)
-                           ? ray_first_block<STATS>(v, oo, od, nearp, nfar, eps, &st_par[0][threadIdx.x],
-                                                    &st_tmax[0][threadIdx.x], nnodes)
-                           : -1.f;
     f3 hit = mk3(0.f, 0.f, 0.f);
     float hw = 0.f;
     if (STATS) npairs++;
-    if (tmin > 0.f) hw = march<STATS>(v, oo, od, tmin * v.size / (float)v.res, nfar, mu, step, largestep, hit, nsteps, ninterp);
+    // rays of a tile outside the object's projected volume: the miss outcome
+    // (no hit, hit point 0) without the traversal
+    if ((tile_objs >> oi) & 1) {
+      const ObjParams& P = *O.prm;
+      const float step = v.size / (float)v.res;
+      const float largestep = step * 8;
+      const f3 wo = translation(P.view);
+      const f3 oo = affine(P.inv_pose, wo);
+      const f3 wd = rotate(P.view, mk3((float)x, (float)y, 1.f));
+      const f3 od = normalize(rotate(P.inv_pose, wd));
+      const float nfar = fmin_r(nearest, farp);
+      const float tmin = ray_first_block<STATS>(v, oo, od, nearp, nfar, eps, &st_par[0][threadIdx.x],
+                                                &st_tmax[0][threadIdx.x], nnodes);
+      if (tmin > 0.f)
+        hw = march<STATS>(v, oo, od, tmin * v.size / (float)v.res, nfar, mu, step, largestep, hit, nsteps, ninterp);
+    }
     const float hd = (O.label == 0) ? hw + 0.05f : hw;
     if (hd > nearest) continue;
     const float inv_vs = (float)v.res / v.size;
     VCache c;
-    const float fg = interp<true>(v, c, hit * inv_vs);
+    // a miss leaves hit 0 (a hit has t >= the near plane > 0)
+    const float fg = hw == 0.f ? fg_miss[oi] : interp<true>(v, c, hit * inv_vs);
     if (STATS) ninterp++;
     if (fg <= 0.5f) continue;
     bool take = false;
