@@ -323,10 +323,10 @@ TrackLaunch base_launch(mf_ctx* c, const float k[4], int level, int object) {
   return p;
 }
 
-// Camera ICP + photometric rows run by pixel (both rows of a pixel on one
-// thread) when the level's rows per thread is even.
+// ICP + photometric rows run by pixel (both rows of a pixel on one thread),
+// camera and objects, when the level's rows per thread is even.
 void set_shape(TrackLaunch& p, int rows_per_thread) {
-  p.paired = (!p.object && p.use_icp && p.use_rgb && rows_per_thread % 2 == 0) ? 1 : 0;
+  p.paired = (p.use_icp && p.use_rgb && rows_per_thread % 2 == 0) ? 1 : 0;
 }
 
 // Rendered reference depth pyramids (tracking.cpp:218-238): vertex2depth at

@@ -238,12 +238,9 @@ __device__ __forceinline__ int obj_icp_core(const TrackLaunch& p, const TrackIns
 // normal test.  Most of an object's rows cover pixels far from it and end at
 // the projection or at the taps.
 template <class MT>
-__device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst& in, const MT& M, int x, int y,
-                                           size_t i, float& e, float J[6]) {
+__device__ __forceinline__ int obj_icp_rest(const TrackLaunch& p, const TrackInst& in, const MT& M, size_t i,
+                                            const f3& c2v, float nlx, float& e, float J[6]) {
   const float* nb = p.live + (size_t)PL_N * p.lstride;
-  const f3 c2v = lv3(p, PL_V, i);
-  const float nlx = ldx(nb, (uint32_t)i);
-  hold(c2v); hold(nlx);
   if (nlx == INVALID) return -1;
   if (c2v.x == 0 && c2v.y == 0 && c2v.z == 0) return -1;
   const f3 o2v = affine(M.invT, c2v);
@@ -278,6 +275,14 @@ __device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst
   const f3 j1 = w * cross(c2v, jt);
   J[0] = j0.x; J[1] = j0.y; J[2] = j0.z; J[3] = j1.x; J[4] = j1.y; J[5] = j1.z;
   return 1;
+}
+template <class MT>
+__device__ __forceinline__ int obj_icp_row(const TrackLaunch& p, const TrackInst& in, const MT& M, int x, int y,
+                                           size_t i, float& e, float J[6]) {
+  const f3 c2v = lv3(p, PL_V, i);
+  const float nlx = ldx(p.live + (size_t)PL_N * p.lstride, (uint32_t)i);
+  hold(c2v); hold(nlx);
+  return obj_icp_rest(p, in, M, i, c2v, nlx, e, J);
 }
 
 // Photometric rows: trackRGB (tracking.cpp:1023-1111) for the camera,
@@ -611,6 +616,59 @@ __device__ __forceinline__ void camera_pixels(const TrackLaunch& p, const TrackI
         if (in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
         if (dbg) { TrackRow& d = dbg[N0 + ro]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
       }
+    }
+  }
+  wg_reduce(s, part, red);
+}
+
+// Object rows by pixel: each thread takes whole pixels and computes the
+// object's ICP row (track_Obj_ICP_Kernel) and photometric row
+// (track_obj_RGB_kernel) of each, issuing both rows' leading loads (live
+// vertex and normal x, rendered depth) in one batch, so the pixel costs one
+// memory round trip where two rows on separate threads cost one each.  Most
+// of an object's rows end after that batch (away from the object).  Pixels
+// i = wg * TB + tid + k * nactive * TB; ICP row then photometric row of each
+// pixel into the thread's FP32 sums.
+template <int RB, class MT>
+__device__ __forceinline__ void object_pixels(const TrackLaunch& p, const TrackInst& inst, const MT& M, int state,
+                                              int wg, int nactive, float* __restrict__ part, float (*red)[32]) {
+  const int tid = threadIdx.x;
+  const TrackInst in = inst;
+  float s[32];
+#pragma unroll
+  for (int k = 0; k < 32; k++) s[k] = 0.f;
+  const int n = p.w * p.h;
+  const size_t N0 = (size_t)p.W * p.H;
+  TrackRow* dbg = (p.dbg && state < 2) ? p.dbg + ((size_t)state * p.n_levels + p.level) * 2 * N0 : nullptr;
+  const float* nb = p.live + (size_t)PL_N * p.lstride;
+  for (int i = wg * TB + tid; i < n; i += nactive * TB) {
+    asm volatile("" ::: "memory");   // keep the LDS matrix reads inside the loop
+    const int x = i % p.w, y = i / p.w;
+    const size_t ro = (size_t)x + (size_t)y * p.W;
+    const f3 c2v = lv3(p, PL_V, (size_t)i);
+    const float nlx = ldx(nb, (uint32_t)i);
+    const float dr = ldx(in.rD + p.rd_off, (uint32_t)i);
+    hold(c2v); hold(nlx); hold(dr);
+    {
+      float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
+      const int res = obj_icp_rest(p, in, M, (size_t)i, c2v, nlx, e, J);
+      accum<RB>(s, res, e, J, p.robust, p.robust_k);
+      if (in.codes && p.level == 0) in.codes[ro] = (int8_t)res;
+      if (dbg) { TrackRow& d = dbg[ro]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
+    }
+    {
+      float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
+      int res = -1;
+      if (dr > 0.0f) {
+        const f3 vl = rf3(p, PL_V, (size_t)i);
+        const float ir = ldx(p.ref + (size_t)PL_I * p.lstride, (uint32_t)i);
+        hold(vl); hold(ir);
+        res = rgb_core<true>(p, M, x, y, dr, vl, ir, e, J);
+      }
+      accum<RB>(s, res, e, J, p.robust, p.robust_k);
+      if (in.codes && p.level == 0) in.codes[N0 + ro] = (int8_t)res;
+      if (in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
+      if (dbg) { TrackRow& d = dbg[N0 + ro]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
     }
   }
   wg_reduce(s, part, red);
@@ -1056,6 +1114,9 @@ __global__ __launch_bounds__(TB) void k_track(TrackLaunch p, const TrackInst* __
   // object rows cluster where the object is: they keep the round-robin
   // spread over the XCDs, which balances them better than L2 locality pays
   const int c = OBJ ? (int)blockIdx.x : xcd_chunk(blockIdx.x, nblk);
+  if constexpr (ICP && RGB && OBJ) {
+    if (p.paired) { object_pixels<RB>(p, insts[j], Msh, state, c, nblk, partial + ((size_t)j * nblk + c) * 32, red); return; }
+  }
   track_rows<ICP, RGB, OBJ, RB>(p, insts[j], Msh, state, c, nblk, partial + ((size_t)j * nblk + c) * 32, red);
 }
 
@@ -1241,6 +1302,9 @@ __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* 
   float* part = part_cur + ((size_t)j * nblk + blockIdx.x) * 32;
   if constexpr (ICP && RGB && !OBJ) {
     if (p.paired) { camera_pixels<RB, 1>(p, insts[j], Msh, state, blockIdx.x, nblk, part, red); return; }
+  }
+  if constexpr (ICP && RGB && OBJ) {
+    if (p.paired) { object_pixels<RB>(p, insts[j], Msh, state, blockIdx.x, nblk, part, red); return; }
   }
   track_rows<ICP, RGB, OBJ, RB>(p, insts[j], Msh, state, blockIdx.x, nblk, part, red);
 }
