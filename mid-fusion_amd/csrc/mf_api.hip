@@ -296,11 +296,9 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   // mapping work lists cover every object's pool (grown with each object;
   // the previous lists stay allocated until the context is destroyed)
   const long work_cap = c->work_cap + cap;
-  int2* new_list = dalloc<int2>(c, work_cap);
   int2* act_list = dalloc<int2>(c, work_cap);
-  if (!new_list || !act_list) return rollback(MF_E_OOM, "device allocation failed");
+  if (!act_list) return rollback(MF_E_OOM, "device allocation failed");
   c->work_cap = work_cap;
-  c->work.new_list = new_list;
   c->work.act_list = act_list;
   c->objs.push_back(o);
   return o->id;
@@ -603,10 +601,8 @@ mf_ctx* mf_create(const mf_config* cfg, const float init_pose[16], float bg_size
   c->gauss = dalloc<float>(c, 5);
   c->stats = dalloc<unsigned long long>(c, (size_t)N_STATS * STAT_SLOTS);
   c->obj_ids = dalloc<int>(c, MAX_INST);
-  c->work.pool_pre = dalloc<long>(c, MAX_INST + 1);
-  c->work.new_count = dalloc<int>(c, 1);
   c->work.act_count = dalloc<int>(c, 1);
-  ok = ok && c->work.pool_pre && c->work.new_count && c->work.act_count;
+  ok = ok && c->work.act_count;
   ok = ok && c->vbf[2] && c->label_bf && c->motion_codes && c->md && c->vcam && c->rayobjs_bf && c->rayobjs_lab &&
        c->motion_tab && c->ray_pose && c->view_dev && c->palette_dev;
   ok = ok && c->stats && c->obj_ids && c->depth && c->grey && c->rgb[2] && c->vtx[2] && c->nrm[2] && c->label && c->mats && c->gn &&

@@ -200,14 +200,12 @@ struct MapObj {
 // Per-launch work lists of the mapping kernels.
 struct MapWork {
   long table_pre[MAX_INST + 1];   // prefix of the objects' Morton-table sizes (host computed)
-  long* pool_pre;                 // [MAX_INST + 1] prefix of allocated-block counts (device)
-  int* new_count;                 // new blocks this frame
-  int2* new_list;                 // (object, slot) of each new block
   int* act_count;                 // active blocks this frame
   int2* act_list;                 // (object, slot) of each active block
 };
-// flags: OBJ_ALLOC_RESET starts an allocation pass (new-block count and the
-// volumes' frame-start counters), OBJ_TBOX_RESET empties the objects' model-map
+// flags: OBJ_ALLOC_RESET starts a mapping pass (the active-list count, and the
+// volumes' frame-start block counts that mark this pass's new blocks for
+// k_integrate to initialise), OBJ_TBOX_RESET empties the objects' model-map
 // tile boxes ahead of the raycast that fills them.
 enum { OBJ_ALLOC_RESET = 1, OBJ_TBOX_RESET = 2 };
 void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx,

@@ -42,9 +42,9 @@ __device__ void obj_params(const Mat4& Twc, const Mat4& Two, float fx, float fy,
 // Matrices of every object of a stage (one thread each), and the start of
 // this frame's allocation bookkeeping when `alloc_reset` is set.
 __global__ void k_obj_params(const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx, float fy,
-                             float cx, float cy, int flags, int* new_count) {
+                             float cx, float cy, int flags, int* act_count) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if ((flags & OBJ_ALLOC_RESET) && j == 0) *new_count = 0;
+  if ((flags & OBJ_ALLOC_RESET) && j == 0) *act_count = 0;
   if (j >= n) return;
   const MapObj& o = objs[j];
   obj_params(*T_w_c, mats[3 + o.id], fx, fy, cx, cy, o.prm);
@@ -138,13 +138,11 @@ __global__ void k_alloc_commit(const MapObj* __restrict__ objs, int n, MapWork w
       if (s >= v.cap) { v.counters[1] = 1; atomicSub(&v.counters[0], 1); }
       else fresh = true;
     }
-    const int slot = wave_append(wl.new_count, fresh);
     if (fresh) {
       v.slot[m] = s;
       v.blk_morton[s] = (int32_t)m;
       v.blk_active[s] = 1;
       for (int l = v.leaf_level - 1; l >= 1; l--) v.nodes[v.node_off[l] + (m >> (3 * (v.leaf_level - l)))] = 1;
-      wl.new_list[slot] = make_int2(j, s);
     }
   }
 }
@@ -176,36 +174,22 @@ void launch_export_voxels(hipStream_t st, const VolDev& v, const uint64_t* keys,
   if (n > 0) k_export_voxels<<<n < 4096 ? n : 4096, BLOCK_VOX, 0, st>>>(v, keys, n, out, found);
 }
 
-// Voxel initialisation of this frame's new blocks (voxel_traits<SDF>::initValue,
-// continuous/volume_traits.hpp:38), one 512-thread workgroup per block.
-__global__ __launch_bounds__(512) void k_init_new(const MapObj* __restrict__ objs, MapWork wl) {
-  const int n_new = *wl.new_count;
-  for (int a = blockIdx.x; a < n_new; a += gridDim.x) {
-    const int2 e = wl.new_list[a];
-    const VolDev& v = objs[e.x].v;
-    const size_t i = (size_t)e.y * BLOCK_VOX + threadIdx.x;
-    v.tsdf[i] = 1.f; v.wt[i] = 0.f; v.r[i] = 0.f; v.g[i] = 0.f; v.b[i] = 0.f; v.c[i] = 0; v.fg[i] = 0.0; v.fgw[i] = 0;
-  }
-}
-
 __device__ __forceinline__ void block_coords(int32_t m, int& x, int& y, int& z) {
   x = (int)compact21((uint64_t)m) * 8; y = (int)compact21((uint64_t)m >> 1) * 8; z = (int)compact21((uint64_t)m >> 2) * 8;
 }
 
 // projective_functor::build_active_list: active || in_frustum(block corner),
-// over the concatenated block pools of the launch's objects (prefix of their
-// allocated counts in `wl.pool_pre`, filled by k_pool_prefix); entries
-// (object, slot) of one list for the whole launch.
-__global__ void k_pool_prefix(const MapObj* __restrict__ objs, int n, MapWork wl) {
-  if (threadIdx.x != 0) return;
-  long t = 0;
-  wl.pool_pre[0] = 0;
-  for (int j = 0; j < n; j++) { t += min(objs[j].v.counters[0], (int)objs[j].v.cap); wl.pool_pre[j + 1] = t; }
-  *wl.act_count = 0;
-}
+// over the concatenated block pools of the launch's objects (the prefix of
+// their allocated counts, computed by every workgroup); entries (object,
+// slot) of one list for the whole launch, counted in wl.act_count (zeroed by
+// k_obj_params at the start of the mapping pass).
 __global__ void k_active_list(const MapObj* __restrict__ objs, int n, int w, int h, MapWork wl) {
   __shared__ long pre[MAX_INST + 1];
-  if ((int)threadIdx.x <= n) pre[threadIdx.x] = wl.pool_pre[threadIdx.x];
+  if (threadIdx.x == 0) {
+    long t = 0;
+    pre[0] = 0;
+    for (int j = 0; j < n; j++) { t += min(objs[j].v.counters[0], (int)objs[j].v.cap); pre[j + 1] = t; }
+  }
   __syncthreads();
   const long total = pre[n];
   const long stride = (long)gridDim.x * blockDim.x;
@@ -259,6 +243,14 @@ __global__ __launch_bounds__(512) void k_integrate(const MapObj* __restrict__ ob
     const f3 delta = rotate(Tco, mk3(vs, 0.f, 0.f));
     const f3 cdelta = rotate(K, delta);
     const int s = e.y;
+    if (s >= v.counters[3]) {
+      // a block allocated this frame (slot at or past the frame-start count,
+      // k_obj_params): voxel_traits<SDF>::initValue (continuous/
+      // volume_traits.hpp:38) before its first update; every new block is
+      // active, so it is on this list
+      const size_t i = (size_t)s * BLOCK_VOX + tid;
+      v.tsdf[i] = 1.f; v.wt[i] = 0.f; v.r[i] = 0.f; v.g[i] = 0.f; v.b[i] = 0.f; v.c[i] = 0; v.fg[i] = 0.0; v.fgw[i] = 0;
+    }
     int bx, by, bz;
     block_coords(v.blk_morton[s], bx, by, bz);
     const int y = by + ly, z = bz + lz;
@@ -316,7 +308,7 @@ __global__ __launch_bounds__(512) void k_integrate(const MapObj* __restrict__ ob
 // launch; a one-object-per-launch caller passes n = 1 and a table offset.
 void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx,
                        float fy, float cx, float cy, int flags, const MapWork& wl) {
-  k_obj_params<<<(n + 63) / 64, 64, 0, s>>>(T_w_c, mats, objs, n, fx, fy, cx, cy, flags, wl.new_count);
+  k_obj_params<<<(n + 63) / 64, 64, 0, s>>>(T_w_c, mats, objs, n, fx, fy, cx, cy, flags, wl.act_count);
 }
 void launch_alloc(hipStream_t s, const MapObj* objs, int n, const float* depth, int refine, int use_icp,
                   int use_rgb, int w, int h, float band, unsigned long long* stats) {
@@ -327,7 +319,6 @@ void launch_alloc_commit(hipStream_t s, const MapObj* objs, int n, const MapWork
   long g = (wl.table_pre[n] + 255) / 256;
   if (g > 4096) g = 4096;
   k_alloc_commit<<<(unsigned)g, 256, 0, s>>>(objs, n, wl);
-  k_init_new<<<1024, 512, 0, s>>>(objs, wl);
 }
 void launch_integrate(hipStream_t s, const MapObj* objs, int n, long sum_cap, float fx, float fy, float cx,
                       float cy, const float* depth, const float* r, const float* g, const float* b, int refine,
@@ -335,7 +326,6 @@ void launch_integrate(hipStream_t s, const MapObj* objs, int n, long sum_cap, fl
                       unsigned long long* stats) {
   long ga = (sum_cap + 255) / 256;
   if (ga > 2048) ga = 2048;
-  k_pool_prefix<<<1, 64, 0, s>>>(objs, n, wl);
   k_active_list<<<(unsigned)ga, 256, 0, s>>>(objs, n, w, h, wl);
   k_integrate<<<4096, 512, 0, s>>>(objs, fx, fy, cx, cy, depth, r, g, b, refine, use_icp, use_rgb, w, h, mu,
                                    maxweight, wl, stats);
