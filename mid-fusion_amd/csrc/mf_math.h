@@ -117,6 +117,17 @@ MF_HD uint64_t compact21(uint64_t v) {
   return x;
 }
 MF_HD uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) { return expand21(x) | (expand21(y) << 1) | (expand21(z) << 2); }
+// expand21 for v < 1024 in 32-bit arithmetic (block coordinates of volumes
+// up to 8192^3: the same bits, interleaved into a 30-bit Morton index)
+MF_HD uint32_t expand10(uint32_t v) {
+  uint32_t x = v & 0x3ffu;
+  x = (x | x << 16) & 0x030000ffu;
+  x = (x | x << 8) & 0x0300f00fu;
+  x = (x | x << 4) & 0x030c30c3u;
+  x = (x | x << 2) & 0x09249249u;
+  return x;
+}
+MF_HD uint32_t morton3_10(uint32_t x, uint32_t y, uint32_t z) { return expand10(x) | (expand10(y) << 1) | (expand10(z) << 2); }
 
 // Per-pixel TrackData (commons.h:624-628)
 struct TrackRow { int result; float error; float J[6]; };

@@ -16,7 +16,7 @@ namespace mf {
 
 __device__ __forceinline__ int32_t vslot(const VolDev& v, int x, int y, int z) {
   if (x < 0 || y < 0 || z < 0 || x >= v.res || y >= v.res || z >= v.res) return -1;
-  return gload(v.slot + morton3((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3));
+  return gload(v.slot + morton3_10((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3));
 }
 
 // One-entry block cache: consecutive voxel reads of a ray (march steps, the
@@ -33,7 +33,7 @@ __device__ __forceinline__ int32_t cslot(const VolDev& v, VCache& c, int x, int 
   const int key = ((x >> 3) << 20) | ((y >> 3) << 10) | (z >> 3);
   if (key != c.key) {
     c.key = key;
-    c.slot = gload(v.slot + morton3((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3));
+    c.slot = gload(v.slot + morton3_10((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3));
   }
   return c.slot;
 }
@@ -73,9 +73,29 @@ __device__ __forceinline__ float interp(const VolDev& v, VCache& c, f3 pos) {
 #pragma unroll
     for (int i = 0; i < 8; i++) p[i] = field_at<FG>(v, sl, lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1));
   } else {
+    // the corners span 2 blocks along some axis: each corner's block-table
+    // index from the two spread block coordinates per axis, and the 8 table
+    // loads issued together (an out-of-range corner, only at the volume's
+    // far faces, reads a clamped entry and takes -1 instead)
+    const int nb = v.res >> 3;
+    uint32_t e[3][2];
+    bool in[3][2];
+    const int L[3] = {lx, ly, lz};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int d = 0; d < 2; d++) {
+        const int c = L[a] + d;
+        in[a][d] = c < v.res;
+        e[a][d] = expand10((uint32_t)min(c >> 3, nb - 1)) << a;
+      }
     int32_t sl[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) sl[i] = vslot(v, lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1));
+    for (int i = 0; i < 8; i++) {
+      const int dx = i & 1, dy = (i >> 1) & 1, dz = (i >> 2) & 1;
+      const int32_t s = gload(v.slot + (e[0][dx] | e[1][dy] | e[2][dz]));
+      sl[i] = (in[0][dx] && in[1][dy] && in[2][dz]) ? s : -1;
+    }
 #pragma unroll
     for (int i = 0; i < 8; i++) p[i] = field_at<FG>(v, sl[i], lx + (i & 1), ly + ((i >> 1) & 1), lz + ((i >> 2) & 1));
   }
@@ -208,13 +228,14 @@ __device__ float ray_first_block(const VolDev& v, f3 o, f3 d_in, float nearp, fl
       scale--;
       scale_exp2 = half;
       idx = (t_center[0] > t_min) | ((t_center[1] > t_min) << 1) | ((t_center[2] > t_min) << 2);
-      for (int a = 0; a < 3; a++) pos[a] += scale_exp2 * (float)(bool)(idx & (1 << a));
+      // pos + scale_exp2 * bit, as a select: pos + 0 is pos (pos in [1, 2))
+      for (int a = 0; a < 3; a++) pos[a] = (idx & (1 << a)) ? pos[a] + scale_exp2 : pos[a];
       t_max = tv_max;
       continue;
     }
     // advance
     const int step_mask = (t_corner[0] <= tc_max) | ((t_corner[1] <= tc_max) << 1) | ((t_corner[2] <= tc_max) << 2);
-    for (int a = 0; a < 3; a++) pos[a] -= scale_exp2 * (float)(bool)(step_mask & (1 << a));
+    for (int a = 0; a < 3; a++) pos[a] = (step_mask & (1 << a)) ? pos[a] - scale_exp2 : pos[a];
     t_min = tc_max;
     idx ^= step_mask;
     if ((idx & step_mask) != 0) {
