@@ -382,6 +382,19 @@ __device__ __forceinline__ void accum(float* s, int res, float e, const float* J
 __device__ __forceinline__ void wg_reduce(float (&s)[32], float* __restrict__ part, float (*red)[32]) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
+  if (__ballot(s[28] != 0.f) == 0) {
+    // No lane of this wave has a valid row (sum 28 counts them), so sums
+    // 0-28 are +0 in every lane and the butterfly would give +0; only the
+    // status counts 29-31 are reduced.  They are small integers, exact in
+    // FP32 in any order, so the wave's column of `red` is the same.  (Most
+    // of an object's rows lie away from the object and end invalid.)
+    float c0 = s[29], c1 = s[30], c2 = s[31];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      c0 += __shfl_xor(c0, o); c1 += __shfl_xor(c1, o); c2 += __shfl_xor(c2, o);
+    }
+    if (lane < 32) red[tid >> 6][lane] = lane == 29 ? c0 : (lane == 30 ? c1 : (lane == 31 ? c2 : 0.f));
+  } else {
 #define MF_BFLY(HALF, OFF)                                                   \
   {                                                                          \
     const bool hi = (lane & OFF) != 0;                                       \
@@ -397,8 +410,9 @@ __device__ __forceinline__ void wg_reduce(float (&s)[32], float* __restrict__ pa
   MF_BFLY(2, 4)
   MF_BFLY(1, 2)
 #undef MF_BFLY
-  const float v = s[0] + __shfl_xor(s[0], 1);
-  if ((lane & 1) == 0) red[tid >> 6][lane >> 1] = v;
+    const float v = s[0] + __shfl_xor(s[0], 1);
+    if ((lane & 1) == 0) red[tid >> 6][lane >> 1] = v;
+  }
   __syncthreads();
   if (tid < 32) {
     float a = 0.f;
@@ -641,13 +655,24 @@ __device__ __forceinline__ void object_pixels(const TrackLaunch& p, const TrackI
   const size_t N0 = (size_t)p.W * p.H;
   TrackRow* dbg = (p.dbg && state < 2) ? p.dbg + ((size_t)state * p.n_levels + p.level) * 2 * N0 : nullptr;
   const float* nb = p.live + (size_t)PL_N * p.lstride;
+  // The rendered depth is vertex2depth of the object's model vertex map, 0
+  // at every raycast miss, and each coarser level's pixel is a robust
+  // half-sample of its 2x2 finer pixels (0 where all four are 0).  So it is
+  // 0 at pixels whose level-0 footprint lies outside the box of model-map
+  // tiles with a surface (M.tb, in MAP_TILE tiles; empty: x0 > x1): their
+  // photometric row ends at the depth test, and the load is skipped.
+  const bool boxed = in.tbox != nullptr;
+  const int l = p.level;
   for (int i = wg * TB + tid; i < n; i += nactive * TB) {
     asm volatile("" ::: "memory");   // keep the LDS matrix reads inside the loop
     const int x = i % p.w, y = i / p.w;
     const size_t ro = (size_t)x + (size_t)y * p.W;
+    const int fx0 = x << l, fy0 = y << l, fx1 = fx0 + (1 << l) - 1, fy1 = fy0 + (1 << l) - 1;
+    const bool rgb_out = boxed && (fx1 / MAP_TILE < M.tb[0] || fx0 / MAP_TILE > M.tb[2] ||
+                                   fy1 / MAP_TILE < M.tb[1] || fy0 / MAP_TILE > M.tb[3]);
     const f3 c2v = lv3(p, PL_V, (size_t)i);
     const float nlx = ldx(nb, (uint32_t)i);
-    const float dr = ldx(in.rD + p.rd_off, (uint32_t)i);
+    const float dr = rgb_out ? 0.0f : ldx(in.rD + p.rd_off, (uint32_t)i);
     hold(c2v); hold(nlx); hold(dr);
     {
       float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
