@@ -204,7 +204,8 @@ void mf_clear_trace(mf_ctx* ctx);
 int mf_debug_rows(mf_ctx* ctx, int on);
 int mf_get_rows(mf_ctx* ctx, int state, int level, int32_t* result, float* error, float* J);
 /* Pyramid buffers of the live frame: which = 0 depth, 1 grey, 2 vertex (3/px),
- * 3 normal (3/px), 4 grad x, 5 grad y, 6 icp covariance (3/px). */
+ * 3 normal (3/px), 4 grad x, 5 grad y, 6 icp covariance (3/px), 7 the
+ * camera's rendered reference depth (vertex2depth pyramid, r_D_render). */
 long mf_get_level(mf_ctx* ctx, int level, int which, float* out);
 /* Preprocessed frame: depth (m), grey, rgb (3/px). */
 int mf_get_frame(mf_ctx* ctx, float* depth, float* grey, float* rgb);
@@ -269,7 +270,8 @@ long mf_device_bytes(mf_ctx* ctx);
  *   once for every object, with 1 or 2 pixels per thread; 0 (default) gives
  *   each object its own row grid in the same launch. */
 /*   MF_TUNE_OBJ_ROWS_PER_THREAD_L0 + l: rows per thread of the moving
- *   objects' row grids at level l, 1..64, or 0 (default) for the camera's. */
+ *   objects' row grids at level l, 1..64, or 0 (default): {8, 4, 4, 4} when
+ *   two or more objects share the launch, else the camera's. */
 enum { MF_TUNE_TRACK_ROWS_PER_THREAD_L0 = 0, MF_TUNE_TRACK_FUSED_L0 = 8, MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 = 16,
        MF_TUNE_OBJ_ROWS_PER_THREAD_L0 = 24 };
 int mf_set_tuning(mf_ctx* ctx, int knob, int value);

@@ -81,7 +81,8 @@ struct mf_ctx {
   // thread (1, 2), or 0: one row grid per object (grid.y), per pyramid level.
   // 0 everywhere measured fastest on MI355X at 1, 8 and 16 objects.
   int track_obj_ppt[MAX_LEVELS] = {0, 0, 0, 0};
-  // rows per thread of the objects' row grids per pyramid level (0: the camera's)
+  // rows per thread of the objects' row grids per pyramid level (0: automatic,
+  // obj_rows_per_thread)
   int track_obj_rpt[MAX_LEVELS] = {0, 0, 0, 0};
   uint8_t* moving = nullptr;     // [MAX_INST] label-indexed flags
   TraceRec* trace = nullptr;
@@ -337,6 +338,21 @@ void render_reference(mf_ctx* c, int first, int n) {
   launch_render_reference(c->stream, c->inst + first, n, c->W, c->H, c->cfg.n_levels, c->mats + 2, off);
 }
 
+// Rows per thread of the objects' row grids at level l in a launch of n
+// objects: the knob, or when it is 0, {8, 4, 4, 4} for two or more objects
+// and the camera's for one.  Most of an object's rows lie away from it and
+// end at their first loads, so with several objects the launch fills the
+// chip with fewer, longer workgroups (fewer partials to reduce); one object
+// alone needs the camera's shape to fill it.  Measured on MI355X at 640x480:
+// 8 objects 532 -> 548 frames/s, 16 objects 427 -> 450, 1280x720 with 16
+// objects 215 -> 236; {8, 4, 4} for a single object lost 1 % (5 % in the
+// one-object-per-launch form at 8 objects).
+int obj_rows_per_thread(const mf_ctx* c, int l, int n) {
+  static constexpr int kBatched[MAX_LEVELS] = {8, 4, 4, 4};
+  if (c->track_obj_rpt[l] > 0) return c->track_obj_rpt[l];
+  return n >= 2 ? kBatched[l] : c->track_rpt[l];
+}
+
 // One coarse-to-fine GN run over `n` instances sharing a launch
 // (trackLiveFrame / trackEachObject loops, tracking.cpp:242-335, 624-690):
 // one fused track+reduce+finalize launch per (level, iteration).
@@ -366,7 +382,7 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
     if (mask_labels) p.moving = mask_labels;
     const bool fused = c->track_fused[l] != 0;
     const bool objs_grid = object && c->cfg.batch_objects && !fused && c->track_obj_ppt[l] > 0;
-    const int rpt = (object && c->track_obj_rpt[l] > 0) ? c->track_obj_rpt[l] : c->track_rpt[l];
+    const int rpt = object ? obj_rows_per_thread(c, l, n) : c->track_rpt[l];
     set_shape(p, rpt);
     const int nblk = objs_grid ? track_obj_blocks(p.w * p.h, c->track_obj_ppt[l]) : track_blocks(p.w * p.h * kinds, rpt);
     if (!fused) finish();
@@ -1630,6 +1646,7 @@ long mf_get_level(mf_ctx* c, int level, int which, float* out) {
     case 4: hipMemcpy(out, L.gx, n * 4, hipMemcpyDeviceToHost); break;
     case 5: hipMemcpy(out, L.gy, n * 4, hipMemcpyDeviceToHost); break;
     case 6: download_planes(c, L.cov, out, n); break;
+    case 7: hipMemcpy(out, c->objs[0]->rD[level], n * 4, hipMemcpyDeviceToHost); break;   // camera's rendered reference depth
     default: return fail(c, MF_E_ARG, "bad buffer");
   }
   return (long)n;

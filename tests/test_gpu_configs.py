@@ -47,13 +47,18 @@ def test_batched_objects_640x480(k):
 def test_batched_plain_and_per_object_calls_agree():
     """The batched launches, the one-object-per-launch form and per-object
     mf_track_object calls compute every object's rows and FP64 sums in the
-    same order, so poses, rows and keys are bit-identical across the three."""
+    same order at the same launch shape, so poses, rows and keys are
+    bit-identical across the three.  The objects' rows per thread are set
+    alike in the three contexts: by default a batch of objects takes another
+    shape than a single object, which changes the FP32 partial sums' order."""
     movers = scene.batch_objects(4, 4004)
     sc = scene.Scene(seed=4, width=W, height=H, K=tuple(K640), moving=movers)
     P0 = sc.camera_pose(0).astype(np.float32)
     ctx = []
     for batch in (True, False, True):
         g = midfusion.Kfusion(W, H, P0, bg_size=5.12, bg_res=512, use_rgb=True, batch_objects=batch)
+        for lvl, v in enumerate((8, 4, 4)):
+            g.set_tuning(g.TUNE_OBJ_ROWS_PER_THREAD_L0 + lvl, v)
         for i, m in enumerate(movers):
             g.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
             g.set_moving(i + 1, True)
