@@ -34,8 +34,8 @@ void launch_level(hipStream_t s, const Level& L, const Mat4& invK, int negY, flo
                   float baseline, bool grads, bool cov);
 void launch_vertex2depth(hipStream_t s, const float* const v[3], int n, const Mat4* invT, float* out);
 // Rendered reference depth pyramids (vertex2depth + robust half-sampling) of
-// `n` tracking instances in one launch per level; rd_off[l] is the offset of
-// level l in each instance's pyramid.
+// `n` tracking instances, every level in one launch; rd_off[l] is the offset
+// of level l in each instance's pyramid.
 void launch_render_reference(hipStream_t s, const TrackInst* insts, int n, int w, int h, int n_levels,
                              const Mat4* invT, const int* rd_off);
 

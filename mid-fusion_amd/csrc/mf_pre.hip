@@ -168,45 +168,76 @@ __global__ void k_vertex2depth(const float* __restrict__ vx, const float* __rest
   out[i] = (v.z > 0.0f) ? affine(M, v).z : 0.0f;
 }
 
-// Rendered reference depth pyramids of `gridDim.y` tracking instances in one
-// pass per level: level 0 is vertex2depth of the instance's model vertex map,
-// level l the robust half-sample of level l-1 (tracking.cpp:218-238).
-__global__ void k_render_ref0(const TrackInst* __restrict__ insts, int n0, const Mat4* __restrict__ invT) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n0) return;
-  const TrackInst& in = insts[blockIdx.y];
+// Rendered reference depth pyramids of `gridDim.z` tracking instances in one
+// launch (tracking.cpp:218-238): each workgroup renders a 32x16 tile of level
+// 0, vertex2depth of the instance's model vertex map at the reference pose
+// (preprocessing.cpp:292-309), and half-samples it level by level in LDS with
+// the robust rule (the 2x2 values within e_d of the top-left one, averaged).
+// Level l's tile is (32 >> l) x (16 >> l) pixels, and the 2x2 sources of
+// every level-l pixel (x < w_l = w_{l-1} / 2) lie inside the image and inside
+// the tile, so the levels come out as separate half-sampling passes over
+// the whole image would make them.  n_levels <= 4.
+static constexpr int RR_TX = 32, RR_TY = 16;
+struct RefOffsets { int off[MAX_LEVELS]; };
+__global__ __launch_bounds__(256) void k_render_ref(const TrackInst* __restrict__ insts, int W, int H, int n_levels,
+                                                   const Mat4* __restrict__ invT, RefOffsets ro, float e_d) {
+  __shared__ float tile[RR_TX * RR_TY + (RR_TX / 2) * (RR_TY / 2) + (RR_TX / 4) * (RR_TY / 4) + (RR_TX / 8) * (RR_TY / 8)];
+  const int tid = threadIdx.x;
+  const TrackInst& ti = insts[blockIdx.z];
+  float* rD = const_cast<float*>(ti.rD);
   const Mat4 M = *invT;
-  const f3 v = mk3(in.mvn[i], in.mvn[(size_t)n0 + i], in.mvn[2 * (size_t)n0 + i]);
-  const_cast<float*>(in.rD)[i] = (v.z > 0.0f) ? affine(M, v).z : 0.0f;
-}
-__global__ void k_render_half(const TrackInst* __restrict__ insts, int off_in, int off_out, int iw, int ih, float e_d) {
-  const int ow = iw / 2, oh = ih / 2;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.z;
-  if (x >= ow || y >= oh) return;
-  const TrackInst& ti = insts[blockIdx.y];
-  const float* in = ti.rD + off_in;
-  float* out = const_cast<float*>(ti.rD) + off_out;
-  const int cx = 2 * x, cy = 2 * y;
-  const float c = in[cx + (size_t)cy * iw];
-  float sum = 0.0f, t = 0.0f;
-  for (int i = 0; i <= 1; ++i)
-    for (int j = 0; j <= 1; ++j) {
-      const int ux = min(cx + j, 2 * ow - 1), uy = min(cy + i, 2 * oh - 1);
-      const float cur = in[ux + (size_t)uy * iw];
-      if (fabsf(cur - c) < e_d) { sum = sum + 1.0f; t = t + cur; }
+  const int X0 = blockIdx.x * RR_TX, Y0 = blockIdx.y * RR_TY;
+  const size_t n0 = (size_t)W * H;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int lx = tid & (RR_TX - 1), ly = (tid >> 5) + (RR_TY / 2) * k;
+    const int x = X0 + lx, y = Y0 + ly;
+    float d = 0.0f;
+    if (x < W && y < H) {
+      const size_t i = (size_t)x + (size_t)y * W;
+      const f3 v = mk3(ti.mvn[i], ti.mvn[n0 + i], ti.mvn[2 * n0 + i]);
+      d = (v.z > 0.0f) ? affine(M, v).z : 0.0f;
+      rD[i] = d;
     }
-  out[x + (size_t)y * ow] = t / sum;
+    tile[ly * RR_TX + lx] = d;
+  }
+  __syncthreads();
+  int iw = W, ih = H, src = 0, dst = RR_TX * RR_TY;
+  for (int l = 1; l < n_levels; l++) {
+    const int ow = iw / 2, oh = ih / 2;
+    const int sw = RR_TX >> (l - 1), tw = RR_TX >> l, th = RR_TY >> l;
+    if (tid < tw * th) {
+      const int lx = tid % tw, ly = tid / tw;
+      const int x = (X0 >> l) + lx, y = (Y0 >> l) + ly;
+      if (x < ow && y < oh) {
+        const float* in = tile + src + (2 * ly) * sw + 2 * lx;
+        const float c = in[0];
+        float sum = 0.0f, t = 0.0f;
+        for (int i = 0; i <= 1; ++i)
+          for (int j = 0; j <= 1; ++j) {
+            const float cur = in[i * sw + j];
+            if (fabsf(cur - c) < e_d) { sum = sum + 1.0f; t = t + cur; }
+          }
+        const float r = t / sum;
+        rD[ro.off[l] + x + (size_t)y * ow] = r;
+        tile[dst + ly * tw + lx] = r;
+      }
+    }
+    __syncthreads();
+    iw = ow; ih = oh;
+    src = dst;
+    dst += tw * th;
+  }
 }
 
 // ------------------------------------------------------------ launchers
 void launch_render_reference(hipStream_t s, const TrackInst* insts, int n, int w, int h, int n_levels,
                              const Mat4* invT, const int* rd_off) {
-  const int n0 = w * h;
-  k_render_ref0<<<dim3((n0 + 255) / 256, n), 256, 0, s>>>(insts, n0, invT);
-  for (int l = 1; l < n_levels; l++) {
-    const int iw = w >> (l - 1), ih = h >> (l - 1);
-    k_render_half<<<dim3((iw / 2 + 255) / 256, n, ih / 2), 256, 0, s>>>(insts, rd_off[l - 1], rd_off[l], iw, ih, 0.1f * 3);
-  }
+  RefOffsets ro;
+  for (int l = 0; l < MAX_LEVELS; l++) ro.off[l] = rd_off[l];
+  const int nl = n_levels < MAX_LEVELS ? n_levels : MAX_LEVELS;
+  k_render_ref<<<dim3((w + RR_TX - 1) / RR_TX, (h + RR_TY - 1) / RR_TY, n), 256, 0, s>>>(insts, w, h, nl, invT, ro,
+                                                                                         0.1f * 3);
 }
 void launch_convert(hipStream_t s, const uint16_t* din, const uint8_t* rgb, int iw, int ratio, int ow, int oh,
                     float* depth, float* grey, float* r, float* g, float* b) {

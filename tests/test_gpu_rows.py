@@ -63,6 +63,13 @@ def test_camera_rows(level, use_rgb):
         g.sync()
         if f == 1:
             compare_rows(g.rows(0, level), o.rows(level), 'camera L%d rgb=%d' % (level, use_rgb))
+            if use_rgb:
+                # T1: the rendered reference depth pyramid the photometric
+                # rows read, every level bit-exact
+                for lv in range(3):
+                    rg, ro = g.level(lv, 7), o.level(lv, 7)
+                    assert np.array_equal(rg, ro), (lv, int(np.count_nonzero(rg != ro)))
+                    assert np.count_nonzero(rg) > 100
         g.setPose(-1, o.pose().astype(np.float32))
         g.integration(K320, f)
         o.integrate(K320, f)
