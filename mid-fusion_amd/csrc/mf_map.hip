@@ -87,17 +87,53 @@ __global__ void k_alloc(const MapObj* __restrict__ objs, const float* __restrict
   const f3 origin = ov - (band * 0.5f) * dir;
   const f3 step = (dir * band) / (float)num_steps;
   const float size = (float)v.res;
+  // Samples in batches of ALLOC_BATCH: the batch's block-table loads are
+  // issued together and its flag stores follow them.  (One sample at a time,
+  // each byte store may alias the next table load, so every sample waited a
+  // full memory round trip.)  A sample in the same block as the sample
+  // before it repeats that sample's load and store, so it is skipped.  The
+  // flags are set-to-1 stores, so the set of requested and active blocks is
+  // the one the sample-by-sample loop leaves.  Block coordinates are below
+  // 1024 (res <= 8192), so the 30-bit Morton code is the table index.
+  constexpr int ALLOC_BATCH = 8;
+  // descriptor pointers held in registers: a byte store may alias the
+  // descriptor, which would otherwise be re-read before every store
+  const int32_t* const slot = v.slot;
+  __attribute__((address_space(1))) uint8_t* const req = (__attribute__((address_space(1))) uint8_t*)v.req;
+  __attribute__((address_space(1))) uint8_t* const act = (__attribute__((address_space(1))) uint8_t*)v.blk_active;
   f3 p = origin;
-  for (int s = 0; s < num_steps; s++) {
-    const float sx = floorf(p.x * inv_vox), sy = floorf(p.y * inv_vox), sz = floorf(p.z * inv_vox);
-    if (sx < size && sy < size && sz < size && sx >= 0 && sy >= 0 && sz >= 0) {
-      const uint32_t vx = (uint32_t)(int)sx, vy = (uint32_t)(int)sy, vz = (uint32_t)(int)sz;
-      const uint64_t m = morton3(vx >> 3, vy >> 3, vz >> 3);
-      const int32_t sl = v.slot[m];
-      if (sl < 0) v.req[m] = 1;
-      else v.blk_active[sl] = 1;
+  uint32_t last = 0xffffffffu;
+  for (int s0 = 0; s0 < num_steps; s0 += ALLOC_BATCH) {
+    uint32_t m[ALLOC_BATCH];
+    bool use[ALLOC_BATCH];
+#pragma unroll
+    for (int k = 0; k < ALLOC_BATCH; k++) {
+      use[k] = false;
+      m[k] = 0;
+      if (s0 + k < num_steps) {
+        const float sx = floorf(p.x * inv_vox), sy = floorf(p.y * inv_vox), sz = floorf(p.z * inv_vox);
+        if (sx < size && sy < size && sz < size && sx >= 0 && sy >= 0 && sz >= 0) {
+          const uint32_t vx = (uint32_t)(int)sx, vy = (uint32_t)(int)sy, vz = (uint32_t)(int)sz;
+          const uint32_t mk = morton3_10(vx >> 3, vy >> 3, vz >> 3);
+          use[k] = mk != last;
+          m[k] = mk;
+          last = mk;
+        }
+        p = p + step;
+      }
     }
-    p = p + step;
+    int32_t sl[ALLOC_BATCH];
+#pragma unroll
+    for (int k = 0; k < ALLOC_BATCH; k++) sl[k] = use[k] ? ldx(slot, m[k]) : 0;
+    // one wait for the batch: the stores below then carry no load waits
+#pragma unroll
+    for (int k = 0; k < ALLOC_BATCH; k++) asm volatile("" ::"v"(sl[k]));
+#pragma unroll
+    for (int k = 0; k < ALLOC_BATCH; k++) {
+      if (!use[k]) continue;
+      if (sl[k] < 0) req[m[k]] = 1;
+      else act[sl[k]] = 1;
+    }
   }
 }
 
@@ -219,6 +255,17 @@ __global__ void k_active_list(const MapObj* __restrict__ objs, int n, int w, int
 // update_block + sdf_update: one 512-thread workgroup per active block, one
 // voxel per thread (x fastest, the block's storage order).  The row start
 // and x-increment are formed exactly as the reference forms them.
+//
+// Memory round trips per block: the list entry (prefetched one block ahead),
+// the descriptor fields (held in registers, so that the voxel stores, which
+// may alias the descriptor as far as the compiler knows, do not force
+// re-reads), the block's Morton code, one batch of pixel gathers (depth,
+// mask and, for the motion-refined mask, both track codes) and one batch of
+// voxel fields with the colour gathers; the stores come last.  A block
+// allocated this frame (slot at or past the frame-start count) holds
+// voxel_traits<SDF>::initValue() (continuous/volume_traits.hpp:38) before
+// its first update, so its voxels take those values without loading them
+// and store them whether they update or not.
 __global__ __launch_bounds__(512) void k_integrate(const MapObj* __restrict__ objs, float fx, float fy,
                                                    float cx, float cy, const float* __restrict__ depth,
                                                    const float* __restrict__ rr, const float* __restrict__ gg,
@@ -230,68 +277,97 @@ __global__ __launch_bounds__(512) void k_integrate(const MapObj* __restrict__ ob
   const Mat4 K = camK(fx, fy, cx, cy);
   const size_t npx = (size_t)w * h;
   const int nact = *wl.act_count;
-  for (int a = blockIdx.x; a < nact; a += gridDim.x) {
-    const int2 e = wl.act_list[a];
-    const int j = e.x;
+  const int G = (int)gridDim.x;
+  int2 next = (int)blockIdx.x < nact ? wl.act_list[blockIdx.x] : make_int2(0, 0);
+  for (int a = blockIdx.x; a < nact; a += G) {
+    const int2 e = next;
+    if (a + G < nact) next = wl.act_list[a + G];
+    const int j = e.x, s = e.y;
     const MapObj& O = objs[j];
-    const VolDev& v = O.v;
-    const ObjParams* __restrict__ prm = O.prm;
-    const float* __restrict__ mask = O.mask;
-    const int8_t* __restrict__ codes = O.codes;
-    const Mat4 Tco = prm->Tco;
-    const float vs = v.voxel;
+    const Mat4 Tco = O.prm->Tco;
+    const float vs = O.v.voxel;
+    const bool fresh = s >= O.v.counters[3];
+    const int32_t morton = O.v.blk_morton[s];
+    float* const tsdf = O.v.tsdf;
+    float* const wt = O.v.wt;
+    float* const vr = O.v.r;
+    float* const vg = O.v.g;
+    float* const vb = O.v.b;
+    int32_t* const vc = O.v.c;
+    int32_t* const vfgw = O.v.fgw;
+    double* const vfg = O.v.fg;
+    uint8_t* const blk_active = O.v.blk_active;
+    const float* const mask = O.mask;
+    const int8_t* const codes = O.codes;
     const f3 delta = rotate(Tco, mk3(vs, 0.f, 0.f));
     const f3 cdelta = rotate(K, delta);
-    const int s = e.y;
-    if (s >= v.counters[3]) {
-      // a block allocated this frame (slot at or past the frame-start count,
-      // k_obj_params): voxel_traits<SDF>::initValue (continuous/
-      // volume_traits.hpp:38) before its first update; every new block is
-      // active, so it is on this list
-      const size_t i = (size_t)s * BLOCK_VOX + tid;
-      v.tsdf[i] = 1.f; v.wt[i] = 0.f; v.r[i] = 0.f; v.g[i] = 0.f; v.b[i] = 0.f; v.c[i] = 0; v.fg[i] = 0.0; v.fgw[i] = 0;
-    }
     int bx, by, bz;
-    block_coords(v.blk_morton[s], bx, by, bz);
+    block_coords(morton, bx, by, bz);
     const int y = by + ly, z = bz + lz;
     const f3 start = affine(Tco, mk3((float)bx * vs, (float)y * vs, (float)z * vs));
     const f3 cstart = affine(K, start);
     const f3 cv = cstart + (float)lx * cdelta;
     const f3 pos = start + (float)lx * delta;
-    bool vis = false, upd = false;
+    bool vis = false;
+    size_t pi = 0;
     if (!(pos.z < 0.0001f)) {
       const float invd = 1.f / cv.z;
       const float pxf = cv.x * invd + 0.5f, pyf = cv.y * invd + 0.5f;
       if (!(pxf < 0.5f || pxf > (float)w - 1.5f || pyf < 0.5f || pyf > (float)h - 1.5f)) {
         vis = true;
-        const size_t pi = (size_t)(unsigned)pxf + (size_t)w * (unsigned)pyf;
-        const float ds = depth[pi];
-        const float m = refined_mask(mask, codes, pi, npx, refine, use_icp, use_rgb);
-        if (!(ds <= 0.f || m < 0.f)) {
-          const float qx = pos.x / pos.z, qy = pos.y / pos.z;
-          const float diff = (ds - pos.z) * sqrtf(1.f + qx * qx + qy * qy);
-          if (diff > -mu) {
-            upd = true;
-            const float sdf = fmin_r(1.f, diff / mu);
-            const size_t vi = (size_t)s * BLOCK_VOX + tid;
-            const float t0 = v.tsdf[vi], w0 = v.wt[vi];
-            v.tsdf[vi] = fmax_r(-1.f, fmin_r((w0 * t0 + sdf) / (w0 + 1), 1.f));
-            v.wt[vi] = fmin_r(w0 + 1, (float)maxweight);
-            const int c0 = v.c[vi];
-            const float fc = (float)c0, fc1 = (float)(c0 + 1);
-            v.r[vi] = (rr[pi] + v.r[vi] * fc) / fc1;
-            v.g[vi] = (gg[pi] + v.g[vi] * fc) / fc1;
-            v.b[vi] = (bb[pi] + v.b[vi] * fc) / fc1;
-            v.c[vi] = c0 + 1;
-            const int fw = v.fgw[vi];
-            v.fg[vi] = ((double)m + v.fg[vi] * (double)fw) / (double)(fw + 1);
-            v.fgw[vi] = fw + 1;
-          }
-        }
+        pi = (size_t)(unsigned)pxf + (size_t)w * (unsigned)pyf;
       }
     }
+    // pixel gathers, one batch (refine_mask_use_motion reads both codes)
+    const bool use_codes = refine && codes;
+    float ds = 0.f, m = -1.f;
+    int8_t ci = 0, cr = 0;
+    if (vis) {
+      ds = depth[pi];
+      m = mask[pi];
+      if (use_codes) { ci = codes[pi]; cr = codes[npx + pi]; }
+    }
+    if (use_codes && m >= 0.f) {
+      if (use_icp && (ci == -4 || ci == -6)) m = -1.f;
+      else if (use_rgb && cr < -3) m = -1.f;
+    }
+    bool upd = false;
+    float sdf = 0.f;
+    if (vis && !(ds <= 0.f || m < 0.f)) {
+      const float qx = pos.x / pos.z, qy = pos.y / pos.z;
+      const float diff = (ds - pos.z) * sqrtf(1.f + qx * qx + qy * qy);
+      if (diff > -mu) {
+        upd = true;
+        sdf = fmin_r(1.f, diff / mu);
+      }
+    }
+    // voxel fields and colour gathers, one batch
+    const size_t vi = (size_t)s * BLOCK_VOX + tid;
+    float t0 = 1.f, w0 = 0.f, r0 = 0.f, g0 = 0.f, b0 = 0.f, pr = 0.f, pg = 0.f, pb = 0.f;
+    int c0 = 0, fw = 0;
+    double fg0 = 0.0;
+    if (upd) {
+      if (!fresh) {
+        t0 = tsdf[vi]; w0 = wt[vi]; r0 = vr[vi]; g0 = vg[vi]; b0 = vb[vi];
+        c0 = vc[vi]; fw = vfgw[vi]; fg0 = vfg[vi];
+      }
+      pr = rr[pi]; pg = gg[pi]; pb = bb[pi];
+    }
+    if (upd) {
+      tsdf[vi] = fmax_r(-1.f, fmin_r((w0 * t0 + sdf) / (w0 + 1), 1.f));
+      wt[vi] = fmin_r(w0 + 1, (float)maxweight);
+      const float fc = (float)c0, fc1 = (float)(c0 + 1);
+      vr[vi] = (pr + r0 * fc) / fc1;
+      vg[vi] = (pg + g0 * fc) / fc1;
+      vb[vi] = (pb + b0 * fc) / fc1;
+      vc[vi] = c0 + 1;
+      vfg[vi] = ((double)m + fg0 * (double)fw) / (double)(fw + 1);
+      vfgw[vi] = fw + 1;
+    } else if (fresh) {
+      tsdf[vi] = 1.f; wt[vi] = 0.f; vr[vi] = 0.f; vg[vi] = 0.f; vb[vi] = 0.f; vc[vi] = 0; vfg[vi] = 0.0; vfgw[vi] = 0;
+    }
     const int any = __syncthreads_or(vis ? 1 : 0);
-    if (tid == 0) v.blk_active[s] = any ? 1 : 0;
+    if (tid == 0) blk_active[s] = any ? 1 : 0;
     if (stats) {
       const int nupd = __syncthreads_count(upd ? 1 : 0);
       if (tid == 0) {
