@@ -466,16 +466,24 @@ __device__ __forceinline__ void track_rows(const TrackLaunch& p, const TrackInst
 }
 
 
-// Camera rows by pixel: each thread takes whole pixels, U at a time, and
-// computes both rows of each (track_ICP_Kernel's and trackRGB's, the same
-// arithmetic as icp_row and rgb_core<false>), issuing the own inputs of all
-// 2U rows as one batch and then all their gathers as one batch, so the 2U
-// rows wait on two memory round trips instead of 2U pairs (U = 1 keeps the
-// kernel at 143 VGPRs, 3 waves per SIMD; U = 2 needs 172, 2 waves, and a
-// level-0 launch of 2400 waves then runs in two rounds).  Pixels i = wg *
-// TB + tid + k * nactive * TB; ICP row then photometric row of each pixel
-// into the thread's FP32 sums.
-template <int RB, int U, class MT>
+// Camera rows by pixel: each thread takes whole pixels and computes both
+// rows of each (track_ICP_Kernel's and trackRGB's, the same arithmetic as
+// icp_row and rgb_core<false>): the pixel's own inputs of both rows in one
+// batch, then all its gathers in one batch.  The next pixel's own inputs are
+// issued as soon as this pixel's have arrived, so they are in flight with
+// this pixel's gathers: k pixels per thread wait on k + 1 memory round trips
+// instead of 2k.  (Both pixels' loads in one batch
+// took 172 VGPRs, 2 waves per SIMD; this form stays within 3.)  Pixels
+// i = wg * TB + tid + k * nactive * TB; ICP row then photometric row of each
+// pixel into the thread's FP32 sums.
+struct PixelOwn { f3 nl, vl, P, rvl; float dr, ir; LabelTaps lt; };
+__device__ __forceinline__ void pixel_own(const TrackLaunch& p, const TrackInst& in, int idx, int n, PixelOwn& o) {
+  const uint32_t ii = idx < n ? (uint32_t)idx : 0u;   // past the end: pixel 0, loaded and unused
+  o.nl = lv3(p, PL_N, ii); o.vl = lv3(p, PL_V, ii); o.P = lv3(p, PL_COV, ii);
+  if (p.label) label_taps(p, (int)ii % p.w, (int)ii / p.w, o.lt);
+  o.dr = ldx(in.rD + p.rd_off, ii); o.rvl = rf3(p, PL_V, ii); o.ir = ldx(p.ref + (size_t)PL_I * p.lstride, ii);
+}
+template <int RB, class MT>
 __device__ __forceinline__ void camera_pixels(const TrackLaunch& p, const TrackInst& inst, const MT& M, int state,
                                               int wg, int nactive, float* __restrict__ part, float (*red)[32]) {
   const int tid = threadIdx.x;
@@ -492,145 +500,118 @@ __device__ __forceinline__ void camera_pixels(const TrackLaunch& p, const TrackI
   const float* Ip = p.live + (size_t)PL_I * p.lstride;
   const float* Gx = p.live + (size_t)PL_GX * p.lstride;
   const float* Gy = p.live + (size_t)PL_GY * p.lstride;
-  for (int base = wg * TB + tid; base < n; base += U * stride) {
+  int idx = wg * TB + tid;
+  PixelOwn cur;
+  pixel_own(p, in, idx, n, cur);
+  for (; idx < n; idx += stride) {
     asm volatile("" ::: "memory");   // keep the LDS matrix reads inside the loop
-    int idx[U];
-    f3 nl[U], vl[U], P[U], rvl[U];
-    float dr[U], ir[U];
-    LabelTaps lt[U];
-    // own inputs of both rows of every pixel
+    hold(cur.nl); hold(cur.vl); hold(cur.P); hold(cur.dr); hold(cur.rvl); hold(cur.ir);
+    if (p.label) {
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      idx[u] = base + u * stride;
-      const uint32_t ii = idx[u] < n ? (uint32_t)idx[u] : 0u;
-      nl[u] = lv3(p, PL_N, ii); vl[u] = lv3(p, PL_V, ii); P[u] = lv3(p, PL_COV, ii);
-      if (p.label) label_taps(p, (int)ii % p.w, (int)ii / p.w, lt[u]);
-      dr[u] = ldx(in.rD + p.rd_off, ii); rvl[u] = rf3(p, PL_V, ii); ir[u] = ldx(p.ref + (size_t)PL_I * p.lstride, ii);
+      for (int k = 0; k < 4; k++) hold(cur.lt.l[k]);
     }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      hold(nl[u]); hold(vl[u]); hold(P[u]); hold(dr[u]); hold(rvl[u]); hold(ir[u]);
-      if (p.label) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) hold(lt[u].l[k]);
-      }
-    }
+    // the next pixel's own inputs: in flight with this pixel's gathers
+    PixelOwn nxt;
+    pixel_own(p, in, idx + stride, n, nxt);
     // projections (codes 0: pending)
-    int ci[U], cr[U];
-    f3 pv[U], wv[U], lvv[U];
-    uint32_t ri[U], i0[U];
-    float tfx[U], tfy[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int x = idx[u] % p.w, y = idx[u] / p.w;
-      ci[u] = 0; cr[u] = 0; ri[u] = 0; i0[u] = 0; tfx[u] = 0.f; tfy[u] = 0.f;
-      pv[u] = mk3(0.f, 0.f, 0.f); wv[u] = pv[u]; lvv[u] = pv[u];
-      // track_ICP_Kernel (tracking.cpp:923-1020), up to the reference lookup
-      if (p.label && taps_moving(p, lt[u], M.mov)) ci[u] = -6;
-      else if (nl[u].x == INVALID) ci[u] = -1;
+    const int x = idx % p.w, y = idx / p.w;
+    int ci = 0, cr = 0;
+    f3 pv = mk3(0.f, 0.f, 0.f), wv = pv, lvv = pv;
+    uint32_t ri = 0, i0 = 0;
+    float tfx = 0.f, tfy = 0.f;
+    // track_ICP_Kernel (tracking.cpp:923-1020), up to the reference lookup
+    if (p.label && taps_moving(p, cur.lt, M.mov)) ci = -6;
+    else if (cur.nl.x == INVALID) ci = -1;
+    else {
+      pv = affine(M.T, cur.vl);
+      const f3 pp = affine(M.B, pv);
+      const float ux = pp.x / pp.z, uy = pp.y / pp.z;
+      if (ux < 1 || ux > (float)(p.W - 2) || uy < 1 || uy > (float)(p.H - 2) || isnan(ux) || isnan(uy)) ci = -2;
+      else ri = (unsigned)ux + (unsigned)uy * (unsigned)p.W;
+    }
+    // trackRGB (tracking.cpp:1023-1111), up to the bilinear taps
+    const f3 vr = cur.dr > 0.0f ? cur.dr * rotate(camKinv(fxl, fyl, cxl, cyl), mk3((float)x, (float)y, 1.0f))
+                                : mk3(0.f, 0.f, 0.f);
+    if (vr.z <= 0.f || vr.z == INVALID) cr = -1;
+    else if (length(vr - cur.rvl) > 0.1f && cur.rvl.z > 0.f) cr = -3;
+    else {
+      wv = affine(M.A, vr);
+      lvv = affine(M.invT, wv);
+      const f3 pj = rotate(camK(fxl, fyl, cxl, cyl), lvv);
+      const float ux = pj.x / pj.z, uy = pj.y / pj.z;
+      if (ux < 1 || ux > (float)(p.w - 2) || uy < 1 || uy > (float)(p.h - 2) || isnan(ux) || isnan(uy)) cr = -2;
       else {
-        pv[u] = affine(M.T, vl[u]);
-        const f3 pp = affine(M.B, pv[u]);
-        const float ux = pp.x / pp.z, uy = pp.y / pp.z;
-        if (ux < 1 || ux > (float)(p.W - 2) || uy < 1 || uy > (float)(p.H - 2) || isnan(ux) || isnan(uy)) ci[u] = -2;
-        else ri[u] = (unsigned)ux + (unsigned)uy * (unsigned)p.W;
+        const float fpx = floorf(ux), fpy = floorf(uy);
+        i0 = (uint32_t)(int)fpx + (uint32_t)(int)fpy * (uint32_t)p.w;
+        tfx = ux - fpx; tfy = uy - fpy;
       }
-      // trackRGB (tracking.cpp:1023-1111), up to the bilinear taps
-      const f3 vr = dr[u] > 0.0f ? dr[u] * rotate(camKinv(fxl, fyl, cxl, cyl), mk3((float)x, (float)y, 1.0f))
-                                 : mk3(0.f, 0.f, 0.f);
-      if (vr.z <= 0.f || vr.z == INVALID) cr[u] = -1;
-      else if (length(vr - rvl[u]) > 0.1f && rvl[u].z > 0.f) cr[u] = -3;
-      else {
-        wv[u] = affine(M.A, vr);
-        lvv[u] = affine(M.invT, wv[u]);
-        const f3 pj = rotate(camK(fxl, fyl, cxl, cyl), lvv[u]);
-        const float ux = pj.x / pj.z, uy = pj.y / pj.z;
-        if (ux < 1 || ux > (float)(p.w - 2) || uy < 1 || uy > (float)(p.h - 2) || isnan(ux) || isnan(uy)) cr[u] = -2;
+    }
+    // gathers of both pending rows
+    const uint32_t r0 = ci == 0 ? ri : 0u;
+    const f3 rn = mdl3(p, in, 3, r0), rv = mdl3(p, in, 0, r0);
+    const uint32_t j0 = cr == 0 ? i0 : 0u;
+    const uint32_t w = (uint32_t)p.w;
+    float ta[12];
+    ta[0] = ldx(Ip, j0); ta[1] = ldx(Ip, j0 + 1); ta[2] = ldx(Ip, j0 + w); ta[3] = ldx(Ip, j0 + w + 1);
+    ta[4] = ldx(Gx, j0); ta[5] = ldx(Gx, j0 + 1); ta[6] = ldx(Gx, j0 + w); ta[7] = ldx(Gx, j0 + w + 1);
+    ta[8] = ldx(Gy, j0); ta[9] = ldx(Gy, j0 + 1); ta[10] = ldx(Gy, j0 + w); ta[11] = ldx(Gy, j0 + w + 1);
+    hold(rn); hold(rv);
+#pragma unroll
+    for (int k = 0; k < 12; k++) hold(ta[k]);
+    // finish both rows
+    const size_t ro = (size_t)x + (size_t)y * p.W;   // row stride W at every level (tracking.cpp:49-50)
+    {
+      float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
+      int res = ci;
+      if (res == 0) {
+        if (rn.x == INVALID) res = -3;
         else {
-          const float fpx = floorf(ux), fpy = floorf(uy);
-          i0[u] = (uint32_t)(int)fpx + (uint32_t)(int)fpy * (uint32_t)p.w;
-          tfx[u] = ux - fpx; tfy[u] = uy - fpy;
-        }
-      }
-      if (idx[u] >= n) { ci[u] = 1; cr[u] = 1; }   // no pixel: nothing to load or sum
-    }
-    // gathers of every pending row
-    f3 rn[U], rv[U];
-    float ta[U][12];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const uint32_t r0 = ci[u] == 0 ? ri[u] : 0u;
-      rn[u] = mdl3(p, in, 3, r0); rv[u] = mdl3(p, in, 0, r0);
-      const uint32_t j0 = cr[u] == 0 ? i0[u] : 0u;
-      const uint32_t w = (uint32_t)p.w;
-      ta[u][0] = ldx(Ip, j0); ta[u][1] = ldx(Ip, j0 + 1); ta[u][2] = ldx(Ip, j0 + w); ta[u][3] = ldx(Ip, j0 + w + 1);
-      ta[u][4] = ldx(Gx, j0); ta[u][5] = ldx(Gx, j0 + 1); ta[u][6] = ldx(Gx, j0 + w); ta[u][7] = ldx(Gx, j0 + w + 1);
-      ta[u][8] = ldx(Gy, j0); ta[u][9] = ldx(Gy, j0 + 1); ta[u][10] = ldx(Gy, j0 + w); ta[u][11] = ldx(Gy, j0 + w + 1);
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      hold(rn[u]); hold(rv[u]);
-#pragma unroll
-      for (int k = 0; k < 12; k++) hold(ta[u][k]);
-    }
-    // finish both rows of every pixel
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      if (idx[u] >= n) continue;
-      const int x = idx[u] % p.w, y = idx[u] / p.w;
-      const size_t ro = (size_t)x + (size_t)y * p.W;   // row stride W at every level (tracking.cpp:49-50)
-      {
-        float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
-        int res = ci[u];
-        if (res == 0) {
-          if (rn[u].x == INVALID) res = -3;
+          const f3 diff = rv - pv;
+          const f3 pn = rotate(M.T, cur.nl);
+          if (length(diff) > 0.1f) res = -4;
+          else if (dot(pn, rn) < 0.8f) res = -5;
           else {
-            const f3 diff = rv[u] - pv[u];
-            const f3 pn = rotate(M.T, nl[u]);
-            if (length(diff) > 0.1f) res = -4;
-            else if (dot(pn, rn[u]) < 0.8f) res = -5;
-            else {
-              const f3 Pu = P[u], rnu = rn[u];
-              const float sig = rnu.x * rnu.x * Pu.x + rnu.y * rnu.y * Pu.y + rnu.z * rnu.z * Pu.z;
-              const float w = sqrtf((float)(1.0 / (double)sig));
-              e = w * dot(rnu, diff);
-              const f3 j0 = w * rnu, j1 = w * cross(pv[u], rnu);
-              J[0] = j0.x; J[1] = j0.y; J[2] = j0.z; J[3] = j1.x; J[4] = j1.y; J[5] = j1.z;
-              res = 1;
-            }
+            const f3 Pu = cur.P;
+            const float sig = rn.x * rn.x * Pu.x + rn.y * rn.y * Pu.y + rn.z * rn.z * Pu.z;
+            const float wgt = sqrtf((float)(1.0 / (double)sig));
+            e = wgt * dot(rn, diff);
+            const f3 j0v = wgt * rn, j1v = wgt * cross(pv, rn);
+            J[0] = j0v.x; J[1] = j0v.y; J[2] = j0v.z; J[3] = j1v.x; J[4] = j1v.y; J[5] = j1v.z;
+            res = 1;
           }
         }
-        accum<RB>(s, res, e, J, p.robust, p.robust_k);
-        if (in.codes && p.level == 0) in.codes[ro] = (int8_t)res;
-        if (dbg) { TrackRow& d = dbg[ro]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
       }
-      {
-        float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
-        int res = cr[u];
-        if (res == 0) {
-          const float fx = tfx[u], fy = tfy[u];
-          const float* t = ta[u];
-          const float li = t[0] * (1.f - fx) * (1.f - fy) + t[1] * (1.f - fy) * fx + t[2] * (1.f - fx) * fy + t[3] * fx * fy;
-          const float g_x = t[4] * (1.f - fx) * (1.f - fy) + t[5] * (1.f - fy) * fx + t[6] * (1.f - fx) * fy + t[7] * fx * fy;
-          const float g_y = t[8] * (1.f - fx) * (1.f - fy) + t[9] * (1.f - fy) * fx + t[10] * (1.f - fx) * fy + t[11] * fx * fy;
-          const float rres = ir[u] - li;
-          const f3 lv = lvv[u];
-          const float sc = 1.f / lv.z;
-          f3 jt = sc * mk3(g_x * fxl, g_y * fyl, -(g_x * lv.x * fxl + g_y * lv.y * fyl) / lv.z);
-          jt = (-1.f) * rotate(M.T, jt);
-          const f3 jr = cross(wv[u], jt);
-          const float ic = p.inv_cov_rgb;
-          const f3 j0 = ic * jt, j1 = ic * jr;
-          J[0] = j0.x; J[1] = j0.y; J[2] = j0.z; J[3] = j1.x; J[4] = j1.y; J[5] = j1.z;
-          e = ic * rres;
-          res = 1;
-        }
-        accum<RB>(s, res, e, J, p.robust, p.robust_k);
-        if (in.codes && p.level == 0) in.codes[N0 + ro] = (int8_t)res;
-        if (in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
-        if (dbg) { TrackRow& d = dbg[N0 + ro]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
-      }
+      accum<RB>(s, res, e, J, p.robust, p.robust_k);
+      if (in.codes && p.level == 0) in.codes[ro] = (int8_t)res;
+      if (dbg) { TrackRow& d = dbg[ro]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
     }
+    {
+      float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
+      int res = cr;
+      if (res == 0) {
+        const float fx = tfx, fy = tfy;
+        const float* t = ta;
+        const float li = t[0] * (1.f - fx) * (1.f - fy) + t[1] * (1.f - fy) * fx + t[2] * (1.f - fx) * fy + t[3] * fx * fy;
+        const float g_x = t[4] * (1.f - fx) * (1.f - fy) + t[5] * (1.f - fy) * fx + t[6] * (1.f - fx) * fy + t[7] * fx * fy;
+        const float g_y = t[8] * (1.f - fx) * (1.f - fy) + t[9] * (1.f - fy) * fx + t[10] * (1.f - fx) * fy + t[11] * fx * fy;
+        const float rres = cur.ir - li;
+        const f3 lv = lvv;
+        const float sc = 1.f / lv.z;
+        f3 jt = sc * mk3(g_x * fxl, g_y * fyl, -(g_x * lv.x * fxl + g_y * lv.y * fyl) / lv.z);
+        jt = (-1.f) * rotate(M.T, jt);
+        const f3 jr = cross(wv, jt);
+        const float ic = p.inv_cov_rgb;
+        const f3 j0v = ic * jt, j1v = ic * jr;
+        J[0] = j0v.x; J[1] = j0v.y; J[2] = j0v.z; J[3] = j1v.x; J[4] = j1v.y; J[5] = j1v.z;
+        e = ic * rres;
+        res = 1;
+      }
+      accum<RB>(s, res, e, J, p.robust, p.robust_k);
+      if (in.codes && p.level == 0) in.codes[N0 + ro] = (int8_t)res;
+      if (in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
+      if (dbg) { TrackRow& d = dbg[N0 + ro]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
+    }
+    cur = nxt;
   }
   wg_reduce(s, part, red);
 }
@@ -640,9 +621,12 @@ __device__ __forceinline__ void camera_pixels(const TrackLaunch& p, const TrackI
 // (track_obj_RGB_kernel) of each, issuing both rows' leading loads (live
 // vertex and normal x, rendered depth) in one batch, so the pixel costs one
 // memory round trip where two rows on separate threads cost one each.  Most
-// of an object's rows end after that batch (away from the object).  Pixels
+// of an object's rows end after that batch (away from the object), so the
+// next pixel's leading loads are issued as soon as this pixel's have
+// arrived: they are in flight while this pixel's rows finish.  Pixels
 // i = wg * TB + tid + k * nactive * TB; ICP row then photometric row of each
 // pixel into the thread's FP32 sums.
+struct ObjLead { f3 c2v; float nlx, dr; };
 template <int RB, class MT>
 __device__ __forceinline__ void object_pixels(const TrackLaunch& p, const TrackInst& inst, const MT& M, int state,
                                               int wg, int nactive, float* __restrict__ part, float (*red)[32]) {
@@ -663,20 +647,30 @@ __device__ __forceinline__ void object_pixels(const TrackLaunch& p, const TrackI
   // photometric row ends at the depth test, and the load is skipped.
   const bool boxed = in.tbox != nullptr;
   const int l = p.level;
-  for (int i = wg * TB + tid; i < n; i += nactive * TB) {
-    asm volatile("" ::: "memory");   // keep the LDS matrix reads inside the loop
-    const int x = i % p.w, y = i / p.w;
-    const size_t ro = (size_t)x + (size_t)y * p.W;
+  const int stride = nactive * TB;
+  auto lead = [&](int i, ObjLead& o) {
+    const uint32_t ii = i < n ? (uint32_t)i : 0u;   // past the end: pixel 0, loaded and unused
+    const int x = (int)ii % p.w, y = (int)ii / p.w;
     const int fx0 = x << l, fy0 = y << l, fx1 = fx0 + (1 << l) - 1, fy1 = fy0 + (1 << l) - 1;
     const bool rgb_out = boxed && (fx1 / MAP_TILE < M.tb[0] || fx0 / MAP_TILE > M.tb[2] ||
                                    fy1 / MAP_TILE < M.tb[1] || fy0 / MAP_TILE > M.tb[3]);
-    const f3 c2v = lv3(p, PL_V, (size_t)i);
-    const float nlx = ldx(nb, (uint32_t)i);
-    const float dr = rgb_out ? 0.0f : ldx(in.rD + p.rd_off, (uint32_t)i);
-    hold(c2v); hold(nlx); hold(dr);
+    o.c2v = lv3(p, PL_V, ii);
+    o.nlx = ldx(nb, ii);
+    o.dr = rgb_out ? 0.0f : ldx(in.rD + p.rd_off, ii);
+  };
+  int i = wg * TB + tid;
+  ObjLead cur;
+  lead(i, cur);
+  for (; i < n; i += stride) {
+    asm volatile("" ::: "memory");   // keep the LDS matrix reads inside the loop
+    hold(cur.c2v); hold(cur.nlx); hold(cur.dr);
+    ObjLead nxt;
+    lead(i + stride, nxt);
+    const int x = i % p.w, y = i / p.w;
+    const size_t ro = (size_t)x + (size_t)y * p.W;
     {
       float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
-      const int res = obj_icp_rest(p, in, M, (size_t)i, c2v, nlx, e, J);
+      const int res = obj_icp_rest(p, in, M, (size_t)i, cur.c2v, cur.nlx, e, J);
       accum<RB>(s, res, e, J, p.robust, p.robust_k);
       if (in.codes && p.level == 0) in.codes[ro] = (int8_t)res;
       if (dbg) { TrackRow& d = dbg[ro]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
@@ -684,17 +678,18 @@ __device__ __forceinline__ void object_pixels(const TrackLaunch& p, const TrackI
     {
       float e = 0.f, J[6] = {0, 0, 0, 0, 0, 0};
       int res = -1;
-      if (dr > 0.0f) {
+      if (cur.dr > 0.0f) {
         const f3 vl = rf3(p, PL_V, (size_t)i);
         const float ir = ldx(p.ref + (size_t)PL_I * p.lstride, (uint32_t)i);
         hold(vl); hold(ir);
-        res = rgb_core<true>(p, M, x, y, dr, vl, ir, e, J);
+        res = rgb_core<true>(p, M, x, y, cur.dr, vl, ir, e, J);
       }
       accum<RB>(s, res, e, J, p.robust, p.robust_k);
       if (in.codes && p.level == 0) in.codes[N0 + ro] = (int8_t)res;
       if (in.rgb_err && p.level == 0) in.rgb_err[ro] = e;
       if (dbg) { TrackRow& d = dbg[N0 + ro]; d.result = res; d.error = e; for (int q = 0; q < 6; q++) d.J[q] = J[q]; }
     }
+    cur = nxt;
   }
   wg_reduce(s, part, red);
 }
@@ -1167,7 +1162,7 @@ __global__ __launch_bounds__(TB) void k_track_px(TrackLaunch p, const TrackInst*
   __syncthreads();
   if (skip) return;
   const int c = xcd_chunk(blockIdx.x, nblk);
-  camera_pixels<RB, 1>(p, insts[j], Msh, state, c, nblk, partial + ((size_t)j * nblk + c) * 32, red);
+  camera_pixels<RB>(p, insts[j], Msh, state, c, nblk, partial + ((size_t)j * nblk + c) * 32, red);
 }
 
 // Two-kernel form, rows of every moving object in one grid over the pixels
@@ -1326,7 +1321,7 @@ __global__ __launch_bounds__(TB) void k_gn_iter(TrackLaunch p, const TrackInst* 
   if (skip) return;
   float* part = part_cur + ((size_t)j * nblk + blockIdx.x) * 32;
   if constexpr (ICP && RGB && !OBJ) {
-    if (p.paired) { camera_pixels<RB, 1>(p, insts[j], Msh, state, blockIdx.x, nblk, part, red); return; }
+    if (p.paired) { camera_pixels<RB>(p, insts[j], Msh, state, blockIdx.x, nblk, part, red); return; }
   }
   if constexpr (ICP && RGB && OBJ) {
     if (p.paired) { object_pixels<RB>(p, insts[j], Msh, state, blockIdx.x, nblk, part, red); return; }
