@@ -264,7 +264,8 @@ long mf_device_bytes(mf_ctx* ctx);
  *   level l, 1..64 (defaults 4, 1, 1, 1). */
 /*   MF_TUNE_TRACK_FUSED_L0 + l: 1 runs each GN iteration at level l as one
  *   launch whose workgroups first finish the previous iteration themselves;
- *   0 (default) as a row launch and a one-workgroup solve launch. */
+ *   0 as a row launch and a one-workgroup solve launch; -1 (default) fused
+ *   at levels l >= 1 when the launch has at most 1200 row workgroups. */
 /*   MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 + l: moving objects' rows at level l in
  *   one grid over the pixels, each thread loading its pixels' frame inputs
  *   once for every object, with 1 or 2 pixels per thread; 0 (default) gives

@@ -76,7 +76,7 @@ struct mf_ctx {
   float* partial = nullptr;
   int nblk0 = 0;
   int track_rpt[MAX_LEVELS] = {4, 1, 1, 1};   // tracking rows per thread, per pyramid level
-  int track_fused[MAX_LEVELS] = {0, 0, 0, 0};   // 1: one fused launch per iteration at that level
+  int track_fused[MAX_LEVELS] = {-1, -1, -1, -1};   // 1: one fused launch per iteration at that level, 0: two, -1: fused_level
   // objects' rows in one grid over the pixels with this many pixels per
   // thread (1, 2), or 0: one row grid per object (grid.y), per pyramid level.
   // 0 everywhere measured fastest on MI355X at 1, 8 and 16 objects.
@@ -353,6 +353,21 @@ int obj_rows_per_thread(const mf_ctx* c, int l, int n) {
   return n >= 2 ? kBatched[l] : c->track_rpt[l];
 }
 
+// Whether the GN iterations of level l run as one fused launch each (its
+// workgroups first finish the previous iteration themselves from all of its
+// partials) for a launch of n instances with nblk row workgroups each: the
+// knob, or when it is -1, at the coarse levels (l >= 1) when the launch has
+// at most 1200 workgroups.  The fused form saves a launch per iteration but
+// every workgroup reads all nblk partials and runs the solve step; measured
+// on MI355X: 640x480 with 1 object 794 -> 810 frames/s, 8 objects 566 -> 570
+// (one object per launch 313 -> 321); level 1 of 1280x720 (1800
+// workgroups) fused 238 -> 220, and at 640x480 the level-0 launch fused is
+// no faster.
+bool fused_level(const mf_ctx* c, int l, int nblk, int n) {
+  if (c->track_fused[l] >= 0) return c->track_fused[l] != 0;
+  return l >= 1 && (long)nblk * n <= 1200;
+}
+
 // One coarse-to-fine GN run over `n` instances sharing a launch
 // (trackLiveFrame / trackEachObject loops, tracking.cpp:242-335, 624-690):
 // one fused track+reduce+finalize launch per (level, iteration).
@@ -380,10 +395,10 @@ void gn_run(mf_ctx* c, const float k[4], int object, int first_inst, int n, cons
     p = base_launch(c, k, l, object);
     p.label = label_mask;
     if (mask_labels) p.moving = mask_labels;
-    const bool fused = c->track_fused[l] != 0;
-    const bool objs_grid = object && c->cfg.batch_objects && !fused && c->track_obj_ppt[l] > 0;
     const int rpt = object ? obj_rows_per_thread(c, l, n) : c->track_rpt[l];
     set_shape(p, rpt);
+    const bool fused = fused_level(c, l, track_blocks(p.w * p.h * kinds, rpt), n);
+    const bool objs_grid = object && c->cfg.batch_objects && !fused && c->track_obj_ppt[l] > 0;
     const int nblk = objs_grid ? track_obj_blocks(p.w * p.h, c->track_obj_ppt[l]) : track_blocks(p.w * p.h * kinds, rpt);
     if (!fused) finish();
     for (int it = 0; it < c->cfg.iters[l]; ++it, ++q) {
@@ -463,7 +478,7 @@ std::vector<uint64_t> base_key(mf_ctx* c, int stage, const float k[4]) {
                                fbits(k[0]), fbits(k[1]), fbits(k[2]), fbits(k[3]), (uint64_t)c->trace_on,
                                (uint64_t)(uintptr_t)c->dbg_rows};
   for (int l = 0; l < MAX_LEVELS; l++)
-    key.push_back((uint64_t)c->track_rpt[l] | ((uint64_t)c->track_fused[l] << 32) | ((uint64_t)c->track_obj_ppt[l] << 40) |
+    key.push_back((uint64_t)c->track_rpt[l] | ((uint64_t)(c->track_fused[l] + 1) << 32) | ((uint64_t)c->track_obj_ppt[l] << 40) |
                   ((uint64_t)c->track_obj_rpt[l] << 48));
   return key;
 }
@@ -1725,7 +1740,7 @@ int mf_set_tuning(mf_ctx* c, int knob, int value) {
     return 0;
   }
   if (knob >= MF_TUNE_TRACK_FUSED_L0 && knob < MF_TUNE_TRACK_FUSED_L0 + MAX_LEVELS) {
-    c->track_fused[knob - MF_TUNE_TRACK_FUSED_L0] = value ? 1 : 0;
+    c->track_fused[knob - MF_TUNE_TRACK_FUSED_L0] = value < 0 ? -1 : (value ? 1 : 0);
     return 0;
   }
   if (knob >= MF_TUNE_TRACK_ROWS_PER_THREAD_L0 && knob < MF_TUNE_TRACK_ROWS_PER_THREAD_L0 + MAX_LEVELS) {
