@@ -297,7 +297,7 @@ int make_object(mf_ctx* c, float size, int res, const float* T, long cap) {
   // mapping work lists cover every object's pool (grown with each object;
   // the previous lists stay allocated until the context is destroyed)
   const long work_cap = c->work_cap + cap;
-  int2* act_list = dalloc<int2>(c, work_cap);
+  int4* act_list = dalloc<int4>(c, work_cap);
   if (!act_list) return rollback(MF_E_OOM, "device allocation failed");
   c->work_cap = work_cap;
   c->work.act_list = act_list;

@@ -201,7 +201,7 @@ struct MapObj {
 struct MapWork {
   long table_pre[MAX_INST + 1];   // prefix of the objects' Morton-table sizes (host computed)
   int* act_count;                 // active blocks this frame
-  int2* act_list;                 // (object, slot) of each active block
+  int4* act_list;                 // (object, slot, Morton index, 0) of each active block
 };
 // flags: OBJ_ALLOC_RESET starts a mapping pass (the active-list count, and the
 // volumes' frame-start block counts that mark this pass's new blocks for

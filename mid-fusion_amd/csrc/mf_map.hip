@@ -217,8 +217,8 @@ __device__ __forceinline__ void block_coords(int32_t m, int& x, int& y, int& z) 
 // projective_functor::build_active_list: active || in_frustum(block corner),
 // over the concatenated block pools of the launch's objects (the prefix of
 // their allocated counts, computed by every workgroup); entries (object,
-// slot) of one list for the whole launch, counted in wl.act_count (zeroed by
-// k_obj_params at the start of the mapping pass).
+// slot, Morton index of the block) of one list for the whole launch, counted
+// in wl.act_count (zeroed by k_obj_params at the start of the mapping pass).
 __global__ void k_active_list(const MapObj* __restrict__ objs, int n, int w, int h, MapWork wl) {
   __shared__ long pre[MAX_INST + 1];
   if (threadIdx.x == 0) {
@@ -235,12 +235,14 @@ __global__ void k_active_list(const MapObj* __restrict__ objs, int n, int w, int
     while (j + 1 < n && pre[j + 1] <= a) j++;
     bool keep = false;
     const int s = (int)(a - pre[j]);
+    int32_t mor = 0;
     if (a < total) {
       const VolDev& v = objs[j].v;
+      mor = v.blk_morton[s];
       keep = v.blk_active[s] != 0;
       if (!keep) {
         int bx, by, bz;
-        block_coords(v.blk_morton[s], bx, by, bz);
+        block_coords(mor, bx, by, bz);
         const f3 bc = mk3((float)bx * v.voxel, (float)by * v.voxel, (float)bz * v.voxel);
         const f3 vc = affine(objs[j].prm->KT, bc);
         const int px = (int)(vc.x / vc.z), py = (int)(vc.y / vc.z);
@@ -248,7 +250,7 @@ __global__ void k_active_list(const MapObj* __restrict__ objs, int n, int w, int
       }
     }
     const int slot = wave_append(wl.act_count, keep);
-    if (keep) wl.act_list[slot] = make_int2(j, s);
+    if (keep) wl.act_list[slot] = make_int4(j, s, mor, 0);
   }
 }
 
@@ -256,49 +258,72 @@ __global__ void k_active_list(const MapObj* __restrict__ objs, int n, int w, int
 // voxel per thread (x fastest, the block's storage order).  The row start
 // and x-increment are formed exactly as the reference forms them.
 //
-// Memory round trips per block: the list entry (prefetched one block ahead),
-// the descriptor fields (held in registers, so that the voxel stores, which
-// may alias the descriptor as far as the compiler knows, do not force
-// re-reads), the block's Morton code, one batch of pixel gathers (depth,
-// mask and, for the motion-refined mask, both track codes) and one batch of
-// voxel fields with the colour gathers; the stores come last.  A block
-// allocated this frame (slot at or past the frame-start count) holds
-// voxel_traits<SDF>::initValue() (continuous/volume_traits.hpp:38) before
-// its first update, so its voxels take those values without loading them
-// and store them whether they update or not.
-__global__ __launch_bounds__(512) void k_integrate(const MapObj* __restrict__ objs, float fx, float fy,
+// Memory round trips per block: the list entry (object, slot, Morton index;
+// prefetched one block ahead), one batch of pixel gathers (depth, mask and,
+// for the motion-refined mask, both track codes) and one batch of voxel
+// fields with the colour gathers; the stores come last.  Each object's
+// descriptor fields are staged in LDS once per workgroup (the voxel stores
+// may alias the descriptor as far as the compiler knows, which would
+// otherwise force re-reads).  A block allocated this frame (slot at or past
+// the frame-start count) holds voxel_traits<SDF>::initValue()
+// (continuous/volume_traits.hpp:38) before its first update, so its voxels
+// take those values without loading them and store them whether they update
+// or not.
+struct IntegObj {
+  Mat4 Tco;
+  float *tsdf, *wt, *r, *g, *b;
+  int32_t *c, *fgw;
+  double* fg;
+  uint8_t* blk_active;
+  const float* mask;
+  const int8_t* codes;
+  float voxel;
+  int start_count;
+};
+__global__ __launch_bounds__(512) void k_integrate(const MapObj* __restrict__ objs, int n, float fx, float fy,
                                                    float cx, float cy, const float* __restrict__ depth,
                                                    const float* __restrict__ rr, const float* __restrict__ gg,
                                                    const float* __restrict__ bb, int refine, int use_icp, int use_rgb,
                                                    int w, int h, float mu, int maxweight, MapWork wl,
                                                    unsigned long long* __restrict__ stats) {
+  __shared__ IntegObj iobj[MAX_INST];
   const int tid = threadIdx.x;
+  const int nact = *wl.act_count;
+  if ((int)blockIdx.x >= nact) return;   // uniform: no block for this workgroup
+  for (int j = tid; j < n; j += (int)blockDim.x) {
+    const MapObj& O = objs[j];
+    IntegObj& d = iobj[j];
+    d.Tco = O.prm->Tco;
+    d.tsdf = O.v.tsdf; d.wt = O.v.wt; d.r = O.v.r; d.g = O.v.g; d.b = O.v.b; d.c = O.v.c; d.fgw = O.v.fgw;
+    d.fg = O.v.fg; d.blk_active = O.v.blk_active; d.mask = O.mask; d.codes = O.codes;
+    d.voxel = O.v.voxel; d.start_count = O.v.counters[3];
+  }
   const int lx = tid & 7, ly = (tid >> 3) & 7, lz = tid >> 6;
   const Mat4 K = camK(fx, fy, cx, cy);
   const size_t npx = (size_t)w * h;
-  const int nact = *wl.act_count;
   const int G = (int)gridDim.x;
-  int2 next = (int)blockIdx.x < nact ? wl.act_list[blockIdx.x] : make_int2(0, 0);
+  int4 next = wl.act_list[blockIdx.x];
+  __syncthreads();
   for (int a = blockIdx.x; a < nact; a += G) {
-    const int2 e = next;
+    const int4 e = next;
     if (a + G < nact) next = wl.act_list[a + G];
-    const int j = e.x, s = e.y;
-    const MapObj& O = objs[j];
-    const Mat4 Tco = O.prm->Tco;
-    const float vs = O.v.voxel;
-    const bool fresh = s >= O.v.counters[3];
-    const int32_t morton = O.v.blk_morton[s];
-    float* const tsdf = O.v.tsdf;
-    float* const wt = O.v.wt;
-    float* const vr = O.v.r;
-    float* const vg = O.v.g;
-    float* const vb = O.v.b;
-    int32_t* const vc = O.v.c;
-    int32_t* const vfgw = O.v.fgw;
-    double* const vfg = O.v.fg;
-    uint8_t* const blk_active = O.v.blk_active;
-    const float* const mask = O.mask;
-    const int8_t* const codes = O.codes;
+    const int s = e.y;
+    const IntegObj& D = iobj[e.x];
+    const Mat4 Tco = D.Tco;
+    const float vs = D.voxel;
+    const bool fresh = s >= D.start_count;
+    const int32_t morton = e.z;
+    float* const tsdf = D.tsdf;
+    float* const wt = D.wt;
+    float* const vr = D.r;
+    float* const vg = D.g;
+    float* const vb = D.b;
+    int32_t* const vc = D.c;
+    int32_t* const vfgw = D.fgw;
+    double* const vfg = D.fg;
+    uint8_t* const blk_active = D.blk_active;
+    const float* const mask = D.mask;
+    const int8_t* const codes = D.codes;
     const f3 delta = rotate(Tco, mk3(vs, 0.f, 0.f));
     const f3 cdelta = rotate(K, delta);
     int bx, by, bz;
@@ -403,7 +428,7 @@ void launch_integrate(hipStream_t s, const MapObj* objs, int n, long sum_cap, fl
   long ga = (sum_cap + 255) / 256;
   if (ga > 2048) ga = 2048;
   k_active_list<<<(unsigned)ga, 256, 0, s>>>(objs, n, w, h, wl);
-  k_integrate<<<4096, 512, 0, s>>>(objs, fx, fy, cx, cy, depth, r, g, b, refine, use_icp, use_rgb, w, h, mu,
+  k_integrate<<<4096, 512, 0, s>>>(objs, n, fx, fy, cx, cy, depth, r, g, b, refine, use_icp, use_rgb, w, h, mu,
                                    maxweight, wl, stats);
 }
 
