@@ -521,10 +521,10 @@ def main():
     stage = {k: round(v[0] / 3, 4) for k, v in tm.items() if v[1]}
     N0 = W * H
     rpt = [g.get_tuning(g.TUNE_TRACK_ROWS_PER_THREAD_L0 + lvl) for lvl in range(4)]
-    # the library's automatic object shape (knob 0): {8, 4, 4, 4} when two or
+    # the library's automatic object shape (knob 0): {8, 8, 4, 4} when two or
     # more objects share a launch, else the camera's
     batched = not args.plain and n_obj >= 2
-    orpt = [g.get_tuning(g.TUNE_OBJ_ROWS_PER_THREAD_L0 + lvl) or ((8, 4, 4, 4)[lvl] if batched else rpt[lvl])
+    orpt = [g.get_tuning(g.TUNE_OBJ_ROWS_PER_THREAD_L0 + lvl) or ((8, 8, 4, 4)[lvl] if batched else rpt[lvl])
             for lvl in range(4)]
     alg = {
         'track_rows': track_bytes(tr, rpt, False),

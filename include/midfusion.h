@@ -271,7 +271,7 @@ long mf_device_bytes(mf_ctx* ctx);
  *   once for every object, with 1 or 2 pixels per thread; 0 (default) gives
  *   each object its own row grid in the same launch. */
 /*   MF_TUNE_OBJ_ROWS_PER_THREAD_L0 + l: rows per thread of the moving
- *   objects' row grids at level l, 1..64, or 0 (default): {8, 4, 4, 4} when
+ *   objects' row grids at level l, 1..64, or 0 (default): {8, 8, 4, 4} when
  *   two or more objects share the launch, else the camera's. */
 enum { MF_TUNE_TRACK_ROWS_PER_THREAD_L0 = 0, MF_TUNE_TRACK_FUSED_L0 = 8, MF_TUNE_OBJ_PIXELS_PER_THREAD_L0 = 16,
        MF_TUNE_OBJ_ROWS_PER_THREAD_L0 = 24 };

@@ -339,16 +339,18 @@ void render_reference(mf_ctx* c, int first, int n) {
 }
 
 // Rows per thread of the objects' row grids at level l in a launch of n
-// objects: the knob, or when it is 0, {8, 4, 4, 4} for two or more objects
+// objects: the knob, or when it is 0, {8, 8, 4, 4} for two or more objects
 // and the camera's for one.  Most of an object's rows lie away from it and
 // end at their first loads, so with several objects the launch fills the
-// chip with fewer, longer workgroups (fewer partials to reduce); one object
-// alone needs the camera's shape to fill it.  Measured on MI355X at 640x480:
-// 8 objects 532 -> 548 frames/s, 16 objects 427 -> 450, 1280x720 with 16
-// objects 215 -> 236; {8, 4, 4} for a single object lost 1 % (5 % in the
-// one-object-per-launch form at 8 objects).
+// chip with fewer, longer workgroups (fewer partials to reduce, and at the
+// fused coarse levels every workgroup reads all of the previous iteration's
+// partials); one object alone needs the camera's shape to fill it.  Measured
+// on MI355X at 640x480: 8 objects 532 -> 548 frames/s with {8, 4, 4}, then
+// 570 -> 575 with 8 at level 1; 16 objects 427 -> 450, 1280x720 with 16
+// objects 215 -> 236 with {8, 4, 4}; {8, 4, 4} for a single object lost 1 %
+// (5 % in the one-object-per-launch form at 8 objects).
 int obj_rows_per_thread(const mf_ctx* c, int l, int n) {
-  static constexpr int kBatched[MAX_LEVELS] = {8, 4, 4, 4};
+  static constexpr int kBatched[MAX_LEVELS] = {8, 8, 4, 4};
   if (c->track_obj_rpt[l] > 0) return c->track_obj_rpt[l];
   return n >= 2 ? kBatched[l] : c->track_rpt[l];
 }
@@ -944,14 +946,15 @@ void upload_ray_tables(mf_ctx* c) {
 // Raycast of every object at the current camera pose and the objects' current
 // poses into the maps before fusion (kfusion.cpp:295-297, 432-434).  Only the
 // vertices and labels of these maps are read downstream (compute_residuals,
-// split_labels), so no normals are computed.
-void raycast_before_fusion(mf_ctx* c, const float k[4], const int* in_view, const int* gate) {
+// split_labels), so no normals are computed; the raycast flags the labels it
+// writes in `present` (the instance set split_labels takes, kfusion.cpp:299).
+void raycast_before_fusion(mf_ctx* c, const float k[4], const int* in_view, const int* gate, int* present) {
   hipStream_t s = c->stream;
   const int n = (int)c->objs.size();
   { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats, c->mapobjs + MAX_INST, n, k[0], k[1], k[2], k[3], 0, c->work); }
   Timed t(c, KC_RAYCAST);
   launch_raycast(s, c->rayobjs_bf, n, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, c->vbf[0], c->vbf[1], c->vbf[2],
-                 nullptr, nullptr, nullptr, c->label_bf, c->timing ? c->stats : nullptr, in_view, gate);
+                 nullptr, nullptr, nullptr, c->label_bf, c->timing ? c->stats : nullptr, in_view, gate, present);
 }
 
 // compute_residuals launch of one instance against model maps `mvn` (6
@@ -1028,9 +1031,10 @@ int mf_track_camera(mf_ctx* c, const float k[4], unsigned frame) {
       gn_run(c, k, 0, 0, 1, nullptr);
     }
     if (!multi) return;
-    { Timed t(c, KC_MOTION); launch_motion_gate(s, md, c->label, (int)c->N0, nobj); }
-    raycast_before_fusion(c, k, nullptr, &md->gate);
-    { Timed t(c, KC_MOTION); launch_motion_inview(s, md, c->label_bf, (int)c->N0, nobj, c->moving, autom ? 0 : 1); }
+    { Timed t(c, KC_MOTION); launch_motion_gate(s, md, nobj); }
+    // the in-view set: the labels of this raycast (cleared by the gate)
+    raycast_before_fusion(c, k, nullptr, &md->gate, md->in_view);
+    { Timed t(c, KC_MOTION); launch_motion_inview(s, md, nobj, c->moving, autom ? 0 : 1); }
     if (autom) {
       MotionLaunch m = motion_launch(c, k, c->mats + 0, c->vtx[0], c->vbf[0], c->motion_codes, &md->gate);
       m.label_bf = c->label_bf; m.counts = md->counts;
@@ -1157,8 +1161,8 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
           MotionLaunch m = motion_launch(c, k, c->mats + 0, c->vtx[0], c->vbf[0], bg->codes, &md->move_any);
           m.rgb_err = bg->rgb_err;
           launch_motion_rows(s, m); }
-        raycast_before_fusion(c, k, md->in_view, &md->move_any);
-        { Timed t(c, KC_MOTION); launch_motion_after(s, md, c->label_bf, (int)c->N0, nobj, c->cfg.use_rgb); }
+        raycast_before_fusion(c, k, md->in_view, &md->move_any, md->present);
+        { Timed t(c, KC_MOTION); launch_motion_after(s, md, nobj, c->cfg.use_rgb); }
         // each moving object's rows: motion residuals at its virtual camera
         // pose (kfusion.cpp:445-467), one launch for all of them
         { Timed t(c, KC_MOTION);
@@ -1166,7 +1170,7 @@ int mf_track_objects(mf_ctx* c, const float k[4], unsigned frame) {
                                (int)mov.size()); }
       } else {
         Timed t(c, KC_MOTION);
-        launch_motion_after(s, md, nullptr, (int)c->N0, nobj, c->cfg.use_rgb);
+        launch_motion_after(s, md, nobj, c->cfg.use_rgb);
       }
       // static objects carry the camera's rows (kfusion.cpp:399-402, 450-454)
       for (Object* o : stat) {
@@ -1237,14 +1241,17 @@ int mf_raycast(mf_ctx* c, const float k[4], unsigned frame) {
   long sum_nb3, sum_cap;
   set_mapobjs(c, n_int, n_all, sum_nb3, sum_cap);
   return run_graph(c, key, [&]() {
-    // the objects' model-map tile boxes restart empty: this raycast fills them
-    { Timed t(c, KC_PARAMS); launch_obj_params(s, c->mats + 0, c->mats, c->mapobjs + MAX_INST, n_all, k[0], k[1], k[2], k[3], OBJ_TBOX_RESET, c->work); }
+    // the objects' model-map tile boxes and the label image's instance set
+    // restart empty (this raycast fills them); raycastPose = camera_pose
+    // (kfusion.cpp:554), nothing in the raycast reads it
+    { Timed t(c, KC_PARAMS);
+      launch_obj_params(s, c->mats + 0, c->mats, c->mapobjs + MAX_INST, n_all, k[0], k[1], k[2], k[3], OBJ_TBOX_RESET,
+                        c->work, c->ray_pose, c->md->present_main); }
     Timed t(c, KC_RAYCAST);
     // objects in view only (kfusion.cpp:557-558)
     launch_raycast(s, c->rayobjs, n, c->W, c->H, c->cfg.mu, 0.1f, 4.0f, c->ray_eps, c->vtx[0], c->vtx[1],
                    c->vtx[2], c->nrm[0], c->nrm[1], c->nrm[2], c->label, c->timing ? c->stats : nullptr,
-                   c->md->in_view, nullptr);
-    launch_copy_mat(s, c->ray_pose, c->mats + 0);   // raycastPose (kfusion.cpp:554)
+                   c->md->in_view, nullptr, c->md->present_main);
   }) < 0 ? -1 : 1;
 }
 
@@ -1441,7 +1448,15 @@ int mf_get_maps(mf_ctx* c, float* vertex, float* normal, int32_t* label) {
 int mf_set_maps(mf_ctx* c, const float* vertex, const float* normal, const int32_t* label) {
   if (vertex) upload_planes(c, c->vtx, vertex, c->N0);
   if (normal) upload_planes(c, c->nrm, normal, c->N0);
-  if (label) { hipMemcpyAsync(c->label, label, c->N0 * sizeof(int32_t), hipMemcpyHostToDevice, c->stream); hipStreamSynchronize(c->stream); }
+  if (label) {
+    hipMemcpyAsync(c->label, label, c->N0 * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+    // the label image's instance set, which the frame-start gate reads
+    int present[MAX_INST] = {};
+    for (size_t i = 0; i < c->N0; i++)
+      if (label[i] >= 0 && label[i] < MAX_INST) present[label[i]] = 1;
+    hipMemcpyAsync(c->md->present_main, present, sizeof present, hipMemcpyHostToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
+  }
   return 0;
 }
 int mf_get_object_maps(mf_ctx* c, int id, float* vertex, float* normal) {

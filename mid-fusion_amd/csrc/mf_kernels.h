@@ -103,7 +103,8 @@ struct MotionDev {
   int copy_static;   // static objects take the camera's rows (kfusion.cpp:399-402, 450-454)
   int rendered_n;    // instances in the last raycast before fusion (persists across frames)
   int pad[2];
-  int present[MAX_INST];       // scratch: labels present in an image
+  int present[MAX_INST];       // labels present in the second raycast before fusion
+  int present_main[MAX_INST];  // labels present in the model maps' label image
   int in_view[MAX_INST];       // objects_in_view_ by label
   int counts[2 * MAX_INST];    // {pixels, inliers} by label of the motion check
   uint8_t move_set[MAX_INST];  // move_obj_set by label
@@ -149,20 +150,20 @@ void launch_generalize(hipStream_t s, const uint8_t* mask, const uint8_t* global
 // *dst = table[*index] (ground-truth camera pose of the frame)
 void launch_pose_from_table(hipStream_t s, Mat4* dst, const Mat4* table, const int* index);
 void launch_set_int(hipStream_t s, int* dst, int v);
-void launch_copy_mat(hipStream_t s, Mat4* dst, const Mat4* src);
-// Frame-start clear of the motion state and the presence of the last
-// raycast's labels, then the gate (and the in-view set cleared under it).
-void launch_motion_gate(hipStream_t s, MotionDev* md, const int32_t* label, int n, int nobj);
-// After the raycast before fusion: in-view set and rendered count; with
-// `manual`, the moving set is the caller's moving flags.
-void launch_motion_inview(hipStream_t s, MotionDev* md, const int32_t* label_bf, int n, int nobj,
-                          const uint8_t* moving, int manual);
+// Frame-start clear of the motion state, then the gate from the labels
+// present in the last raycast (present_main, kept by whatever writes the
+// label image) and the in-view set cleared under it.
+void launch_motion_gate(hipStream_t s, MotionDev* md, int nobj);
+// After the raycast before fusion (which flagged its labels in in_view): the
+// rendered count; with `manual`, the moving set is the caller's moving flags.
+void launch_motion_inview(hipStream_t s, MotionDev* md, int nobj, const uint8_t* moving, int manual);
 // check_static_state decision from the counts (kfusion.cpp:314-354): updates
 // the moving flags and the moving set.
 void launch_motion_classify(hipStream_t s, MotionDev* md, int nobj, uint8_t* moving);
 // Object phase: rendered count after the second raycast before fusion (when
-// objects move), then cond2 and copy_static.
-void launch_motion_after(hipStream_t s, MotionDev* md, const int32_t* label_bf, int n, int nobj, int use_rgb);
+// objects move; that raycast flags its labels in `present`, zero since the
+// frame-start clear), then cond2 and copy_static.
+void launch_motion_after(hipStream_t s, MotionDev* md, int nobj, int use_rgb);
 // virtual_camera_pose_ of tracked objects: vcam[ids[j] - 3] for instances
 // [first, first + n), unless the run was skipped.
 void launch_virtual_cams(hipStream_t s, const GNState* st, int first, int n, const int* ids, Mat4* vcam);
@@ -207,9 +208,11 @@ struct MapWork {
 // volumes' frame-start block counts that mark this pass's new blocks for
 // k_integrate to initialise), OBJ_TBOX_RESET empties the objects' model-map
 // tile boxes ahead of the raycast that fills them.
+// Optionally also *copy_to = *T_w_c and clear[0 .. MAX_INST) = 0.
 enum { OBJ_ALLOC_RESET = 1, OBJ_TBOX_RESET = 2 };
 void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx,
-                       float fy, float cx, float cy, int flags, const MapWork& wl);
+                       float fy, float cx, float cy, int flags, const MapWork& wl, Mat4* copy_to = nullptr,
+                       int* clear = nullptr);
 // Work counters accumulated on the device (algorithmic-byte accounting):
 // [0] voxel updates, [1] blocks integrated, [2] march steps, [3] trilinear
 // evaluations, [4] raycast hits, [5] allocation band samples, [6] pixels
@@ -251,8 +254,10 @@ void launch_render_volume_maps(hipStream_t s, const RayObj* objs, int nobj, int 
 void launch_render_volume_ray(hipStream_t s, const RayObj* objs, int nobj, int w, int h, const Mat4* T_w_c, float mu,
                               float nearp, float farp, float largestep, float eps, int render_color,
                               const uint32_t* palette, int npal, uint32_t* out);
+// present (optional): present[l] = 1 for every label l in [0, MAX_INST) the
+// raycast writes (split_labels' instance set, kfusion.cpp:1527-1553).
 void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,
                     float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label,
-                    unsigned long long* stats, const int* in_view, const int* gate);
+                    unsigned long long* stats, const int* in_view, const int* gate, int* present = nullptr);
 
 }  // namespace mf

@@ -42,9 +42,12 @@ __device__ void obj_params(const Mat4& Twc, const Mat4& Two, float fx, float fy,
 // Matrices of every object of a stage (one thread each), and the start of
 // this frame's allocation bookkeeping when `alloc_reset` is set.
 __global__ void k_obj_params(const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx, float fy,
-                             float cx, float cy, int flags, int* act_count) {
+                             float cx, float cy, int flags, int* act_count, Mat4* copy_to, int* clear) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if ((flags & OBJ_ALLOC_RESET) && j == 0) *act_count = 0;
+  if (copy_to && j == 0) *copy_to = *T_w_c;
+  if (clear && blockIdx.x == 0)
+    for (int k = threadIdx.x; k < MAX_INST; k += blockDim.x) clear[k] = 0;
   if (j >= n) return;
   const MapObj& o = objs[j];
   obj_params(*T_w_c, mats[3 + o.id], fx, fy, cx, cy, o.prm);
@@ -408,8 +411,9 @@ __global__ __launch_bounds__(512) void k_integrate(const MapObj* __restrict__ ob
 // Every launcher covers the `n` objects of `objs` (device table) in one
 // launch; a one-object-per-launch caller passes n = 1 and a table offset.
 void launch_obj_params(hipStream_t s, const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx,
-                       float fy, float cx, float cy, int flags, const MapWork& wl) {
-  k_obj_params<<<(n + 63) / 64, 64, 0, s>>>(T_w_c, mats, objs, n, fx, fy, cx, cy, flags, wl.act_count);
+                       float fy, float cx, float cy, int flags, const MapWork& wl, Mat4* copy_to, int* clear) {
+  k_obj_params<<<(n + 63) / 64, 64, 0, s>>>(T_w_c, mats, objs, n, fx, fy, cx, cy, flags, wl.act_count, copy_to,
+                                            clear);
 }
 void launch_alloc(hipStream_t s, const MapObj* objs, int n, const float* depth, int refine, int use_icp,
                   int use_rgb, int w, int h, float band, unsigned long long* stats) {

@@ -335,7 +335,8 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
                                                  float* __restrict__ nx, float* __restrict__ ny,
                                                  float* __restrict__ nz, int32_t* __restrict__ label,
                                                  unsigned long long* __restrict__ stats,
-                                                 const int* __restrict__ in_view, const int* __restrict__ gate) {
+                                                 const int* __restrict__ in_view, const int* __restrict__ gate,
+                                                 int* __restrict__ present) {
   if (gate && *gate == 0) return;
   __shared__ unsigned cnt[5];
   __shared__ uint32_t st_par[MAX_LG + 1][RT];
@@ -366,6 +367,7 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
   const int y = blockIdx.y * RT_TILE + (wv >> 1) * 8 + (lane >> 3);
   unsigned nsteps = 0, ninterp = 0, nhits = 0, npairs = 0, nnodes = 0;
   int surf = -1;   // object whose maps get a surface pixel here (not the miss constants)
+  int plabel = -1; // the pixel's label, for the instance set
   if (x < w && y < h) {
   const size_t pix = (size_t)x + (size_t)y * w;
   float nearest = 1.0e3f;
@@ -437,6 +439,7 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
   if (vx) { vx[pix] = out_v.x; vy[pix] = out_v.y; vz[pix] = out_v.z; }
   if (nx) { nx[pix] = out_n.x; ny[pix] = out_n.y; nz[pix] = out_n.z; }
   label[pix] = out_l;
+  plabel = out_l;
   for (int oi = 0; oi < nobj; oi++) {
     const RayObj& O = objs[oi];
     const bool mine = (out_l == O.label);
@@ -447,6 +450,16 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
          __float_as_uint(out_n.z) | (__float_as_uint(out_n.x) ^ __float_as_uint((float)INVALID))) != 0u)
       surf = oi;
   }
+  }
+  // The instance set of the label image: one same-value store per distinct
+  // label of the wave.
+  if (present) {
+    bool todo = plabel >= 0 && plabel < MAX_INST;
+    for (uint64_t b = __ballot(todo); b != 0; b = __ballot(todo)) {
+      const int l0 = __shfl(plabel, __ffsll((unsigned long long)b) - 1);
+      if (plabel == l0) todo = false;
+      if (lane == __ffsll((unsigned long long)b) - 1) present[l0] = 1;
+    }
   }
   // Tile flags of the objects' model maps (TrackInst::tiles; each wave
   // covers one 8x8 tile): 1 when a pixel of the tile got anything but the
@@ -624,14 +637,14 @@ void launch_render_volume_ray(hipStream_t s, const RayObj* objs, int nobj, int w
 
 void launch_raycast(hipStream_t s, const RayObj* objs, int nobj, int w, int h, float mu, float nearp, float farp,
                     float eps, float* vx, float* vy, float* vz, float* nx, float* ny, float* nz, int32_t* label,
-                    unsigned long long* stats, const int* in_view, const int* gate) {
+                    unsigned long long* stats, const int* in_view, const int* gate, int* present) {
   dim3 blk(RT), grd((w + RT_TILE - 1) / RT_TILE, (h + RT_TILE - 1) / RT_TILE);
   if (stats)
     k_raycast<true><<<grd, blk, 0, s>>>(objs, nobj, w, h, mu, nearp, farp, eps, vx, vy, vz, nx, ny, nz, label, stats,
-                                        in_view, gate);
+                                        in_view, gate, present);
   else
     k_raycast<false><<<grd, blk, 0, s>>>(objs, nobj, w, h, mu, nearp, farp, eps, vx, vy, vz, nx, ny, nz, label,
-                                         nullptr, in_view, gate);
+                                         nullptr, in_view, gate, present);
 }
 
 }  // namespace mf

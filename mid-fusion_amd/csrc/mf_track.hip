@@ -1399,41 +1399,23 @@ __global__ void k_gn_begin_objects(GNState* st, int first, int n, const int* ids
 // and the moving decision and gating of Kfusion::tracking (kfusion.cpp:
 // 286-471).
 
-// Labels present in an image: flags[l] = 1 for l in [0, MAX_INST) (the
-// split_labels instance set, kfusion.cpp:1527-1553).  Same-value stores, so
-// the order of the workgroups does not matter.
-__device__ void presence(const int32_t* __restrict__ label, int n, int* flags) {
-  __shared__ int f[MAX_INST];
-  for (int j = threadIdx.x; j < MAX_INST; j += blockDim.x) f[j] = 0;
-  __syncthreads();
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int l = label[i];
-    if (l >= 0 && l < MAX_INST) f[l] = 1;
-  }
-  __syncthreads();
-  for (int j = threadIdx.x; j < MAX_INST; j += blockDim.x)
-    if (f[j]) flags[j] = 1;
-}
+// Instance sets (split_labels, kfusion.cpp:1527-1553) are flags by label,
+// flags[l] = 1 for each label l in [0, MAX_INST) of an image, written by the
+// raycast that writes the image (k_raycast's `present`).
 __device__ int count_flags(const int* f, int nobj) {
   int c = 0;
   for (int j = 0; j < nobj; j++) c += f[j] != 0;
   return c;
 }
-__global__ void k_motion_clear(MotionDev* md) {
+// Frame start: the motion state cleared, then gate = more than one instance
+// in the last raycast (kfusion.cpp:292); under it the in-view set is cleared
+// before the raycast of every object (:294).
+__global__ void k_motion_gate(MotionDev* md, int nobj) {
   for (int j = threadIdx.x; j < MAX_INST; j += blockDim.x) {
     md->present[j] = 0; md->move_set[j] = 0; md->counts[2 * j] = 0; md->counts[2 * j + 1] = 0;
   }
-  if (threadIdx.x == 0) { md->gate = 0; md->move_any = 0; md->retrack = 0; }
-}
-__global__ void k_presence(const int32_t* __restrict__ label, int n, int* flags, const int* gate) {
-  if (gate && *gate == 0) return;
-  presence(label, n, flags);
-}
-// gate = more than one instance in the last raycast (kfusion.cpp:292); under
-// it the in-view set is cleared before the raycast of every object (:294)
-__global__ void k_motion_gate(MotionDev* md, int nobj) {
-  const int g = count_flags(md->present, nobj) > 1 ? 1 : 0;
-  if (threadIdx.x == 0) md->gate = g;
+  const int g = count_flags(md->present_main, nobj) > 1 ? 1 : 0;
+  if (threadIdx.x == 0) { md->gate = g; md->move_any = 0; md->retrack = 0; }
   if (g)
     for (int j = threadIdx.x; j < MAX_INST; j += blockDim.x) md->in_view[j] = 0;
 }
@@ -1459,10 +1441,6 @@ __global__ void k_motion_classify(MotionDev* md, int nobj, uint8_t* moving) {
   }
   md->move_any = any;
   md->retrack = (any && md->rendered_n != 0) ? 1 : 0;
-}
-__global__ void k_motion_clear_present(MotionDev* md) {
-  if (!md->move_any) return;
-  for (int j = threadIdx.x; j < MAX_INST; j += blockDim.x) md->present[j] = 0;
 }
 __global__ void k_motion_after(MotionDev* md, int nobj, int use_rgb) {
   if (md->move_any) md->rendered_n = count_flags(md->present, nobj);
@@ -1716,8 +1694,6 @@ void launch_solve6(hipStream_t s, const double* s27, int n, double* x, int* solv
 
 __global__ void k_pose_from_table(Mat4* dst, const Mat4* table, const int* index) { *dst = table[*index]; }
 __global__ void k_set_int(int* dst, int v) { *dst = v; }
-__global__ void k_copy_mat(Mat4* dst, const Mat4* src) { *dst = *src; }
-void launch_copy_mat(hipStream_t s, Mat4* dst, const Mat4* src) { k_copy_mat<<<1, 1, 0, s>>>(dst, src); }
 void launch_set_int(hipStream_t s, int* dst, int v) { k_set_int<<<1, 1, 0, s>>>(dst, v); }
 void launch_pose_from_table(hipStream_t s, Mat4* dst, const Mat4* table, const int* index) {
   k_pose_from_table<<<1, 1, 0, s>>>(dst, table, index);
@@ -1728,25 +1704,14 @@ void launch_motion_rows(hipStream_t s, const MotionLaunch& m) {
 void launch_motion_rows_n(hipStream_t s, const MotionLaunch& base, const MotionObj* objs, int n) {
   if (n > 0) k_motion_rows_n<<<dim3((base.W + 255) / 256, base.H, n), 256, 0, s>>>(base, objs);
 }
-static unsigned presence_blocks(int n) { return (unsigned)std::min(256, (n + 1023) / 1024); }
-void launch_motion_gate(hipStream_t s, MotionDev* md, const int32_t* label, int n, int nobj) {
-  k_motion_clear<<<1, 64, 0, s>>>(md);
-  k_presence<<<presence_blocks(n), 256, 0, s>>>(label, n, md->present, nullptr);
-  k_motion_gate<<<1, 64, 0, s>>>(md, nobj);
-}
-void launch_motion_inview(hipStream_t s, MotionDev* md, const int32_t* label_bf, int n, int nobj,
-                          const uint8_t* moving, int manual) {
-  k_presence<<<presence_blocks(n), 256, 0, s>>>(label_bf, n, md->in_view, &md->gate);
+void launch_motion_gate(hipStream_t s, MotionDev* md, int nobj) { k_motion_gate<<<1, 64, 0, s>>>(md, nobj); }
+void launch_motion_inview(hipStream_t s, MotionDev* md, int nobj, const uint8_t* moving, int manual) {
   k_motion_inview<<<1, 1, 0, s>>>(md, nobj, moving, manual);
 }
 void launch_motion_classify(hipStream_t s, MotionDev* md, int nobj, uint8_t* moving) {
   k_motion_classify<<<1, 1, 0, s>>>(md, nobj, moving);
 }
-void launch_motion_after(hipStream_t s, MotionDev* md, const int32_t* label_bf, int n, int nobj, int use_rgb) {
-  if (label_bf) {
-    k_motion_clear_present<<<1, 64, 0, s>>>(md);
-    k_presence<<<presence_blocks(n), 256, 0, s>>>(label_bf, n, md->present, &md->move_any);
-  }
+void launch_motion_after(hipStream_t s, MotionDev* md, int nobj, int use_rgb) {
   k_motion_after<<<1, 1, 0, s>>>(md, nobj, use_rgb);
 }
 void launch_virtual_cams(hipStream_t s, const GNState* st, int first, int n, const int* ids, Mat4* vcam) {

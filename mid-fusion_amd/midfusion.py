@@ -534,7 +534,7 @@ class Kfusion:
     TUNE_TRACK_ROWS_PER_THREAD_L0 = 0   # + pyramid level
     TUNE_TRACK_FUSED_L0 = 8             # + pyramid level (1 fused, 0 two launches, -1 automatic)
     TUNE_OBJ_PIXELS_PER_THREAD_L0 = 16  # + pyramid level
-    TUNE_OBJ_ROWS_PER_THREAD_L0 = 24    # + pyramid level (0: {8, 4, 4, 4} for 2+ objects per launch, else the camera's)
+    TUNE_OBJ_ROWS_PER_THREAD_L0 = 24    # + pyramid level (0: {8, 8, 4, 4} for 2+ objects per launch, else the camera's)
     TUNE_CANDIDATES = (1, 2, 3, 4, 8, 16)   # rows per thread tried by autotune()
 
     def set_tuning(self, knob, value):

@@ -57,7 +57,7 @@ def test_batched_plain_and_per_object_calls_agree():
     ctx = []
     for batch in (True, False, True):
         g = midfusion.Kfusion(W, H, P0, bg_size=5.12, bg_res=512, use_rgb=True, batch_objects=batch)
-        for lvl, v in enumerate((8, 4, 4)):
+        for lvl, v in enumerate((8, 8, 4)):
             g.set_tuning(g.TUNE_OBJ_ROWS_PER_THREAD_L0 + lvl, v)
         for i, m in enumerate(movers):
             g.add_object(m.vol_size, m.vol_res, m.pose(0).astype(np.float32))
