@@ -876,6 +876,25 @@ __device__ __forceinline__ void inverse_cols(float (&a)[4], int lane) {
       if (cc >= 4) a[u] = a[u] - f[u] * a[i];
   }
 }
+// 1/n! for n = 0..20 (exact to the double's rounding of each quotient).
+struct InvFact {
+  double v[21];
+  constexpr InvFact() : v() {
+    double f = 1.0;
+    for (int n = 0; n < 21; n++) {
+      if (n > 0) f = f * n;
+      v[n] = 1.0 / f;
+    }
+  }
+};
+__device__ constexpr InvFact kInvFact{};
+// sum_{k=0..8} (-t)^k / (2k + o)!, Horner from k = 8.
+__device__ __forceinline__ double alt_series(double t, int o) {
+  double s = kInvFact.v[16 + o];
+#pragma unroll
+  for (int k = 7; k >= 0; k--) s = kInvFact.v[2 * k + o] - t * s;
+  return s;
+}
 // SE3 exponential, x[0..2] translation, x[3..5] rotation (TooN SE3::exp as
 // used at tracking.cpp:1361): Rodrigues + left Jacobian in FP64.
 __device__ Mat4 se3_exp(const double x[6]) {
@@ -884,6 +903,14 @@ __device__ Mat4 se3_exp(const double x[6]) {
   double A, B, C;
   if (th2 < 1e-8) { A = 1.0 - th2 / 6.0; B = 0.5 - th2 / 24.0; C = 1.0 / 6.0 - th2 / 120.0; }
   else if (th2 < 1e-6) { C = (1.0 / 6.0) * (1.0 - th2 / 20.0); A = 1.0 - th2 * C; B = 0.5 - th2 / 24.0; }
+  else if (th2 < 0.25) {
+    // the same three functions by their series in th2: sin(th)/th,
+    // (1 - cos(th))/th2 and (1 - A)/th2 are sum_k (-th2)^k / (2k+1)!, (2k+2)!
+    // and (2k+3)!; below 0.25 the terms past k = 8 are under 1e-22 of the
+    // sums, which keeps sin, cos and three divisions off the solve's serial
+    // path (and the cancellation in 1 - cos(th) out of B and C)
+    A = alt_series(th2, 1); B = alt_series(th2, 2); C = alt_series(th2, 3);
+  }
   else { const double th = sqrt(th2); A = sin(th) / th; B = (1.0 - cos(th)) / th2; C = (1.0 - A) / th2; }
   const double W[3][3] = {{0, -w2, w1}, {w2, 0, -w0}, {-w1, w0, 0}};
   double W2[3][3];
