@@ -428,10 +428,11 @@ def main():
     ap.add_argument('--steps', type=int, default=600)
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--objects', type=int, default=8)
-    ap.add_argument('--frames', type=int, default=30,
+    ap.add_argument('--frames', type=int, default=16,
                     help='distinct frames rendered for the GPU legs; longer runs play them forward and backward '
-                         '(30: the camera stays within a few cm of its ground truth at 8 objects; over 120 '
-                         'distinct frames it loses track after about 50)')
+                         '(16: the camera drifts to about 7 cm from its ground truth over 600 steps at 8 objects; '
+                         'with 30 it can lose track near the turnaround at frames 25-29, and over 120 distinct '
+                         'frames it loses track after about 50)')
     ap.add_argument('--plain', action='store_true', help='headline in the one-object-per-launch form')
     ap.add_argument('--no-plain', action='store_true', help='skip the one-object-per-launch leg')
     ap.add_argument('--no-single', action='store_true', help='skip the single-object workload leg')
