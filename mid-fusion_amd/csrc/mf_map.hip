@@ -39,8 +39,10 @@ __device__ void obj_params(const Mat4& Twc, const Mat4& Two, float fx, float fy,
   p.view = mul(Twc, camKinv(fx, fy, cx, cy));
   *out = p;
 }
-// Matrices of every object of a stage (one thread each), and the start of
-// this frame's allocation bookkeeping when `alloc_reset` is set.
+// Matrices of every object of a stage (one thread each), the start of this
+// frame's allocation bookkeeping with OBJ_ALLOC_RESET, and two small jobs of
+// the launch that precedes a raycast: *copy_to = *T_w_c and clear[0 ..
+// MAX_INST) = 0 (each optional).
 __global__ void k_obj_params(const Mat4* T_w_c, const Mat4* mats, const MapObj* objs, int n, float fx, float fy,
                              float cx, float cy, int flags, int* act_count, Mat4* copy_to, int* clear) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;

@@ -456,9 +456,10 @@ __global__ __launch_bounds__(RT) void k_raycast(const RayObj* __restrict__ objs,
   if (present) {
     bool todo = plabel >= 0 && plabel < MAX_INST;
     for (uint64_t b = __ballot(todo); b != 0; b = __ballot(todo)) {
-      const int l0 = __shfl(plabel, __ffsll((unsigned long long)b) - 1);
+      const int first = __ffsll((unsigned long long)b) - 1;
+      const int l0 = __shfl(plabel, first);
       if (plabel == l0) todo = false;
-      if (lane == __ffsll((unsigned long long)b) - 1) present[l0] = 1;
+      if (lane == first) present[l0] = 1;
     }
   }
   // Tile flags of the objects' model maps (TrackInst::tiles; each wave
